@@ -1,0 +1,248 @@
+"""ctypes binding of the engine's C ABI (``include/lsseg/lsseg.h``).
+
+``Context`` owns one ``lsseg_ctx``: a padded, tiled copy of the volume and all
+device buffers.  Every method maps to one C entry point and raises
+``LssegError`` on a non-OK status.  There is no fallback: if the HIP library
+cannot be loaded, construction fails.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _build
+
+OK = 0
+ERR_INVALID_ARG, ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_CAPACITY, ERR_STATE, ERR_UNSUPPORTED = range(1, 7)
+U8, I8, U16, I16, U32, I32, F32 = range(7)
+MODE_REF, MODE_CLEAN = 0, 1
+
+DTYPE_CODE = {np.dtype(np.uint8): U8, np.dtype(np.int8): I8, np.dtype(np.uint16): U16,
+              np.dtype(np.int16): I16, np.dtype(np.uint32): U32, np.dtype(np.int32): I32,
+              np.dtype(np.float32): F32}
+
+# every entry point declared in include/lsseg/lsseg.h
+EXPORTS = (
+    "lsseg_create", "lsseg_destroy", "lsseg_last_error", "lsseg_get_dims", "lsseg_seed_sphere",
+    "lsseg_set_phi", "lsseg_iterate", "lsseg_iterate_reinit", "lsseg_get_iteration_counts",
+    "lsseg_get_active_count", "lsseg_get_active", "lsseg_get_phi", "lsseg_get_mask",
+    "lsseg_freeze", "lsseg_clear", "lsseg_get_state", "lsseg_set_state",
+    "lsseg_get_device_buffers",
+)
+
+
+class LssegError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"lsseg status {status}: {message}")
+        self.status = status
+
+
+class VolumeDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_uint32), ("ny", ctypes.c_uint32), ("nz", ctypes.c_uint32),
+                ("dtype", ctypes.c_int32), ("z_aniso", ctypes.c_float)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("T", ctypes.c_float), ("eps", ctypes.c_float), ("alpha", ctypes.c_float),
+                ("dt", ctypes.c_float), ("mode", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_uint64), ("sum_active", ctypes.c_uint64),
+                ("sum_candidates", ctypes.c_uint64), ("sum_new_active", ctypes.c_uint64),
+                ("active_final", ctypes.c_uint64), ("few_path_iters", ctypes.c_uint64),
+                ("many_path_iters", ctypes.c_uint64), ("device_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def library_path() -> str:
+    return _build.hip_library()
+
+
+def lib():
+    """Load liblsseg_hip.so and declare its signatures."""
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(library_path())
+        vp, u32, u64, i32, f32 = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32,
+                                  ctypes.c_float)
+        P = ctypes.POINTER
+        sig = {
+            "lsseg_create": (i32, [P(VolumeDesc), vp, P(vp)]),
+            "lsseg_destroy": (i32, [vp]),
+            "lsseg_last_error": (ctypes.c_char_p, [vp]),
+            "lsseg_get_dims": (i32, [vp, P(u32), P(u32), P(u32)]),
+            "lsseg_seed_sphere": (i32, [vp, f32, f32, f32, u32]),
+            "lsseg_set_phi": (i32, [vp, vp, f32]),
+            "lsseg_iterate": (i32, [vp, P(Params), i32, P(Stats)]),
+            "lsseg_iterate_reinit": (i32, [vp, P(Params)]),
+            "lsseg_get_iteration_counts": (i32, [vp, vp, vp, vp, u64, P(u64)]),
+            "lsseg_get_active_count": (i32, [vp, P(u64)]),
+            "lsseg_get_active": (i32, [vp, vp, u64, P(u64)]),
+            "lsseg_get_phi": (i32, [vp, i32, vp]),
+            "lsseg_get_mask": (i32, [vp, vp]),
+            "lsseg_freeze": (i32, [vp]),
+            "lsseg_clear": (i32, [vp, i32]),
+            "lsseg_get_state": (i32, [vp, vp, vp, vp, vp, u64, P(u64)]),
+            "lsseg_set_state": (i32, [vp, vp, vp, vp, vp, u64]),
+            "lsseg_get_device_buffers": (i32, [vp, P(vp), P(vp), P(vp)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclass
+class IterationCounts:
+    n: np.ndarray   # active voxels entering each iteration
+    c: np.ndarray   # candidate emissions before duplicate removal
+    m: np.ndarray   # entries of the next active list
+
+
+class Context:
+    """One segmentation context (the C ABI's ``lsseg_ctx``)."""
+
+    def __init__(self, volume_zyx: np.ndarray, z_aniso: float = 1.0):
+        vol = np.ascontiguousarray(volume_zyx)
+        if vol.ndim != 3:
+            raise ValueError("volume must be 3-D (z, y, x)")
+        if vol.dtype not in DTYPE_CODE:
+            raise ValueError(f"unsupported dtype {vol.dtype}")
+        self.nz, self.ny, self.nx = (int(v) for v in vol.shape)
+        self.dtype = vol.dtype
+        L = lib()
+        desc = VolumeDesc(self.nx, self.ny, self.nz, DTYPE_CODE[vol.dtype], z_aniso)
+        h = ctypes.c_void_p()
+        st = L.lsseg_create(ctypes.byref(desc), _p(vol), ctypes.byref(h))
+        if st != OK:
+            raise LssegError(st, L.lsseg_last_error(None).decode())
+        self._h = h
+        X, Y, Z = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(L.lsseg_get_dims(self._h, ctypes.byref(X), ctypes.byref(Y), ctypes.byref(Z)))
+        self.X, self.Y, self.Z = X.value, Y.value, Z.value
+        self.N = self.X * self.Y * self.Z
+
+    # -- plumbing --
+    def _check(self, st: int):
+        if st != OK:
+            raise LssegError(st, lib().lsseg_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lsseg_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- initialisation --
+    def set_phi(self, phi0_zyx: np.ndarray, band: float = 1.0):
+        phi = np.ascontiguousarray(phi0_zyx, dtype=np.float32)
+        if phi.shape != (self.nz, self.ny, self.nx):
+            raise ValueError("phi0 shape mismatch")
+        self._check(lib().lsseg_set_phi(self._h, _p(phi), band))
+
+    def seed_sphere(self, cx: float, cy: float, cz: float, radius: int):
+        self._check(lib().lsseg_seed_sphere(self._h, cx, cy, cz, int(radius)))
+
+    # -- iteration --
+    def iterate(self, T, eps, alpha, dt, iters, mode=MODE_REF) -> Stats:
+        prm = Params(T, eps, alpha, dt, mode)
+        st = Stats()
+        self._check(lib().lsseg_iterate(self._h, ctypes.byref(prm), int(iters), ctypes.byref(st)))
+        return st
+
+    def iterate_reinit(self, T, eps, alpha, dt, mode=MODE_REF):
+        prm = Params(T, eps, alpha, dt, mode)
+        self._check(lib().lsseg_iterate_reinit(self._h, ctypes.byref(prm)))
+
+    def iteration_counts(self) -> IterationCounts:
+        cnt = ctypes.c_uint64()
+        self._check(lib().lsseg_get_iteration_counts(self._h, None, None, None, 0, ctypes.byref(cnt)))
+        k = cnt.value
+        n = np.zeros(max(k, 1), np.uint32)
+        c = np.zeros(max(k, 1), np.uint64)
+        m = np.zeros(max(k, 1), np.uint32)
+        self._check(lib().lsseg_get_iteration_counts(self._h, _p(n), _p(c), _p(m), k, ctypes.byref(cnt)))
+        return IterationCounts(n[:k], c[:k], m[:k])
+
+    # -- readback --
+    @property
+    def active_count(self) -> int:
+        n = ctypes.c_uint64()
+        self._check(lib().lsseg_get_active_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def active(self) -> np.ndarray:
+        n = self.active_count
+        out = np.zeros(max(n, 1), np.uint32)
+        got = ctypes.c_uint64()
+        self._check(lib().lsseg_get_active(self._h, _p(out), len(out), ctypes.byref(got)))
+        return out[:got.value].copy()
+
+    def phi(self, which: int = 0) -> np.ndarray:
+        out = np.zeros((self.nz, self.ny, self.nx), np.float32)
+        self._check(lib().lsseg_get_phi(self._h, which, _p(out)))
+        return out
+
+    def mask(self) -> np.ndarray:
+        out = np.zeros((self.nz, self.ny, self.nx), np.uint8)
+        self._check(lib().lsseg_get_mask(self._h, _p(out)))
+        return out
+
+    def freeze(self):
+        self._check(lib().lsseg_freeze(self._h))
+
+    def clear(self, all_segmentations: bool = False):
+        self._check(lib().lsseg_clear(self._h, 1 if all_segmentations else 0))
+
+    def state(self):
+        """Raw padded tiled state: (phi_read, phi_write, export, list)."""
+        r = np.zeros(self.N, np.float32)
+        w = np.zeros(self.N, np.float32)
+        e = np.zeros(self.N, np.int8)
+        n = self.active_count
+        lst = np.zeros(max(n, 1), np.uint32)
+        got = ctypes.c_uint64()
+        self._check(lib().lsseg_get_state(self._h, _p(r), _p(w), _p(e), _p(lst), len(lst),
+                                          ctypes.byref(got)))
+        return r, w, e, lst[:got.value].copy()
+
+    def set_state(self, phi_read, phi_write, export, lst):
+        r = np.ascontiguousarray(phi_read, np.float32)
+        w = np.ascontiguousarray(phi_write, np.float32)
+        e = np.ascontiguousarray(export, np.int8)
+        v = np.ascontiguousarray(lst, np.uint32)
+        if r.size != self.N or w.size != self.N or e.size != self.N:
+            raise ValueError("state arrays must have X*Y*Z entries")
+        self._check(lib().lsseg_set_state(self._h, _p(r), _p(w), _p(e), _p(v) if v.size else None, v.size))
+
+    def device_buffers(self):
+        r, w, s = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        self._check(lib().lsseg_get_device_buffers(self._h, ctypes.byref(r), ctypes.byref(w), ctypes.byref(s)))
+        return r.value, w.value, s.value

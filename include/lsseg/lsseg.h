@@ -1,0 +1,202 @@
+/*
+ * lsseg.h -- C ABI of the MI355X (gfx950) sparse level-set segmentation engine.
+ *
+ * This is the drop-in boundary for the reference's GPU segmentation path.  The
+ * reference exposes the same path at two levels, and this ABI replaces both:
+ *
+ *   - the C++ entry point `Segmenter` (code/gpuseg/code/Segmenter.hpp:40-73),
+ *     which owns all device buffers and drives the per-iteration loop
+ *     (Segmenter.cpp:759-1051);
+ *   - the `extern "C"` kernel launchers behind it
+ *     (code/gpuseg/code/cuda/CudaKernels.hpp:9-95), which take raw device
+ *     pointers plus implicitly bound texture references.
+ *
+ * One opaque context owns every device buffer (both phi buffers, the export
+ * byte, the active lists, scratch and counters), so there is no global state and
+ * no texture binding.  Every call returns a status and never aborts; the text of
+ * the last error is available from lsseg_last_error().  A context is externally
+ * synchronised; separate contexts may be used from separate threads.
+ *
+ * Conventions
+ *   - Volumes crossing the ABI are linear, x fastest, unpadded (nx*ny*nz).
+ *     Internally each axis is padded to a multiple of 16
+ *     (Segmenter.cpp:1131-1204) and stored in the reference's 16^3-tiled order
+ *     (Coordinates.cu:65-96).
+ *   - Active-list entries returned by lsseg_get_active() are tiled indices tau of
+ *     the padded volume: tau = ((tz*TY + ty)*TX + tx)*4096 + (lz*16 + ly)*16 + lx
+ *     with (tx, lx) = (x/16, x%16) and likewise for y, z; TX = X/16, TY = Y/16.
+ *   - The caller owns host arrays; the context copies them in and out.
+ */
+#ifndef LSSEG_LSSEG_H
+#define LSSEG_LSSEG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lsseg_ctx lsseg_ctx;
+
+typedef enum lsseg_status {
+    LSSEG_OK = 0,
+    LSSEG_ERR_INVALID_ARG = 1,   /* null pointer, bad size, bad parameter */
+    LSSEG_ERR_OUT_OF_MEMORY = 2, /* device or host allocation failed */
+    LSSEG_ERR_DEVICE = 3,        /* HIP runtime error or device-side error flag */
+    LSSEG_ERR_CAPACITY = 4,      /* caller buffer too small */
+    LSSEG_ERR_STATE = 5,         /* call not valid in the current state */
+    LSSEG_ERR_UNSUPPORTED = 6    /* volume too large, unsupported type */
+} lsseg_status;
+
+/* Source element type: VolumeDesc::numBytesPerVoxel x isSigned
+   (VolumeDesc.hpp:12-38, Textures.cu:5-10), plus 32-bit float sources. */
+typedef enum lsseg_dtype {
+    LSSEG_U8 = 0, LSSEG_I8 = 1, LSSEG_U16 = 2, LSSEG_I16 = 3,
+    LSSEG_U32 = 4, LSSEG_I32 = 5, LSSEG_F32 = 6
+} lsseg_dtype;
+
+/* Replaces VolumeDesc (VolumeDesc.hpp:12-38): dims, type, z anisotropy. */
+typedef struct lsseg_volume_desc {
+    uint32_t nx, ny, nz;
+    int32_t dtype;       /* lsseg_dtype */
+    float z_aniso;       /* VolumeDesc::zAnisotropy, used by the seed sphere */
+} lsseg_volume_desc;
+
+typedef enum lsseg_mode {
+    LSSEG_MODE_REF = 0,   /* shipped rule: list positions p % 4 == 3 never emit
+                             (UpdateActiveElementsVolume.cu:82,329) */
+    LSSEG_MODE_CLEAN = 1  /* every position emits (ablation) */
+} lsseg_mode;
+
+/* Replaces the Segmenter's global parameters growLevel, growWindow,
+   curvatureInfluence and timeStep (Segmenter.cpp:34-40, passed at :811-822).
+   For integer sources T and eps are truncated to int as the reference does
+   (Segmenter.cpp:817-818); for float sources they are used as given. */
+typedef struct lsseg_params {
+    float T;       /* target intensity (growLevel) */
+    float eps;     /* window (growWindow, maxDistanceBeforeShrink) */
+    float alpha;   /* curvature weight (curvatureInfluence) */
+    float dt;      /* time step (timeStep); dt < 0 flips the data term */
+    int32_t mode;  /* lsseg_mode */
+} lsseg_params;
+
+/* Per-call statistics (sums over the iterations of one lsseg_iterate call). */
+typedef struct lsseg_stats {
+    uint64_t iterations;       /* iterations run */
+    uint64_t sum_active;       /* sum of n_k: active-voxel updates */
+    uint64_t sum_candidates;   /* sum of c_k: candidate emissions before dedup */
+    uint64_t sum_new_active;   /* sum of m_k: entries written to the next lists */
+    uint64_t active_final;     /* n after the last iteration */
+    uint64_t few_path_iters;   /* iterations that took the sparse ("few") path */
+    uint64_t many_path_iters;  /* iterations that took the dense ("many") path */
+    double   device_ms;        /* device time of the call (HIP events) */
+} lsseg_stats;
+
+/* ---- lifetime ----------------------------------------------------------- */
+
+/* Replaces Segmenter::LoadVolume (Segmenter.cpp:110-285): pads, tiles and
+   uploads the source; allocates every buffer; phi := 1, export := 127. */
+lsseg_status lsseg_create(const lsseg_volume_desc* desc, const void* host_volume,
+                          lsseg_ctx** out_ctx);
+
+/* Replaces Segmenter::UnloadVolume / ~Segmenter (Segmenter.cpp:97-108, 287-340). */
+lsseg_status lsseg_destroy(lsseg_ctx* ctx);
+
+/* Text of the last error on this context ("" if none).  With ctx == NULL,
+   the last error of a failed lsseg_create on this thread. */
+const char* lsseg_last_error(const lsseg_ctx* ctx);
+
+/* Padded dims (multiples of 16) of the context. */
+lsseg_status lsseg_get_dims(const lsseg_ctx* ctx, uint32_t* X, uint32_t* Y, uint32_t* Z);
+
+/* ---- initialisation ----------------------------------------------------- */
+
+/* Replaces Segmenter::InitializeSeed (Segmenter.cpp:491-595) with the reference
+   semantics: the centre is truncated to unsigned ints, two init passes with a
+   buffer swap after each union the sphere ramp into phi
+   (InitializeLevelSetVolume.cu:55-98), then the active list is every voxel
+   whose 6-neighbour differences exceed 0.001
+   (InitializeActiveElementsVolume.cu:37-74) in ascending tiled order. */
+lsseg_status lsseg_seed_sphere(lsseg_ctx* ctx, float cx, float cy, float cz, uint32_t radius);
+
+/* Injects phi0 (unpadded, linear) into both phi buffers, export := floor(127
+   phi0) saturated, padded voxels := 1.0; active list := {|phi0| < band} in
+   ascending tiled order.  No reference counterpart: the benchmark
+   configurations start from a signed-distance phi0. */
+lsseg_status lsseg_set_phi(lsseg_ctx* ctx, const float* phi0, float band);
+
+/* ---- iteration ---------------------------------------------------------- */
+
+/* Replaces the sub-step loop of Segmenter::RequestUpdateSegmentation
+   (Segmenter.cpp:772-1036) and the kernel launchers it calls
+   (CudaUpdateLevelSetVolumeAsync, CudaOutputNewActiveElements,
+   CudaUpdateActiveElementsVolume, CudaMemSetCharSparse, CudaFilterDuplicates,
+   CudppCompact, CudaMemSetIntSparse; CudaKernels.hpp:25-87, Cuda.cpp:699-758).
+   Runs `iters` iterations on the context's stream with no host round trip
+   between them and synchronises once at return.  stats may be NULL. */
+lsseg_status lsseg_iterate(lsseg_ctx* ctx, const lsseg_params* params, int32_t iters,
+                           lsseg_stats* stats);
+
+/* One iteration whose marking is replaced by a restart of the active set: the
+   update runs with the given parameters, then the active list becomes every
+   voxel whose 6-neighbour differences on the new phi exceed 0.001, in ascending
+   tiled order.  Replaces the parameter-change branch of
+   Segmenter::RequestUpdateSegmentation (Segmenter.cpp:847-876, taken when
+   ShouldInitializeActiveElements, :1371-1374, holds). */
+lsseg_status lsseg_iterate_reinit(lsseg_ctx* ctx, const lsseg_params* params);
+
+/* Per-iteration counters of the last lsseg_iterate call: n_k (active), c_k
+   (candidates), m_k (new active).  Any pointer may be NULL. */
+lsseg_status lsseg_get_iteration_counts(const lsseg_ctx* ctx, uint32_t* n, uint64_t* c,
+                                        uint32_t* m, uint64_t cap, uint64_t* count);
+
+/* ---- readback ----------------------------------------------------------- */
+
+/* Replaces Segmenter::GetActiveElementCount (Segmenter.hpp:66). */
+lsseg_status lsseg_get_active_count(const lsseg_ctx* ctx, uint64_t* n);
+
+/* Active list in list order (tiled indices); *n receives the full count.
+   Returns LSSEG_ERR_CAPACITY (after copying cap entries) if cap < n. */
+lsseg_status lsseg_get_active(const lsseg_ctx* ctx, uint32_t* out, uint64_t cap, uint64_t* n);
+
+/* phi over the unpadded box; which = 0: the read buffer (latest values at
+   active voxels), which = 1: the other buffer. */
+lsseg_status lsseg_get_phi(const lsseg_ctx* ctx, int32_t which, float* out);
+
+/* Replaces Segmenter::GetSaveableSegmentation (Segmenter.cpp:1216-1286) plus
+   UnpadVolume (:1288-1337): out[i] = 1 where the export byte is <= 0, else 0,
+   over the unpadded box.  With the frozen segmentation included
+   (lsseg_freeze), the export is first unioned with it as AddLevelSetKernel does
+   (InitializeLevelSetVolume.cu:151-169). */
+lsseg_status lsseg_get_mask(const lsseg_ctx* ctx, uint8_t* out);
+
+/* Replaces Segmenter::FreezeCurrentSegmentation (Segmenter.cpp:659-679):
+   frozen := union(frozen, export). */
+lsseg_status lsseg_freeze(lsseg_ctx* ctx);
+
+/* Replaces Segmenter::ClearCurrentSegmentation (Segmenter.cpp:597-640): phi := 1,
+   export := 127, empty active list.  all != 0 also clears the frozen
+   segmentation (ClearAllSegmentations, :641-657). */
+lsseg_status lsseg_clear(lsseg_ctx* ctx, int32_t all);
+
+/* ---- raw state (checkpoint / resume, parity tests) ---------------------- */
+
+/* Raw padded tiled state: phi read buffer, phi write buffer, export byte
+   (N = X*Y*Z entries each; any pointer may be NULL) and the active list. */
+lsseg_status lsseg_get_state(const lsseg_ctx* ctx, float* phi_read, float* phi_write,
+                             int8_t* export_bytes, uint32_t* list, uint64_t list_cap,
+                             uint64_t* n);
+lsseg_status lsseg_set_state(lsseg_ctx* ctx, const float* phi_read, const float* phi_write,
+                             const int8_t* export_bytes, const uint32_t* list, uint64_t n);
+
+/* Device pointers of the phi buffers (read, write) and the stream, for callers
+   that keep data resident in HBM (e.g. a torch wrapper). */
+lsseg_status lsseg_get_device_buffers(const lsseg_ctx* ctx, void** phi_read, void** phi_write,
+                                      void** stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSSEG_LSSEG_H */

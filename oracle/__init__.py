@@ -1,0 +1,280 @@
+"""ctypes binding of the CPU oracle (oracle/lsoracle.c).
+
+TEST INFRASTRUCTURE ONLY: the test suite, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may use this module, as the checker or the
+timed CPU baseline.  The engine package never imports it.
+
+All per-voxel arrays exchanged here are padded (multiples of 16) and stored in
+the 16^3-tiled order of the reference (Coordinates.cu:65-96); ``tau`` /
+``untau`` convert.  Lists hold tau values.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liblsoracle.so")
+
+U8, I8, U16, I16, U32, I32, F32 = range(7)
+MODE_REF, MODE_CLEAN = 0, 1
+NZ, NY, NX, SELF, PX, PY, PZ = range(7)
+DIRS = ((0, 0, -1), (0, -1, 0), (-1, 0, 0), (0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1))
+
+NP_DTYPE = {U8: np.uint8, I8: np.int8, U16: np.uint16, I16: np.int16,
+            U32: np.uint32, I32: np.int32, F32: np.float32}
+DTYPE_CODE = {np.dtype(v): k for k, v in NP_DTYPE.items()}
+
+_lib = None
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc, no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        c_i32, c_u32, c_i64, c_u64, c_f, c_d, vp = (ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64,
+                                                    ctypes.c_uint64, ctypes.c_float, ctypes.c_double,
+                                                    ctypes.c_void_p)
+        L.lso_tau.restype = c_u32
+        L.lso_tau.argtypes = [c_i32] * 5
+        L.lso_neighbor.restype = c_u32
+        L.lso_neighbor.argtypes = [c_i32, c_i32, c_i32, c_u32, c_i32]
+        L.lso_align256.restype = c_u64
+        L.lso_align256.argtypes = [c_u64]
+        L.lso_path_is_few.restype = c_i32
+        L.lso_path_is_few.argtypes = [c_u64, c_u64]
+        L.lso_update_f32.restype = None
+        L.lso_update_f32.argtypes = [c_i32, c_i32, c_i32, c_i32, vp, vp, vp, vp, vp, vp, c_i64,
+                                     c_f, c_f, c_f, c_f, c_i32]
+        L.lso_update_f64.restype = None
+        L.lso_update_f64.argtypes = [c_i32, c_i32, c_i32, c_i32, vp, vp, vp, vp, vp, vp, c_i64,
+                                     c_d, c_d, c_d, c_d, c_i32]
+        L.lso_mark.restype = None
+        L.lso_mark.argtypes = [c_i32, c_i32, c_i32, vp, vp, vp, c_i64, c_i32, vp, c_i32]
+        L.lso_compact_few.restype = c_i64
+        L.lso_compact_few.argtypes = [c_i32, c_i32, c_i32, vp, c_i64, vp, vp]
+        L.lso_compact_many.restype = c_i64
+        L.lso_compact_many.argtypes = [c_i32, c_i32, c_i32, vp, c_i64, vp, vp]
+        L.lso_create.restype = vp
+        L.lso_create.argtypes = [c_i32, c_i32, c_i32, c_i32, vp, c_f]
+        L.lso_destroy.restype = None
+        L.lso_destroy.argtypes = [vp]
+        L.lso_dims.restype = None
+        L.lso_dims.argtypes = [vp, vp, vp, vp]
+        L.lso_set_phi.restype = c_i32
+        L.lso_set_phi.argtypes = [vp, vp, c_f]
+        L.lso_seed_sphere.restype = c_i32
+        L.lso_seed_sphere.argtypes = [vp, c_f, c_f, c_f, c_u32]
+        L.lso_iterate.restype = c_i32
+        L.lso_iterate.argtypes = [vp, c_f, c_f, c_f, c_f, c_i32, c_i32, c_i32, vp, vp, vp]
+        L.lso_active_count.restype = c_i64
+        L.lso_active_count.argtypes = [vp]
+        L.lso_get_active.restype = c_i64
+        L.lso_get_active.argtypes = [vp, vp, c_i64]
+        L.lso_get_state.restype = None
+        L.lso_get_state.argtypes = [vp, vp, vp, vp]
+        L.lso_set_state.restype = c_i32
+        L.lso_set_state.argtypes = [vp, vp, vp, vp, vp, c_i64]
+        L.lso_get_phi.restype = None
+        L.lso_get_phi.argtypes = [vp, c_i32, vp]
+        L.lso_get_mask.restype = None
+        L.lso_get_mask.argtypes = [vp, vp]
+        L.lso_reinit_active.restype = None
+        L.lso_reinit_active.argtypes = [vp]
+        L.lso_iterate_reinit.restype = None
+        L.lso_iterate_reinit.argtypes = [vp, c_f, c_f, c_f, c_f]
+        L.lso_freeze.restype = None
+        L.lso_freeze.argtypes = [vp]
+        L.lso_clear.restype = None
+        L.lso_clear.argtypes = [vp, c_i32]
+        L.lso_add_export.restype = None
+        L.lso_add_export.argtypes = [vp, vp, c_i64]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def padded_dims(nx: int, ny: int, nz: int):
+    return (nx + 15) // 16 * 16, (ny + 15) // 16 * 16, (nz + 15) // 16 * 16
+
+
+def tau(X, Y, x, y, z):
+    """16^3-tiled index; numpy-vectorised (Coordinates.cu:65-96)."""
+    x, y, z = (np.asarray(a, dtype=np.int64) for a in (x, y, z))
+    tile = ((z >> 4) * (Y >> 4) + (y >> 4)) * (X >> 4) + (x >> 4)
+    return (tile * 4096 + ((z & 15) * 16 + (y & 15)) * 16 + (x & 15)).astype(np.int64)
+
+
+def untau(X, Y, t):
+    t = np.asarray(t, dtype=np.int64)
+    TX, TY = X >> 4, Y >> 4
+    tile, w = t >> 12, t & 4095
+    tx, ty, tz = tile % TX, (tile // TX) % TY, tile // (TX * TY)
+    return tx * 16 + (w & 15), ty * 16 + ((w >> 4) & 15), tz * 16 + (w >> 8)
+
+
+def tile_volume(vol_zyx: np.ndarray) -> np.ndarray:
+    """(Z, Y, X) array with dims multiples of 16 -> flat tiled array."""
+    Z, Y, X = vol_zyx.shape
+    v = vol_zyx.reshape(Z // 16, 16, Y // 16, 16, X // 16, 16)
+    return np.ascontiguousarray(v.transpose(0, 2, 4, 1, 3, 5)).reshape(-1)
+
+
+def untile_volume(flat: np.ndarray, X: int, Y: int, Z: int) -> np.ndarray:
+    v = flat.reshape(Z // 16, Y // 16, X // 16, 16, 16, 16)
+    return np.ascontiguousarray(v.transpose(0, 3, 1, 4, 2, 5)).reshape(Z, Y, X)
+
+
+def update(X, Y, Z, src_tiled, R, T, eps, alpha, dt, V, W=None, E=None, threads=1):
+    """Stateless update of every tau in V; returns (W, E, flags).  R's dtype picks
+    the float32 or float64 instantiation."""
+    L = lib()
+    code = DTYPE_CODE[src_tiled.dtype]
+    N = X * Y * Z
+    W = R.copy() if W is None else W
+    E = np.zeros(N, np.int8) if E is None else E
+    V = np.ascontiguousarray(V, dtype=np.uint32)
+    flags = np.zeros(max(len(V), 1), np.uint8)
+    fn = L.lso_update_f64 if R.dtype == np.float64 else L.lso_update_f32
+    fn(X, Y, Z, code, _p(src_tiled), _p(R), _p(W), _p(E), _p(flags), _p(V), len(V),
+       T, eps, alpha, dt, threads)
+    return W, E, flags[:len(V)]
+
+
+def mark(X, Y, Z, R, flags, V, mode=MODE_REF, threads=1):
+    L = lib()
+    V = np.ascontiguousarray(V, dtype=np.uint32)
+    flags = np.ascontiguousarray(flags, dtype=np.uint8)
+    masks = np.zeros(max(len(V), 1), np.uint8)
+    L.lso_mark(X, Y, Z, _p(np.ascontiguousarray(R, dtype=np.float32)), _p(flags), _p(V), len(V),
+               mode, _p(masks), threads)
+    return masks[:len(V)]
+
+
+def compact_few(X, Y, Z, V, masks):
+    L = lib()
+    V = np.ascontiguousarray(V, dtype=np.uint32)
+    masks = np.ascontiguousarray(masks, dtype=np.uint8)
+    out = np.zeros(max(7 * len(V), 1), np.uint32)
+    m = L.lso_compact_few(X, Y, Z, _p(V), len(V), _p(masks), _p(out))
+    return out[:m].copy()
+
+
+def compact_many(X, Y, Z, V, masks):
+    L = lib()
+    V = np.ascontiguousarray(V, dtype=np.uint32)
+    masks = np.ascontiguousarray(masks, dtype=np.uint8)
+    out = np.zeros(X * Y * Z, np.uint32)
+    m = L.lso_compact_many(X, Y, Z, _p(V), len(V), _p(masks), _p(out))
+    return out[:m].copy()
+
+
+def path_is_few(n, N):
+    return bool(lib().lso_path_is_few(n, N))
+
+
+class State:
+    """Owning wrapper of lso_state (the oracle's Segmenter)."""
+
+    def __init__(self, volume_zyx: np.ndarray, z_aniso: float = 1.0):
+        L = lib()
+        vol = np.ascontiguousarray(volume_zyx)
+        self.dtype_code = DTYPE_CODE[vol.dtype]
+        self.nz, self.ny, self.nx = vol.shape
+        self._keep = vol
+        self._h = L.lso_create(self.nx, self.ny, self.nz, self.dtype_code, _p(vol), z_aniso)
+        if not self._h:
+            raise ValueError("lso_create failed")
+        self.X, self.Y, self.Z = padded_dims(self.nx, self.ny, self.nz)
+        self.N = self.X * self.Y * self.Z
+
+    def close(self):
+        if self._h:
+            lib().lso_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_phi(self, phi0_zyx: np.ndarray, band: float = 1.0):
+        phi = np.ascontiguousarray(phi0_zyx, dtype=np.float32)
+        assert phi.shape == (self.nz, self.ny, self.nx)
+        lib().lso_set_phi(self._h, _p(phi), band)
+
+    def seed_sphere(self, cx, cy, cz, radius):
+        lib().lso_seed_sphere(self._h, cx, cy, cz, radius)
+
+    def iterate(self, T, eps, alpha, dt, iters, mode=MODE_REF, threads=1):
+        n = np.zeros(max(iters, 1), np.uint32)
+        c = np.zeros(max(iters, 1), np.uint64)
+        m = np.zeros(max(iters, 1), np.uint32)
+        lib().lso_iterate(self._h, T, eps, alpha, dt, mode, iters, threads, _p(n), _p(c), _p(m))
+        return n[:iters], c[:iters], m[:iters]
+
+    @property
+    def n(self) -> int:
+        return int(lib().lso_active_count(self._h))
+
+    def active(self) -> np.ndarray:
+        out = np.zeros(max(self.n, 1), np.uint32)
+        lib().lso_get_active(self._h, _p(out), len(out))
+        return out[:self.n].copy()
+
+    def state(self):
+        r = np.zeros(self.N, np.float32)
+        w = np.zeros(self.N, np.float32)
+        e = np.zeros(self.N, np.int8)
+        lib().lso_get_state(self._h, _p(r), _p(w), _p(e))
+        return r, w, e
+
+    def set_state(self, phi_read, phi_write, E, V):
+        V = np.ascontiguousarray(V, dtype=np.uint32)
+        rc = lib().lso_set_state(self._h, _p(np.ascontiguousarray(phi_read, np.float32)),
+                                 _p(np.ascontiguousarray(phi_write, np.float32)),
+                                 _p(np.ascontiguousarray(E, np.int8)), _p(V), len(V))
+        if rc != 0:
+            raise ValueError("lso_set_state rejected the state")
+
+    def phi(self, which: int = 0) -> np.ndarray:
+        out = np.zeros((self.nz, self.ny, self.nx), np.float32)
+        lib().lso_get_phi(self._h, which, _p(out))
+        return out
+
+    def mask(self) -> np.ndarray:
+        out = np.zeros((self.nz, self.ny, self.nx), np.uint8)
+        lib().lso_get_mask(self._h, _p(out))
+        return out
+
+    def iterate_reinit(self, T, eps, alpha, dt):
+        lib().lso_iterate_reinit(self._h, T, eps, alpha, dt)
+
+    def freeze(self):
+        lib().lso_freeze(self._h)
+
+    def clear(self, all_segmentations: bool = False):
+        lib().lso_clear(self._h, 1 if all_segmentations else 0)
+
+
+def add_export(result: np.ndarray, add: np.ndarray) -> np.ndarray:
+    r = np.ascontiguousarray(result, dtype=np.int8).copy()
+    a = np.ascontiguousarray(add, dtype=np.int8)
+    lib().lso_add_export(_p(r), _p(a), r.size)
+    return r

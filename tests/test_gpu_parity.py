@@ -1,0 +1,278 @@
+"""Parity of the HIP engine with the oracle, through the C ABI (needs a GPU).
+
+Bar: bit-exact lists, export bytes and FP32 phi against the FP32 oracle on the
+same seeded inputs (both sides are built without FMA contraction and with
+correctly rounded division and square root); FP32 phi within 1e-5 relative
+(1e-5 absolute floor, |phi| <= 1) of an FP64 run of the same update; final
+masks at Dice >= 0.9999 (in practice identical).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from aworkefficientgpualgorithmforlevelsetsegmentation_amd import engine, synth
+from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import Context, LssegError
+
+pytestmark = pytest.mark.gpu
+
+FP64_RTOL = 1e-5
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _assert_same_state(ctx: Context, ref: O.State, where=""):
+    r, w, e, lst = ctx.state()
+    rr, rw, re_ = ref.state()
+    rl = ref.active()
+    assert np.array_equal(lst, rl), f"{where}: lists differ (gpu {len(lst)} vs oracle {len(rl)})"
+    assert np.array_equal(r.view(np.uint32), rr.view(np.uint32)), f"{where}: phi read differs"
+    assert np.array_equal(w.view(np.uint32), rw.view(np.uint32)), f"{where}: phi write differs"
+    assert np.array_equal(e, re_), f"{where}: export differs"
+
+
+def _random_state(rng, N, n, wide=False):
+    R = rng.uniform(-1, 1, N).astype(np.float32)
+    if wide:
+        R[rng.random(N) < 0.05] = 4.0
+        R[rng.random(N) < 0.05] = -4.0
+    # coarse levels give exact ties with the 0.008 / 0.1 thresholds
+    lev = rng.random(N) < 0.3
+    R[lev] = rng.choice(np.array([0.0, 0.008, 0.1, 0.5, -0.5, 1.0], np.float32), lev.sum())
+    W = rng.uniform(-1, 1, N).astype(np.float32)
+    E = rng.integers(-128, 128, N).astype(np.int8)
+    V = rng.permutation(N)[:n].astype(np.uint32)
+    return R, W, E, V
+
+
+SOURCES = [
+    (np.uint8, 0, 255, 150.0, 40.0), (np.int8, -128, 127, -10.0, 30.0),
+    (np.uint16, 0, 4000, 1200.0, 250.0), (np.int16, -2000, 2000, 100.0, 300.0),
+    (np.uint32, 0, 100000, 50000.0, 9000.0), (np.int32, -100000, 100000, -300.0, 20000.0),
+    (np.float32, 0.0, 1.0, 0.55, 0.12),
+]
+
+
+@pytest.mark.parametrize("dtype,lo,hi,T,eps", SOURCES, ids=[np.dtype(s[0]).name for s in SOURCES])
+@pytest.mark.parametrize("n", [2000, 6000], ids=["few", "many"])
+def test_one_iteration_from_random_state(dtype, lo, hi, T, eps, n):
+    _say(f"one iteration dtype={np.dtype(dtype).name} n={n}")
+    rng = np.random.default_rng(n + sum(map(ord, np.dtype(dtype).name)))
+    shape = (32, 32, 32)
+    if np.issubdtype(dtype, np.floating):
+        vol = rng.uniform(lo, hi, shape).astype(dtype)
+    else:
+        vol = rng.integers(lo, hi, shape).astype(dtype)
+    ctx = Context(vol)
+    ref = O.State(vol)
+    R, W, E, V = _random_state(rng, ctx.N, n, wide=True)
+    ctx.set_state(R, W, E, V)
+    ref.set_state(R, W, E, V)
+    assert O.path_is_few(n, ctx.N) == (n == 2000)
+    for mode in (O.MODE_REF, O.MODE_CLEAN, O.MODE_REF):
+        st = ctx.iterate(T, eps, 0.15, 0.5, 1, mode=mode)
+        nh, ch, mh = ref.iterate(T, eps, 0.15, 0.5, 1, mode=mode)
+        _assert_same_state(ctx, ref, f"mode {mode}")
+        cnt = ctx.iteration_counts()
+        assert (int(cnt.n[0]), int(cnt.c[0]), int(cnt.m[0])) == (int(nh[0]), int(ch[0]), int(mh[0]))
+        assert st.sum_active == int(nh[0])
+    ctx.close()
+
+
+@pytest.mark.parametrize("dt", [0.5, -0.3])
+def test_alpha_zero_and_negative_dt(dt):
+    _say(f"alpha=0 dt={dt}")
+    rng = np.random.default_rng(5)
+    vol = rng.integers(0, 255, (16, 48, 32)).astype(np.uint8)
+    ctx = Context(vol)
+    ref = O.State(vol)
+    R, W, E, V = _random_state(rng, ctx.N, 1500)
+    ctx.set_state(R, W, E, V)
+    ref.set_state(R, W, E, V)
+    ctx.iterate(120, 50, 0.0, dt, 3)
+    ref.iterate(120, 50, 0.0, dt, 3)
+    _assert_same_state(ctx, ref)
+
+
+def _run_pair(w, iters, chunk, mode=O.MODE_REF):
+    ctx = Context(w.volume)
+    ref = O.State(w.volume)
+    ctx.set_phi(w.phi0, w.band)
+    ref.set_phi(w.phi0, w.band)
+    _assert_same_state(ctx, ref, "init")
+    done = 0
+    while done < iters:
+        k = min(chunk, iters - done)
+        ctx.iterate(w.T, w.eps, w.alpha, w.dt, k, mode=mode)
+        nh, ch, mh = ref.iterate(w.T, w.eps, w.alpha, w.dt, k, mode=mode, threads=8)
+        cnt = ctx.iteration_counts()
+        assert np.array_equal(cnt.n, nh) and np.array_equal(cnt.c, ch) and np.array_equal(cnt.m, mh), \
+            f"counts diverge in iterations {done}..{done + k}"
+        done += k
+        _assert_same_state(ctx, ref, f"after {done} iterations")
+    return ctx, ref
+
+
+def test_ellipsoids_small_multi_iteration():
+    _say("ellipsoids 64^3, 120 iterations in chunks (graph and direct launches)")
+    w = synth.ellipsoids(edge=64)
+    ctx, ref = _run_pair(w, 120, 37)
+    assert synth.dice(ctx.mask(), ref.mask()) == 1.0
+    ctx.close()
+
+
+def test_ellipsoids_full_workload():
+    _say("ellipsoids 128^3, 300 iterations")
+    w = synth.ellipsoids()
+    ctx, ref = _run_pair(w, w.iterations, 100)
+    m_gpu, m_ref = ctx.mask(), ref.mask()
+    assert synth.dice(m_gpu, m_ref) >= 0.9999
+    assert np.array_equal(m_gpu, m_ref)
+    assert synth.dice(m_gpu, w.truth) > 0.9   # it segments the ellipsoids
+    ctx.close()
+
+
+def test_phantom_float_source_reduced():
+    _say("phantom 181x217x181 float32, 150 iterations")
+    w = synth.phantom()
+    ctx, ref = _run_pair(w, 150, 75)
+    assert np.array_equal(ctx.mask(), ref.mask())
+    ctx.close()
+
+
+def test_vessels_small_uint16():
+    _say("vessels 128^3 uint16, 200 iterations")
+    w = synth.vessels(edge=128, n_tubes=6)
+    ctx, ref = _run_pair(w, 200, 100)
+    assert np.array_equal(ctx.mask(), ref.mask())
+    ctx.close()
+
+
+def test_clean_mode_multi_iteration():
+    _say("clean mode 48^3")
+    w = synth.ellipsoids(edge=48)
+    ctx, _ = _run_pair(w, 60, 30, mode=O.MODE_CLEAN)
+    ctx.close()
+
+
+def test_many_path_multi_iteration():
+    _say("dense path: 32^3 with a wide band")
+    n = 32
+    z, y, x = np.mgrid[0:n, 0:n, 0:n]
+    vol = ((x + y + z) % 7 * 30).astype(np.uint8)
+    phi0 = np.clip((x - 15.5) / 3.0, -4, 4).astype(np.float32)
+    w = synth.Workload("dense", vol, phi0, 100.0, 60.0, 0.2, 0.5, 40, band=3.5)
+    assert not O.path_is_few(int((np.abs(phi0) < 3.5).sum()), n ** 3)
+    ctx, ref = _run_pair(w, 40, 10)
+    ctx2 = Context(vol)
+    ctx2.set_phi(phi0, 3.5)
+    st = ctx2.iterate(w.T, w.eps, w.alpha, w.dt, 40)
+    assert st.many_path_iters > 0 and st.few_path_iters + st.many_path_iters == 40
+    ctx.close()
+    ctx2.close()
+
+
+def test_fp64_tolerance_per_iteration():
+    _say("fp64 tolerance along the ellipsoids run")
+    w = synth.ellipsoids(edge=64)
+    ctx = Context(w.volume)
+    ctx.set_phi(w.phi0, w.band)
+    X, Y, Z = ctx.X, ctx.Y, ctx.Z
+    src = O.tile_volume(np.pad(w.volume, [(0, Z - w.volume.shape[0]), (0, Y - w.volume.shape[1]),
+                                          (0, X - w.volume.shape[2])]))
+    worst = 0.0
+    for step in range(0, 90, 3):
+        r, _, _, lst = ctx.state()
+        ctx.iterate(w.T, w.eps, w.alpha, w.dt, 1)
+        new_r = ctx.state()[0]
+        W64, _, _ = O.update(X, Y, Z, src, r.astype(np.float64), w.T, w.eps, w.alpha, w.dt, lst)
+        got = new_r[lst].astype(np.float64)
+        want = W64[lst]
+        err = np.abs(got - want) / np.maximum(1.0, np.abs(want))
+        worst = max(worst, float(err.max()) if len(lst) else 0.0)
+        ctx.iterate(w.T, w.eps, w.alpha, w.dt, 2)
+    _say(f"worst relative error vs fp64: {worst:.3e}")
+    assert worst <= FP64_RTOL
+    ctx.close()
+
+
+def test_seed_sphere_and_iterate():
+    _say("seed sphere 40x36x33 (padded), anisotropy 1.5")
+    rng = np.random.default_rng(11)
+    vol = np.clip(rng.normal(100, 15, (33, 36, 40)), 0, 255).astype(np.uint8)
+    vol[8:25, 10:28, 10:30] = np.clip(rng.normal(200, 10, (17, 18, 20)), 0, 255).astype(np.uint8)
+    ctx = Context(vol, z_aniso=1.5)
+    ref = O.State(vol, z_aniso=1.5)
+    for c, r in (((20.6, 19.2, 16.9), 5), ((12.0, 14.0, 12.0), 3)):   # second seed unions
+        ctx.seed_sphere(*c, r)
+        ref.seed_sphere(*c, r)
+        _assert_same_state(ctx, ref, f"seed {c}")
+        ctx.iterate(200, 50, 0.2, 0.4, 25)
+        ref.iterate(200, 50, 0.2, 0.4, 25)
+        _assert_same_state(ctx, ref, f"after seed {c}")
+    ctx.close()
+
+
+def test_reinit_freeze_clear_and_mask():
+    _say("parameter-change restart, freeze, clear")
+    w = synth.ellipsoids(edge=48)
+    ctx = Context(w.volume)
+    ref = O.State(w.volume)
+    for s in (ctx, ref):
+        s.set_phi(w.phi0, 1.0)
+    ctx.iterate(w.T, w.eps, w.alpha, w.dt, 20)
+    ref.iterate(w.T, w.eps, w.alpha, w.dt, 20)
+    ctx.iterate_reinit(w.T - 10, w.eps, w.alpha, w.dt)
+    ref.iterate_reinit(w.T - 10, w.eps, w.alpha, w.dt)
+    _assert_same_state(ctx, ref, "reinit")
+    ctx.iterate(w.T - 10, w.eps, w.alpha, w.dt, 15)
+    ref.iterate(w.T - 10, w.eps, w.alpha, w.dt, 15)
+    _assert_same_state(ctx, ref, "after reinit")
+    assert np.array_equal(ctx.mask(), ref.mask())
+    ctx.freeze()
+    ref.freeze()
+    ctx.clear(False)
+    ref.clear(False)
+    assert np.array_equal(ctx.mask(), ref.mask()) and ctx.mask().sum() > 0
+    assert ctx.active_count == 0
+    ctx.clear(True)
+    assert ctx.mask().sum() == 0
+    ctx.close()
+
+
+def test_phi_readback_unpadded():
+    w = synth.phantom(dims=(37, 21, 19), iterations=5)
+    ctx = Context(w.volume)
+    ref = O.State(w.volume)
+    ctx.set_phi(w.phi0, 1.0)
+    ref.set_phi(w.phi0, 1.0)
+    ctx.iterate(w.T, w.eps, w.alpha, w.dt, 5)
+    ref.iterate(w.T, w.eps, w.alpha, w.dt, 5)
+    for which in (0, 1):
+        assert np.array_equal(ctx.phi(which), ref.phi(which))
+    ctx.close()
+
+
+def test_errors_are_reported_not_aborted():
+    vol = np.zeros((16, 16, 16), np.uint8)
+    ctx = Context(vol)
+    with pytest.raises(LssegError) as e:
+        ctx.iterate(100, 0.5, 0.1, 0.5, 1)      # integer sources need eps >= 1
+    assert e.value.status == engine.ERR_INVALID_ARG
+    with pytest.raises(LssegError):
+        ctx.iterate(100, 10, float("nan"), 0.5, 1)
+    with pytest.raises(LssegError):
+        ctx.iterate(100, 10, 0.1, 0.5, 1, mode=7)
+    with pytest.raises(LssegError) as e:
+        ctx.set_state(np.zeros(ctx.N, np.float32), np.zeros(ctx.N, np.float32), np.zeros(ctx.N, np.int8),
+                      np.array([1, 1], np.uint32))
+    assert "duplicate" in str(e.value)
+    with pytest.raises(LssegError):
+        ctx.set_state(np.zeros(ctx.N, np.float32), np.zeros(ctx.N, np.float32), np.zeros(ctx.N, np.int8),
+                      np.array([ctx.N], np.uint32))
+    # the context still works after rejected calls
+    ctx.set_phi(np.full((16, 16, 16), 0.5, np.float32), 1.0)
+    assert ctx.active_count == 4096
+    ctx.iterate(100, 10, 0.1, 0.5, 2)
+    ctx.close()
