@@ -723,6 +723,16 @@ struct lsseg_ctx {
     uint32_t stats_cap = 0;
     uint64_t last_iters = 0;
     Graphs graphs;
+    struct Checkpoint {
+        float* phi[2] = {nullptr, nullptr};
+        int8_t* E = nullptr;
+        uint32_t* list = nullptr;
+        uint32_t n = 0;
+        bool valid = false;
+    } ckpt[LSSEG_MAX_CHECKPOINTS];
+    bool profile = false;               // record per-kernel events
+    std::vector<hipEvent_t> prof_events;
+    size_t prof_next = 0;
     std::vector<void*> allocs;
     std::string err;
 };
@@ -803,11 +813,21 @@ void launch_update(lsseg_ctx* c, int q)
     }
 }
 
+void prof_mark(lsseg_ctx* c)
+{
+    if (c->profile && c->prof_next < c->prof_events.size())
+        (void)hipEventRecord(c->prof_events[c->prof_next++], c->stream);
+}
+
 void launch_iteration(lsseg_ctx* c, int q)
 {
+    prof_mark(c);
     launch_update(c, q);
+    prof_mark(c);
     hipLaunchKernelGGL(k_mark, dim3(c->grid), dim3(kBlock), 0, c->stream, c->a, q);
+    prof_mark(c);
     hipLaunchKernelGGL(k_compact, dim3(c->grid), dim3(kBlock), 0, c->stream, c->a, q, 0);
+    prof_mark(c);
 }
 
 // dense compaction of F into list[target]; list[target^1] count := 0
@@ -1004,6 +1024,8 @@ lsseg_status lsseg_destroy(lsseg_ctx* ctx)
     if (!ctx) return LSSEG_ERR_INVALID_ARG;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     destroy_graphs(ctx);
+    for (hipEvent_t e : ctx->prof_events) (void)hipEventDestroy(e);
+    ctx->prof_events.clear();
     for (void* p : ctx->allocs) (void)hipFree(p);
     ctx->allocs.clear();
     if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
@@ -1173,7 +1195,7 @@ lsseg_status lsseg_iterate(lsseg_ctx* c, const lsseg_params* prm, int32_t iters,
     const int chunk = env_int("LSSEG_GRAPH_ITERS", 32);
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     int remaining = iters;
-    if (chunk > 1 && remaining >= chunk) {
+    if (!c->profile && chunk > 1 && remaining >= chunk) {
         if (c->graphs.chunk_iters != chunk) {
             lsseg_status st = build_graphs(c, chunk);
             if (st != LSSEG_OK) return st;
@@ -1230,7 +1252,10 @@ lsseg_status lsseg_get_iteration_counts(const lsseg_ctx* c, uint32_t* n, uint64_
         if (cc) HIP_TRY(cm, hipMemcpy(cc, c->a.c_hist, k * sizeof(uint64_t), hipMemcpyDeviceToHost));
         if (m) HIP_TRY(cm, hipMemcpy(m, c->a.m_hist, k * sizeof(uint32_t), hipMemcpyDeviceToHost));
     }
-    return cap < c->last_iters ? LSSEG_ERR_CAPACITY : LSSEG_OK;
+    if (!n && !cc && !m) return LSSEG_OK;  // count query
+    return cap < c->last_iters ? fail(cm, LSSEG_ERR_CAPACITY, "%llu iterations, capacity %llu",
+                                      (unsigned long long)c->last_iters, (unsigned long long)cap)
+                               : LSSEG_OK;
 }
 
 lsseg_status lsseg_get_active_count(const lsseg_ctx* c, uint64_t* n)
@@ -1368,6 +1393,90 @@ lsseg_status lsseg_set_state(lsseg_ctx* c, const float* phi_read, const float* p
     HIP_TRY(c, hipMemcpyAsync(c->a.ctl->n, counts, sizeof(counts), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipGetLastError());
     return sync_and_check(c);
+}
+
+lsseg_status lsseg_iterate_profiled(lsseg_ctx* c, const lsseg_params* prm, int32_t iters, lsseg_stats* stats,
+                                    lsseg_kernel_times* times)
+{
+    if (!c || !times) return LSSEG_ERR_INVALID_ARG;
+    if (iters < 0) return fail(c, LSSEG_ERR_INVALID_ARG, "negative iteration count");
+    const size_t need = (size_t)iters * 4;
+    while (c->prof_events.size() < need) {
+        hipEvent_t e;
+        HIP_TRY(c, hipEventCreate(&e));
+        c->prof_events.push_back(e);
+    }
+    c->profile = true;
+    c->prof_next = 0;
+    lsseg_status st = lsseg_iterate(c, prm, iters, stats);
+    c->profile = false;
+    if (st != LSSEG_OK) return st;
+    memset(times, 0, sizeof(*times));
+    for (int32_t i = 0; i < iters; ++i) {
+        float a = 0, b = 0, d = 0;
+        HIP_TRY(c, hipEventElapsedTime(&a, c->prof_events[4 * i + 0], c->prof_events[4 * i + 1]));
+        HIP_TRY(c, hipEventElapsedTime(&b, c->prof_events[4 * i + 1], c->prof_events[4 * i + 2]));
+        HIP_TRY(c, hipEventElapsedTime(&d, c->prof_events[4 * i + 2], c->prof_events[4 * i + 3]));
+        times->update_ms += a;
+        times->mark_ms += b;
+        times->compact_ms += d;
+    }
+    times->iterations = (uint64_t)iters;
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_checkpoint_save(lsseg_ctx* c, int32_t slot)
+{
+    if (!c) return LSSEG_ERR_INVALID_ARG;
+    if (slot < 0 || slot >= LSSEG_MAX_CHECKPOINTS) return fail(c, LSSEG_ERR_INVALID_ARG, "bad checkpoint slot %d", slot);
+    auto& k = c->ckpt[slot];
+    const uint64_t N = c->a.g.N;
+    if (!k.phi[0]) {
+        void* p = nullptr;
+        lsseg_status st;
+        if ((st = dalloc(c, &p, N * sizeof(float))) != LSSEG_OK) return st;
+        k.phi[0] = (float*)p;
+        if ((st = dalloc(c, &p, N * sizeof(float))) != LSSEG_OK) return st;
+        k.phi[1] = (float*)p;
+        if ((st = dalloc(c, &p, N)) != LSSEG_OK) return st;
+        k.E = (int8_t*)p;
+        if ((st = dalloc(c, &p, N * sizeof(uint32_t))) != LSSEG_OK) return st;
+        k.list = (uint32_t*)p;
+    }
+    uint64_t n = 0;
+    lsseg_status st = lsseg_get_active_count(c, &n);
+    if (st != LSSEG_OK) return st;
+    HIP_TRY(c, hipMemcpyAsync(k.phi[0], c->a.phi[c->q], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(k.phi[1], c->a.phi[c->q ^ 1], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(k.E, c->a.E, N, hipMemcpyDeviceToDevice, c->stream));
+    if (n) HIP_TRY(c, hipMemcpyAsync(k.list, c->a.list[c->q], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    k.n = (uint32_t)n;
+    k.valid = true;
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_checkpoint_restore(lsseg_ctx* c, int32_t slot)
+{
+    if (!c) return LSSEG_ERR_INVALID_ARG;
+    if (slot < 0 || slot >= LSSEG_MAX_CHECKPOINTS || !c->ckpt[slot].valid)
+        return fail(c, LSSEG_ERR_STATE, "checkpoint slot %d is empty", slot);
+    auto& k = c->ckpt[slot];
+    const uint64_t N = c->a.g.N;
+    c->q = 0;
+    HIP_TRY(c, hipMemcpyAsync(c->a.phi[0], k.phi[0], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->a.phi[1], k.phi[1], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->a.E, k.E, N, hipMemcpyDeviceToDevice, c->stream));
+    if (k.n) HIP_TRY(c, hipMemcpyAsync(c->a.list[0], k.list, k.n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    // per-voxel emission masks must be zero outside an iteration in flight
+    hipLaunchKernelGGL(k_fill_u8, dim3(dense_grid(N)), dim3(kBlock), 0, c->stream, c->a.M, N, (uint8_t)0);
+    uint32_t* counts = &c->h_ctl->n[0];
+    counts[0] = k.n;
+    counts[1] = 0;
+    HIP_TRY(c, hipMemcpyAsync(c->a.ctl->n, counts, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return LSSEG_OK;
 }
 
 lsseg_status lsseg_get_device_buffers(const lsseg_ctx* c, void** phi_read, void** phi_write, void** stream)

@@ -29,7 +29,8 @@ EXPORTS = (
     "lsseg_set_phi", "lsseg_iterate", "lsseg_iterate_reinit", "lsseg_get_iteration_counts",
     "lsseg_get_active_count", "lsseg_get_active", "lsseg_get_phi", "lsseg_get_mask",
     "lsseg_freeze", "lsseg_clear", "lsseg_get_state", "lsseg_set_state",
-    "lsseg_get_device_buffers",
+    "lsseg_get_device_buffers", "lsseg_checkpoint_save", "lsseg_checkpoint_restore",
+    "lsseg_iterate_profiled",
 )
 
 
@@ -54,6 +55,14 @@ class Stats(ctypes.Structure):
                 ("sum_candidates", ctypes.c_uint64), ("sum_new_active", ctypes.c_uint64),
                 ("active_final", ctypes.c_uint64), ("few_path_iters", ctypes.c_uint64),
                 ("many_path_iters", ctypes.c_uint64), ("device_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class KernelTimes(ctypes.Structure):
+    _fields_ = [("update_ms", ctypes.c_double), ("mark_ms", ctypes.c_double),
+                ("compact_ms", ctypes.c_double), ("iterations", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -93,6 +102,9 @@ def lib():
             "lsseg_get_state": (i32, [vp, vp, vp, vp, vp, u64, P(u64)]),
             "lsseg_set_state": (i32, [vp, vp, vp, vp, vp, u64]),
             "lsseg_get_device_buffers": (i32, [vp, P(vp), P(vp), P(vp)]),
+            "lsseg_checkpoint_save": (i32, [vp, i32]),
+            "lsseg_checkpoint_restore": (i32, [vp, i32]),
+            "lsseg_iterate_profiled": (i32, [vp, P(Params), i32, P(Stats), P(KernelTimes)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -176,6 +188,20 @@ class Context:
         st = Stats()
         self._check(lib().lsseg_iterate(self._h, ctypes.byref(prm), int(iters), ctypes.byref(st)))
         return st
+
+    def iterate_profiled(self, T, eps, alpha, dt, iters, mode=MODE_REF):
+        prm = Params(T, eps, alpha, dt, mode)
+        st = Stats()
+        kt = KernelTimes()
+        self._check(lib().lsseg_iterate_profiled(self._h, ctypes.byref(prm), int(iters), ctypes.byref(st),
+                                                 ctypes.byref(kt)))
+        return st, kt
+
+    def checkpoint_save(self, slot: int = 0):
+        self._check(lib().lsseg_checkpoint_save(self._h, slot))
+
+    def checkpoint_restore(self, slot: int = 0):
+        self._check(lib().lsseg_checkpoint_restore(self._h, slot))
 
     def iterate_reinit(self, T, eps, alpha, dt, mode=MODE_REF):
         prm = Params(T, eps, alpha, dt, mode)

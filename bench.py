@@ -1,0 +1,255 @@
+#!/usr/bin/env python3
+"""Throughput of the sparse level-set engine on one MI355X.
+
+A step is one complete segmentation run of the workload: restore the initial
+state from an HBM-resident checkpoint, then run the workload's iterations
+(update, marking, duplicate removal and compaction) through the C ABI.  Inputs
+are resident in HBM before the timed region starts.
+
+Printed (one JSON line): active-voxel updates/s (sum of n_k over the run's
+iterations divided by the wall time), the roofline of the dominant kernel (its
+algorithmic bytes per launch, from the per-iteration counters, over its average
+launch time measured with HIP events on the engine's stream), the whole-path
+fraction of HBM peak, and the CPU baseline (the oracle: the reference's update
+equations compiled as host C, timed on this machine's cores).
+
+Algorithmic bytes per iteration k, FP32 phi (4 bytes), source S bytes/voxel:
+  update   (38 + S) n_k   coordinate, 7 phi taps, source, phi write, flag, mask byte
+  mark     33 n_k         coordinate, flag, 7 phi taps
+  compact  10 c_k + 5 m_k candidate write/read-back, tag probe/set; list write, clear
+with n_k active voxels, c_k candidate emissions and m_k new active voxels.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "active-voxel updates/s + % HBM peak; batched/adjoint vs plain at Dice≥0.9999"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def device_sync():
+    hip = ctypes.CDLL("libamdhip64.so")
+    rc = hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
+
+
+def host_cores() -> int:
+    try:
+        return max(1, min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "64"))))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def algorithmic_bytes(n, c, m, S):
+    n = n.astype(np.float64)
+    c = c.astype(np.float64)
+    m = m.astype(np.float64)
+    return {"update": (38 + S) * n, "mark": 33.0 * n, "compact": 10.0 * c + 5.0 * m}
+
+
+def make_workload(name, iterations):
+    from aworkefficientgpualgorithmforlevelsetsegmentation_amd import synth
+    if name == "ellipsoids":
+        w = synth.ellipsoids()
+    elif name == "phantom":
+        w = synth.phantom()
+    elif name == "vessels":
+        w = synth.vessels()
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    if iterations:
+        w.iterations = iterations
+    return w
+
+
+def describe(w, X, Y, Z):
+    z, y, x = w.volume.shape
+    return (f"{w.name} {x}x{y}x{z} {w.volume.dtype.name} (padded {X}x{Y}x{Z}), {w.iterations} iterations, "
+            f"T={w.T:g} eps={w.eps:g} alpha={w.alpha:g} dt={w.dt:g}, phi0 = sphere SDF clamped +-4, "
+            f"active band |phi0|<{w.band:g}")
+
+
+def cpu_baseline(w, gpu_mask, seconds_1core):
+    """The oracle on this host: full workload on all cores (also gives the mask
+    for the Dice check), plus a bounded single-core sample."""
+    import oracle as O
+    cores = host_cores()
+    ref = O.State(w.volume)
+    ref.set_phi(w.phi0, w.band)
+    t0 = time.perf_counter()
+    n_tot = 0
+    done = 0
+    while done < w.iterations:
+        k = min(50, w.iterations - done)
+        n, _, _ = ref.iterate(w.T, w.eps, w.alpha, w.dt, k, threads=cores)
+        n_tot += int(n.sum())
+        done += k
+    t_all = time.perf_counter() - t0
+    mask = ref.mask()
+    from aworkefficientgpualgorithmforlevelsetsegmentation_amd import synth
+    dice = synth.dice(gpu_mask, mask)
+    identical = bool(np.array_equal(gpu_mask, mask))
+    # single core: the first iterations of the same workload, bounded in time
+    one = O.State(w.volume)
+    one.set_phi(w.phi0, w.band)
+    t0 = time.perf_counter()
+    n1 = 0
+    it1 = 0
+    while it1 < w.iterations and time.perf_counter() - t0 < seconds_1core:
+        n, _, _ = one.iterate(w.T, w.eps, w.alpha, w.dt, 10, threads=1)
+        n1 += int(n.sum())
+        it1 += 10
+    t1 = time.perf_counter() - t0
+    return {
+        "value": n_tot / t_all, "unit": "active-voxel updates/s", "cores": cores, "kind": "port",
+        "sample": f"full workload ({w.iterations} iterations, {n_tot} updates) in {t_all:.2f} s, "
+                  f"oracle/lsoracle.c -O3 OpenMP over the active list, {cpu_model()}",
+        "single_core": {"value": n1 / t1 if t1 > 0 else 0.0, "cores": 1,
+                        "sample": f"first {it1} iterations ({n1} updates) in {t1:.2f} s"},
+    }, dice, identical
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="phantom", choices=["ellipsoids", "phantom", "vessels"])
+    ap.add_argument("--iterations", type=int, default=0, help="override the workload's iteration count")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="single-core baseline sample budget")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import Context
+
+    t = time.perf_counter()
+    w = make_workload(args.workload, args.iterations)
+    log(f"workload generated in {time.perf_counter() - t:.1f} s")
+    ctx = Context(w.volume)
+    ctx.set_phi(w.phi0, w.band)
+    ctx.checkpoint_save(0)
+    S = w.volume.dtype.itemsize
+    log(f"context ready: {describe(w, ctx.X, ctx.Y, ctx.Z)}; n0={ctx.active_count}")
+
+    for _ in range(args.warmup):
+        ctx.checkpoint_restore(0)
+        ctx.iterate(w.T, w.eps, w.alpha, w.dt, w.iterations)
+    log("warmup done")
+
+    device_sync()
+    t0 = time.perf_counter()
+    last = None
+    for _ in range(args.steps):
+        ctx.checkpoint_restore(0)
+        last = ctx.iterate(w.T, w.eps, w.alpha, w.dt, w.iterations)
+    device_sync()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / max(args.steps, 1)
+    updates_per_step = int(last.sum_active) if last else 0
+    value = updates_per_step * args.steps / elapsed if elapsed > 0 else 0.0
+    gpu_mask = ctx.mask()
+    counts = ctx.iteration_counts()
+    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {updates_per_step} updates/step, "
+        f"few/many iterations {last.few_path_iters}/{last.many_path_iters}, final n={last.active_final}")
+
+    # per-kernel device times (HIP events on the engine's stream, one launch per kernel)
+    ctx.checkpoint_restore(0)
+    pst, kt = ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, w.iterations)
+    pc = ctx.iteration_counts()
+    assert np.array_equal(pc.n, counts.n), "profiled run diverged from the timed run"
+    B = algorithmic_bytes(pc.n, pc.c, pc.m, S)
+    times = {"update": kt.update_ms, "mark": kt.mark_ms, "compact": kt.compact_ms}
+    names = {"update": "k_update", "mark": "k_mark", "compact": "k_compact"}
+    kernels = {}
+    for k in times:
+        launches = int(kt.iterations)
+        avg_us = 1e3 * times[k] / max(launches, 1)
+        bytes_per_launch = float(B[k].sum()) / max(launches, 1)
+        kernels[k] = {"kernel": names[k], "avg_launch_us": avg_us, "bytes_per_launch": bytes_per_launch,
+                      "achieved_GBs": bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0,
+                      "total_ms": times[k]}
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    d = kernels[dom]
+    path_bytes = float(sum(v.sum() for v in B.values()))
+    path_gbs = path_bytes / (ms_per_step * 1e-3) / 1e9
+    log("per-kernel: " + ", ".join(f"{v['kernel']} {v['avg_launch_us']:.2f} us/launch "
+                                   f"{v['achieved_GBs']:.0f} GB/s" for v in kernels.values()))
+
+    cpu = None
+    dice = None
+    identical = None
+    if not args.no_cpu_baseline:
+        log("running the CPU oracle (full workload on all cores, then a single-core sample)")
+        cpu, dice, identical = cpu_baseline(w, gpu_mask, args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.3e} updates/s on {cpu['cores']} cores; Dice vs oracle {dice:.6f}")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "active-voxel updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": describe(w, ctx.X, ctx.Y, ctx.Z),
+            "volume_xyz": list(w.shape_xyz),
+            "iterations": w.iterations,
+            "active_voxel_updates_per_step": updates_per_step,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": d["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
+            "kernel": d["kernel"], "bytes_per_launch": d["bytes_per_launch"],
+            "avg_launch_us": d["avg_launch_us"],
+        },
+        "path_roofline": {"achieved": path_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": path_gbs / HBM_PEAK_GBS, "bytes_per_step": path_bytes},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+        "dice_vs_oracle": dice,
+        "mask_identical_to_oracle": identical,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -147,7 +147,9 @@ lsseg_status lsseg_iterate(lsseg_ctx* ctx, const lsseg_params* params, int32_t i
 lsseg_status lsseg_iterate_reinit(lsseg_ctx* ctx, const lsseg_params* params);
 
 /* Per-iteration counters of the last lsseg_iterate call: n_k (active), c_k
-   (candidates), m_k (new active).  Any pointer may be NULL. */
+   (candidates), m_k (new active).  Any pointer may be NULL; with all three NULL
+   the call only reports *count.  Returns LSSEG_ERR_CAPACITY (after copying cap
+   entries) if cap < *count. */
 lsseg_status lsseg_get_iteration_counts(const lsseg_ctx* ctx, uint32_t* n, uint64_t* c,
                                         uint32_t* m, uint64_t cap, uint64_t* count);
 
@@ -189,6 +191,30 @@ lsseg_status lsseg_get_state(const lsseg_ctx* ctx, float* phi_read, float* phi_w
                              uint64_t* n);
 lsseg_status lsseg_set_state(lsseg_ctx* ctx, const float* phi_read, const float* phi_write,
                              const int8_t* export_bytes, const uint32_t* list, uint64_t n);
+
+/* Device-resident checkpoint of the complete segmentation state (both phi
+   buffers, export byte, active list) in slot 0..LSSEG_MAX_CHECKPOINTS-1; the slot
+   is allocated on first use.  Restore makes the saved state current again.  No
+   reference counterpart (the reference keeps no engine checkpoint); used for
+   resumable runs, repeated timed runs from one resident input, and the
+   adjoint's phi checkpoints. */
+#define LSSEG_MAX_CHECKPOINTS 64
+lsseg_status lsseg_checkpoint_save(lsseg_ctx* ctx, int32_t slot);
+lsseg_status lsseg_checkpoint_restore(lsseg_ctx* ctx, int32_t slot);
+
+/* Summed device time of each kernel of the iteration, measured with HIP events
+   recorded on the context's stream around every launch. */
+typedef struct lsseg_kernel_times {
+    double update_ms;    /* level-set update (UpdateLevelSetVolume.cu) */
+    double mark_ms;      /* marking (UpdateActiveElementsVolume.cu) */
+    double compact_ms;   /* duplicate removal + compaction (FilterDuplicates.cu, CUDPP) */
+    uint64_t iterations;
+} lsseg_kernel_times;
+
+/* lsseg_iterate with per-kernel event timing (kernels launched one by one, no
+   graph).  Results and statistics are identical to lsseg_iterate. */
+lsseg_status lsseg_iterate_profiled(lsseg_ctx* ctx, const lsseg_params* params, int32_t iters,
+                                    lsseg_stats* stats, lsseg_kernel_times* times);
 
 /* Device pointers of the phi buffers (read, write) and the stream, for callers
    that keep data resident in HBM (e.g. a torch wrapper). */
