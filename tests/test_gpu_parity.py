@@ -242,7 +242,10 @@ def test_reinit_freeze_clear_and_mask():
 
 
 def test_phi_readback_unpadded():
-    w = synth.phantom(dims=(37, 21, 19), iterations=5)
+    rng = np.random.default_rng(2)
+    vol = rng.uniform(0, 1, (19, 21, 37)).astype(np.float32)
+    phi0 = synth.sphere_sdf(37, 21, 19, (18, 10, 9), 4.0)
+    w = synth.Workload("odd", vol, phi0, 0.5, 0.3, 0.1, 0.5, 5)
     ctx = Context(w.volume)
     ref = O.State(w.volume)
     ctx.set_phi(w.phi0, 1.0)
