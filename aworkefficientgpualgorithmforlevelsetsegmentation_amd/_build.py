@@ -58,7 +58,7 @@ def _atomic_out(path: str) -> str:
 
 def build_hip() -> str:
     out = _atomic_out(HIP_LIB)
-    srcs = [os.path.join(CSRC, "lsseg.hip"), os.path.join(CSRC, "segmenter.cpp")]
+    srcs = [os.path.join(CSRC, "lsseg.hip"), os.path.join(CSRC, "segmenter.cpp")]  # + lsseg_device.hpp
     _run([_hipcc(), *HIPCC_FLAGS, "-shared", f"-I{INCLUDE}", "-o", out, *srcs])
     os.replace(out, HIP_LIB)
     return HIP_LIB

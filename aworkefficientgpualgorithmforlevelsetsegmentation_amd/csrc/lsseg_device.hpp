@@ -1,0 +1,738 @@
+// lsseg_device.hpp -- device code of the sparse level-set engine (gfx950).
+//
+// Reference paths in comments are relative to code/gpuseg/code/ of the
+// reference repository.
+//
+// One kernel launch per iteration.  Launch k:
+//   1. compacts the candidates that iteration k-1 emitted into the active list
+//      V_k (duplicate removal + stable compaction, Segmenter.cpp:883-1019) with
+//      a single-pass decoupled look-back scan (Merrill & Garland, "Single-pass
+//      Parallel Prefix Scan with Decoupled Look-back", 2016);
+//   2. updates every voxel it just placed (UpdateLevelSetVolume.cu:222-342) and
+//      marks it (UpdateActiveElements.cu:75-103), emitting candidates for V_{k+1}.
+// Marking compares the new phi of a voxel with the new phi of its neighbours.
+// A neighbour that is itself active is being updated by some other workgroup of
+// the same launch, so its new value is recomputed locally with the identical
+// arithmetic instead of waiting for a grid-wide barrier; an inactive neighbour
+// keeps the value already in the write buffer (the reference never copies
+// phi between its two buffers, Segmenter.cpp:841,1339-1351).
+//
+// Emission records: C[j] (three rotating buffers, 8 bytes per voxel) holds the
+// candidates of iteration j: byte e of C[j][u] is 1 iff u was emitted into list
+// e (0:-z 1:-y 2:-x 3:self 4:+x 5:+y 6:+z, UpdateActiveElementsVolume.cu:168-174).
+// First-wins duplicate removal over the list-major concatenation
+// (FilterDuplicates.cu:42-66) then keeps candidate (e, p) iff the lowest set
+// byte of C[u] is e, and the dense path keeps every u with C[u] != 0 in tiled
+// order.  The set of voxels with C[j][u] != 0 is exactly V_{j+1}, which also
+// answers "is this neighbour active" for the recomputation above.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lsseg {
+
+// Config.hpp:55-66
+constexpr float kTemporalThreshold = 0.008f;  // TEMPORAL_DERIVATIVE_THRESHOLD
+constexpr float kSpatialThreshold = 0.001f;   // SPATIAL_DERIVATIVE_THRESHOLD
+constexpr float kRescale = 0.01f;             // LEVEL_SET_RESCALE_AMOUNT
+constexpr float kSmoothDistance = 4.0f;       // LEVEL_SET_INITIALIZE_SMOOTH_DISTANCE
+constexpr float kConnTol = 0.1f;              // UpdateLevelSetVolume.cu:282-287
+
+constexpr int kBlock = 256;                   // threads per workgroup (4 waves of 64)
+constexpr int kItems = 8;                     // compaction slots per thread
+constexpr int kScanTile = kBlock * kItems;    // slots per look-back tile
+constexpr uint32_t kSpinLimit = 1u << 24;     // bounded look-back wait
+
+enum : uint32_t { kErrSpin = 1u, kErrOverflow = 2u };
+
+enum KMode : int {
+    kFirst = 0,  // update + mark of the materialised list (no compaction)
+    kMid = 1,    // compaction of the previous candidates + update + mark
+    kLast = 2,   // compaction only, with statistics
+    kInit = 3    // dense compaction of C[ck] only (initialisation), no statistics
+};
+
+// ---------------------------------------------------------------------------
+// geometry
+// ---------------------------------------------------------------------------
+struct FastDiv {
+    uint32_t d, m, s;
+};
+
+// q = (umulhi(n, m) + n) >> s with s = ceil(log2 d), m = 2^32 (2^s - d) / d + 1
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    FastDiv f;
+    f.d = d;
+    uint32_t s = 0;
+    while ((1ull << s) < d) ++s;
+    f.s = s;
+    f.m = (uint32_t)(((1ull << 32) * ((1ull << s) - d)) / d + 1);
+    return f;
+}
+
+__host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv& f)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    const uint32_t hi = __umulhi(n, f.m);
+#else
+    const uint32_t hi = (uint32_t)(((uint64_t)n * f.m) >> 32);
+#endif
+    return (uint32_t)(((uint64_t)hi + n) >> f.s);
+}
+
+struct Geom {
+    uint32_t X, Y, Z;        // padded dims
+    uint32_t TX, TY, TZ;     // tiles per axis
+    uint32_t sTY, sTZ;       // tiled strides of one tile row (y) and one tile slab (z)
+    uint64_t N;
+    FastDiv dTX, dTXY;
+};
+
+// The six face neighbours in list order; an out-of-bounds neighbour is the
+// voxel itself (OffsetCoordinatesUnsafe, Coordinates.cu:157-184) and bit e of
+// *inb tells whether neighbour e is inside the volume.
+__device__ inline void neighbors(const Geom& g, uint32_t t, uint32_t nb[7], uint32_t* inb)
+{
+    const uint32_t lx = t & 15u, ly = (t >> 4) & 15u, lz = (t >> 8) & 15u;
+    const uint32_t tile = t >> 12;
+    const uint32_t qy = fdiv(tile, g.dTX);
+    const uint32_t tx = tile - qy * g.TX;
+    const uint32_t tz = fdiv(tile, g.dTXY);
+    const uint32_t ty = qy - tz * g.TY;
+    uint32_t in = 0;
+    nb[0] = lz ? t - 256u : (tz ? t - g.sTZ + 3840u : t);
+    in |= (lz | tz) ? 1u : 0u;
+    nb[1] = ly ? t - 16u : (ty ? t - g.sTY + 240u : t);
+    in |= (ly | ty) ? 2u : 0u;
+    nb[2] = lx ? t - 1u : (tx ? t - 4096u + 15u : t);
+    in |= (lx | tx) ? 4u : 0u;
+    nb[3] = t;
+    nb[4] = lx < 15u ? t + 1u : (tx + 1u < g.TX ? t + 4096u - 15u : t);
+    in |= (lx < 15u || tx + 1u < g.TX) ? 16u : 0u;
+    nb[5] = ly < 15u ? t + 16u : (ty + 1u < g.TY ? t + g.sTY - 240u : t);
+    in |= (ly < 15u || ty + 1u < g.TY) ? 32u : 0u;
+    nb[6] = lz < 15u ? t + 256u : (tz + 1u < g.TZ ? t + g.sTZ - 3840u : t);
+    in |= (lz < 15u || tz + 1u < g.TZ) ? 64u : 0u;
+    *inb = in;
+}
+
+// Neighbour e of t when it is known to be inside the volume (every emitted
+// candidate is): only the in-tile coordinate decides whether a tile is crossed.
+__device__ inline uint32_t neighbor_in(const Geom& g, uint32_t t, uint32_t e)
+{
+    const uint32_t lx = t & 15u, ly = (t >> 4) & 15u, lz = (t >> 8) & 15u;
+    switch (e) {
+    case 0: return lz ? t - 256u : t - g.sTZ + 3840u;
+    case 1: return ly ? t - 16u : t - g.sTY + 240u;
+    case 2: return lx ? t - 1u : t - 4096u + 15u;
+    case 4: return lx < 15u ? t + 1u : t + 4096u - 15u;
+    case 5: return ly < 15u ? t + 16u : t + g.sTY - 240u;
+    case 6: return lz < 15u ? t + 256u : t + g.sTZ - 3840u;
+    default: return t;
+    }
+}
+
+__device__ inline void decode_xyz(const Geom& g, uint32_t t, uint32_t* x, uint32_t* y, uint32_t* z)
+{
+    const uint32_t tile = t >> 12;
+    const uint32_t qy = fdiv(tile, g.dTX);
+    const uint32_t tx = tile - qy * g.TX;
+    const uint32_t tz = fdiv(tile, g.dTXY);
+    const uint32_t ty = qy - tz * g.TY;
+    *x = tx * 16u + (t & 15u);
+    *y = ty * 16u + ((t >> 4) & 15u);
+    *z = tz * 16u + ((t >> 8) & 15u);
+}
+
+__host__ __device__ inline uint64_t align256(uint64_t n) { return n + ((256u - (n % 256u)) % 256u); }
+
+// ShouldOptimizeForFewActiveVoxels (Segmenter.cpp:1363-1369)
+__host__ __device__ inline bool path_is_few(uint64_t n, uint64_t N) { return 7u * align256(n) < N; }
+
+// ---------------------------------------------------------------------------
+// device-resident control block, parameters, kernel arguments
+// ---------------------------------------------------------------------------
+struct DevCtl {
+    uint32_t n[2];        // entries of list[0], list[1]
+    uint32_t iter;        // statistics index within the current lsseg_iterate call
+    uint32_t epoch;       // +1 per compaction launch; stamps the look-back states
+    uint32_t stats_cap;
+    uint32_t error;
+    uint64_t c_acc[2];    // candidate emissions, by list parity of the marked list
+    uint32_t few_iters, many_iters;
+};
+
+struct DevParams {
+    float negInv;         // -1 / eps (eps truncated to an integer for int sources)
+    float Tf;             // T for float sources
+    int32_t Ti;           // (int)T for int sources
+    float oneMinusAlpha;  // 1 - alpha
+    float alpha;
+    float dtAbs;          // |dt|
+    int32_t flipDensity;  // dt < 0 (UpdateLevelSetVolume.cu:297-301)
+    int32_t alphaPositive;
+    int32_t refMode;      // skip positions p % 4 == 3
+    int32_t pad_;
+};
+
+struct KArgs {
+    Geom g;
+    float* phi[2];
+    const void* src;
+    int8_t* E;            // export byte (tiled)
+    int8_t* frozen;       // frozen export (tiled)
+    uint64_t* C[3];       // emission records
+    uint32_t* list[2];
+    uint8_t* mlist[2];    // per-position emission masks
+    uint64_t* tstate;     // look-back tile states
+    DevCtl* ctl;
+    const DevParams* prm;
+    uint32_t* n_hist;
+    uint64_t* c_hist;
+    uint32_t* m_hist;
+};
+
+// ---------------------------------------------------------------------------
+// wave helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t lanemask_lt(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
+
+__device__ inline uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v, int lane)
+{
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__device__ inline uint64_t ld_state(const uint64_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void st_state(uint64_t* p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Look-back tile state: [epoch:30][flag:2][value:32]; flag 1 = aggregate,
+// 2 = inclusive prefix.  A state written by an earlier launch carries another
+// epoch and reads as "not yet published", so the states need no clearing.
+__device__ inline uint32_t state_flag(uint64_t s, uint32_t ep)
+{
+    return ((uint32_t)(s >> 34) == ep) ? (uint32_t)((s >> 32) & 3u) : 0u;
+}
+
+// Decoupled look-back for tile `tile` with aggregate `total`, run by all 64
+// lanes of one wave.  The count travels inside the 64-bit state word, so no
+// other memory has to be ordered with it.  Returns the exclusive prefix.
+__device__ uint32_t lookback(uint64_t* states, uint32_t tile, uint32_t total, uint32_t epoch, uint32_t* err)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t ep = epoch & 0x3FFFFFFFu;
+    const uint64_t stamp = (uint64_t)ep << 34;
+    if (tile == 0) {
+        if (lane == 0) st_state(&states[0], stamp | (2ull << 32) | total);
+        return 0;
+    }
+    if (lane == 0) st_state(&states[tile], stamp | (1ull << 32) | total);
+    uint32_t excl = 0;
+    int64_t look = (int64_t)tile - 1;
+    uint32_t spins = 0;
+    for (;;) {
+        const int64_t idx = look - lane;
+        uint64_t s = idx >= 0 ? ld_state(&states[idx]) : (stamp | (2ull << 32));
+        for (;;) {
+            const bool pending = state_flag(s, ep) == 0;
+            if (__ballot(pending) == 0) break;
+            if (++spins > kSpinLimit) {
+                if (lane == 0) atomicOr(err, kErrSpin);
+                s = stamp | (2ull << 32);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            if (pending) s = ld_state(&states[idx]);
+        }
+        const uint64_t pm = __ballot(state_flag(s, ep) == 2u);
+        const uint32_t v = (uint32_t)s;
+        if (pm) {
+            const int k = __ffsll((unsigned long long)pm) - 1;
+            excl += wave_sum(lane <= k ? v : 0u);
+            break;
+        }
+        excl += wave_sum(v);
+        look -= 64;
+    }
+    if (lane == 0) st_state(&states[tile], stamp | (2ull << 32) | (uint64_t)(excl + total));
+    return excl;
+}
+
+// ---------------------------------------------------------------------------
+// the update rule (UpdateLevelSetVolume.cu:222-342), operation order kept
+// ---------------------------------------------------------------------------
+__device__ inline float fmin_ref(float a, float b) { return b < a ? b : a; }
+__device__ inline float fmax_ref(float a, float b) { return b > a ? b : a; }
+
+__device__ inline int8_t export_byte(float v)
+{
+    // __float2int_rd(v * 127) (UpdateLevelSetVolume.cu:323); saturated for
+    // injected values outside [-1, 1]
+    float q = floorf(v * 127.0f);
+    q = q > 127.0f ? 127.0f : (q < -128.0f ? -128.0f : q);
+    return (int8_t)(int32_t)q;
+}
+
+template <typename SrcT, bool kFloatSrc>
+__device__ inline float update_value(const KArgs& a, const DevParams& P, const float* R, uint32_t t,
+                                     const uint32_t nb[7], float* u4_out)
+{
+    const float unz = R[nb[0]];
+    const float u1 = R[nb[1]];
+    const float u3 = R[nb[2]];
+    const float u4 = R[t];
+    const float u5 = R[nb[4]];
+    const float u7 = R[nb[5]];
+    const float upz = R[nb[6]];
+    float f;
+    if constexpr (kFloatSrc) {
+        const float s = static_cast<const float*>(a.src)[t];
+        const float d = s - P.Tf;
+        f = P.negInv * fabsf(d) + 1.0f;
+    } else {
+        const int32_t s = (int32_t)(static_cast<const SrcT*>(a.src)[t]);
+        const int32_t d = s - P.Ti;
+        const int32_t ad = d < 0 ? -d : d;
+        f = P.negInv * (float)ad + 1.0f;  // ComputeDensityTerm, :335-342
+    }
+    const float dcx = (u5 - u3) / 2.0f;
+    const float dcy = (u7 - u1) / 2.0f;
+    const float dcz = (upz - unz) / 2.0f;
+    const float fx = u5 - u4, fy = u7 - u4, fz = upz - u4;
+    const float bx = u4 - u3, by = u4 - u1, bz = u4 - unz;
+    const float sdx = fx - bx, sdy = fy - by, sdz = fz - bz;
+    float conn = 0.0f;
+    if (P.alphaPositive) {
+        if (fabsf(upz - unz) <= kConnTol && fabsf(unz - u1) <= kConnTol && fabsf(u1 - u3) <= kConnTol &&
+            fabsf(u3 - u5) <= kConnTol && fabsf(u5 - u7) <= kConnTol && fabsf(u7 - upz) <= kConnTol) {
+            const float avg = (upz + u1 + u3 + u5 + u7 + unz) / 6.0f;
+            conn = avg - u4;
+        }
+    }
+    const float curv = sdx + sdy + sdz;
+    const float dens = P.flipDensity ? f : -f;
+    const float speed = (P.oneMinusAlpha * dens) + (P.alpha * curv);
+    const float g = __builtin_sqrtf(dcx * dcx + dcy * dcy + dcz * dcz);
+    const float delta = ((P.dtAbs * speed * g)) + conn;
+    float nv = fmin_ref(1.0f, fmax_ref(-1.0f, u4 + delta));
+    nv = fmin_ref(1.0f, fmax_ref(-1.0f, ((1.0f + kRescale) * nv) + kRescale));
+    *u4_out = u4;
+    return nv;
+}
+
+// Update voxel u at list position p of the list with parity `op`, then mark it.
+// Cmem marks the members of the list being updated; emissions go to Cemit.
+// Returns the number of candidates emitted.
+template <typename SrcT, bool kFloatSrc>
+__device__ inline uint32_t update_mark(const KArgs& a, const DevParams& P, uint32_t u, uint32_t p, int op,
+                                       const uint64_t* Cmem, uint8_t* Cemit)
+{
+    const float* R = a.phi[op];
+    float* W = a.phi[op ^ 1];
+    uint32_t nb[7], inb;
+    neighbors(a.g, u, nb, &inb);
+    float r4;
+    const float nv = update_value<SrcT, kFloatSrc>(a, P, R, u, nb, &r4);
+    W[u] = nv;
+    a.E[u] = export_byte(nv);
+    uint32_t m = 0;
+    // temporal AND spatial test (UpdateActiveElements.cu:75-103) on the new phi,
+    // positions p % 4 == 3 skipped (UpdateActiveElementsVolume.cu:329)
+    if (fabsf(nv - r4) > kTemporalThreshold && (!P.refMode || (p & 3u) != 3u)) {
+#pragma unroll
+        for (int e = 0; e < 7; ++e) {
+            if (e == 3 || !((inb >> e) & 1u)) continue;
+            const uint32_t w = nb[e];
+            float vw;
+            if (Cmem[w] != 0ull) {
+                uint32_t nbw[7], inbw;
+                neighbors(a.g, w, nbw, &inbw);
+                float dummy;
+                vw = update_value<SrcT, kFloatSrc>(a, P, R, w, nbw, &dummy);
+            } else {
+                vw = W[w];
+            }
+            if (!(fabsf(nv - vw) <= kTemporalThreshold)) m |= 1u << e;
+        }
+        if (m) m |= 8u;
+    }
+    a.mlist[op][p] = (uint8_t)m;
+    if (m) {
+#pragma unroll
+        for (int e = 0; e < 7; ++e)
+            if ((m >> e) & 1u) Cemit[(uint64_t)nb[e] * 8u + (uint32_t)e] = 1;
+    }
+    return (uint32_t)__popc(m);
+}
+
+__device__ inline void block_add_candidates(DevCtl* ctl, int slot, uint32_t c, uint32_t* s_red)
+{
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < kBlock / 64; ++w) tot += s_red[w];
+        if (tot) atomicAdd((unsigned long long*)&ctl->c_acc[slot], (unsigned long long)tot);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the iteration kernel
+// ---------------------------------------------------------------------------
+// op: parity of the list this launch produces (kMid, kLast, kInit) or updates
+//     (kFirst); the source list of a compaction has parity op ^ 1.
+// ck: emission record holding the source candidates (kMid, kLast), the members
+//     of the list being updated (kFirst), or the dense flags (kInit).
+template <typename SrcT, bool kFloatSrc, int kMode>
+__global__ __launch_bounds__(kBlock) void k_iter(KArgs a, int op, int ck)
+{
+    __shared__ uint32_t s_stage[kScanTile];
+    __shared__ uint32_t s_cnt[kItems * 4];
+    __shared__ uint32_t s_excl[kItems * 4];
+    __shared__ uint32_t s_prefix, s_total;
+    __shared__ uint32_t s_red[kBlock / 64];
+
+    const DevParams P = *a.prm;
+    const int ce = ck == 2 ? 0 : ck + 1;   // emission record of this launch
+    const int cc = ck == 0 ? 2 : ck - 1;   // record to clear (two iterations old)
+    uint32_t emitted = 0;
+
+    if constexpr (kMode == kFirst) {
+        const uint32_t n = a.ctl->n[op];
+        const uint32_t* V = a.list[op];
+        const uint32_t stride = gridDim.x * blockDim.x;
+        for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride)
+            emitted += update_mark<SrcT, kFloatSrc>(a, P, V[p], p, op, a.C[ck], (uint8_t*)a.C[ce]);
+        block_add_candidates(a.ctl, op, emitted, s_red);
+        return;
+    } else {
+        const int sq = op ^ 1;
+        const uint32_t n_src = a.ctl->n[sq];
+        const uint32_t epoch = a.ctl->epoch;
+        const bool dense = (kMode == kInit) || !path_is_few(n_src, a.g.N);
+        const uint64_t slots = dense ? a.g.N : 7ull * n_src;
+        const uint32_t ntiles = (uint32_t)((slots + kScanTile - 1) / kScanTile);
+        const uint32_t* __restrict__ Vs = a.list[sq];
+        const uint8_t* __restrict__ Ms = a.mlist[sq];
+        const uint64_t* __restrict__ Ck = a.C[ck];
+        uint64_t* __restrict__ Cc = a.C[cc];
+        uint32_t* __restrict__ out = a.list[op];
+        const int lane = threadIdx.x & 63;
+        const int wave = threadIdx.x >> 6;
+        const uint64_t lt = lanemask_lt(lane);
+
+        // retire the record of two iterations ago (non-zero exactly at the
+        // source list); the sparse path does it in its first segment below
+        if (kMode != kInit && dense) {
+            const uint32_t stride = gridDim.x * blockDim.x;
+            for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n_src; p += stride) Cc[Vs[p]] = 0ull;
+        }
+
+        for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+            uint64_t bal[kItems];
+            uint32_t val[kItems];
+            const uint64_t base = (uint64_t)tile * kScanTile;
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                const uint64_t s = base + (uint64_t)j * kBlock + threadIdx.x;
+                bool keep = false;
+                uint32_t v = 0;
+                if (s < slots) {
+                    if (dense) {
+                        keep = Ck[s] != 0ull;
+                        v = (uint32_t)s;
+                    } else {
+                        // slot s = e * n_src + p: list-major concatenation of the
+                        // seven candidate lists (Cuda.cpp:699-758 over 7 A slots)
+                        const uint32_t e = (uint32_t)(s >= n_src) + (uint32_t)(s >= 2ull * n_src) +
+                                           (uint32_t)(s >= 3ull * n_src) + (uint32_t)(s >= 4ull * n_src) +
+                                           (uint32_t)(s >= 5ull * n_src) + (uint32_t)(s >= 6ull * n_src);
+                        const uint32_t p = (uint32_t)(s - (uint64_t)e * n_src);
+                        const uint32_t m = Ms[p];
+                        const bool bit = (m >> e) & 1u;
+                        if (e == 0u || bit) {
+                            const uint32_t t = Vs[p];
+                            if (kMode != kInit && e == 0u) Cc[t] = 0ull;
+                            if (bit) {
+                                const uint32_t u = neighbor_in(a.g, t, e);
+                                const uint64_t rec = Ck[u];
+                                // first-wins: the lowest list that emitted u keeps it
+                                keep = (uint32_t)((__ffsll((unsigned long long)rec) - 1) >> 3) == e;
+                                v = u;
+                            }
+                        }
+                    }
+                }
+                bal[j] = __ballot(keep);
+                val[j] = v;
+                if (lane == 0) s_cnt[j * 4 + wave] = (uint32_t)__popcll(bal[j]);
+            }
+            __syncthreads();
+            if (wave == 0) {
+                const uint32_t c = lane < kItems * 4 ? s_cnt[lane] : 0u;
+                const uint32_t incl = wave_inclusive_scan(c, lane);
+                if (lane < kItems * 4) s_excl[lane] = incl - c;
+                const uint32_t total = __shfl(incl, 63, 64);
+                const uint32_t prefix = lookback(a.tstate, tile, total, epoch, &a.ctl->error);
+                if (lane == 0) {
+                    s_prefix = prefix;
+                    s_total = total;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                if ((bal[j] >> lane) & 1ull) {
+                    const uint32_t pos = s_excl[j * 4 + wave] + (uint32_t)__popcll(bal[j] & lt);
+                    s_stage[pos] = val[j];
+                }
+            }
+            __syncthreads();
+            const uint32_t total = s_total, prefix = s_prefix;
+            if ((uint64_t)prefix + total > a.g.N) {  // cannot happen with consistent records
+                if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
+                break;
+            }
+            for (uint32_t i = threadIdx.x; i < total; i += kBlock) out[prefix + i] = s_stage[i];
+            if constexpr (kMode == kMid) {
+                for (uint32_t i = threadIdx.x; i < total; i += kBlock)
+                    emitted += update_mark<SrcT, kFloatSrc>(a, P, s_stage[i], prefix + i, op, Ck,
+                                                            (uint8_t*)a.C[ce]);
+            }
+            if (threadIdx.x == 0 && tile == ntiles - 1) {
+                const uint32_t m = prefix + total;
+                a.ctl->n[op] = m;
+                if constexpr (kMode == kInit) {
+                    a.ctl->n[sq] = 0;
+                } else {
+                    const uint32_t k = a.ctl->iter;
+                    if (k < a.ctl->stats_cap) {
+                        a.n_hist[k] = n_src;
+                        a.c_hist[k] = a.ctl->c_acc[sq];
+                        a.m_hist[k] = m;
+                    }
+                    if (dense) a.ctl->many_iters += 1; else a.ctl->few_iters += 1;
+                    a.ctl->c_acc[sq] = 0;
+                    a.ctl->iter = k + 1;
+                }
+                a.ctl->epoch = epoch + 1;
+            }
+            __syncthreads();
+        }
+        if (ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+            a.ctl->n[op] = 0;
+            if constexpr (kMode == kInit) {
+                a.ctl->n[sq] = 0;
+            } else {
+                const uint32_t k = a.ctl->iter;
+                if (k < a.ctl->stats_cap) {
+                    a.n_hist[k] = n_src;
+                    a.c_hist[k] = a.ctl->c_acc[sq];
+                    a.m_hist[k] = 0;
+                }
+                if (dense) a.ctl->many_iters += 1; else a.ctl->few_iters += 1;
+                a.ctl->c_acc[sq] = 0;
+                a.ctl->iter = k + 1;
+            }
+            a.ctl->epoch = epoch + 1;
+        }
+        if constexpr (kMode == kMid) block_add_candidates(a.ctl, op, emitted, s_red);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dense helpers (initialisation, readback)
+// ---------------------------------------------------------------------------
+#define LSSEG_GRID_STRIDE(i, count)                                                        \
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < (count);       \
+         i += (uint64_t)gridDim.x * blockDim.x)
+
+__global__ void k_fill_f32(float* p, uint64_t count, float v)
+{
+    LSSEG_GRID_STRIDE(i, count) p[i] = v;
+}
+
+__global__ void k_fill_u8(uint8_t* p, uint64_t count, uint8_t v)
+{
+    LSSEG_GRID_STRIDE(i, count) p[i] = v;
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t count, uint64_t v)
+{
+    LSSEG_GRID_STRIDE(i, count) p[i] = v;
+}
+
+// pad + tile the linear source (LoadVolumeHostPadded, Segmenter.cpp:1131-1204)
+__global__ void k_pad_tile_source(Geom g, const uint8_t* lin, uint8_t* dst, uint32_t nx, uint32_t ny,
+                                  uint32_t nz, uint32_t bytes, uint8_t pad)
+{
+    LSSEG_GRID_STRIDE(t, g.N)
+    {
+        uint32_t x, y, z;
+        decode_xyz(g, (uint32_t)t, &x, &y, &z);
+        if (x < nx && y < ny && z < nz) {
+            const uint64_t li = ((uint64_t)z * ny + y) * nx + x;
+            for (uint32_t b = 0; b < bytes; ++b) dst[t * bytes + b] = lin[li * bytes + b];
+        } else {
+            for (uint32_t b = 0; b < bytes; ++b) dst[t * bytes + b] = pad;
+        }
+    }
+}
+
+// phi0 injection: both buffers, export byte, dense band flags in C[ck]
+__global__ void k_set_phi(KArgs a, const float* lin, uint32_t nx, uint32_t ny, uint32_t nz, float band, int ck)
+{
+    LSSEG_GRID_STRIDE(t, a.g.N)
+    {
+        uint32_t x, y, z;
+        decode_xyz(a.g, (uint32_t)t, &x, &y, &z);
+        float v = 1.0f;
+        if (x < nx && y < ny && z < nz) v = lin[((uint64_t)z * ny + y) * nx + x];
+        a.phi[0][t] = v;
+        a.phi[1][t] = v;
+        a.E[t] = export_byte(v);
+        a.C[ck][t] = fabsf(v) < band ? 1ull : 0ull;
+    }
+}
+
+// InitializeLevelSetKernel (InitializeLevelSetVolume.cu:55-98)
+__global__ void k_seed_pass(KArgs a, int rd, uint32_t sx, uint32_t sy, uint32_t sz, int32_t radius, float aniso)
+{
+    const float* R = a.phi[rd];
+    float* W = a.phi[rd ^ 1];
+    LSSEG_GRID_STRIDE(t, a.g.N)
+    {
+        uint32_t x, y, z;
+        decode_xyz(a.g, (uint32_t)t, &x, &y, &z);
+        const int32_t vx = (int32_t)(sx - x);
+        const int32_t vy = (int32_t)(sy - y);
+        const float vz = ((float)sz - (float)z) * aniso;
+        const float d2 = (float)vx * (float)vx + (float)vy * (float)vy + vz * vz;
+        const float d = __builtin_sqrtf(d2);
+        float c = 1.0f;
+        if (d < (float)radius) c = -1.0f;
+        else if (d < (float)radius + kSmoothDistance) c = -1.0f + ((d - (float)radius) / (kSmoothDistance / 2.0f));
+        float nv = R[t] + c - 1.0f;
+        nv = nv < -1.0f ? -1.0f : (nv > 1.0f ? 1.0f : nv);
+        W[t] = nv;
+        a.E[t] = export_byte(nv);
+    }
+}
+
+// InitializeActiveElementsVolumeKernel (InitializeActiveElementsVolume.cu:37-74):
+// dense keep flags into C[ck]
+__global__ void k_keep_flags(KArgs a, int rd, int ck)
+{
+    const float* R = a.phi[rd];
+    LSSEG_GRID_STRIDE(t, a.g.N)
+    {
+        uint32_t nb[7], inb;
+        neighbors(a.g, (uint32_t)t, nb, &inb);
+        const float u4 = R[t];
+        bool flat = true;
+#pragma unroll
+        for (int e = 0; e < 7; ++e) {
+            if (e == 3) continue;
+            if (!(fabsf(u4 - R[nb[e]]) <= kSpatialThreshold)) flat = false;
+        }
+        a.C[ck][t] = flat ? 0ull : 1ull;
+    }
+}
+
+// update only (the parameter-change sub-step, Segmenter.cpp:811-876)
+template <typename SrcT, bool kFloatSrc>
+__global__ __launch_bounds__(kBlock) void k_update_only(KArgs a, int op)
+{
+    const DevParams P = *a.prm;
+    const uint32_t n = a.ctl->n[op];
+    const float* R = a.phi[op];
+    float* W = a.phi[op ^ 1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        const uint32_t u = a.list[op][p];
+        uint32_t nb[7], inb;
+        neighbors(a.g, u, nb, &inb);
+        float r4;
+        const float nv = update_value<SrcT, kFloatSrc>(a, P, R, u, nb, &r4);
+        W[u] = nv;
+        a.E[u] = export_byte(nv);
+    }
+}
+
+// membership record of an injected list
+__global__ void k_scatter_membership(const uint32_t* list, uint32_t n, uint64_t* C)
+{
+    LSSEG_GRID_STRIDE(p, n) C[list[p]] = 1ull;
+}
+
+// AddLevelSetKernel (InitializeLevelSetVolume.cu:151-169)
+__device__ inline int8_t add_export(int8_t ra, int8_t rb)
+{
+    const float av = (float)ra / 127.0f;
+    const float bv = (float)rb / 127.0f;
+    float r = av + bv - 1.0f;
+    r = r < -1.0f ? -1.0f : (r > 1.0f ? 1.0f : r);
+    return (int8_t)(int32_t)floorf(r * 127.0f);
+}
+
+__global__ void k_freeze(KArgs a)
+{
+    LSSEG_GRID_STRIDE(t, a.g.N) a.frozen[t] = add_export(a.frozen[t], a.E[t]);
+}
+
+__device__ inline uint32_t tau_of(const Geom& g, uint32_t x, uint32_t y, uint32_t z)
+{
+    return (((z >> 4) * (g.Y >> 4) + (y >> 4)) * (g.X >> 4) + (x >> 4)) * 4096u +
+           ((z & 15u) * 16u + (y & 15u)) * 16u + (x & 15u);
+}
+
+__global__ void k_get_phi(KArgs a, int which, float* out, uint32_t nx, uint32_t ny, uint32_t nz)
+{
+    const uint64_t cnt = (uint64_t)nx * ny * nz;
+    LSSEG_GRID_STRIDE(i, cnt)
+    {
+        const uint32_t x = (uint32_t)(i % nx);
+        const uint32_t y = (uint32_t)((i / nx) % ny);
+        const uint32_t z = (uint32_t)(i / ((uint64_t)nx * ny));
+        out[i] = a.phi[which][tau_of(a.g, x, y, z)];
+    }
+}
+
+// GetSaveableSegmentation (Segmenter.cpp:1216-1286): export unioned with the
+// frozen export, predicate <= 0, unpadded (UnpadVolume, :1288-1337)
+__global__ void k_get_mask(KArgs a, uint8_t* out, uint32_t nx, uint32_t ny, uint32_t nz)
+{
+    const uint64_t cnt = (uint64_t)nx * ny * nz;
+    LSSEG_GRID_STRIDE(i, cnt)
+    {
+        const uint32_t x = (uint32_t)(i % nx);
+        const uint32_t y = (uint32_t)((i / nx) % ny);
+        const uint32_t z = (uint32_t)(i / ((uint64_t)nx * ny));
+        const uint32_t t = tau_of(a.g, x, y, z);
+        out[i] = add_export(a.E[t], a.frozen[t]) <= 0 ? 1 : 0;
+    }
+}
+
+}  // namespace lsseg

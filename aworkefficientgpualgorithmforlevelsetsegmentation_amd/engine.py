@@ -61,8 +61,8 @@ class Stats(ctypes.Structure):
 
 
 class KernelTimes(ctypes.Structure):
-    _fields_ = [("update_ms", ctypes.c_double), ("mark_ms", ctypes.c_double),
-                ("compact_ms", ctypes.c_double), ("iterations", ctypes.c_uint64)]
+    _fields_ = [("total_ms", ctypes.c_double), ("first_ms", ctypes.c_double),
+                ("last_ms", ctypes.c_double), ("launches", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -104,7 +104,7 @@ def lib():
             "lsseg_get_device_buffers": (i32, [vp, P(vp), P(vp), P(vp)]),
             "lsseg_checkpoint_save": (i32, [vp, i32]),
             "lsseg_checkpoint_restore": (i32, [vp, i32]),
-            "lsseg_iterate_profiled": (i32, [vp, P(Params), i32, P(Stats), P(KernelTimes)]),
+            "lsseg_iterate_profiled": (i32, [vp, P(Params), i32, P(Stats), P(KernelTimes), vp, u64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -190,12 +190,14 @@ class Context:
         return st
 
     def iterate_profiled(self, T, eps, alpha, dt, iters, mode=MODE_REF):
+        """Returns (Stats, KernelTimes, per-launch milliseconds)."""
         prm = Params(T, eps, alpha, dt, mode)
         st = Stats()
         kt = KernelTimes()
+        per = np.zeros(int(iters) + 1, np.float32)
         self._check(lib().lsseg_iterate_profiled(self._h, ctypes.byref(prm), int(iters), ctypes.byref(st),
-                                                 ctypes.byref(kt)))
-        return st, kt
+                                                 ctypes.byref(kt), _p(per), per.size))
+        return st, kt, per
 
     def checkpoint_save(self, slot: int = 0):
         self._check(lib().lsseg_checkpoint_save(self._h, slot))

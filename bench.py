@@ -17,7 +17,9 @@ Algorithmic bytes per iteration k, FP32 phi (4 bytes), source S bytes/voxel:
   update   (38 + S) n_k   coordinate, 7 phi taps, source, phi write, flag, mask byte
   mark     33 n_k         coordinate, flag, 7 phi taps
   compact  10 c_k + 5 m_k candidate write/read-back, tag probe/set; list write, clear
-with n_k active voxels, c_k candidate emissions and m_k new active voxels.
+with n_k active voxels, c_k candidate emissions and m_k new active voxels.  One
+launch of the iteration kernel compacts iteration k's candidates and updates
+and marks iteration k + 1, so its bytes are 10 c_k + 5 m_k + (71 + S) n_{k+1}.
 """
 from __future__ import annotations
 
@@ -183,28 +185,42 @@ def main():
     log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {updates_per_step} updates/step, "
         f"few/many iterations {last.few_path_iters}/{last.many_path_iters}, final n={last.active_final}")
 
-    # per-kernel device times (HIP events on the engine's stream, one launch per kernel)
+    # per-launch device times (HIP events on the engine's stream).  Launch 0
+    # updates + marks iteration 1; launch i (1 <= i < iters) compacts iteration
+    # i's candidates and updates + marks iteration i + 1; the last launch only
+    # compacts.  The middle launches (k_iter, middle mode) are the dominant kernel.
     ctx.checkpoint_restore(0)
-    pst, kt = ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, w.iterations)
+    pst, kt, per_launch = ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, w.iterations)
     pc = ctx.iteration_counts()
     assert np.array_equal(pc.n, counts.n), "profiled run diverged from the timed run"
     B = algorithmic_bytes(pc.n, pc.c, pc.m, S)
-    times = {"update": kt.update_ms, "mark": kt.mark_ms, "compact": kt.compact_ms}
-    names = {"update": "k_update", "mark": "k_mark", "compact": "k_compact"}
-    kernels = {}
-    for k in times:
-        launches = int(kt.iterations)
-        avg_us = 1e3 * times[k] / max(launches, 1)
-        bytes_per_launch = float(B[k].sum()) / max(launches, 1)
-        kernels[k] = {"kernel": names[k], "avg_launch_us": avg_us, "bytes_per_launch": bytes_per_launch,
-                      "achieved_GBs": bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0,
-                      "total_ms": times[k]}
-    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
-    d = kernels[dom]
-    path_bytes = float(sum(v.sum() for v in B.values()))
+    um = B["update"] + B["mark"]          # per iteration k (index k-1)
+    cp = B["compact"]
+    it = w.iterations
+    launch_bytes = np.zeros(it + 1)
+    launch_bytes[0] = um[0]
+    launch_bytes[1:it] = cp[:it - 1] + um[1:it]
+    launch_bytes[it] = cp[it - 1]
+    mids = slice(1, it)
+    mid_ms = float(per_launch[mids].sum())
+    mid_launches = it - 1
+    avg_us = 1e3 * mid_ms / max(mid_launches, 1)
+    bytes_per_launch = float(launch_bytes[mids].sum()) / max(mid_launches, 1)
+    achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
+    path_bytes = float(launch_bytes.sum())
     path_gbs = path_bytes / (ms_per_step * 1e-3) / 1e9
-    log("per-kernel: " + ", ".join(f"{v['kernel']} {v['avg_launch_us']:.2f} us/launch "
-                                   f"{v['achieved_GBs']:.0f} GB/s" for v in kernels.values()))
+    big = pc.n >= np.percentile(pc.n, 90)
+    big_launch = np.zeros(it + 1, bool)
+    big_launch[1:it] = big[1:it]
+    big_gbs = (launch_bytes[big_launch].sum() / (per_launch[big_launch].sum() * 1e-3) / 1e9
+               if big_launch.any() else 0.0)
+    log(f"k_iter middle launches: {avg_us:.2f} us/launch, {bytes_per_launch / 1e6:.2f} MB/launch, "
+        f"{achieved:.0f} GB/s; busiest decile {big_gbs:.0f} GB/s; first {kt.first_ms * 1e3:.1f} us, "
+        f"last {kt.last_ms * 1e3:.1f} us; event-timed sum {kt.total_ms:.2f} ms vs {ms_per_step:.2f} ms/step")
+    kernels = {"k_iter_mid": {"avg_launch_us": avg_us, "bytes_per_launch": bytes_per_launch,
+                              "achieved_GBs": achieved, "launches": mid_launches},
+               "k_iter_first_us": kt.first_ms * 1e3, "k_iter_last_us": kt.last_ms * 1e3,
+               "busiest_decile_GBs": big_gbs}
 
     cpu = None
     dice = None
@@ -235,10 +251,10 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": {
-            "bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": d["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
-            "kernel": d["kernel"], "bytes_per_launch": d["bytes_per_launch"],
-            "avg_launch_us": d["avg_launch_us"],
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_iter (middle launch: compaction + update + marking)",
+            "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_us,
         },
         "path_roofline": {"achieved": path_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": path_gbs / HBM_PEAK_GBS, "bytes_per_step": path_bytes},

@@ -202,19 +202,25 @@ lsseg_status lsseg_set_state(lsseg_ctx* ctx, const float* phi_read, const float*
 lsseg_status lsseg_checkpoint_save(lsseg_ctx* ctx, int32_t slot);
 lsseg_status lsseg_checkpoint_restore(lsseg_ctx* ctx, int32_t slot);
 
-/* Summed device time of each kernel of the iteration, measured with HIP events
-   recorded on the context's stream around every launch. */
+/* Device time of the iteration kernels, measured with HIP events recorded on
+   the context's stream around every launch.  An lsseg_iterate call of `iters`
+   iterations makes iters + 1 launches: the first updates and marks the current
+   list; each later one compacts the previous iteration's candidates
+   (duplicate removal + compaction) and updates and marks the result; the last
+   one only compacts. */
 typedef struct lsseg_kernel_times {
-    double update_ms;    /* level-set update (UpdateLevelSetVolume.cu) */
-    double mark_ms;      /* marking (UpdateActiveElementsVolume.cu) */
-    double compact_ms;   /* duplicate removal + compaction (FilterDuplicates.cu, CUDPP) */
-    uint64_t iterations;
+    double total_ms;     /* all iteration launches */
+    double first_ms;     /* the first launch */
+    double last_ms;      /* the final compaction */
+    uint64_t launches;   /* iters + 1 */
 } lsseg_kernel_times;
 
-/* lsseg_iterate with per-kernel event timing (kernels launched one by one, no
-   graph).  Results and statistics are identical to lsseg_iterate. */
+/* lsseg_iterate with per-launch event timing (launches issued one by one, no
+   graph).  Results and statistics are identical to lsseg_iterate.  launch_ms
+   (may be NULL) receives up to launch_cap per-launch times in launch order. */
 lsseg_status lsseg_iterate_profiled(lsseg_ctx* ctx, const lsseg_params* params, int32_t iters,
-                                    lsseg_stats* stats, lsseg_kernel_times* times);
+                                    lsseg_stats* stats, lsseg_kernel_times* times,
+                                    float* launch_ms, uint64_t launch_cap);
 
 /* Device pointers of the phi buffers (read, write) and the stream, for callers
    that keep data resident in HBM (e.g. a torch wrapper). */
