@@ -294,28 +294,26 @@ __device__ inline int8_t export_byte(float v)
     return (int8_t)(int32_t)q;
 }
 
+// ComputeDensityTerm (UpdateLevelSetVolume.cu:335-342): f = (-1/eps) |I - T| + 1
 template <typename SrcT, bool kFloatSrc>
-__device__ inline float update_value(const KArgs& a, const DevParams& P, const float* R, uint32_t t,
-                                     const uint32_t nb[7], float* u4_out)
+__device__ inline float density_factor(const KArgs& a, const DevParams& P, uint32_t t)
 {
-    const float unz = R[nb[0]];
-    const float u1 = R[nb[1]];
-    const float u3 = R[nb[2]];
-    const float u4 = R[t];
-    const float u5 = R[nb[4]];
-    const float u7 = R[nb[5]];
-    const float upz = R[nb[6]];
-    float f;
     if constexpr (kFloatSrc) {
         const float s = static_cast<const float*>(a.src)[t];
         const float d = s - P.Tf;
-        f = P.negInv * fabsf(d) + 1.0f;
+        return P.negInv * fabsf(d) + 1.0f;
     } else {
         const int32_t s = (int32_t)(static_cast<const SrcT*>(a.src)[t]);
         const int32_t d = s - P.Ti;
         const int32_t ad = d < 0 ? -d : d;
-        f = P.negInv * (float)ad + 1.0f;  // ComputeDensityTerm, :335-342
+        return P.negInv * (float)ad + 1.0f;
     }
+}
+
+// the update of one voxel from its 7-point neighbourhood and density factor f
+__device__ inline float update_core(const DevParams& P, float unz, float u1, float u3, float u4, float u5, float u7,
+                                    float upz, float f)
+{
     const float dcx = (u5 - u3) / 2.0f;
     const float dcy = (u7 - u1) / 2.0f;
     const float dcz = (upz - unz) / 2.0f;
@@ -337,41 +335,103 @@ __device__ inline float update_value(const KArgs& a, const DevParams& P, const f
     const float delta = ((P.dtAbs * speed * g)) + conn;
     float nv = fmin_ref(1.0f, fmax_ref(-1.0f, u4 + delta));
     nv = fmin_ref(1.0f, fmax_ref(-1.0f, ((1.0f + kRescale) * nv) + kRescale));
-    *u4_out = u4;
     return nv;
 }
 
+__device__ inline uint32_t tau_of(const Geom& g, uint32_t x, uint32_t y, uint32_t z)
+{
+    return (((z >> 4) * g.TY + (y >> 4)) * g.TX + (x >> 4)) * 4096u + ((z & 15u) * 16u + (y & 15u)) * 16u +
+           (x & 15u);
+}
+
+// list-direction offsets (0:-z 1:-y 2:-x 3:self 4:+x 5:+y 6:+z)
+constexpr int kDx[7] = {0, 0, -1, 0, 1, 0, 0};
+constexpr int kDy[7] = {0, -1, 0, 0, 0, 1, 0};
+constexpr int kDz[7] = {-1, 0, 0, 0, 0, 0, 1};
+
 // Update voxel u at list position p of the list with parity `op`, then mark it.
 // Cmem marks the members of the list being updated; emissions go to Cemit.
-// Returns the number of candidates emitted.
+// Memory is read in two independent rounds: the voxel's own 7-point stencil and
+// source, then (only if the voxel can emit) its radius-2 diamond, the sources,
+// membership records and write-buffer values of its face neighbours.  Returns
+// the number of candidates emitted.
 template <typename SrcT, bool kFloatSrc>
 __device__ inline uint32_t update_mark(const KArgs& a, const DevParams& P, uint32_t u, uint32_t p, int op,
                                        const uint64_t* Cmem, uint8_t* Cemit)
 {
     const float* R = a.phi[op];
     float* W = a.phi[op ^ 1];
-    uint32_t nb[7], inb;
-    neighbors(a.g, u, nb, &inb);
-    float r4;
-    const float nv = update_value<SrcT, kFloatSrc>(a, P, R, u, nb, &r4);
+    const Geom& g = a.g;
+    uint32_t x, y, z;
+    decode_xyz(g, u, &x, &y, &z);
+    const int ix = (int)x, iy = (int)y, iz = (int)z;
+    bool fb[7];
+    uint32_t ft[7];
+#pragma unroll
+    for (int e = 0; e < 7; ++e) {
+        const int nx = ix + kDx[e], ny = iy + kDy[e], nz = iz + kDz[e];
+        fb[e] = e != 3 && nx >= 0 && ny >= 0 && nz >= 0 && nx < (int)g.X && ny < (int)g.Y && nz < (int)g.Z;
+        ft[e] = fb[e] ? tau_of(g, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz) : u;  // clamp to self
+    }
+    // round 1: the voxel's own neighbourhood (UpdateLevelSetVolume.cu:248-258)
+    const float c = R[u];
+    float F[7];
+#pragma unroll
+    for (int e = 0; e < 7; ++e) F[e] = e == 3 ? c : R[ft[e]];
+    const float fu = density_factor<SrcT, kFloatSrc>(a, P, u);
+    const float nv = update_core(P, F[0], F[1], F[2], c, F[4], F[5], F[6], fu);
     W[u] = nv;
     a.E[u] = export_byte(nv);
     uint32_t m = 0;
     // temporal AND spatial test (UpdateActiveElements.cu:75-103) on the new phi,
     // positions p % 4 == 3 skipped (UpdateActiveElementsVolume.cu:329)
-    if (fabsf(nv - r4) > kTemporalThreshold && (!P.refMode || (p & 3u) != 3u)) {
+    if (fabsf(nv - c) > kTemporalThreshold && (!P.refMode || (p & 3u) != 3u)) {
+        // round 2: everything the neighbours' new values depend on
+        uint64_t mem[7];
+        float wv[7], fw[7], ax[7], ed[7][7];
 #pragma unroll
         for (int e = 0; e < 7; ++e) {
-            if (e == 3 || !((inb >> e) & 1u)) continue;
-            const uint32_t w = nb[e];
+            if (e == 3) continue;
+            mem[e] = fb[e] ? Cmem[ft[e]] : 0ull;
+            wv[e] = fb[e] ? W[ft[e]] : 0.0f;
+            fw[e] = fb[e] ? density_factor<SrcT, kFloatSrc>(a, P, ft[e]) : 0.0f;
+            // axis point u + 2 d_e
+            const int x2 = ix + 2 * kDx[e], y2 = iy + 2 * kDy[e], z2 = iz + 2 * kDz[e];
+            const bool in2 = x2 >= 0 && y2 >= 0 && z2 >= 0 && x2 < (int)g.X && y2 < (int)g.Y && z2 < (int)g.Z;
+            ax[e] = in2 ? R[tau_of(g, (uint32_t)x2, (uint32_t)y2, (uint32_t)z2)] : F[e];
+        }
+        // edge points u + d_a + d_b (different axes), each loaded once
+#pragma unroll
+        for (int ea = 0; ea < 7; ++ea) {
+#pragma unroll
+            for (int eb = ea + 1; eb < 7; ++eb) {
+                if (ea == 3 || eb == 3 || ea + eb == 6) continue;
+                const bool in = fb[ea] && fb[eb];
+                const float v = in ? R[tau_of(g, (uint32_t)(ix + kDx[ea] + kDx[eb]), (uint32_t)(iy + kDy[ea] + kDy[eb]),
+                                              (uint32_t)(iz + kDz[ea] + kDz[eb]))]
+                                   : 0.0f;
+                ed[ea][eb] = v;
+                ed[eb][ea] = v;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 7; ++e) {
+            if (e == 3 || !fb[e]) continue;
             float vw;
-            if (Cmem[w] != 0ull) {
-                uint32_t nbw[7], inbw;
-                neighbors(a.g, w, nbw, &inbw);
-                float dummy;
-                vw = update_value<SrcT, kFloatSrc>(a, P, R, w, nbw, &dummy);
+            if (mem[e] != 0ull) {
+                // neighbour w = u + d_e is active: its new value, recomputed with
+                // the same arithmetic; its out-of-bounds neighbours are w itself
+                float s[7];
+#pragma unroll
+                for (int e2 = 0; e2 < 7; ++e2) {
+                    if (e2 == 3) continue;
+                    if (e2 + e == 6) s[e2] = c;                  // back to u
+                    else if (e2 == e) s[e2] = ax[e];              // u + 2 d_e
+                    else s[e2] = fb[e2] ? ed[e][e2] : F[e];       // u + d_e + d_e2
+                }
+                vw = update_core(P, s[0], s[1], s[2], F[e], s[4], s[5], s[6], fw[e]);
             } else {
-                vw = W[w];
+                vw = wv[e];  // inactive: the write buffer keeps its older value
             }
             if (!(fabsf(nv - vw) <= kTemporalThreshold)) m |= 1u << e;
         }
@@ -381,7 +441,7 @@ __device__ inline uint32_t update_mark(const KArgs& a, const DevParams& P, uint3
     if (m) {
 #pragma unroll
         for (int e = 0; e < 7; ++e)
-            if ((m >> e) & 1u) Cemit[(uint64_t)nb[e] * 8u + (uint32_t)e] = 1;
+            if ((m >> e) & 1u) Cemit[(uint64_t)(e == 3 ? u : ft[e]) * 8u + (uint32_t)e] = 1;
     }
     return (uint32_t)__popc(m);
 }
@@ -471,17 +531,15 @@ __global__ __launch_bounds__(kBlock) void k_iter(KArgs a, int op, int ck)
                                            (uint32_t)(s >= 5ull * n_src) + (uint32_t)(s >= 6ull * n_src);
                         const uint32_t p = (uint32_t)(s - (uint64_t)e * n_src);
                         const uint32_t m = Ms[p];
+                        const uint32_t t = Vs[p];
                         const bool bit = (m >> e) & 1u;
-                        if (e == 0u || bit) {
-                            const uint32_t t = Vs[p];
-                            if (kMode != kInit && e == 0u) Cc[t] = 0ull;
-                            if (bit) {
-                                const uint32_t u = neighbor_in(a.g, t, e);
-                                const uint64_t rec = Ck[u];
-                                // first-wins: the lowest list that emitted u keeps it
-                                keep = (uint32_t)((__ffsll((unsigned long long)rec) - 1) >> 3) == e;
-                                v = u;
-                            }
+                        if (kMode != kInit && e == 0u) Cc[t] = 0ull;
+                        if (bit) {
+                            const uint32_t u = neighbor_in(a.g, t, e);
+                            const uint64_t rec = Ck[u];
+                            // first-wins: the lowest list that emitted u keeps it
+                            keep = (uint32_t)((__ffsll((unsigned long long)rec) - 1) >> 3) == e;
+                            v = u;
                         }
                     }
                 }
@@ -674,8 +732,8 @@ __global__ __launch_bounds__(kBlock) void k_update_only(KArgs a, int op)
         const uint32_t u = a.list[op][p];
         uint32_t nb[7], inb;
         neighbors(a.g, u, nb, &inb);
-        float r4;
-        const float nv = update_value<SrcT, kFloatSrc>(a, P, R, u, nb, &r4);
+        const float nv = update_core(P, R[nb[0]], R[nb[1]], R[nb[2]], R[u], R[nb[4]], R[nb[5]], R[nb[6]],
+                                     density_factor<SrcT, kFloatSrc>(a, P, u));
         W[u] = nv;
         a.E[u] = export_byte(nv);
     }
@@ -700,12 +758,6 @@ __device__ inline int8_t add_export(int8_t ra, int8_t rb)
 __global__ void k_freeze(KArgs a)
 {
     LSSEG_GRID_STRIDE(t, a.g.N) a.frozen[t] = add_export(a.frozen[t], a.E[t]);
-}
-
-__device__ inline uint32_t tau_of(const Geom& g, uint32_t x, uint32_t y, uint32_t z)
-{
-    return (((z >> 4) * (g.Y >> 4) + (y >> 4)) * (g.X >> 4) + (x >> 4)) * 4096u +
-           ((z & 15u) * 16u + (y & 15u)) * 16u + (x & 15u);
 }
 
 __global__ void k_get_phi(KArgs a, int which, float* out, uint32_t nx, uint32_t ny, uint32_t nz)
