@@ -4,12 +4,11 @@
 // lsseg_device.hpp.  Reference paths in comments are relative to
 // code/gpuseg/code/ of the reference repository.
 //
-// An lsseg_iterate call of `iters` iterations launches one kernel per
-// iteration plus one final compaction: first (update + mark of the current
-// list), then iters - 1 launches that each compact the previous iteration's
-// candidates and update + mark the result, then last (compaction only).  All
-// counts stay in device memory; the host never reads n between iterations.
-// Runs of middle launches are captured in hipGraphs and replayed.
+// An lsseg_iterate call of `iters` iterations launches: an update + mark of the
+// current list, then for each later iteration a compaction of the previous
+// iteration's candidates followed by an update + mark of the new list, then a
+// final compaction.  All counts stay in device memory; the host never reads n
+// between iterations.  Runs of iterations are captured in hipGraphs.
 //
 // State at rest (between calls): list[q] holds the active list (count n[q]),
 // phi[q] is the read buffer, emission record C[cm] is non-zero exactly at the
@@ -34,7 +33,7 @@ using namespace lsseg;
 
 namespace {
 thread_local std::string g_create_error;
-constexpr int kGraphMids = 30;  // middle launches per captured graph (multiple of 6)
+constexpr int kGraphPairs = 30;  // iterations per captured graph (multiple of 6)
 }  // namespace
 
 struct lsseg_ctx {
@@ -198,12 +197,20 @@ lsseg_status dense_init_list(lsseg_ctx* c, int target)
     return LSSEG_OK;
 }
 
-// Middle launches (iterations 2..iters): the first one has state (op, ck); each
-// launch flips op and advances ck.  Runs of kGraphMids launches (a multiple of
-// 6, so the state returns to its start) replay a captured graph.
+// Iterations 2..iters: each is a compaction launch (the previous iteration's
+// candidates into the next list) followed by an update + mark launch over that
+// list.  The first pair has state (op, ck); each pair flips op and advances ck.
+// Runs of kGraphPairs pairs (a multiple of 6, so the state returns to its start)
+// replay a captured graph.
+void launch_pair(lsseg_ctx* c, int op, int ck)
+{
+    launch_iter<kLast>(c, op, ck);
+    launch_iter<kFirst>(c, op, ck);
+}
+
 lsseg_status launch_mids(lsseg_ctx* c, int count, int op, int ck)
 {
-    const int chunk = env_int("LSSEG_GRAPH", 1) ? kGraphMids : 0;
+    const int chunk = env_int("LSSEG_GRAPH", 1) ? kGraphPairs : 0;
     while (chunk && !c->profile && count >= chunk) {
         hipGraphExec_t& ge = c->graphs[op][ck];
         if (!ge) {
@@ -211,7 +218,7 @@ lsseg_status launch_mids(lsseg_ctx* c, int count, int op, int ck)
             HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int o = op, k = ck;
             for (int i = 0; i < chunk; ++i) {
-                launch_iter<kMid>(c, o, k);
+                launch_pair(c, o, k);
                 o ^= 1;
                 k = rot(k, 1);
             }
@@ -228,7 +235,7 @@ lsseg_status launch_mids(lsseg_ctx* c, int count, int op, int ck)
         count -= chunk;
     }
     for (int i = 0; i < count; ++i) {
-        launch_iter<kMid>(c, op, ck);
+        launch_pair(c, op, ck);
         op ^= 1;
         ck = rot(ck, 1);
     }
@@ -622,7 +629,7 @@ lsseg_status lsseg_iterate_profiled(lsseg_ctx* c, const lsseg_params* prm, int32
 {
     if (!c || !times) return LSSEG_ERR_INVALID_ARG;
     if (iters < 0) return fail(c, LSSEG_ERR_INVALID_ARG, "negative iteration count");
-    const size_t need = (size_t)iters + 2;
+    const size_t need = 2 * (size_t)iters + 2;
     while (c->prof_events.size() < need) {
         hipEvent_t e;
         HIP_TRY(c, hipEventCreate(&e));
@@ -635,17 +642,18 @@ lsseg_status lsseg_iterate_profiled(lsseg_ctx* c, const lsseg_params* prm, int32
     if (st != LSSEG_OK) return st;
     memset(times, 0, sizeof(*times));
     if (iters == 0) return LSSEG_OK;
-    // events: before the first launch, before each middle launch, before the
-    // last launch and after it
-    for (int32_t i = 0; i <= iters; ++i) {
+    // events precede every launch (first update, then compaction / update pairs,
+    // then the final compaction) and follow the last one
+    const int32_t launches = 2 * iters;
+    for (int32_t i = 0; i < launches; ++i) {
         float ms = 0;
         HIP_TRY(c, hipEventElapsedTime(&ms, c->prof_events[i], c->prof_events[i + 1]));
         times->total_ms += ms;
         if (i == 0) times->first_ms = ms;
-        if (i == iters) times->last_ms = ms;
+        if (i == launches - 1) times->last_ms = ms;
         if (launch_ms && (uint64_t)i < launch_cap) launch_ms[i] = ms;
     }
-    times->launches = (uint64_t)iters + 1;
+    times->launches = (uint64_t)launches;
     return LSSEG_OK;
 }
 

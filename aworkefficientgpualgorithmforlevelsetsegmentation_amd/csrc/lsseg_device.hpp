@@ -351,10 +351,11 @@ constexpr int kDz[7] = {-1, 0, 0, 0, 0, 0, 1};
 
 // Update voxel u at list position p of the list with parity `op`, then mark it.
 // Cmem marks the members of the list being updated; emissions go to Cemit.
-// Memory is read in two independent rounds: the voxel's own 7-point stencil and
-// source, then (only if the voxel can emit) its radius-2 diamond, the sources,
-// membership records and write-buffer values of its face neighbours.  Returns
-// the number of candidates emitted.
+// Every load is issued in one round: the voxel's 7-point stencil and source
+// and, when its list position may emit at all, its radius-2 diamond and the
+// sources, membership records and write-buffer values of its face neighbours
+// (used only if the voxel's own change passes the temporal threshold).
+// Returns the number of candidates emitted.
 template <typename SrcT, bool kFloatSrc>
 __device__ inline uint32_t update_mark(const KArgs& a, const DevParams& P, uint32_t u, uint32_t p, int op,
                                        const uint64_t* Cmem, uint8_t* Cemit)
@@ -373,22 +374,17 @@ __device__ inline uint32_t update_mark(const KArgs& a, const DevParams& P, uint3
         fb[e] = e != 3 && nx >= 0 && ny >= 0 && nz >= 0 && nx < (int)g.X && ny < (int)g.Y && nz < (int)g.Z;
         ft[e] = fb[e] ? tau_of(g, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz) : u;  // clamp to self
     }
-    // round 1: the voxel's own neighbourhood (UpdateLevelSetVolume.cu:248-258)
+    // the voxel's own neighbourhood (UpdateLevelSetVolume.cu:248-258)
     const float c = R[u];
     float F[7];
 #pragma unroll
     for (int e = 0; e < 7; ++e) F[e] = e == 3 ? c : R[ft[e]];
     const float fu = density_factor<SrcT, kFloatSrc>(a, P, u);
-    const float nv = update_core(P, F[0], F[1], F[2], c, F[4], F[5], F[6], fu);
-    W[u] = nv;
-    a.E[u] = export_byte(nv);
-    uint32_t m = 0;
-    // temporal AND spatial test (UpdateActiveElements.cu:75-103) on the new phi,
-    // positions p % 4 == 3 skipped (UpdateActiveElementsVolume.cu:329)
-    if (fabsf(nv - c) > kTemporalThreshold && (!P.refMode || (p & 3u) != 3u)) {
-        // round 2: everything the neighbours' new values depend on
-        uint64_t mem[7];
-        float wv[7], fw[7], ax[7], ed[7][7];
+    // positions p % 4 == 3 never emit (UpdateActiveElementsVolume.cu:329)
+    const bool may_emit = !P.refMode || (p & 3u) != 3u;
+    uint64_t mem[7];
+    float wv[7], fw[7], ax[7], ed[7][7];
+    if (may_emit) {
 #pragma unroll
         for (int e = 0; e < 7; ++e) {
             if (e == 3) continue;
@@ -414,6 +410,13 @@ __device__ inline uint32_t update_mark(const KArgs& a, const DevParams& P, uint3
                 ed[eb][ea] = v;
             }
         }
+    }
+    const float nv = update_core(P, F[0], F[1], F[2], c, F[4], F[5], F[6], fu);
+    W[u] = nv;
+    a.E[u] = export_byte(nv);
+    uint32_t m = 0;
+    // temporal AND spatial test (UpdateActiveElements.cu:75-103) on the new phi
+    if (may_emit && fabsf(nv - c) > kTemporalThreshold) {
 #pragma unroll
         for (int e = 0; e < 7; ++e) {
             if (e == 3 || !fb[e]) continue;

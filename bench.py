@@ -17,9 +17,9 @@ Algorithmic bytes per iteration k, FP32 phi (4 bytes), source S bytes/voxel:
   update   (38 + S) n_k   coordinate, 7 phi taps, source, phi write, flag, mask byte
   mark     33 n_k         coordinate, flag, 7 phi taps
   compact  10 c_k + 5 m_k candidate write/read-back, tag probe/set; list write, clear
-with n_k active voxels, c_k candidate emissions and m_k new active voxels.  One
-launch of the iteration kernel compacts iteration k's candidates and updates
-and marks iteration k + 1, so its bytes are 10 c_k + 5 m_k + (71 + S) n_{k+1}.
+with n_k active voxels, c_k candidate emissions and m_k new active voxels.  Each
+iteration is two launches of the iteration kernel: update + marking ((71 + S) n_k
+bytes) and duplicate removal + compaction (10 c_k + 5 m_k bytes).
 """
 from __future__ import annotations
 
@@ -185,42 +185,40 @@ def main():
     log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {updates_per_step} updates/step, "
         f"few/many iterations {last.few_path_iters}/{last.many_path_iters}, final n={last.active_final}")
 
-    # per-launch device times (HIP events on the engine's stream).  Launch 0
-    # updates + marks iteration 1; launch i (1 <= i < iters) compacts iteration
-    # i's candidates and updates + marks iteration i + 1; the last launch only
-    # compacts.  The middle launches (k_iter, middle mode) are the dominant kernel.
+    # per-launch device times (HIP events on the engine's stream).  Launches
+    # alternate: update + mark of iteration k (k_iter, update mode), then the
+    # compaction of its candidates (k_iter, compaction mode).
     ctx.checkpoint_restore(0)
     pst, kt, per_launch = ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, w.iterations)
     pc = ctx.iteration_counts()
     assert np.array_equal(pc.n, counts.n), "profiled run diverged from the timed run"
     B = algorithmic_bytes(pc.n, pc.c, pc.m, S)
-    um = B["update"] + B["mark"]          # per iteration k (index k-1)
-    cp = B["compact"]
     it = w.iterations
-    launch_bytes = np.zeros(it + 1)
-    launch_bytes[0] = um[0]
-    launch_bytes[1:it] = cp[:it - 1] + um[1:it]
-    launch_bytes[it] = cp[it - 1]
-    mids = slice(1, it)
-    mid_ms = float(per_launch[mids].sum())
-    mid_launches = it - 1
-    avg_us = 1e3 * mid_ms / max(mid_launches, 1)
-    bytes_per_launch = float(launch_bytes[mids].sum()) / max(mid_launches, 1)
-    achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
-    path_bytes = float(launch_bytes.sum())
+    t_upd = per_launch[0::2][:it].astype(np.float64)
+    t_cmp = per_launch[1::2][:it].astype(np.float64)
+    parts = {
+        "update": {"kernel": "k_iter (update + marking mode)", "bytes": B["update"] + B["mark"], "ms": t_upd},
+        "compact": {"kernel": "k_iter (duplicate removal + compaction mode)", "bytes": B["compact"], "ms": t_cmp},
+    }
+    kernels = {}
+    for k, v in parts.items():
+        tot_ms = float(v["ms"].sum())
+        avg_us = 1e3 * tot_ms / it
+        bpl = float(v["bytes"].sum()) / it
+        kernels[k] = {"kernel": v["kernel"], "avg_launch_us": avg_us, "bytes_per_launch": bpl,
+                      "achieved_GBs": bpl / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0, "total_ms": tot_ms}
+        big = pc.n >= np.percentile(pc.n, 90)
+        kernels[k]["busiest_decile_GBs"] = (float(v["bytes"][big].sum()) / (float(v["ms"][big].sum()) * 1e-3) / 1e9
+                                            if big.any() and v["ms"][big].sum() > 0 else 0.0)
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    d = kernels[dom]
+    avg_us, bytes_per_launch, achieved = d["avg_launch_us"], d["bytes_per_launch"], d["achieved_GBs"]
+    path_bytes = float(sum(v.sum() for v in B.values()))
     path_gbs = path_bytes / (ms_per_step * 1e-3) / 1e9
-    big = pc.n >= np.percentile(pc.n, 90)
-    big_launch = np.zeros(it + 1, bool)
-    big_launch[1:it] = big[1:it]
-    big_gbs = (launch_bytes[big_launch].sum() / (per_launch[big_launch].sum() * 1e-3) / 1e9
-               if big_launch.any() else 0.0)
-    log(f"k_iter middle launches: {avg_us:.2f} us/launch, {bytes_per_launch / 1e6:.2f} MB/launch, "
-        f"{achieved:.0f} GB/s; busiest decile {big_gbs:.0f} GB/s; first {kt.first_ms * 1e3:.1f} us, "
-        f"last {kt.last_ms * 1e3:.1f} us; event-timed sum {kt.total_ms:.2f} ms vs {ms_per_step:.2f} ms/step")
-    kernels = {"k_iter_mid": {"avg_launch_us": avg_us, "bytes_per_launch": bytes_per_launch,
-                              "achieved_GBs": achieved, "launches": mid_launches},
-               "k_iter_first_us": kt.first_ms * 1e3, "k_iter_last_us": kt.last_ms * 1e3,
-               "busiest_decile_GBs": big_gbs}
+    log("per launch: " + "; ".join(f"{k} {v['avg_launch_us']:.2f} us, {v['bytes_per_launch'] / 1e6:.2f} MB, "
+                                   f"{v['achieved_GBs']:.0f} GB/s (busiest decile {v['busiest_decile_GBs']:.0f})"
+                                   for k, v in kernels.items())
+        + f"; event-timed sum {kt.total_ms:.2f} ms vs {ms_per_step:.2f} ms/step")
 
     cpu = None
     dice = None
@@ -253,8 +251,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_iter (middle launch: compaction + update + marking)",
-            "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_us,
+            "kernel": d["kernel"], "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_us,
         },
         "path_roofline": {"achieved": path_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": path_gbs / HBM_PEAK_GBS, "bytes_per_step": path_bytes},

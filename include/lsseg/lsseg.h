@@ -204,15 +204,15 @@ lsseg_status lsseg_checkpoint_restore(lsseg_ctx* ctx, int32_t slot);
 
 /* Device time of the iteration kernels, measured with HIP events recorded on
    the context's stream around every launch.  An lsseg_iterate call of `iters`
-   iterations makes iters + 1 launches: the first updates and marks the current
-   list; each later one compacts the previous iteration's candidates
-   (duplicate removal + compaction) and updates and marks the result; the last
-   one only compacts. */
+   iterations makes 2 * iters launches: an update + mark of the current list,
+   then per later iteration a compaction (duplicate removal + compaction of the
+   previous iteration's candidates) and an update + mark of the new list, then a
+   final compaction. */
 typedef struct lsseg_kernel_times {
     double total_ms;     /* all iteration launches */
     double first_ms;     /* the first launch */
     double last_ms;      /* the final compaction */
-    uint64_t launches;   /* iters + 1 */
+    uint64_t launches;   /* 2 * iters */
 } lsseg_kernel_times;
 
 /* lsseg_iterate with per-launch event timing (launches issued one by one, no
