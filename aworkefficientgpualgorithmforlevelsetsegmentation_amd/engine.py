@@ -194,7 +194,7 @@ class Context:
         prm = Params(T, eps, alpha, dt, mode)
         st = Stats()
         kt = KernelTimes()
-        per = np.zeros(int(iters) + 1, np.float32)
+        per = np.zeros(2 * int(iters) + 1, np.float32)
         self._check(lib().lsseg_iterate_profiled(self._h, ctypes.byref(prm), int(iters), ctypes.byref(st),
                                                  ctypes.byref(kt), _p(per), per.size))
         return st, kt, per
