@@ -4,15 +4,14 @@
 // lsseg_device.hpp.  Reference paths in comments are relative to
 // code/gpuseg/code/ of the reference repository.
 //
-// An lsseg_iterate call of `iters` iterations launches: an update + mark of the
-// current list, then for each later iteration a compaction of the previous
-// iteration's candidates followed by an update + mark of the new list, then a
-// final compaction.  All counts stay in device memory; the host never reads n
-// between iterations.  Runs of iterations are captured in hipGraphs.
+// An lsseg_iterate call of `iters` iterations launches, per iteration, the
+// update, the marking and the compaction of the current list.  All counts stay
+// in device memory; the host never reads n between iterations.  Runs of
+// iterations are captured in hipGraphs.
 //
 // State at rest (between calls): list[q] holds the active list (count n[q]),
-// phi[q] is the read buffer, emission record C[cm] is non-zero exactly at the
-// list's voxels and the other two records are zero.
+// phi[q] is the read buffer and the emission record C is non-zero exactly at the
+// list's voxels.
 
 #include <hip/hip_runtime.h>
 
@@ -33,7 +32,7 @@ using namespace lsseg;
 
 namespace {
 thread_local std::string g_create_error;
-constexpr int kGraphPairs = 30;  // iterations per captured graph (multiple of 6)
+constexpr int kGraphIters = 32;  // iterations per captured graph (even: the parity returns)
 }  // namespace
 
 struct lsseg_ctx {
@@ -45,8 +44,8 @@ struct lsseg_ctx {
     uint32_t S = 1;
     float z_aniso = 1.0f;
     int q = 0;                        // parity of the current list and phi read buffer
-    int cm = 0;                       // emission record holding the current list's members
-    uint32_t grid = 1024;             // co-resident workgroups of the iteration kernel
+    uint32_t grid = 1024;             // co-resident workgroups of the compaction kernel
+    uint32_t sgrid = 2048;            // workgroups of the update and marking kernels
     uint32_t cus = 256;
     KArgs a{};
     DevParams* d_prm = nullptr;
@@ -54,7 +53,7 @@ struct lsseg_ctx {
     DevCtl* h_ctl = nullptr;          // pinned mirror
     uint32_t stats_cap = 0;
     uint64_t last_iters = 0;
-    hipGraphExec_t graphs[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    hipGraphExec_t graphs[2] = {nullptr, nullptr};
     struct Checkpoint {
         float* phi[2] = {nullptr, nullptr};
         int8_t* E = nullptr;
@@ -62,7 +61,7 @@ struct lsseg_ctx {
         uint32_t n = 0;
         bool valid = false;
     } ckpt[LSSEG_MAX_CHECKPOINTS];
-    bool profile = false;             // record an event around every iteration launch
+    bool profile = false;             // record an event before every iteration launch
     std::vector<hipEvent_t> prof_events;
     size_t prof_next = 0;
     std::vector<void*> allocs;
@@ -112,11 +111,10 @@ void dfree(lsseg_ctx* c, void* p)
 
 void destroy_graphs(lsseg_ctx* c)
 {
-    for (auto& row : c->graphs)
-        for (auto& g : row) {
-            if (g) (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
+    for (auto& g : c->graphs) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
 }
 
 uint32_t dense_grid(uint64_t count)
@@ -131,20 +129,17 @@ int env_int(const char* name, int dflt)
     return v && *v ? atoi(v) : dflt;
 }
 
-inline int rot(int c, int k) { return ((c + k) % 3 + 3) % 3; }
-
-// k_iter instantiation for the context's source type
-template <int kMode>
-const void* iter_kernel(int32_t dtype)
+// k_update instantiation for the context's source type
+const void* update_kernel(int32_t dtype)
 {
     switch (dtype) {
-    case LSSEG_U8: return (const void*)&k_iter<uint8_t, false, kMode>;
-    case LSSEG_I8: return (const void*)&k_iter<int8_t, false, kMode>;
-    case LSSEG_U16: return (const void*)&k_iter<uint16_t, false, kMode>;
-    case LSSEG_I16: return (const void*)&k_iter<int16_t, false, kMode>;
-    case LSSEG_U32: return (const void*)&k_iter<uint32_t, false, kMode>;
-    case LSSEG_I32: return (const void*)&k_iter<int32_t, false, kMode>;
-    default: return (const void*)&k_iter<float, true, kMode>;
+    case LSSEG_U8: return (const void*)&k_update<uint8_t, false>;
+    case LSSEG_I8: return (const void*)&k_update<int8_t, false>;
+    case LSSEG_U16: return (const void*)&k_update<uint16_t, false>;
+    case LSSEG_I16: return (const void*)&k_update<int16_t, false>;
+    case LSSEG_U32: return (const void*)&k_update<uint32_t, false>;
+    case LSSEG_I32: return (const void*)&k_update<int32_t, false>;
+    default: return (const void*)&k_update<float, true>;
     }
 }
 
@@ -154,13 +149,19 @@ void prof_mark(lsseg_ctx* c)
         (void)hipEventRecord(c->prof_events[c->prof_next++], c->stream);
 }
 
-template <int kMode>
-void launch_iter(lsseg_ctx* c, int op, int ck)
+void launch(lsseg_ctx* c, const void* fn, uint32_t grid, int q)
 {
-    void* args[] = {(void*)&c->a, (void*)&op, (void*)&ck};
-    const void* fn = kMode == kInit ? (const void*)&k_iter<uint8_t, false, kInit> : iter_kernel<kMode>(c->dtype);
+    void* args[] = {(void*)&c->a, (void*)&q};
     prof_mark(c);
-    (void)hipLaunchKernel(fn, dim3(c->grid), dim3(kBlock), args, 0, c->stream);
+    (void)hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args, 0, c->stream);
+}
+
+// one iteration over list[q]: update, marking, compaction into list[q ^ 1]
+void launch_iteration(lsseg_ctx* c, int q)
+{
+    launch(c, update_kernel(c->dtype), c->sgrid, q);
+    launch(c, (const void*)&k_mark, c->sgrid, q);
+    launch(c, (const void*)&k_compact<kCompact>, c->grid, q);
 }
 
 void launch_update_only(lsseg_ctx* c, int op)
@@ -184,44 +185,27 @@ void fill_u64(lsseg_ctx* c, uint64_t* p, uint64_t count, uint64_t v)
     hipLaunchKernelGGL(k_fill_u64, dim3(dense_grid(count)), dim3(kBlock), 0, c->stream, p, count, v);
 }
 
-// Dense compaction of the flags in C[c->cm] into list[target]; the other two
-// emission records are cleared; afterwards the context is at rest.
+// Dense compaction of the flags in C into list[target]; C stays non-zero
+// exactly at the list's voxels, so afterwards the context is at rest.
 lsseg_status dense_init_list(lsseg_ctx* c, int target)
 {
-    const uint64_t N = c->a.g.N;
-    fill_u64(c, c->a.C[rot(c->cm, 1)], N, 0);
-    fill_u64(c, c->a.C[rot(c->cm, 2)], N, 0);
-    launch_iter<kInit>(c, target, c->cm);
+    launch(c, (const void*)&k_compact<kInit>, c->grid, target ^ 1);
     HIP_TRY(c, hipGetLastError());
     c->q = target;
     return LSSEG_OK;
 }
 
-// Iterations 2..iters: each is a compaction launch (the previous iteration's
-// candidates into the next list) followed by an update + mark launch over that
-// list.  The first pair has state (op, ck); each pair flips op and advances ck.
-// Runs of kGraphPairs pairs (a multiple of 6, so the state returns to its start)
-// replay a captured graph.
-void launch_pair(lsseg_ctx* c, int op, int ck)
+// `count` iterations starting from list parity q; runs of kGraphIters
+// iterations (even, so the parity returns to its start) replay a captured graph.
+lsseg_status launch_iterations(lsseg_ctx* c, int count, int q)
 {
-    launch_iter<kLast>(c, op, ck);
-    launch_iter<kFirst>(c, op, ck);
-}
-
-lsseg_status launch_mids(lsseg_ctx* c, int count, int op, int ck)
-{
-    const int chunk = env_int("LSSEG_GRAPH", 1) ? kGraphPairs : 0;
+    const int chunk = env_int("LSSEG_GRAPH", 1) ? kGraphIters : 0;
     while (chunk && !c->profile && count >= chunk) {
-        hipGraphExec_t& ge = c->graphs[op][ck];
+        hipGraphExec_t& ge = c->graphs[q];
         if (!ge) {
             hipGraph_t g = nullptr;
             HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int o = op, k = ck;
-            for (int i = 0; i < chunk; ++i) {
-                launch_pair(c, o, k);
-                o ^= 1;
-                k = rot(k, 1);
-            }
+            for (int i = 0; i < chunk; ++i) launch_iteration(c, q ^ (i & 1));
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             if (e != hipSuccess) return fail(c, LSSEG_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
             e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
@@ -235,9 +219,8 @@ lsseg_status launch_mids(lsseg_ctx* c, int count, int op, int ck)
         count -= chunk;
     }
     for (int i = 0; i < count; ++i) {
-        launch_pair(c, op, ck);
-        op ^= 1;
-        ck = rot(ck, 1);
+        launch_iteration(c, q);
+        q ^= 1;
     }
     return LSSEG_OK;
 }
@@ -311,16 +294,16 @@ lsseg_status ensure_stats(lsseg_ctx* c, uint64_t iters)
     return LSSEG_OK;
 }
 
-// Makes list[0] (n entries, already on the device) the current list: rebuilds
-// the membership record C[cm] and clears the other two.
+// Makes list[0] (n entries, already on the device) the current list and
+// rebuilds the membership record C.
 lsseg_status inject_list_counts(lsseg_ctx* c, uint32_t n)
 {
     const uint64_t N = c->a.g.N;
     c->q = 0;
-    for (int k = 0; k < 3; ++k) fill_u64(c, c->a.C[k], N, 0);
+    fill_u64(c, c->a.C, N, 0);
     if (n)
         hipLaunchKernelGGL(k_scatter_membership, dim3(dense_grid(n)), dim3(kBlock), 0, c->stream, c->a.list[0], n,
-                           c->a.C[c->cm]);
+                           c->a.C);
     c->h_ctl->n[0] = n;
     c->h_ctl->n[1] = 0;
     HIP_TRY(c, hipMemcpyAsync(c->a.ctl->n, c->h_ctl->n, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
@@ -376,9 +359,10 @@ lsseg_status lsseg_create(const lsseg_volume_desc* desc, const void* host_volume
     int cus = 0;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
     c->cus = (uint32_t)std::max(1, cus);
-    // the look-back needs every workgroup of the iteration kernel resident at once
+    // the look-back needs every workgroup of the compaction kernel resident at once
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, iter_kernel<kMid>(c->dtype), kBlock, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)&k_compact<kCompact>, kBlock, 0) !=
+            hipSuccess ||
         occ < 1) {
         (void)hipGetLastError();
         occ = 1;
@@ -386,6 +370,9 @@ lsseg_status lsseg_create(const lsseg_volume_desc* desc, const void* host_volume
     const int want = env_int("LSSEG_GRID_BLOCKS", 0);
     c->grid = (uint32_t)occ * c->cus;
     if (want > 0) c->grid = std::min<uint32_t>(c->grid, (uint32_t)want);
+    // update and marking: grid-stride loops, eight workgroups per CU
+    const int swant = env_int("LSSEG_STREAM_BLOCKS", 0);
+    c->sgrid = swant > 0 ? (uint32_t)swant : 8u * c->cus;
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
         c->err = std::string("stream/event creation failed: ") + hipGetErrorString(e);
@@ -421,13 +408,10 @@ lsseg_status lsseg_create(const lsseg_volume_desc* desc, const void* host_volume
     ALLOC(phi[1], float, N);
     ALLOC(E, int8_t, N);
     ALLOC(frozen, int8_t, N);
-    ALLOC(C[0], uint64_t, N);
-    ALLOC(C[1], uint64_t, N);
-    ALLOC(C[2], uint64_t, N);
+    ALLOC(C, uint64_t, N);
     ALLOC(list[0], uint32_t, N);
     ALLOC(list[1], uint32_t, N);
-    ALLOC(mlist[0], uint8_t, N);
-    ALLOC(mlist[1], uint8_t, N);
+    ALLOC(mlist, uint8_t, N);
     ALLOC(tstate, uint64_t, ntiles);
     ALLOC(ctl, DevCtl, 1);
 #undef ALLOC
@@ -464,7 +448,7 @@ lsseg_status lsseg_create(const lsseg_volume_desc* desc, const void* host_volume
     hipLaunchKernelGGL(k_fill_f32, dim3(dg), dim3(kBlock), 0, c->stream, c->a.phi[1], N, 1.0f);
     hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, (uint8_t*)c->a.E, N, (uint8_t)127);
     hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, (uint8_t*)c->a.frozen, N, (uint8_t)127);
-    for (int k = 0; k < 3; ++k) fill_u64(c, c->a.C[k], N, 0);
+    fill_u64(c, c->a.C, N, 0);
     fill_u64(c, c->a.tstate, ntiles, 0);
     e = hipMemsetAsync(c->a.ctl, 0, sizeof(DevCtl), c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -473,7 +457,6 @@ lsseg_status lsseg_create(const lsseg_volume_desc* desc, const void* host_volume
         return bail(LSSEG_ERR_DEVICE);
     }
     c->q = 0;
-    c->cm = 0;
     *out_ctx = c;
     return LSSEG_OK;
 }
@@ -518,7 +501,7 @@ lsseg_status lsseg_set_phi(lsseg_ctx* c, const float* phi0, float band)
     hipError_t e = hipMemcpyAsync(lin, phi0, cnt * sizeof(float), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_set_phi, dim3(dense_grid(c->a.g.N)), dim3(kBlock), 0, c->stream, c->a, lin, c->nx,
-                           c->ny, c->nz, band, c->cm);
+                           c->ny, c->nz, band);
         e = hipStreamSynchronize(c->stream);
     }
     (void)hipFree(lin);
@@ -541,7 +524,7 @@ lsseg_status lsseg_seed_sphere(lsseg_ctx* c, float cx, float cy, float cz, uint3
                            (int32_t)radius, c->z_aniso);
         rd ^= 1;
     }
-    hipLaunchKernelGGL(k_keep_flags, dim3(dg), dim3(kBlock), 0, c->stream, c->a, rd, c->cm);
+    hipLaunchKernelGGL(k_keep_flags, dim3(dg), dim3(kBlock), 0, c->stream, c->a, rd);
     HIP_TRY(c, hipGetLastError());
     // two swaps leave the read parity unchanged: the list goes to that parity
     lsseg_status st = dense_init_list(c, rd);
@@ -558,7 +541,7 @@ lsseg_status lsseg_iterate_reinit(lsseg_ctx* c, const lsseg_params* prm)
     if ((st = upload_params(c, P)) != LSSEG_OK) return st;
     launch_update_only(c, c->q);
     const int rd = c->q ^ 1;
-    hipLaunchKernelGGL(k_keep_flags, dim3(dense_grid(c->a.g.N)), dim3(kBlock), 0, c->stream, c->a, rd, c->cm);
+    hipLaunchKernelGGL(k_keep_flags, dim3(dense_grid(c->a.g.N)), dim3(kBlock), 0, c->stream, c->a, rd);
     HIP_TRY(c, hipGetLastError());
     if ((st = dense_init_list(c, rd)) != LSSEG_OK) return st;
     return sync_and_check(c);
@@ -579,21 +562,17 @@ lsseg_status lsseg_iterate(lsseg_ctx* c, const lsseg_params* prm, int32_t iters,
     c->h_ctl->iter = 0;
     c->h_ctl->stats_cap = c->stats_cap;
     c->h_ctl->error = 0;
-    c->h_ctl->c_acc[0] = 0;
-    c->h_ctl->c_acc[1] = 0;
+    c->h_ctl->c_acc = 0;
     c->h_ctl->few_iters = 0;
     c->h_ctl->many_iters = 0;
     HIP_TRY(c, hipMemcpyAsync(c->a.ctl, c->h_ctl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
 
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     if (iters > 0) {
-        const int q0 = c->q, c0 = c->cm;
-        launch_iter<kFirst>(c, q0, c0);
-        if ((st = launch_mids(c, iters - 1, q0 ^ 1, rot(c0, 1))) != LSSEG_OK) return st;
-        launch_iter<kLast>(c, q0 ^ (iters & 1), rot(c0, iters));
+        const int q0 = c->q;
+        if ((st = launch_iterations(c, iters, q0)) != LSSEG_OK) return st;
         prof_mark(c);
         c->q = q0 ^ (iters & 1);
-        c->cm = rot(c0, iters);
     }
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
@@ -629,7 +608,7 @@ lsseg_status lsseg_iterate_profiled(lsseg_ctx* c, const lsseg_params* prm, int32
 {
     if (!c || !times) return LSSEG_ERR_INVALID_ARG;
     if (iters < 0) return fail(c, LSSEG_ERR_INVALID_ARG, "negative iteration count");
-    const size_t need = 2 * (size_t)iters + 2;
+    const size_t need = 3 * (size_t)iters + 1;
     while (c->prof_events.size() < need) {
         hipEvent_t e;
         HIP_TRY(c, hipEventCreate(&e));
@@ -642,9 +621,9 @@ lsseg_status lsseg_iterate_profiled(lsseg_ctx* c, const lsseg_params* prm, int32
     if (st != LSSEG_OK) return st;
     memset(times, 0, sizeof(*times));
     if (iters == 0) return LSSEG_OK;
-    // events precede every launch (first update, then compaction / update pairs,
-    // then the final compaction) and follow the last one
-    const int32_t launches = 2 * iters;
+    // events precede every launch (update, marking, compaction of each
+    // iteration) and follow the last one
+    const int32_t launches = 3 * iters;
     for (int32_t i = 0; i < launches; ++i) {
         float ms = 0;
         HIP_TRY(c, hipEventElapsedTime(&ms, c->prof_events[i], c->prof_events[i + 1]));

@@ -3,28 +3,25 @@
 // Reference paths in comments are relative to code/gpuseg/code/ of the
 // reference repository.
 //
-// One kernel launch per iteration.  Launch k:
-//   1. compacts the candidates that iteration k-1 emitted into the active list
-//      V_k (duplicate removal + stable compaction, Segmenter.cpp:883-1019) with
-//      a single-pass decoupled look-back scan (Merrill & Garland, "Single-pass
-//      Parallel Prefix Scan with Decoupled Look-back", 2016);
-//   2. updates every voxel it just placed (UpdateLevelSetVolume.cu:222-342) and
-//      marks it (UpdateActiveElements.cu:75-103), emitting candidates for V_{k+1}.
-// Marking compares the new phi of a voxel with the new phi of its neighbours.
-// A neighbour that is itself active is being updated by some other workgroup of
-// the same launch, so its new value is recomputed locally with the identical
-// arithmetic instead of waiting for a grid-wide barrier; an inactive neighbour
-// keeps the value already in the write buffer (the reference never copies
-// phi between its two buffers, Segmenter.cpp:841,1339-1351).
+// Three kernel launches per iteration k (all counts in device memory):
+//   update   one lane per list position: the 7-point update of phi
+//            (UpdateLevelSetVolume.cu:222-342) from the read buffer into the
+//            write buffer, plus the export byte;
+//   mark     one lane per list position: the temporal AND spatial test on the
+//            new phi (UpdateActiveElements.cu:75-103), positions p % 4 == 3
+//            skipped (UpdateActiveElementsVolume.cu:329), candidates recorded;
+//   compact  duplicate removal + stable compaction of the candidates into the
+//            next list (Segmenter.cpp:883-1019) with a single-pass decoupled
+//            look-back scan (Merrill & Garland, "Single-pass Parallel Prefix Scan
+//            with Decoupled Look-back", 2016).
 //
-// Emission records: C[j] (three rotating buffers, 8 bytes per voxel) holds the
-// candidates of iteration j: byte e of C[j][u] is 1 iff u was emitted into list
-// e (0:-z 1:-y 2:-x 3:self 4:+x 5:+y 6:+z, UpdateActiveElementsVolume.cu:168-174).
-// First-wins duplicate removal over the list-major concatenation
-// (FilterDuplicates.cu:42-66) then keeps candidate (e, p) iff the lowest set
-// byte of C[u] is e, and the dense path keeps every u with C[u] != 0 in tiled
-// order.  The set of voxels with C[j][u] != 0 is exactly V_{j+1}, which also
-// answers "is this neighbour active" for the recomputation above.
+// Emission record C (8 bytes per voxel): byte e of C[u] is 1 iff u was emitted
+// into candidate list e (0:-z 1:-y 2:-x 3:self 4:+x 5:+y 6:+z,
+// UpdateActiveElementsVolume.cu:168-174).  First-wins duplicate removal over the
+// list-major concatenation (FilterDuplicates.cu:42-66) keeps candidate (e, p)
+// iff the lowest set byte of C[u] is e; the dense path keeps every u with
+// C[u] != 0 in tiled order.  The voxels with C[u] != 0 are exactly the next
+// list, so the next update clears C at the voxels it visits.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -48,10 +45,10 @@ constexpr uint32_t kSpinLimit = 1u << 24;     // bounded look-back wait
 enum : uint32_t { kErrSpin = 1u, kErrOverflow = 2u };
 
 enum KMode : int {
-    kFirst = 0,  // update + mark of the materialised list (no compaction)
-    kMid = 1,    // compaction of the previous candidates + update + mark
-    kLast = 2,   // compaction only, with statistics
-    kInit = 3    // dense compaction of C[ck] only (initialisation), no statistics
+    kUpdate = 0,   // update of the current list
+    kMark = 1,     // marking of the current list
+    kCompact = 2,  // duplicate removal + compaction into the next list, statistics
+    kInit = 3      // dense compaction of C only (initialisation), no statistics
 };
 
 // ---------------------------------------------------------------------------
@@ -161,7 +158,7 @@ struct DevCtl {
     uint32_t epoch;       // +1 per compaction launch; stamps the look-back states
     uint32_t stats_cap;
     uint32_t error;
-    uint64_t c_acc[2];    // candidate emissions, by list parity of the marked list
+    uint64_t c_acc;       // candidate emissions of the iteration being marked
     uint32_t few_iters, many_iters;
 };
 
@@ -184,9 +181,9 @@ struct KArgs {
     const void* src;
     int8_t* E;            // export byte (tiled)
     int8_t* frozen;       // frozen export (tiled)
-    uint64_t* C[3];       // emission records
+    uint64_t* C;          // emission record
     uint32_t* list[2];
-    uint8_t* mlist[2];    // per-position emission masks
+    uint8_t* mlist;       // per-position emission masks
     uint64_t* tstate;     // look-back tile states
     DevCtl* ctl;
     const DevParams* prm;
@@ -344,112 +341,7 @@ __device__ inline uint32_t tau_of(const Geom& g, uint32_t x, uint32_t y, uint32_
            (x & 15u);
 }
 
-// list-direction offsets (0:-z 1:-y 2:-x 3:self 4:+x 5:+y 6:+z)
-constexpr int kDx[7] = {0, 0, -1, 0, 1, 0, 0};
-constexpr int kDy[7] = {0, -1, 0, 0, 0, 1, 0};
-constexpr int kDz[7] = {-1, 0, 0, 0, 0, 0, 1};
-
-// Update voxel u at list position p of the list with parity `op`, then mark it.
-// Cmem marks the members of the list being updated; emissions go to Cemit.
-// Every load is issued in one round: the voxel's 7-point stencil and source
-// and, when its list position may emit at all, its radius-2 diamond and the
-// sources, membership records and write-buffer values of its face neighbours
-// (used only if the voxel's own change passes the temporal threshold).
-// Returns the number of candidates emitted.
-template <typename SrcT, bool kFloatSrc>
-__device__ inline uint32_t update_mark(const KArgs& a, const DevParams& P, uint32_t u, uint32_t p, int op,
-                                       const uint64_t* Cmem, uint8_t* Cemit)
-{
-    const float* R = a.phi[op];
-    float* W = a.phi[op ^ 1];
-    const Geom& g = a.g;
-    uint32_t x, y, z;
-    decode_xyz(g, u, &x, &y, &z);
-    const int ix = (int)x, iy = (int)y, iz = (int)z;
-    bool fb[7];
-    uint32_t ft[7];
-#pragma unroll
-    for (int e = 0; e < 7; ++e) {
-        const int nx = ix + kDx[e], ny = iy + kDy[e], nz = iz + kDz[e];
-        fb[e] = e != 3 && nx >= 0 && ny >= 0 && nz >= 0 && nx < (int)g.X && ny < (int)g.Y && nz < (int)g.Z;
-        ft[e] = fb[e] ? tau_of(g, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz) : u;  // clamp to self
-    }
-    // the voxel's own neighbourhood (UpdateLevelSetVolume.cu:248-258)
-    const float c = R[u];
-    float F[7];
-#pragma unroll
-    for (int e = 0; e < 7; ++e) F[e] = e == 3 ? c : R[ft[e]];
-    const float fu = density_factor<SrcT, kFloatSrc>(a, P, u);
-    // positions p % 4 == 3 never emit (UpdateActiveElementsVolume.cu:329)
-    const bool may_emit = !P.refMode || (p & 3u) != 3u;
-    uint64_t mem[7];
-    float wv[7], fw[7], ax[7], ed[7][7];
-    if (may_emit) {
-#pragma unroll
-        for (int e = 0; e < 7; ++e) {
-            if (e == 3) continue;
-            mem[e] = fb[e] ? Cmem[ft[e]] : 0ull;
-            wv[e] = fb[e] ? W[ft[e]] : 0.0f;
-            fw[e] = fb[e] ? density_factor<SrcT, kFloatSrc>(a, P, ft[e]) : 0.0f;
-            // axis point u + 2 d_e
-            const int x2 = ix + 2 * kDx[e], y2 = iy + 2 * kDy[e], z2 = iz + 2 * kDz[e];
-            const bool in2 = x2 >= 0 && y2 >= 0 && z2 >= 0 && x2 < (int)g.X && y2 < (int)g.Y && z2 < (int)g.Z;
-            ax[e] = in2 ? R[tau_of(g, (uint32_t)x2, (uint32_t)y2, (uint32_t)z2)] : F[e];
-        }
-        // edge points u + d_a + d_b (different axes), each loaded once
-#pragma unroll
-        for (int ea = 0; ea < 7; ++ea) {
-#pragma unroll
-            for (int eb = ea + 1; eb < 7; ++eb) {
-                if (ea == 3 || eb == 3 || ea + eb == 6) continue;
-                const bool in = fb[ea] && fb[eb];
-                const float v = in ? R[tau_of(g, (uint32_t)(ix + kDx[ea] + kDx[eb]), (uint32_t)(iy + kDy[ea] + kDy[eb]),
-                                              (uint32_t)(iz + kDz[ea] + kDz[eb]))]
-                                   : 0.0f;
-                ed[ea][eb] = v;
-                ed[eb][ea] = v;
-            }
-        }
-    }
-    const float nv = update_core(P, F[0], F[1], F[2], c, F[4], F[5], F[6], fu);
-    W[u] = nv;
-    a.E[u] = export_byte(nv);
-    uint32_t m = 0;
-    // temporal AND spatial test (UpdateActiveElements.cu:75-103) on the new phi
-    if (may_emit && fabsf(nv - c) > kTemporalThreshold) {
-#pragma unroll
-        for (int e = 0; e < 7; ++e) {
-            if (e == 3 || !fb[e]) continue;
-            float vw;
-            if (mem[e] != 0ull) {
-                // neighbour w = u + d_e is active: its new value, recomputed with
-                // the same arithmetic; its out-of-bounds neighbours are w itself
-                float s[7];
-#pragma unroll
-                for (int e2 = 0; e2 < 7; ++e2) {
-                    if (e2 == 3) continue;
-                    if (e2 + e == 6) s[e2] = c;                  // back to u
-                    else if (e2 == e) s[e2] = ax[e];              // u + 2 d_e
-                    else s[e2] = fb[e2] ? ed[e][e2] : F[e];       // u + d_e + d_e2
-                }
-                vw = update_core(P, s[0], s[1], s[2], F[e], s[4], s[5], s[6], fw[e]);
-            } else {
-                vw = wv[e];  // inactive: the write buffer keeps its older value
-            }
-            if (!(fabsf(nv - vw) <= kTemporalThreshold)) m |= 1u << e;
-        }
-        if (m) m |= 8u;
-    }
-    a.mlist[op][p] = (uint8_t)m;
-    if (m) {
-#pragma unroll
-        for (int e = 0; e < 7; ++e)
-            if ((m >> e) & 1u) Cemit[(uint64_t)(e == 3 ? u : ft[e]) * 8u + (uint32_t)e] = 1;
-    }
-    return (uint32_t)__popc(m);
-}
-
-__device__ inline void block_add_candidates(DevCtl* ctl, int slot, uint32_t c, uint32_t* s_red)
+__device__ inline void block_add_candidates(DevCtl* ctl, uint32_t c, uint32_t* s_red)
 {
     c = wave_sum(c);
     if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = c;
@@ -457,169 +349,200 @@ __device__ inline void block_add_candidates(DevCtl* ctl, int slot, uint32_t c, u
     if (threadIdx.x == 0) {
         uint32_t tot = 0;
         for (int w = 0; w < kBlock / 64; ++w) tot += s_red[w];
-        if (tot) atomicAdd((unsigned long long*)&ctl->c_acc[slot], (unsigned long long)tot);
+        if (tot) atomicAdd((unsigned long long*)&ctl->c_acc, (unsigned long long)tot);
     }
 }
 
 // ---------------------------------------------------------------------------
-// the iteration kernel
+// the iteration kernels; q is the parity of the current list (list[q]) and of
+// the phi read buffer (phi[q]); the write buffer and the next list have parity
+// q ^ 1
 // ---------------------------------------------------------------------------
-// op: parity of the list this launch produces (kMid, kLast, kInit) or updates
-//     (kFirst); the source list of a compaction has parity op ^ 1.
-// ck: emission record holding the source candidates (kMid, kLast), the members
-//     of the list being updated (kFirst), or the dense flags (kInit).
-template <typename SrcT, bool kFloatSrc, int kMode>
-__global__ __launch_bounds__(kBlock) void k_iter(KArgs a, int op, int ck)
+template <typename SrcT, bool kFloatSrc>
+__global__ __launch_bounds__(kBlock) void k_update(KArgs a, int q)
+{
+    const DevParams P = *a.prm;
+    const uint32_t n = a.ctl->n[q];
+    const uint32_t* __restrict__ V = a.list[q];
+    const float* __restrict__ R = a.phi[q];
+    float* __restrict__ W = a.phi[q ^ 1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        const uint32_t u = V[p];
+        uint32_t nb[7], inb;
+        neighbors(a.g, u, nb, &inb);
+        const float nv = update_core(P, R[nb[0]], R[nb[1]], R[nb[2]], R[u], R[nb[4]], R[nb[5]], R[nb[6]],
+                                     density_factor<SrcT, kFloatSrc>(a, P, u));
+        W[u] = nv;
+        a.E[u] = export_byte(nv);
+        a.C[u] = 0ull;  // the record of the candidates that produced this list
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mark(KArgs a, int q)
+{
+    __shared__ uint32_t s_red[kBlock / 64];
+    const bool ref = a.prm->refMode != 0;
+    const uint32_t n = a.ctl->n[q];
+    const uint32_t* __restrict__ V = a.list[q];
+    const float* __restrict__ R = a.phi[q];
+    const float* __restrict__ W = a.phi[q ^ 1];  // the freshly written values
+    uint8_t* __restrict__ Cb = (uint8_t*)a.C;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t emitted = 0;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        uint32_t m = 0;
+        if (!ref || (p & 3u) != 3u) {  // UpdateActiveElementsVolume.cu:329
+            const uint32_t u = V[p];
+            uint32_t nb[7], inb;
+            neighbors(a.g, u, nb, &inb);
+            const float nv = W[u];
+            float wn[7];
+#pragma unroll
+            for (int e = 0; e < 7; ++e) wn[e] = W[nb[e]];
+            // temporal test: the time derivative of the update (UpdateLevelSetVolume.cu:327-331)
+            if (fabsf(nv - R[u]) > kTemporalThreshold) {
+#pragma unroll
+                for (int e = 0; e < 7; ++e) {
+                    if (e == 3 || !((inb >> e) & 1u)) continue;
+                    if (!(fabsf(nv - wn[e]) <= kTemporalThreshold)) m |= 1u << e;
+                }
+                if (m) m |= 8u;
+            }
+            if (m) {
+#pragma unroll
+                for (int e = 0; e < 7; ++e)
+                    if ((m >> e) & 1u) Cb[(uint64_t)nb[e] * 8u + (uint32_t)e] = 1;
+            }
+        }
+        a.mlist[p] = (uint8_t)m;
+        emitted += (uint32_t)__popc(m);
+    }
+    block_add_candidates(a.ctl, emitted, s_red);
+}
+
+// kCompact: the candidates of list[q] (masks in mlist, record in C) into
+// list[q ^ 1]; kInit: the voxels with C != 0 into list[q ^ 1] in tiled order.
+template <int kMode>
+__global__ __launch_bounds__(kBlock) void k_compact(KArgs a, int q)
 {
     __shared__ uint32_t s_stage[kScanTile];
     __shared__ uint32_t s_cnt[kItems * 4];
     __shared__ uint32_t s_excl[kItems * 4];
     __shared__ uint32_t s_prefix, s_total;
-    __shared__ uint32_t s_red[kBlock / 64];
 
-    const DevParams P = *a.prm;
-    const int ce = ck == 2 ? 0 : ck + 1;   // emission record of this launch
-    const int cc = ck == 0 ? 2 : ck - 1;   // record to clear (two iterations old)
-    uint32_t emitted = 0;
+    const int op = q ^ 1;
+    const uint32_t n_src = a.ctl->n[q];
+    const uint32_t epoch = a.ctl->epoch;
+    const bool dense = (kMode == kInit) || !path_is_few(n_src, a.g.N);
+    const uint64_t slots = dense ? a.g.N : 7ull * n_src;
+    const uint32_t ntiles = (uint32_t)((slots + kScanTile - 1) / kScanTile);
+    const uint32_t* __restrict__ Vs = a.list[q];
+    const uint8_t* __restrict__ Ms = a.mlist;
+    const uint64_t* __restrict__ Ck = a.C;
+    uint32_t* __restrict__ out = a.list[op];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt(lane);
 
-    if constexpr (kMode == kFirst) {
-        const uint32_t n = a.ctl->n[op];
-        const uint32_t* V = a.list[op];
-        const uint32_t stride = gridDim.x * blockDim.x;
-        for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride)
-            emitted += update_mark<SrcT, kFloatSrc>(a, P, V[p], p, op, a.C[ck], (uint8_t*)a.C[ce]);
-        block_add_candidates(a.ctl, op, emitted, s_red);
-        return;
-    } else {
-        const int sq = op ^ 1;
-        const uint32_t n_src = a.ctl->n[sq];
-        const uint32_t epoch = a.ctl->epoch;
-        const bool dense = (kMode == kInit) || !path_is_few(n_src, a.g.N);
-        const uint64_t slots = dense ? a.g.N : 7ull * n_src;
-        const uint32_t ntiles = (uint32_t)((slots + kScanTile - 1) / kScanTile);
-        const uint32_t* __restrict__ Vs = a.list[sq];
-        const uint8_t* __restrict__ Ms = a.mlist[sq];
-        const uint64_t* __restrict__ Ck = a.C[ck];
-        uint64_t* __restrict__ Cc = a.C[cc];
-        uint32_t* __restrict__ out = a.list[op];
-        const int lane = threadIdx.x & 63;
-        const int wave = threadIdx.x >> 6;
-        const uint64_t lt = lanemask_lt(lane);
-
-        // retire the record of two iterations ago (non-zero exactly at the
-        // source list); the sparse path does it in its first segment below
-        if (kMode != kInit && dense) {
-            const uint32_t stride = gridDim.x * blockDim.x;
-            for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n_src; p += stride) Cc[Vs[p]] = 0ull;
-        }
-
-        for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-            uint64_t bal[kItems];
-            uint32_t val[kItems];
-            const uint64_t base = (uint64_t)tile * kScanTile;
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t bal[kItems];
+        uint32_t val[kItems];
+        const uint64_t base = (uint64_t)tile * kScanTile;
 #pragma unroll
-            for (int j = 0; j < kItems; ++j) {
-                const uint64_t s = base + (uint64_t)j * kBlock + threadIdx.x;
-                bool keep = false;
-                uint32_t v = 0;
-                if (s < slots) {
-                    if (dense) {
-                        keep = Ck[s] != 0ull;
-                        v = (uint32_t)s;
-                    } else {
-                        // slot s = e * n_src + p: list-major concatenation of the
-                        // seven candidate lists (Cuda.cpp:699-758 over 7 A slots)
-                        const uint32_t e = (uint32_t)(s >= n_src) + (uint32_t)(s >= 2ull * n_src) +
-                                           (uint32_t)(s >= 3ull * n_src) + (uint32_t)(s >= 4ull * n_src) +
-                                           (uint32_t)(s >= 5ull * n_src) + (uint32_t)(s >= 6ull * n_src);
-                        const uint32_t p = (uint32_t)(s - (uint64_t)e * n_src);
-                        const uint32_t m = Ms[p];
-                        const uint32_t t = Vs[p];
-                        const bool bit = (m >> e) & 1u;
-                        if (kMode != kInit && e == 0u) Cc[t] = 0ull;
-                        if (bit) {
-                            const uint32_t u = neighbor_in(a.g, t, e);
-                            const uint64_t rec = Ck[u];
-                            // first-wins: the lowest list that emitted u keeps it
-                            keep = (uint32_t)((__ffsll((unsigned long long)rec) - 1) >> 3) == e;
-                            v = u;
-                        }
-                    }
-                }
-                bal[j] = __ballot(keep);
-                val[j] = v;
-                if (lane == 0) s_cnt[j * 4 + wave] = (uint32_t)__popcll(bal[j]);
-            }
-            __syncthreads();
-            if (wave == 0) {
-                const uint32_t c = lane < kItems * 4 ? s_cnt[lane] : 0u;
-                const uint32_t incl = wave_inclusive_scan(c, lane);
-                if (lane < kItems * 4) s_excl[lane] = incl - c;
-                const uint32_t total = __shfl(incl, 63, 64);
-                const uint32_t prefix = lookback(a.tstate, tile, total, epoch, &a.ctl->error);
-                if (lane == 0) {
-                    s_prefix = prefix;
-                    s_total = total;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < kItems; ++j) {
-                if ((bal[j] >> lane) & 1ull) {
-                    const uint32_t pos = s_excl[j * 4 + wave] + (uint32_t)__popcll(bal[j] & lt);
-                    s_stage[pos] = val[j];
-                }
-            }
-            __syncthreads();
-            const uint32_t total = s_total, prefix = s_prefix;
-            if ((uint64_t)prefix + total > a.g.N) {  // cannot happen with consistent records
-                if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
-                break;
-            }
-            for (uint32_t i = threadIdx.x; i < total; i += kBlock) out[prefix + i] = s_stage[i];
-            if constexpr (kMode == kMid) {
-                for (uint32_t i = threadIdx.x; i < total; i += kBlock)
-                    emitted += update_mark<SrcT, kFloatSrc>(a, P, s_stage[i], prefix + i, op, Ck,
-                                                            (uint8_t*)a.C[ce]);
-            }
-            if (threadIdx.x == 0 && tile == ntiles - 1) {
-                const uint32_t m = prefix + total;
-                a.ctl->n[op] = m;
-                if constexpr (kMode == kInit) {
-                    a.ctl->n[sq] = 0;
+        for (int j = 0; j < kItems; ++j) {
+            const uint64_t s = base + (uint64_t)j * kBlock + threadIdx.x;
+            bool keep = false;
+            uint32_t v = 0;
+            if (s < slots) {
+                if (dense) {
+                    keep = Ck[s] != 0ull;
+                    v = (uint32_t)s;
                 } else {
-                    const uint32_t k = a.ctl->iter;
-                    if (k < a.ctl->stats_cap) {
-                        a.n_hist[k] = n_src;
-                        a.c_hist[k] = a.ctl->c_acc[sq];
-                        a.m_hist[k] = m;
+                    // slot s = e * n_src + p: list-major concatenation of the
+                    // seven candidate lists (Cuda.cpp:699-758 over 7 A slots)
+                    const uint32_t e = (uint32_t)(s >= n_src) + (uint32_t)(s >= 2ull * n_src) +
+                                       (uint32_t)(s >= 3ull * n_src) + (uint32_t)(s >= 4ull * n_src) +
+                                       (uint32_t)(s >= 5ull * n_src) + (uint32_t)(s >= 6ull * n_src);
+                    const uint32_t p = (uint32_t)(s - (uint64_t)e * n_src);
+                    const uint32_t m = Ms[p];
+                    const uint32_t t = Vs[p];
+                    if ((m >> e) & 1u) {
+                        const uint32_t u = neighbor_in(a.g, t, e);
+                        const uint64_t rec = Ck[u];
+                        // first-wins: the lowest list that emitted u keeps it
+                        keep = (uint32_t)((__ffsll((unsigned long long)rec) - 1) >> 3) == e;
+                        v = u;
                     }
-                    if (dense) a.ctl->many_iters += 1; else a.ctl->few_iters += 1;
-                    a.ctl->c_acc[sq] = 0;
-                    a.ctl->iter = k + 1;
                 }
-                a.ctl->epoch = epoch + 1;
             }
-            __syncthreads();
+            bal[j] = __ballot(keep);
+            val[j] = v;
+            if (lane == 0) s_cnt[j * 4 + wave] = (uint32_t)__popcll(bal[j]);
         }
-        if (ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
-            a.ctl->n[op] = 0;
+        __syncthreads();
+        if (wave == 0) {
+            const uint32_t c = lane < kItems * 4 ? s_cnt[lane] : 0u;
+            const uint32_t incl = wave_inclusive_scan(c, lane);
+            if (lane < kItems * 4) s_excl[lane] = incl - c;
+            const uint32_t total = __shfl(incl, 63, 64);
+            const uint32_t prefix = lookback(a.tstate, tile, total, epoch, &a.ctl->error);
+            if (lane == 0) {
+                s_prefix = prefix;
+                s_total = total;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            if ((bal[j] >> lane) & 1ull) {
+                const uint32_t pos = s_excl[j * 4 + wave] + (uint32_t)__popcll(bal[j] & lt);
+                s_stage[pos] = val[j];
+            }
+        }
+        __syncthreads();
+        const uint32_t total = s_total, prefix = s_prefix;
+        if ((uint64_t)prefix + total > a.g.N) {  // cannot happen with a consistent record
+            if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
+            break;
+        }
+        for (uint32_t i = threadIdx.x; i < total; i += kBlock) out[prefix + i] = s_stage[i];
+        if (threadIdx.x == 0 && tile == ntiles - 1) {
+            const uint32_t m = prefix + total;
+            a.ctl->n[op] = m;
             if constexpr (kMode == kInit) {
-                a.ctl->n[sq] = 0;
+                a.ctl->n[q] = 0;
             } else {
                 const uint32_t k = a.ctl->iter;
                 if (k < a.ctl->stats_cap) {
                     a.n_hist[k] = n_src;
-                    a.c_hist[k] = a.ctl->c_acc[sq];
-                    a.m_hist[k] = 0;
+                    a.c_hist[k] = a.ctl->c_acc;
+                    a.m_hist[k] = m;
                 }
                 if (dense) a.ctl->many_iters += 1; else a.ctl->few_iters += 1;
-                a.ctl->c_acc[sq] = 0;
+                a.ctl->c_acc = 0;
                 a.ctl->iter = k + 1;
             }
             a.ctl->epoch = epoch + 1;
         }
-        if constexpr (kMode == kMid) block_add_candidates(a.ctl, op, emitted, s_red);
+        __syncthreads();
+    }
+    if (ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctl->n[op] = 0;
+        if constexpr (kMode == kInit) {
+            a.ctl->n[q] = 0;
+        } else {
+            const uint32_t k = a.ctl->iter;
+            if (k < a.ctl->stats_cap) {
+                a.n_hist[k] = n_src;
+                a.c_hist[k] = a.ctl->c_acc;
+                a.m_hist[k] = 0;
+            }
+            if (dense) a.ctl->many_iters += 1; else a.ctl->few_iters += 1;
+            a.ctl->c_acc = 0;
+            a.ctl->iter = k + 1;
+        }
+        a.ctl->epoch = epoch + 1;
     }
 }
 
@@ -662,8 +585,8 @@ __global__ void k_pad_tile_source(Geom g, const uint8_t* lin, uint8_t* dst, uint
     }
 }
 
-// phi0 injection: both buffers, export byte, dense band flags in C[ck]
-__global__ void k_set_phi(KArgs a, const float* lin, uint32_t nx, uint32_t ny, uint32_t nz, float band, int ck)
+// phi0 injection: both buffers, export byte, dense band flags in C
+__global__ void k_set_phi(KArgs a, const float* lin, uint32_t nx, uint32_t ny, uint32_t nz, float band)
 {
     LSSEG_GRID_STRIDE(t, a.g.N)
     {
@@ -674,7 +597,7 @@ __global__ void k_set_phi(KArgs a, const float* lin, uint32_t nx, uint32_t ny, u
         a.phi[0][t] = v;
         a.phi[1][t] = v;
         a.E[t] = export_byte(v);
-        a.C[ck][t] = fabsf(v) < band ? 1ull : 0ull;
+        a.C[t] = fabsf(v) < band ? 1ull : 0ull;
     }
 }
 
@@ -703,8 +626,8 @@ __global__ void k_seed_pass(KArgs a, int rd, uint32_t sx, uint32_t sy, uint32_t 
 }
 
 // InitializeActiveElementsVolumeKernel (InitializeActiveElementsVolume.cu:37-74):
-// dense keep flags into C[ck]
-__global__ void k_keep_flags(KArgs a, int rd, int ck)
+// dense keep flags into C
+__global__ void k_keep_flags(KArgs a, int rd)
 {
     const float* R = a.phi[rd];
     LSSEG_GRID_STRIDE(t, a.g.N)
@@ -718,7 +641,7 @@ __global__ void k_keep_flags(KArgs a, int rd, int ck)
             if (e == 3) continue;
             if (!(fabsf(u4 - R[nb[e]]) <= kSpatialThreshold)) flat = false;
         }
-        a.C[ck][t] = flat ? 0ull : 1ull;
+        a.C[t] = flat ? 0ull : 1ull;
     }
 }
 

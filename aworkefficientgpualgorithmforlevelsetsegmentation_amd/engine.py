@@ -190,14 +190,15 @@ class Context:
         return st
 
     def iterate_profiled(self, T, eps, alpha, dt, iters, mode=MODE_REF):
-        """Returns (Stats, KernelTimes, per-launch milliseconds)."""
+        """Returns (Stats, KernelTimes, per-launch milliseconds in launch order:
+        update, marking, compaction of iteration 0, then of iteration 1, ...)."""
         prm = Params(T, eps, alpha, dt, mode)
         st = Stats()
         kt = KernelTimes()
-        per = np.zeros(2 * int(iters) + 1, np.float32)
+        per = np.zeros(4 * int(iters) + 1, np.float32)
         self._check(lib().lsseg_iterate_profiled(self._h, ctypes.byref(prm), int(iters), ctypes.byref(st),
                                                  ctypes.byref(kt), _p(per), per.size))
-        return st, kt, per
+        return st, kt, per[:int(kt.launches)]
 
     def checkpoint_save(self, slot: int = 0):
         self._check(lib().lsseg_checkpoint_save(self._h, slot))

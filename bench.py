@@ -18,8 +18,7 @@ Algorithmic bytes per iteration k, FP32 phi (4 bytes), source S bytes/voxel:
   mark     33 n_k         coordinate, flag, 7 phi taps
   compact  10 c_k + 5 m_k candidate write/read-back, tag probe/set; list write, clear
 with n_k active voxels, c_k candidate emissions and m_k new active voxels.  Each
-iteration is two launches of the iteration kernel: update + marking ((71 + S) n_k
-bytes) and duplicate removal + compaction (10 c_k + 5 m_k bytes).
+iteration is three launches: k_update, k_mark and k_compact.
 """
 from __future__ import annotations
 
@@ -185,20 +184,20 @@ def main():
     log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {updates_per_step} updates/step, "
         f"few/many iterations {last.few_path_iters}/{last.many_path_iters}, final n={last.active_final}")
 
-    # per-launch device times (HIP events on the engine's stream).  Launches
-    # alternate: update + mark of iteration k (k_iter, update mode), then the
-    # compaction of its candidates (k_iter, compaction mode).
+    # per-launch device times (HIP events on the engine's stream): update,
+    # marking and compaction of each iteration in turn.
     ctx.checkpoint_restore(0)
     pst, kt, per_launch = ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, w.iterations)
     pc = ctx.iteration_counts()
     assert np.array_equal(pc.n, counts.n), "profiled run diverged from the timed run"
     B = algorithmic_bytes(pc.n, pc.c, pc.m, S)
     it = w.iterations
-    t_upd = per_launch[0::2][:it].astype(np.float64)
-    t_cmp = per_launch[1::2][:it].astype(np.float64)
+    assert per_launch.size == 3 * it, f"expected {3 * it} launches, got {per_launch.size}"
+    per_launch = per_launch.astype(np.float64)
     parts = {
-        "update": {"kernel": "k_iter (update + marking mode)", "bytes": B["update"] + B["mark"], "ms": t_upd},
-        "compact": {"kernel": "k_iter (duplicate removal + compaction mode)", "bytes": B["compact"], "ms": t_cmp},
+        "update": {"kernel": "k_update", "bytes": B["update"], "ms": per_launch[0::3]},
+        "mark": {"kernel": "k_mark", "bytes": B["mark"], "ms": per_launch[1::3]},
+        "compact": {"kernel": "k_compact", "bytes": B["compact"], "ms": per_launch[2::3]},
     }
     kernels = {}
     for k, v in parts.items():
