@@ -13,6 +13,7 @@
 // phi[q] is the read buffer and the emission record C is non-zero exactly at the
 // list's voxels.
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -61,7 +62,7 @@ struct lsseg_ctx {
         uint32_t n = 0;
         bool valid = false;
     } ckpt[LSSEG_MAX_CHECKPOINTS];
-    bool profile = false;             // record an event before every iteration launch
+    bool profile = false;             // time every iteration launch with its own start/stop events
     std::vector<hipEvent_t> prof_events;
     size_t prof_next = 0;
     std::vector<void*> allocs;
@@ -143,16 +144,16 @@ const void* update_kernel(int32_t dtype)
     }
 }
 
-void prof_mark(lsseg_ctx* c)
-{
-    if (c->profile && c->prof_next < c->prof_events.size())
-        (void)hipEventRecord(c->prof_events[c->prof_next++], c->stream);
-}
-
 void launch(lsseg_ctx* c, const void* fn, uint32_t grid, int q)
 {
     void* args[] = {(void*)&c->a, (void*)&q};
-    prof_mark(c);
+    if (c->profile && c->prof_next + 2 <= c->prof_events.size()) {
+        // the events take the kernel's own start and end timestamps
+        hipEvent_t e0 = c->prof_events[c->prof_next], e1 = c->prof_events[c->prof_next + 1];
+        c->prof_next += 2;
+        (void)hipExtLaunchKernel(fn, dim3(grid), dim3(kBlock), args, 0, c->stream, e0, e1, 0);
+        return;
+    }
     (void)hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args, 0, c->stream);
 }
 
@@ -370,9 +371,17 @@ lsseg_status lsseg_create(const lsseg_volume_desc* desc, const void* host_volume
     const int want = env_int("LSSEG_GRID_BLOCKS", 0);
     c->grid = (uint32_t)occ * c->cus;
     if (want > 0) c->grid = std::min<uint32_t>(c->grid, (uint32_t)want);
-    // update and marking: grid-stride loops, eight workgroups per CU
+    // update and marking: eight workgroups per CU (a multiple of the eight
+    // XCDs), list positions dealt to the XCDs in runs of 4096
     const int swant = env_int("LSSEG_STREAM_BLOCKS", 0);
-    c->sgrid = swant > 0 ? (uint32_t)swant : 8u * c->cus;
+    c->sgrid = swant > 0 ? ((uint32_t)swant + 7u) / 8u * 8u : 8u * c->cus;
+    const int run = env_int("LSSEG_XCD_RUN", 4096);
+    c->a.run_log2 = 0;
+    if (run >= kBlock) {
+        uint32_t lr = 0;
+        while ((1u << (lr + 1)) <= (uint32_t)run && lr < 24) ++lr;
+        c->a.run_log2 = lr;
+    }
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
         c->err = std::string("stream/event creation failed: ") + hipGetErrorString(e);
@@ -571,7 +580,6 @@ lsseg_status lsseg_iterate(lsseg_ctx* c, const lsseg_params* prm, int32_t iters,
     if (iters > 0) {
         const int q0 = c->q;
         if ((st = launch_iterations(c, iters, q0)) != LSSEG_OK) return st;
-        prof_mark(c);
         c->q = q0 ^ (iters & 1);
     }
     HIP_TRY(c, hipGetLastError());
@@ -608,7 +616,7 @@ lsseg_status lsseg_iterate_profiled(lsseg_ctx* c, const lsseg_params* prm, int32
 {
     if (!c || !times) return LSSEG_ERR_INVALID_ARG;
     if (iters < 0) return fail(c, LSSEG_ERR_INVALID_ARG, "negative iteration count");
-    const size_t need = 3 * (size_t)iters + 1;
+    const size_t need = 6 * (size_t)iters;
     while (c->prof_events.size() < need) {
         hipEvent_t e;
         HIP_TRY(c, hipEventCreate(&e));
@@ -621,12 +629,12 @@ lsseg_status lsseg_iterate_profiled(lsseg_ctx* c, const lsseg_params* prm, int32
     if (st != LSSEG_OK) return st;
     memset(times, 0, sizeof(*times));
     if (iters == 0) return LSSEG_OK;
-    // events precede every launch (update, marking, compaction of each
-    // iteration) and follow the last one
-    const int32_t launches = 3 * iters;
+    // a start/stop event pair per launch: update, marking, compaction of each
+    // iteration in turn
+    const int32_t launches = (int32_t)(c->prof_next / 2);
     for (int32_t i = 0; i < launches; ++i) {
         float ms = 0;
-        HIP_TRY(c, hipEventElapsedTime(&ms, c->prof_events[i], c->prof_events[i + 1]));
+        HIP_TRY(c, hipEventElapsedTime(&ms, c->prof_events[2 * i], c->prof_events[2 * i + 1]));
         times->total_ms += ms;
         if (i == 0) times->first_ms = ms;
         if (i == launches - 1) times->last_ms = ms;

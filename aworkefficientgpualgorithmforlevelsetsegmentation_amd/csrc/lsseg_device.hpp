@@ -177,6 +177,7 @@ struct DevParams {
 
 struct KArgs {
     Geom g;
+    uint32_t run_log2;    // log2 of the list positions per XCD run (0: plain grid stride)
     float* phi[2];
     const void* src;
     int8_t* E;            // export byte (tiled)
@@ -354,6 +355,38 @@ __device__ inline void block_add_candidates(DevCtl* ctl, uint32_t c, uint32_t* s
 }
 
 // ---------------------------------------------------------------------------
+// list traversal of the update and marking kernels.  Workgroups are dispatched
+// to the eight XCDs round-robin (workgroup b on XCD b % 8) and each XCD has its
+// own L2, so list positions are dealt to XCDs in runs of 2^run_log2 (>= the
+// workgroup size): run j goes to XCD j % 8 and neighbouring list entries, which
+// gather from the same cache lines, share one L2.  The grid is a multiple of 8.
+// ---------------------------------------------------------------------------
+struct ListWalk {
+    uint32_t vt, step, xcd, lr;
+    __device__ ListWalk(uint32_t run_log2)
+    {
+        lr = run_log2;
+        if (lr) {
+            xcd = blockIdx.x & 7u;
+            vt = (blockIdx.x >> 3) * kBlock + threadIdx.x;
+            step = (gridDim.x >> 3) * kBlock;
+        } else {
+            xcd = 0;
+            vt = blockIdx.x * kBlock + threadIdx.x;
+            step = gridDim.x * kBlock;
+        }
+    }
+    // 64-bit so that no position near 2^32 wraps back into the list
+    __device__ uint64_t pos() const
+    {
+        if (!lr) return vt;
+        const uint32_t mask = (1u << lr) - 1u;
+        return ((((uint64_t)(vt >> lr) << 3) + xcd) << lr) | (vt & mask);
+    }
+    __device__ void next() { vt += step; }
+};
+
+// ---------------------------------------------------------------------------
 // the iteration kernels; q is the parity of the current list (list[q]) and of
 // the phi read buffer (phi[q]); the write buffer and the next list have parity
 // q ^ 1
@@ -366,8 +399,8 @@ __global__ __launch_bounds__(kBlock) void k_update(KArgs a, int q)
     const uint32_t* __restrict__ V = a.list[q];
     const float* __restrict__ R = a.phi[q];
     float* __restrict__ W = a.phi[q ^ 1];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+    ListWalk w(a.run_log2);
+    for (uint64_t p = w.pos(); p < n; w.next(), p = w.pos()) {
         const uint32_t u = V[p];
         uint32_t nb[7], inb;
         neighbors(a.g, u, nb, &inb);
@@ -388,9 +421,9 @@ __global__ __launch_bounds__(kBlock) void k_mark(KArgs a, int q)
     const float* __restrict__ R = a.phi[q];
     const float* __restrict__ W = a.phi[q ^ 1];  // the freshly written values
     uint8_t* __restrict__ Cb = (uint8_t*)a.C;
-    const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t emitted = 0;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+    ListWalk w(a.run_log2);
+    for (uint64_t p = w.pos(); p < n; w.next(), p = w.pos()) {
         uint32_t m = 0;
         if (!ref || (p & 3u) != 3u) {  // UpdateActiveElementsVolume.cu:329
             const uint32_t u = V[p];

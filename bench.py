@@ -73,6 +73,67 @@ def algorithmic_bytes(n, c, m, S):
     return {"update": (38 + S) * n, "mark": 33.0 * n, "compact": 10.0 * c + 5.0 * m}
 
 
+KERNELS = ("k_update", "k_mark", "k_compact")
+
+
+def _pmc_means(path):
+    """Mean counter value (KiB) per dispatch for each kernel in a rocprofv3
+    counter_collection.csv."""
+    import csv
+    acc = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            name = r["Kernel_Name"].split("(")[0]
+            name = name[5:] if name.startswith("void ") else name
+            acc.setdefault(name, []).append(float(r["Counter_Value"]))
+    return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+
+def summarize_pmc(root, workload):
+    """HBM traffic per launch from two rocprofv3 --pmc runs of this benchmark
+    (FETCH_SIZE, then WRITE_SIZE) and the same two counters over a plain copy of
+    known size (--calibrate-copy), which give the scale factor of each counter."""
+    def csv_in(d):
+        import glob
+        hits = glob.glob(os.path.join(root, d, "**", "*counter_collection.csv"), recursive=True)
+        if not hits:
+            raise SystemExit(f"no counter_collection.csv under {os.path.join(root, d)}")
+        return hits[0]
+    copy_bytes = None
+    with open(os.path.join(root, "copy_fetch.json")) as f:
+        for line in f:
+            if line.startswith("{"):
+                copy_bytes = json.loads(line)["calibrate_copy_bytes"]
+    cf = _pmc_means(csv_in("copy_fetch"))
+    cw = _pmc_means(csv_in("copy_write"))
+    copy_name = max(cf, key=lambda k: cf[k][0])
+    f_scale = copy_bytes / (cf[copy_name][0] * 1024.0)
+    w_scale = copy_bytes / (cw[copy_name][0] * 1024.0)
+    fe = _pmc_means(csv_in("fetch"))
+    wr = _pmc_means(csv_in("write"))
+    out = {"workload": workload, "copy_kernel": copy_name, "copy_bytes": copy_bytes,
+           "fetch_scale": f_scale, "write_scale": w_scale, "kernels": {}}
+    for k in KERNELS:
+        fk = [v for n, v in fe.items() if n.startswith("lsseg::" + k)]
+        wk = [v for n, v in wr.items() if n.startswith("lsseg::" + k)]
+        if not fk or not wk:
+            continue
+        fk = max(fk, key=lambda v: v[1])
+        wk = max(wk, key=lambda v: v[1])
+        out["kernels"][k] = {"dispatches": fk[1], "fetch_KiB_raw": fk[0], "write_KiB_raw": wk[0],
+                             "read_bytes": fk[0] * 1024.0 * f_scale, "write_bytes": wk[0] * 1024.0 * w_scale,
+                             "bytes_per_launch": fk[0] * 1024.0 * f_scale + wk[0] * 1024.0 * w_scale}
+    return out
+
+
+def load_traffic(workload):
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        return json.load(f), os.path.relpath(path, ROOT)
+
+
 def make_workload(name, iterations):
     from aworkefficientgpualgorithmforlevelsetsegmentation_amd import synth
     if name == "ellipsoids":
@@ -144,7 +205,36 @@ def main():
     ap.add_argument("--iterations", type=int, default=0, help="override the workload's iteration count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="single-core baseline sample budget")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB three times (HBM counter calibration)")
+    ap.add_argument("--skip-profiled", action="store_true",
+                    help="stop after the timed steps (for hardware-counter passes over the timed dispatches only)")
+    ap.add_argument("--summarize-pmc", default="", metavar="DIR",
+                    help="only summarize rocprofv3 --pmc runs under DIR (fetch/, write/, copy_fetch/, copy_write/, "
+                         "copy_fetch.json) into profiles/hbm_traffic_<workload>.json")
     args = ap.parse_args()
+
+    if args.summarize_pmc:
+        out = summarize_pmc(args.summarize_pmc, args.workload)
+        path = os.path.join(ROOT, "profiles", f"hbm_traffic_{args.workload}.json")
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1)
+        print(json.dumps(out, indent=1))
+        return
+
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": 4 * n, "reads_per_launch_KiB": 4 * n / 1024,
+                          "writes_per_launch_KiB": 4 * n / 1024}), flush=True)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -183,6 +273,9 @@ def main():
     counts = ctx.iteration_counts()
     log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {updates_per_step} updates/step, "
         f"few/many iterations {last.few_path_iters}/{last.many_path_iters}, final n={last.active_final}")
+    if args.skip_profiled:
+        ctx.close()
+        return
 
     # per-launch device times (HIP events on the engine's stream): update,
     # marking and compaction of each iteration in turn.
@@ -211,13 +304,17 @@ def main():
                                             if big.any() and v["ms"][big].sum() > 0 else 0.0)
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     d = kernels[dom]
+    traffic, traffic_src = load_traffic(args.workload)
+    traffic_bytes = None
+    if traffic and traffic.get("kernels", {}).get(d["kernel"]):
+        traffic_bytes = traffic["kernels"][d["kernel"]]["bytes_per_launch"]
     avg_us, bytes_per_launch, achieved = d["avg_launch_us"], d["bytes_per_launch"], d["achieved_GBs"]
     path_bytes = float(sum(v.sum() for v in B.values()))
     path_gbs = path_bytes / (ms_per_step * 1e-3) / 1e9
     log("per launch: " + "; ".join(f"{k} {v['avg_launch_us']:.2f} us, {v['bytes_per_launch'] / 1e6:.2f} MB, "
                                    f"{v['achieved_GBs']:.0f} GB/s (busiest decile {v['busiest_decile_GBs']:.0f})"
                                    for k, v in kernels.items())
-        + f"; event-timed sum {kt.total_ms:.2f} ms vs {ms_per_step:.2f} ms/step")
+        + f"; kernel time sum {kt.total_ms:.2f} ms vs {ms_per_step:.2f} ms/step")
 
     cpu = None
     dice = None
@@ -249,7 +346,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes,
+            "traffic_source": traffic_src,
             "kernel": d["kernel"], "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_us,
         },
         "path_roofline": {"achieved": path_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -202,17 +202,16 @@ lsseg_status lsseg_set_state(lsseg_ctx* ctx, const float* phi_read, const float*
 lsseg_status lsseg_checkpoint_save(lsseg_ctx* ctx, int32_t slot);
 lsseg_status lsseg_checkpoint_restore(lsseg_ctx* ctx, int32_t slot);
 
-/* Device time of the iteration kernels, measured with HIP events recorded on
-   the context's stream around every launch.  An lsseg_iterate call of `iters`
-   iterations makes 2 * iters launches: an update + mark of the current list,
-   then per later iteration a compaction (duplicate removal + compaction of the
-   previous iteration's candidates) and an update + mark of the new list, then a
-   final compaction. */
+/* Device time of the iteration kernels.  Every launch carries its own start
+   and stop HIP events on the context's stream (hipExtLaunchKernel), which take
+   the kernel's start and end timestamps.  An lsseg_iterate call of `iters`
+   iterations makes 3 * iters launches: update, marking, and duplicate removal +
+   compaction of each iteration in turn. */
 typedef struct lsseg_kernel_times {
-    double total_ms;     /* all iteration launches */
+    double total_ms;     /* sum over all iteration launches */
     double first_ms;     /* the first launch */
-    double last_ms;      /* the final compaction */
-    uint64_t launches;   /* 2 * iters */
+    double last_ms;      /* the last launch (the final compaction) */
+    uint64_t launches;   /* 3 * iters */
 } lsseg_kernel_times;
 
 /* lsseg_iterate with per-launch event timing (launches issued one by one, no
