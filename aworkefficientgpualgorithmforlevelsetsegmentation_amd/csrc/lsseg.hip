@@ -39,6 +39,8 @@ constexpr int kGraphIters = 32;  // iterations per captured graph (even: the par
 struct lsseg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool owns_stream = true;          // false for the volumes of a batch (shared stream)
+    lsseg_batch* batch = nullptr;     // the owning batch, if any
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint32_t nx = 0, ny = 0, nz = 0;
     int32_t dtype = 0;
@@ -324,7 +326,15 @@ const char* lsseg_last_error(const lsseg_ctx* ctx)
     return ctx ? ctx->err.c_str() : g_create_error.c_str();
 }
 
-lsseg_status lsseg_create(const lsseg_volume_desc* desc, const void* host_volume, lsseg_ctx** out_ctx)
+}  // extern "C"
+
+namespace {
+
+void destroy_ctx(lsseg_ctx* ctx);
+
+// lsseg_create on the given stream (nullptr: a stream of its own)
+lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, hipStream_t shared,
+                        lsseg_ctx** out_ctx)
 {
     if (!desc || !host_volume || !out_ctx) return fail(nullptr, LSSEG_ERR_INVALID_ARG, "null argument");
     *out_ctx = nullptr;
@@ -343,7 +353,7 @@ lsseg_status lsseg_create(const lsseg_volume_desc* desc, const void* host_volume
     if (!c) return fail(nullptr, LSSEG_ERR_OUT_OF_MEMORY, "host allocation failed");
     auto bail = [&](lsseg_status st) {
         g_create_error = c->err;
-        lsseg_destroy(c);
+        destroy_ctx(c);
         return st;
     };
     c->nx = desc->nx;
@@ -382,8 +392,14 @@ lsseg_status lsseg_create(const lsseg_volume_desc* desc, const void* host_volume
         while ((1u << (lr + 1)) <= (uint32_t)run && lr < 24) ++lr;
         c->a.run_log2 = lr;
     }
-    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
+    if (shared) {
+        c->stream = shared;
+        c->owns_stream = false;
+    } else if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+        c->err = std::string("stream creation failed: ") + hipGetErrorString(e);
+        return bail(LSSEG_ERR_DEVICE);
+    }
+    if ((e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
         c->err = std::string("stream/event creation failed: ") + hipGetErrorString(e);
         return bail(LSSEG_ERR_DEVICE);
     }
@@ -470,9 +486,8 @@ lsseg_status lsseg_create(const lsseg_volume_desc* desc, const void* host_volume
     return LSSEG_OK;
 }
 
-lsseg_status lsseg_destroy(lsseg_ctx* ctx)
+void destroy_ctx(lsseg_ctx* ctx)
 {
-    if (!ctx) return LSSEG_ERR_INVALID_ARG;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     destroy_graphs(ctx);
     for (hipEvent_t e : ctx->prof_events) (void)hipEventDestroy(e);
@@ -482,8 +497,24 @@ lsseg_status lsseg_destroy(lsseg_ctx* ctx)
     if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream && ctx->owns_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+}  // namespace
+
+extern "C" {
+
+lsseg_status lsseg_create(const lsseg_volume_desc* desc, const void* host_volume, lsseg_ctx** out_ctx)
+{
+    return create_ctx(desc, host_volume, nullptr, out_ctx);
+}
+
+lsseg_status lsseg_destroy(lsseg_ctx* ctx)
+{
+    if (!ctx) return LSSEG_ERR_INVALID_ARG;
+    if (ctx->batch) return fail(ctx, LSSEG_ERR_STATE, "this context belongs to a batch; destroy the batch");
+    destroy_ctx(ctx);
     return LSSEG_OK;
 }
 
@@ -842,6 +873,414 @@ lsseg_status lsseg_get_device_buffers(const lsseg_ctx* c, void** phi_read, void*
     if (phi_read) *phi_read = c->a.phi[c->q];
     if (phi_write) *phi_write = c->a.phi[c->q ^ 1];
     if (stream) *stream = (void*)c->stream;
+    return LSSEG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched form
+// ---------------------------------------------------------------------------
+struct lsseg_batch {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<lsseg_ctx*> vols;
+    int32_t dtype = 0;
+    uint32_t cus = 256;
+    uint32_t max_compact = 1024;      // co-resident workgroups of k_compact_batch
+    uint32_t grid = 1024;             // compaction workgroups
+    uint32_t sgrid = 2048;            // update and marking workgroups
+    KArgs* d_table = nullptr;
+    DevCtl* d_ctls = nullptr;
+    DevCtl* h_ctls = nullptr;         // pinned mirror
+    std::vector<KArgs> h_table;
+    hipGraphExec_t graphs[2] = {nullptr, nullptr};
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_batch_create_error;
+
+lsseg_status bfail(lsseg_batch* b, lsseg_status st, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (b) b->err = buf;
+    else g_batch_create_error = buf;
+    return st;
+}
+
+#define BHIP_TRY(b, expr)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return bfail((b), LSSEG_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+    } while (0)
+
+// a failure of a volume's context, reported by the batch
+lsseg_status bforward(lsseg_batch* b, uint32_t v, lsseg_status st)
+{
+    if (st == LSSEG_OK) return st;
+    return bfail(b, st, "volume %u: %s", v, b->vols[v]->err.c_str());
+}
+
+void destroy_batch_graphs(lsseg_batch* b)
+{
+    for (auto& g : b->graphs) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
+const void* update_batch_kernel(int32_t dtype)
+{
+    switch (dtype) {
+    case LSSEG_U8: return (const void*)&k_update_batch<uint8_t, false>;
+    case LSSEG_I8: return (const void*)&k_update_batch<int8_t, false>;
+    case LSSEG_U16: return (const void*)&k_update_batch<uint16_t, false>;
+    case LSSEG_I16: return (const void*)&k_update_batch<int16_t, false>;
+    case LSSEG_U32: return (const void*)&k_update_batch<uint32_t, false>;
+    case LSSEG_I32: return (const void*)&k_update_batch<int32_t, false>;
+    default: return (const void*)&k_update_batch<float, true>;
+    }
+}
+
+void launch_batch_iteration(lsseg_batch* b, int q)
+{
+    BatchArgs args{b->d_table, b->d_ctls, (uint32_t)b->vols.size()};
+    void* kargs[] = {(void*)&args, (void*)&q};
+    (void)hipLaunchKernel(update_batch_kernel(b->dtype), dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
+    (void)hipLaunchKernel((const void*)&k_mark_batch, dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
+    (void)hipLaunchKernel((const void*)&k_compact_batch, dim3(b->grid), dim3(kBlock), kargs, 0, b->stream);
+}
+
+lsseg_status launch_batch_iterations(lsseg_batch* b, int count, int q)
+{
+    const int chunk = env_int("LSSEG_GRAPH", 1) ? kGraphIters : 0;
+    while (chunk && count >= chunk) {
+        hipGraphExec_t& ge = b->graphs[q];
+        if (!ge) {
+            hipGraph_t g = nullptr;
+            BHIP_TRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < chunk; ++i) launch_batch_iteration(b, q ^ (i & 1));
+            hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (e != hipSuccess) return bfail(b, LSSEG_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                ge = nullptr;
+                return bfail(b, LSSEG_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
+            }
+        }
+        BHIP_TRY(b, hipGraphLaunch(ge, b->stream));
+        count -= chunk;
+    }
+    for (int i = 0; i < count; ++i) {
+        launch_batch_iteration(b, q);
+        q ^= 1;
+    }
+    return LSSEG_OK;
+}
+
+// Makes parity 0 the current one: swapping the two phi and list buffers (and
+// their counts) is a renaming, not a copy.
+lsseg_status normalise_parity(lsseg_ctx* c)
+{
+    if (c->q == 0) return LSSEG_OK;
+    HIP_TRY(c, hipMemcpyAsync(c->h_ctl, c->a.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::swap(c->h_ctl->n[0], c->h_ctl->n[1]);
+    HIP_TRY(c, hipMemcpyAsync(c->a.ctl, c->h_ctl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::swap(c->a.phi[0], c->a.phi[1]);
+    std::swap(c->a.list[0], c->a.list[1]);
+    c->q = 0;
+    destroy_graphs(c);
+    return LSSEG_OK;
+}
+
+lsseg_status batch_sync_and_check(lsseg_batch* b)
+{
+    const size_t cnt = b->vols.size();
+    BHIP_TRY(b, hipMemcpyAsync(b->h_ctls, b->d_ctls, cnt * sizeof(DevCtl), hipMemcpyDeviceToHost, b->stream));
+    BHIP_TRY(b, hipStreamSynchronize(b->stream));
+    for (size_t v = 0; v < cnt; ++v) {
+        *b->vols[v]->h_ctl = b->h_ctls[v];
+        if (b->h_ctls[v].error)
+            return bfail(b, LSSEG_ERR_DEVICE, "volume %zu: device error flag 0x%x (%s)", v, b->h_ctls[v].error,
+                         (b->h_ctls[v].error & kErrSpin) ? "look-back wait exceeded its bound" : "list overflow");
+    }
+    return LSSEG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lsseg_batch_last_error(const lsseg_batch* b)
+{
+    return b ? b->err.c_str() : g_batch_create_error.c_str();
+}
+
+lsseg_status lsseg_batch_destroy(lsseg_batch* b)
+{
+    if (!b) return LSSEG_ERR_INVALID_ARG;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    destroy_batch_graphs(b);
+    for (lsseg_ctx* c : b->vols) {
+        c->batch = nullptr;
+        destroy_ctx(c);
+    }
+    b->vols.clear();
+    if (b->d_table) (void)hipFree(b->d_table);
+    if (b->d_ctls) (void)hipFree(b->d_ctls);
+    if (b->h_ctls) (void)hipHostFree(b->h_ctls);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_batch_create(const lsseg_volume_desc* descs, const void* const* host_volumes, uint32_t count,
+                                lsseg_batch** out)
+{
+    if (!descs || !host_volumes || !out) return bfail(nullptr, LSSEG_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (count == 0 || count > LSSEG_MAX_BATCH)
+        return bfail(nullptr, LSSEG_ERR_INVALID_ARG, "batch size %u outside 1..%d", count, LSSEG_MAX_BATCH);
+    for (uint32_t v = 0; v < count; ++v)
+        if (descs[v].dtype != descs[0].dtype)
+            return bfail(nullptr, LSSEG_ERR_UNSUPPORTED, "volume %u has another source type than volume 0", v);
+    lsseg_batch* b = new (std::nothrow) lsseg_batch();
+    if (!b) return bfail(nullptr, LSSEG_ERR_OUT_OF_MEMORY, "host allocation failed");
+    auto bail = [&](lsseg_status st) {
+        g_batch_create_error = b->err;
+        lsseg_batch_destroy(b);
+        return st;
+    };
+    b->dtype = descs[0].dtype;
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess) {
+        b->err = std::string("stream/event creation failed: ") + hipGetErrorString(e);
+        return bail(LSSEG_ERR_DEVICE);
+    }
+    for (uint32_t v = 0; v < count; ++v) {
+        lsseg_ctx* c = nullptr;
+        lsseg_status st = create_ctx(&descs[v], host_volumes[v], b->stream, &c);
+        if (st != LSSEG_OK) {
+            b->err = "volume " + std::to_string(v) + ": " + g_create_error;
+            return bail(st);
+        }
+        c->batch = b;
+        b->vols.push_back(c);
+    }
+    // one contiguous array of control blocks: the batched kernels read every
+    // volume's counts at once
+    if (hipMalloc((void**)&b->d_ctls, count * sizeof(DevCtl)) != hipSuccess ||
+        hipMalloc((void**)&b->d_table, count * sizeof(KArgs)) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_ctls, count * sizeof(DevCtl), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        b->err = "batch allocation failed";
+        return bail(LSSEG_ERR_OUT_OF_MEMORY);
+    }
+    for (uint32_t v = 0; v < count; ++v) {
+        lsseg_ctx* c = b->vols[v];
+        if ((e = hipMemcpyAsync(&b->d_ctls[v], c->a.ctl, sizeof(DevCtl), hipMemcpyDeviceToDevice, b->stream)) !=
+            hipSuccess) {
+            b->err = std::string("control block copy failed: ") + hipGetErrorString(e);
+            return bail(LSSEG_ERR_DEVICE);
+        }
+        c->a.ctl = &b->d_ctls[v];
+        destroy_graphs(c);
+    }
+    if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) {
+        b->err = std::string("batch initialisation failed: ") + hipGetErrorString(e);
+        return bail(LSSEG_ERR_DEVICE);
+    }
+    b->cus = b->vols[0]->cus;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)&k_compact_batch, kBlock, 0) != hipSuccess ||
+        occ < 1) {
+        (void)hipGetLastError();
+        occ = 1;
+    }
+    b->max_compact = (uint32_t)occ * b->cus;
+    b->grid = b->max_compact;
+    b->sgrid = 8u * b->cus;
+    *out = b;
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_batch_count(const lsseg_batch* b, uint32_t* count)
+{
+    if (!b || !count) return LSSEG_ERR_INVALID_ARG;
+    *count = (uint32_t)b->vols.size();
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_batch_volume(lsseg_batch* b, uint32_t index, lsseg_ctx** out)
+{
+    if (!b || !out) return LSSEG_ERR_INVALID_ARG;
+    if (index >= b->vols.size()) return bfail(b, LSSEG_ERR_INVALID_ARG, "volume index %u out of range", index);
+    *out = b->vols[index];
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_batch_iterate(lsseg_batch* b, const lsseg_params* params, int32_t iters, lsseg_stats* stats)
+{
+    if (!b || !params) return LSSEG_ERR_INVALID_ARG;
+    if (iters < 0) return bfail(b, LSSEG_ERR_INVALID_ARG, "negative iteration count");
+    const uint32_t cnt = (uint32_t)b->vols.size();
+    lsseg_status st;
+    for (uint32_t v = 0; v < cnt; ++v) {
+        lsseg_ctx* c = b->vols[v];
+        DevParams P;
+        if ((st = bforward(b, v, make_params(c, &params[v], &P))) != LSSEG_OK) return st;
+        if ((st = bforward(b, v, ensure_stats(c, (uint64_t)iters))) != LSSEG_OK) return st;
+        if ((st = bforward(b, v, upload_params(c, P))) != LSSEG_OK) return st;
+        if ((st = bforward(b, v, normalise_parity(c))) != LSSEG_OK) return st;
+    }
+    b->h_table.resize(cnt);
+    for (uint32_t v = 0; v < cnt; ++v) b->h_table[v] = b->vols[v]->a;
+    BHIP_TRY(b, hipMemcpyAsync(b->d_table, b->h_table.data(), cnt * sizeof(KArgs), hipMemcpyHostToDevice, b->stream));
+    // reset the per-call counters (keep n[], epoch)
+    BHIP_TRY(b, hipMemcpyAsync(b->h_ctls, b->d_ctls, cnt * sizeof(DevCtl), hipMemcpyDeviceToHost, b->stream));
+    BHIP_TRY(b, hipStreamSynchronize(b->stream));
+    for (uint32_t v = 0; v < cnt; ++v) {
+        DevCtl& h = b->h_ctls[v];
+        h.iter = 0;
+        h.stats_cap = b->vols[v]->stats_cap;
+        h.error = 0;
+        h.c_acc = 0;
+        h.few_iters = 0;
+        h.many_iters = 0;
+    }
+    BHIP_TRY(b, hipMemcpyAsync(b->d_ctls, b->h_ctls, cnt * sizeof(DevCtl), hipMemcpyHostToDevice, b->stream));
+    BHIP_TRY(b, hipEventRecord(b->ev0, b->stream));
+    if (iters > 0 && (st = launch_batch_iterations(b, iters, 0)) != LSSEG_OK) return st;
+    BHIP_TRY(b, hipGetLastError());
+    BHIP_TRY(b, hipEventRecord(b->ev1, b->stream));
+    if ((st = batch_sync_and_check(b)) != LSSEG_OK) return st;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    for (uint32_t v = 0; v < cnt; ++v) {
+        lsseg_ctx* c = b->vols[v];
+        c->q = iters & 1;
+        c->last_iters = (uint64_t)iters;
+    }
+    if (stats) {
+        for (uint32_t v = 0; v < cnt; ++v) {
+            lsseg_ctx* c = b->vols[v];
+            lsseg_stats& o = stats[v];
+            memset(&o, 0, sizeof(o));
+            o.iterations = (uint64_t)iters;
+            o.device_ms = ms;
+            o.active_final = c->h_ctl->n[c->q];
+            o.few_path_iters = c->h_ctl->few_iters;
+            o.many_path_iters = c->h_ctl->many_iters;
+            if (iters > 0) {
+                std::vector<uint32_t> hn(iters), hm(iters);
+                std::vector<uint64_t> hc(iters);
+                BHIP_TRY(b, hipMemcpy(hn.data(), c->a.n_hist, iters * sizeof(uint32_t), hipMemcpyDeviceToHost));
+                BHIP_TRY(b, hipMemcpy(hc.data(), c->a.c_hist, iters * sizeof(uint64_t), hipMemcpyDeviceToHost));
+                BHIP_TRY(b, hipMemcpy(hm.data(), c->a.m_hist, iters * sizeof(uint32_t), hipMemcpyDeviceToHost));
+                for (int i = 0; i < iters; ++i) {
+                    o.sum_active += hn[i];
+                    o.sum_candidates += hc[i];
+                    o.sum_new_active += hm[i];
+                }
+            }
+        }
+    }
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_batch_checkpoint_save(lsseg_batch* b, int32_t slot)
+{
+    if (!b) return LSSEG_ERR_INVALID_ARG;
+    for (uint32_t v = 0; v < b->vols.size(); ++v) {
+        lsseg_status st = bforward(b, v, lsseg_checkpoint_save(b->vols[v], slot));
+        if (st != LSSEG_OK) return st;
+    }
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_batch_checkpoint_restore(lsseg_batch* b, int32_t slot)
+{
+    if (!b) return LSSEG_ERR_INVALID_ARG;
+    for (uint32_t v = 0; v < b->vols.size(); ++v) {
+        lsseg_status st = bforward(b, v, lsseg_checkpoint_restore(b->vols[v], slot));
+        if (st != LSSEG_OK) return st;
+    }
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_batch_get_tuning(const lsseg_batch* b, lsseg_tuning* out)
+{
+    if (!b || !out) return LSSEG_ERR_INVALID_ARG;
+    memset(out, 0, sizeof(*out));
+    out->update_blocks = b->sgrid;
+    out->compact_blocks = b->grid;
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_batch_set_tuning(lsseg_batch* b, const lsseg_tuning* t)
+{
+    if (!b || !t) return LSSEG_ERR_INVALID_ARG;
+    if (t->update_blocks == 0 || t->compact_blocks == 0 || t->compact_blocks > b->max_compact)
+        return bfail(b, LSSEG_ERR_INVALID_ARG, "launch shape %u/%u outside 1.. / 1..%u", t->update_blocks,
+                     t->compact_blocks, b->max_compact);
+    if (t->update_blocks != b->sgrid || t->compact_blocks != b->grid) destroy_batch_graphs(b);
+    b->sgrid = t->update_blocks;
+    b->grid = t->compact_blocks;
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_batch_autotune(lsseg_batch* b, const lsseg_params* params, int32_t iters, lsseg_tuning* out)
+{
+    if (!b || !params) return LSSEG_ERR_INVALID_ARG;
+    if (iters <= 0) return bfail(b, LSSEG_ERR_INVALID_ARG, "autotune needs a positive iteration count");
+    const int32_t slot = LSSEG_MAX_CHECKPOINTS - 1;
+    lsseg_status st = lsseg_batch_checkpoint_save(b, slot);
+    if (st != LSSEG_OK) return st;
+    const uint32_t occ = b->max_compact / b->cus;
+    std::vector<lsseg_tuning> cands;
+    for (uint32_t su : {2u, 4u, 8u, 16u})
+        for (uint32_t cd : {occ, (occ + 1) / 2, 1u}) {
+            lsseg_tuning t{};
+            t.update_blocks = su * b->cus;
+            t.compact_blocks = std::max(1u, cd) * b->cus;
+            bool dup = false;
+            for (auto& o : cands) dup |= o.update_blocks == t.update_blocks && o.compact_blocks == t.compact_blocks;
+            if (!dup) cands.push_back(t);
+        }
+    lsseg_tuning best{};
+    best.ms = 1e30;
+    for (auto& t : cands) {
+        if ((st = lsseg_batch_set_tuning(b, &t)) != LSSEG_OK) return st;
+        // first run captures the graphs, second one is timed
+        double ms = 0;
+        for (int rep = 0; rep < 2; ++rep) {
+            if ((st = lsseg_batch_checkpoint_restore(b, slot)) != LSSEG_OK) return st;
+            std::vector<lsseg_stats> sts(b->vols.size());
+            if ((st = lsseg_batch_iterate(b, params, iters, sts.data())) != LSSEG_OK) return st;
+            ms = sts[0].device_ms;
+        }
+        if (ms < best.ms) {
+            best = t;
+            best.ms = ms;
+        }
+    }
+    best.candidates = (uint32_t)cands.size();
+    if ((st = lsseg_batch_set_tuning(b, &best)) != LSSEG_OK) return st;
+    if ((st = lsseg_batch_checkpoint_restore(b, slot)) != LSSEG_OK) return st;
+    if (out) *out = best;
     return LSSEG_OK;
 }
 

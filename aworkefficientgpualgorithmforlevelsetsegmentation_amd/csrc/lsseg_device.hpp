@@ -342,17 +342,6 @@ __device__ inline uint32_t tau_of(const Geom& g, uint32_t x, uint32_t y, uint32_
            (x & 15u);
 }
 
-__device__ inline void block_add_candidates(DevCtl* ctl, uint32_t c, uint32_t* s_red)
-{
-    c = wave_sum(c);
-    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t tot = 0;
-        for (int w = 0; w < kBlock / 64; ++w) tot += s_red[w];
-        if (tot) atomicAdd((unsigned long long*)&ctl->c_acc, (unsigned long long)tot);
-    }
-}
 
 // ---------------------------------------------------------------------------
 // list traversal of the update and marking kernels.  Workgroups are dispatched
@@ -387,29 +376,218 @@ struct ListWalk {
 };
 
 // ---------------------------------------------------------------------------
-// the iteration kernels; q is the parity of the current list (list[q]) and of
+// the iteration steps.  q is the parity of the current list (list[q]) and of
 // the phi read buffer (phi[q]); the write buffer and the next list have parity
-// q ^ 1
+// q ^ 1.  Each step is a device function over one list position or one scan
+// tile; the single-volume kernels and the batched kernels (which take each
+// volume's arguments from a table) share them.
+// ---------------------------------------------------------------------------
+template <typename SrcT, bool kFloatSrc>
+__device__ inline void update_at(const KArgs& a, const DevParams& P, const uint32_t* __restrict__ V,
+                                 const float* __restrict__ R, float* __restrict__ W, uint64_t p)
+{
+    const uint32_t u = V[p];
+    uint32_t nb[7], inb;
+    neighbors(a.g, u, nb, &inb);
+    const float nv = update_core(P, R[nb[0]], R[nb[1]], R[nb[2]], R[u], R[nb[4]], R[nb[5]], R[nb[6]],
+                                 density_factor<SrcT, kFloatSrc>(a, P, u));
+    W[u] = nv;
+    a.E[u] = export_byte(nv);
+    a.C[u] = 0ull;  // the record of the candidates that produced this list
+}
+
+// marking of list position p (UpdateActiveElements.cu:75-103); returns the
+// number of candidate emissions
+__device__ inline uint32_t mark_at(const KArgs& a, bool ref, const uint32_t* __restrict__ V,
+                                   const float* __restrict__ R, const float* __restrict__ W,
+                                   uint8_t* __restrict__ Cb, uint64_t p)
+{
+    uint32_t m = 0;
+    if (!ref || (p & 3u) != 3u) {  // UpdateActiveElementsVolume.cu:329
+        const uint32_t u = V[p];
+        uint32_t nb[7], inb;
+        neighbors(a.g, u, nb, &inb);
+        const float nv = W[u];
+        float wn[7];
+#pragma unroll
+        for (int e = 0; e < 7; ++e) wn[e] = W[nb[e]];
+        // temporal test: the time derivative of the update (UpdateLevelSetVolume.cu:327-331)
+        if (fabsf(nv - R[u]) > kTemporalThreshold) {
+#pragma unroll
+            for (int e = 0; e < 7; ++e) {
+                if (e == 3 || !((inb >> e) & 1u)) continue;
+                if (!(fabsf(nv - wn[e]) <= kTemporalThreshold)) m |= 1u << e;
+            }
+            if (m) m |= 8u;
+        }
+        if (m) {
+#pragma unroll
+            for (int e = 0; e < 7; ++e)
+                if ((m >> e) & 1u) Cb[(uint64_t)nb[e] * 8u + (uint32_t)e] = 1;
+        }
+    }
+    a.mlist[p] = (uint8_t)m;
+    return (uint32_t)__popc(m);
+}
+
+// block-wide sum of c, added to *acc by one thread; every thread calls it
+__device__ inline void block_add(uint64_t* acc, uint32_t c, uint32_t* s_red)
+{
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < kBlock / 64; ++w) tot += s_red[w];
+        if (tot) atomicAdd((unsigned long long*)acc, (unsigned long long)tot);
+    }
+    __syncthreads();
+}
+
+struct CompactShared {
+    uint32_t stage[kScanTile];
+    uint32_t cnt[kItems * 4];
+    uint32_t excl[kItems * 4];
+    uint32_t prefix, total;
+};
+
+// one compaction launch over one volume: few path over the 7 n candidate slots
+// in list-major order, dense path over all N voxels in tiled order
+struct CompactPlan {
+    uint32_t n_src;
+    bool dense;
+    uint64_t slots;
+    uint32_t ntiles;
+};
+
+template <int kMode>
+__device__ inline CompactPlan compact_plan(const Geom& g, uint32_t n_src)
+{
+    CompactPlan c;
+    c.n_src = n_src;
+    c.dense = (kMode == kInit) || !path_is_few(n_src, g.N);
+    c.slots = c.dense ? g.N : 7ull * n_src;
+    c.ntiles = (uint32_t)((c.slots + kScanTile - 1) / kScanTile);
+    return c;
+}
+
+// counts and statistics once the last tile is placed (one thread)
+template <int kMode>
+__device__ inline void compact_finish(const KArgs& a, int q, const CompactPlan& c, uint32_t m, uint32_t epoch)
+{
+    DevCtl* ctl = a.ctl;
+    ctl->n[q ^ 1] = m;
+    if constexpr (kMode == kInit) {
+        ctl->n[q] = 0;
+    } else {
+        const uint32_t k = ctl->iter;
+        if (k < ctl->stats_cap) {
+            a.n_hist[k] = c.n_src;
+            a.c_hist[k] = ctl->c_acc;
+            a.m_hist[k] = m;
+        }
+        if (c.dense) ctl->many_iters += 1;
+        else ctl->few_iters += 1;
+        ctl->c_acc = 0;
+        ctl->iter = k + 1;
+    }
+    ctl->epoch = epoch + 1;
+}
+
+// One look-back tile of the compaction of list[q] (masks in mlist, record in
+// C) into list[q ^ 1]; kInit: the voxels with C != 0 in tiled order.  Every
+// thread of the block calls it.  Returns false (with the error flag set) if the
+// next list would overflow, which a consistent record never does.
+template <int kMode>
+__device__ inline bool compact_tile(const KArgs& a, int q, const CompactPlan& c, uint32_t tile, uint32_t epoch,
+                                    CompactShared& sh)
+{
+    const uint32_t* __restrict__ Vs = a.list[q];
+    const uint8_t* __restrict__ Ms = a.mlist;
+    const uint64_t* __restrict__ Ck = a.C;
+    uint32_t* __restrict__ out = a.list[q ^ 1];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt(lane);
+    const uint32_t n_src = c.n_src;
+
+    uint64_t bal[kItems];
+    uint32_t val[kItems];
+    const uint64_t base = (uint64_t)tile * kScanTile;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const uint64_t s = base + (uint64_t)j * kBlock + threadIdx.x;
+        bool keep = false;
+        uint32_t v = 0;
+        if (s < c.slots) {
+            if (c.dense) {
+                keep = Ck[s] != 0ull;
+                v = (uint32_t)s;
+            } else {
+                // slot s = e * n_src + p: list-major concatenation of the
+                // seven candidate lists (Cuda.cpp:699-758 over 7 A slots)
+                const uint32_t e = (uint32_t)(s >= n_src) + (uint32_t)(s >= 2ull * n_src) +
+                                   (uint32_t)(s >= 3ull * n_src) + (uint32_t)(s >= 4ull * n_src) +
+                                   (uint32_t)(s >= 5ull * n_src) + (uint32_t)(s >= 6ull * n_src);
+                const uint32_t p = (uint32_t)(s - (uint64_t)e * n_src);
+                const uint32_t m = Ms[p];
+                const uint32_t t = Vs[p];
+                if ((m >> e) & 1u) {
+                    const uint32_t u = neighbor_in(a.g, t, e);
+                    const uint64_t rec = Ck[u];
+                    // first-wins: the lowest list that emitted u keeps it
+                    keep = (uint32_t)((__ffsll((unsigned long long)rec) - 1) >> 3) == e;
+                    v = u;
+                }
+            }
+        }
+        bal[j] = __ballot(keep);
+        val[j] = v;
+        if (lane == 0) sh.cnt[j * 4 + wave] = (uint32_t)__popcll(bal[j]);
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const uint32_t cc = lane < kItems * 4 ? sh.cnt[lane] : 0u;
+        const uint32_t incl = wave_inclusive_scan(cc, lane);
+        if (lane < kItems * 4) sh.excl[lane] = incl - cc;
+        const uint32_t total = __shfl(incl, 63, 64);
+        const uint32_t prefix = lookback(a.tstate, tile, total, epoch, &a.ctl->error);
+        if (lane == 0) {
+            sh.prefix = prefix;
+            sh.total = total;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        if ((bal[j] >> lane) & 1ull) {
+            const uint32_t pos = sh.excl[j * 4 + wave] + (uint32_t)__popcll(bal[j] & lt);
+            sh.stage[pos] = val[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t total = sh.total, prefix = sh.prefix;
+    if ((uint64_t)prefix + total > a.g.N) {
+        if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
+        return false;
+    }
+    for (uint32_t i = threadIdx.x; i < total; i += kBlock) out[prefix + i] = sh.stage[i];
+    if (threadIdx.x == 0 && tile == c.ntiles - 1) compact_finish<kMode>(a, q, c, prefix + total, epoch);
+    __syncthreads();
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// single-volume iteration kernels
 // ---------------------------------------------------------------------------
 template <typename SrcT, bool kFloatSrc>
 __global__ __launch_bounds__(kBlock) void k_update(KArgs a, int q)
 {
     const DevParams P = *a.prm;
     const uint32_t n = a.ctl->n[q];
-    const uint32_t* __restrict__ V = a.list[q];
-    const float* __restrict__ R = a.phi[q];
-    float* __restrict__ W = a.phi[q ^ 1];
     ListWalk w(a.run_log2);
-    for (uint64_t p = w.pos(); p < n; w.next(), p = w.pos()) {
-        const uint32_t u = V[p];
-        uint32_t nb[7], inb;
-        neighbors(a.g, u, nb, &inb);
-        const float nv = update_core(P, R[nb[0]], R[nb[1]], R[nb[2]], R[u], R[nb[4]], R[nb[5]], R[nb[6]],
-                                     density_factor<SrcT, kFloatSrc>(a, P, u));
-        W[u] = nv;
-        a.E[u] = export_byte(nv);
-        a.C[u] = 0ull;  // the record of the candidates that produced this list
-    }
+    for (uint64_t p = w.pos(); p < n; w.next(), p = w.pos())
+        update_at<SrcT, kFloatSrc>(a, P, a.list[q], a.phi[q], a.phi[q ^ 1], p);
 }
 
 __global__ __launch_bounds__(kBlock) void k_mark(KArgs a, int q)
@@ -417,166 +595,143 @@ __global__ __launch_bounds__(kBlock) void k_mark(KArgs a, int q)
     __shared__ uint32_t s_red[kBlock / 64];
     const bool ref = a.prm->refMode != 0;
     const uint32_t n = a.ctl->n[q];
-    const uint32_t* __restrict__ V = a.list[q];
-    const float* __restrict__ R = a.phi[q];
-    const float* __restrict__ W = a.phi[q ^ 1];  // the freshly written values
-    uint8_t* __restrict__ Cb = (uint8_t*)a.C;
     uint32_t emitted = 0;
     ListWalk w(a.run_log2);
-    for (uint64_t p = w.pos(); p < n; w.next(), p = w.pos()) {
-        uint32_t m = 0;
-        if (!ref || (p & 3u) != 3u) {  // UpdateActiveElementsVolume.cu:329
-            const uint32_t u = V[p];
-            uint32_t nb[7], inb;
-            neighbors(a.g, u, nb, &inb);
-            const float nv = W[u];
-            float wn[7];
-#pragma unroll
-            for (int e = 0; e < 7; ++e) wn[e] = W[nb[e]];
-            // temporal test: the time derivative of the update (UpdateLevelSetVolume.cu:327-331)
-            if (fabsf(nv - R[u]) > kTemporalThreshold) {
-#pragma unroll
-                for (int e = 0; e < 7; ++e) {
-                    if (e == 3 || !((inb >> e) & 1u)) continue;
-                    if (!(fabsf(nv - wn[e]) <= kTemporalThreshold)) m |= 1u << e;
-                }
-                if (m) m |= 8u;
-            }
-            if (m) {
-#pragma unroll
-                for (int e = 0; e < 7; ++e)
-                    if ((m >> e) & 1u) Cb[(uint64_t)nb[e] * 8u + (uint32_t)e] = 1;
-            }
-        }
-        a.mlist[p] = (uint8_t)m;
-        emitted += (uint32_t)__popc(m);
-    }
-    block_add_candidates(a.ctl, emitted, s_red);
+    for (uint64_t p = w.pos(); p < n; w.next(), p = w.pos())
+        emitted += mark_at(a, ref, a.list[q], a.phi[q], a.phi[q ^ 1], (uint8_t*)a.C, p);
+    block_add(&a.ctl->c_acc, emitted, s_red);
 }
 
-// kCompact: the candidates of list[q] (masks in mlist, record in C) into
-// list[q ^ 1]; kInit: the voxels with C != 0 into list[q ^ 1] in tiled order.
 template <int kMode>
 __global__ __launch_bounds__(kBlock) void k_compact(KArgs a, int q)
 {
-    __shared__ uint32_t s_stage[kScanTile];
-    __shared__ uint32_t s_cnt[kItems * 4];
-    __shared__ uint32_t s_excl[kItems * 4];
-    __shared__ uint32_t s_prefix, s_total;
-
-    const int op = q ^ 1;
-    const uint32_t n_src = a.ctl->n[q];
+    __shared__ CompactShared sh;
     const uint32_t epoch = a.ctl->epoch;
-    const bool dense = (kMode == kInit) || !path_is_few(n_src, a.g.N);
-    const uint64_t slots = dense ? a.g.N : 7ull * n_src;
-    const uint32_t ntiles = (uint32_t)((slots + kScanTile - 1) / kScanTile);
-    const uint32_t* __restrict__ Vs = a.list[q];
-    const uint8_t* __restrict__ Ms = a.mlist;
-    const uint64_t* __restrict__ Ck = a.C;
-    uint32_t* __restrict__ out = a.list[op];
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const uint64_t lt = lanemask_lt(lane);
+    const CompactPlan c = compact_plan<kMode>(a.g, a.ctl->n[q]);
+    for (uint32_t tile = blockIdx.x; tile < c.ntiles; tile += gridDim.x)
+        if (!compact_tile<kMode>(a, q, c, tile, epoch, sh)) break;
+    if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kMode>(a, q, c, 0, epoch);
+}
 
-    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        uint64_t bal[kItems];
-        uint32_t val[kItems];
-        const uint64_t base = (uint64_t)tile * kScanTile;
+// ---------------------------------------------------------------------------
+// batched iteration kernels: many volumes of uneven size, each with its own
+// parameters, advanced by one launch per step.  Work items (blocks of kBlock
+// list positions, or look-back tiles) are numbered volume after volume; every
+// workgroup rebuilds the per-volume offsets from the device counts, so the
+// launches need no host input and replay unchanged from a captured graph.
+// Volume v's control block is ctls[v] (also vols[v].ctl).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kMaxBatch = 1024;
+
+struct BatchArgs {
+    const KArgs* vols;
+    DevCtl* ctls;
+    uint32_t count;
+};
+
+// exclusive prefix s_pre[0..count] of f(v) over the volumes (all threads)
+template <typename F>
+__device__ inline void batch_prefix(uint32_t count, F f, uint32_t* s_pre, uint32_t* s_tmp)
+{
+    const uint32_t per = (count + kBlock - 1) / kBlock;  // <= kMaxBatch / kBlock
+    const uint32_t v0 = threadIdx.x * per;
+    uint32_t vals[kMaxBatch / kBlock];
+    uint32_t sum = 0;
 #pragma unroll
-        for (int j = 0; j < kItems; ++j) {
-            const uint64_t s = base + (uint64_t)j * kBlock + threadIdx.x;
-            bool keep = false;
-            uint32_t v = 0;
-            if (s < slots) {
-                if (dense) {
-                    keep = Ck[s] != 0ull;
-                    v = (uint32_t)s;
-                } else {
-                    // slot s = e * n_src + p: list-major concatenation of the
-                    // seven candidate lists (Cuda.cpp:699-758 over 7 A slots)
-                    const uint32_t e = (uint32_t)(s >= n_src) + (uint32_t)(s >= 2ull * n_src) +
-                                       (uint32_t)(s >= 3ull * n_src) + (uint32_t)(s >= 4ull * n_src) +
-                                       (uint32_t)(s >= 5ull * n_src) + (uint32_t)(s >= 6ull * n_src);
-                    const uint32_t p = (uint32_t)(s - (uint64_t)e * n_src);
-                    const uint32_t m = Ms[p];
-                    const uint32_t t = Vs[p];
-                    if ((m >> e) & 1u) {
-                        const uint32_t u = neighbor_in(a.g, t, e);
-                        const uint64_t rec = Ck[u];
-                        // first-wins: the lowest list that emitted u keeps it
-                        keep = (uint32_t)((__ffsll((unsigned long long)rec) - 1) >> 3) == e;
-                        v = u;
-                    }
-                }
-            }
-            bal[j] = __ballot(keep);
-            val[j] = v;
-            if (lane == 0) s_cnt[j * 4 + wave] = (uint32_t)__popcll(bal[j]);
-        }
-        __syncthreads();
-        if (wave == 0) {
-            const uint32_t c = lane < kItems * 4 ? s_cnt[lane] : 0u;
-            const uint32_t incl = wave_inclusive_scan(c, lane);
-            if (lane < kItems * 4) s_excl[lane] = incl - c;
-            const uint32_t total = __shfl(incl, 63, 64);
-            const uint32_t prefix = lookback(a.tstate, tile, total, epoch, &a.ctl->error);
-            if (lane == 0) {
-                s_prefix = prefix;
-                s_total = total;
-            }
-        }
-        __syncthreads();
+    for (uint32_t i = 0; i < kMaxBatch / kBlock; ++i) {
+        const uint32_t v = v0 + i;
+        vals[i] = (i < per && v < count) ? f(v) : 0u;
+        sum += vals[i];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t incl = wave_inclusive_scan(sum, lane);
+    if (lane == 63) s_tmp[wave] = incl;
+    __syncthreads();
+    uint32_t wbase = 0;
+    for (int w = 0; w < wave; ++w) wbase += s_tmp[w];
+    uint32_t run = wbase + incl - sum;
 #pragma unroll
-        for (int j = 0; j < kItems; ++j) {
-            if ((bal[j] >> lane) & 1ull) {
-                const uint32_t pos = s_excl[j * 4 + wave] + (uint32_t)__popcll(bal[j] & lt);
-                s_stage[pos] = val[j];
-            }
-        }
-        __syncthreads();
-        const uint32_t total = s_total, prefix = s_prefix;
-        if ((uint64_t)prefix + total > a.g.N) {  // cannot happen with a consistent record
-            if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
-            break;
-        }
-        for (uint32_t i = threadIdx.x; i < total; i += kBlock) out[prefix + i] = s_stage[i];
-        if (threadIdx.x == 0 && tile == ntiles - 1) {
-            const uint32_t m = prefix + total;
-            a.ctl->n[op] = m;
-            if constexpr (kMode == kInit) {
-                a.ctl->n[q] = 0;
-            } else {
-                const uint32_t k = a.ctl->iter;
-                if (k < a.ctl->stats_cap) {
-                    a.n_hist[k] = n_src;
-                    a.c_hist[k] = a.ctl->c_acc;
-                    a.m_hist[k] = m;
-                }
-                if (dense) a.ctl->many_iters += 1; else a.ctl->few_iters += 1;
-                a.ctl->c_acc = 0;
-                a.ctl->iter = k + 1;
-            }
-            a.ctl->epoch = epoch + 1;
-        }
-        __syncthreads();
+    for (uint32_t i = 0; i < kMaxBatch / kBlock; ++i) {
+        const uint32_t v = v0 + i;
+        if (i < per && v < count) s_pre[v] = run;
+        run += vals[i];
     }
-    if (ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
-        a.ctl->n[op] = 0;
-        if constexpr (kMode == kInit) {
-            a.ctl->n[q] = 0;
-        } else {
-            const uint32_t k = a.ctl->iter;
-            if (k < a.ctl->stats_cap) {
-                a.n_hist[k] = n_src;
-                a.c_hist[k] = a.ctl->c_acc;
-                a.m_hist[k] = 0;
-            }
-            if (dense) a.ctl->many_iters += 1; else a.ctl->few_iters += 1;
-            a.ctl->c_acc = 0;
-            a.ctl->iter = k + 1;
-        }
-        a.ctl->epoch = epoch + 1;
+    if (threadIdx.x == kBlock - 1) s_pre[count] = wbase + incl;
+    __syncthreads();
+}
+
+// the volume owning work item j: s_pre[v] <= j < s_pre[v + 1]
+__device__ inline uint32_t batch_find(const uint32_t* s_pre, uint32_t count, uint32_t j)
+{
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_pre[mid] <= j) lo = mid;
+        else hi = mid;
     }
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
+}
+
+template <typename SrcT, bool kFloatSrc>
+__global__ __launch_bounds__(kBlock) void k_update_batch(BatchArgs b, int q)
+{
+    __shared__ uint32_t s_pre[kMaxBatch + 1];
+    __shared__ uint32_t s_tmp[kBlock / 64];
+    batch_prefix(b.count, [&](uint32_t v) { return (b.ctls[v].n[q] + kBlock - 1) / kBlock; }, s_pre, s_tmp);
+    const uint32_t items = s_pre[b.count];
+    for (uint32_t j = blockIdx.x; j < items; j += gridDim.x) {
+        const uint32_t v = batch_find(s_pre, b.count, j);
+        const KArgs& a = b.vols[v];
+        const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
+        if (p < b.ctls[v].n[q]) {
+            const DevParams P = *a.prm;
+            update_at<SrcT, kFloatSrc>(a, P, a.list[q], a.phi[q], a.phi[q ^ 1], p);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mark_batch(BatchArgs b, int q)
+{
+    __shared__ uint32_t s_pre[kMaxBatch + 1];
+    __shared__ uint32_t s_tmp[kBlock / 64];
+    __shared__ uint32_t s_red[kBlock / 64];
+    batch_prefix(b.count, [&](uint32_t v) { return (b.ctls[v].n[q] + kBlock - 1) / kBlock; }, s_pre, s_tmp);
+    const uint32_t items = s_pre[b.count];
+    for (uint32_t j = blockIdx.x; j < items; j += gridDim.x) {
+        const uint32_t v = batch_find(s_pre, b.count, j);
+        const KArgs& a = b.vols[v];
+        const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
+        uint32_t emitted = 0;
+        if (p < b.ctls[v].n[q])
+            emitted = mark_at(a, a.prm->refMode != 0, a.list[q], a.phi[q], a.phi[q ^ 1], (uint8_t*)a.C, p);
+        block_add(&b.ctls[v].c_acc, emitted, s_red);
+    }
+}
+
+// Every workgroup of the grid is resident (occupancy-sized grid), so a tile
+// may wait on a tile of the same volume held by any workgroup.  A volume's
+// epoch is read at each of its tiles: its last tile, which advances the epoch,
+// completes only after every other tile of the volume has published.
+__global__ __launch_bounds__(kBlock) void k_compact_batch(BatchArgs b, int q)
+{
+    __shared__ CompactShared sh;
+    __shared__ uint32_t s_pre[kMaxBatch + 1];
+    __shared__ uint32_t s_tmp[kBlock / 64];
+    batch_prefix(b.count, [&](uint32_t v) { return compact_plan<kCompact>(b.vols[v].g, b.ctls[v].n[q]).ntiles; },
+                 s_pre, s_tmp);
+    const uint32_t items = s_pre[b.count];
+    for (uint32_t j = blockIdx.x; j < items; j += gridDim.x) {
+        const uint32_t v = batch_find(s_pre, b.count, j);
+        const KArgs& a = b.vols[v];
+        const CompactPlan c = compact_plan<kCompact>(a.g, b.ctls[v].n[q]);
+        if (!compact_tile<kCompact>(a, q, c, j - s_pre[v], b.ctls[v].epoch, sh)) break;
+    }
+    // volumes with nothing to compact still advance their counts
+    if (threadIdx.x == 0)
+        for (uint32_t v = blockIdx.x; v < b.count; v += gridDim.x)
+            if (s_pre[v + 1] == s_pre[v])
+                compact_finish<kCompact>(b.vols[v], q, compact_plan<kCompact>(b.vols[v].g, b.ctls[v].n[q]), 0,
+                                         b.ctls[v].epoch);
 }
 
 // ---------------------------------------------------------------------------

@@ -31,6 +31,9 @@ EXPORTS = (
     "lsseg_freeze", "lsseg_clear", "lsseg_get_state", "lsseg_set_state",
     "lsseg_get_device_buffers", "lsseg_checkpoint_save", "lsseg_checkpoint_restore",
     "lsseg_iterate_profiled",
+    "lsseg_batch_create", "lsseg_batch_destroy", "lsseg_batch_last_error", "lsseg_batch_count",
+    "lsseg_batch_volume", "lsseg_batch_iterate", "lsseg_batch_checkpoint_save", "lsseg_batch_checkpoint_restore",
+    "lsseg_batch_get_tuning", "lsseg_batch_set_tuning", "lsseg_batch_autotune",
 )
 
 
@@ -55,6 +58,14 @@ class Stats(ctypes.Structure):
                 ("sum_candidates", ctypes.c_uint64), ("sum_new_active", ctypes.c_uint64),
                 ("active_final", ctypes.c_uint64), ("few_path_iters", ctypes.c_uint64),
                 ("many_path_iters", ctypes.c_uint64), ("device_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("update_blocks", ctypes.c_uint32), ("compact_blocks", ctypes.c_uint32),
+                ("ms", ctypes.c_double), ("candidates", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -105,6 +116,17 @@ def lib():
             "lsseg_checkpoint_save": (i32, [vp, i32]),
             "lsseg_checkpoint_restore": (i32, [vp, i32]),
             "lsseg_iterate_profiled": (i32, [vp, P(Params), i32, P(Stats), P(KernelTimes), vp, u64]),
+            "lsseg_batch_create": (i32, [P(VolumeDesc), P(vp), u32, P(vp)]),
+            "lsseg_batch_destroy": (i32, [vp]),
+            "lsseg_batch_last_error": (ctypes.c_char_p, [vp]),
+            "lsseg_batch_count": (i32, [vp, P(u32)]),
+            "lsseg_batch_volume": (i32, [vp, u32, P(vp)]),
+            "lsseg_batch_iterate": (i32, [vp, P(Params), i32, P(Stats)]),
+            "lsseg_batch_checkpoint_save": (i32, [vp, i32]),
+            "lsseg_batch_checkpoint_restore": (i32, [vp, i32]),
+            "lsseg_batch_get_tuning": (i32, [vp, P(Tuning)]),
+            "lsseg_batch_set_tuning": (i32, [vp, P(Tuning)]),
+            "lsseg_batch_autotune": (i32, [vp, P(Params), i32, P(Tuning)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -145,8 +167,24 @@ class Context:
         if st != OK:
             raise LssegError(st, L.lsseg_last_error(None).decode())
         self._h = h
+        self._owned = True
+        self._dims()
+
+    @classmethod
+    def _borrow(cls, handle, shape_zyx, dtype, owner):
+        """A view of a context owned by someone else (a batch volume)."""
+        self = cls.__new__(cls)
+        self.nz, self.ny, self.nx = (int(v) for v in shape_zyx)
+        self.dtype = np.dtype(dtype)
+        self._h = handle
+        self._owned = False
+        self._owner = owner
+        self._dims()
+        return self
+
+    def _dims(self):
         X, Y, Z = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
-        self._check(L.lsseg_get_dims(self._h, ctypes.byref(X), ctypes.byref(Y), ctypes.byref(Z)))
+        self._check(lib().lsseg_get_dims(self._h, ctypes.byref(X), ctypes.byref(Y), ctypes.byref(Z)))
         self.X, self.Y, self.Z = X.value, Y.value, Z.value
         self.N = self.X * self.Y * self.Z
 
@@ -157,7 +195,8 @@ class Context:
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().lsseg_destroy(self._h)
+            if getattr(self, "_owned", True):
+                lib().lsseg_destroy(self._h)
             self._h = None
 
     def __enter__(self):
@@ -275,3 +314,91 @@ class Context:
         r, w, s = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
         self._check(lib().lsseg_get_device_buffers(self._h, ctypes.byref(r), ctypes.byref(w), ctypes.byref(s)))
         return r.value, w.value, s.value
+
+
+class Batch:
+    """Many volumes advanced together (the C ABI's ``lsseg_batch``).  ``volume(i)``
+    is a ``Context`` view of volume i for initialisation and readback."""
+
+    def __init__(self, volumes_zyx, z_aniso: float = 1.0):
+        vols = [np.ascontiguousarray(v) for v in volumes_zyx]
+        if not vols:
+            raise ValueError("empty batch")
+        for v in vols:
+            if v.ndim != 3 or v.dtype not in DTYPE_CODE:
+                raise ValueError("volumes must be 3-D arrays of a supported dtype")
+        L = lib()
+        descs = (VolumeDesc * len(vols))(*[VolumeDesc(v.shape[2], v.shape[1], v.shape[0], DTYPE_CODE[v.dtype],
+                                                      z_aniso) for v in vols])
+        ptrs = (ctypes.c_void_p * len(vols))(*[v.ctypes.data for v in vols])
+        h = ctypes.c_void_p()
+        st = L.lsseg_batch_create(descs, ptrs, len(vols), ctypes.byref(h))
+        if st != OK:
+            raise LssegError(st, L.lsseg_batch_last_error(None).decode())
+        self._h = h
+        self._vols = []
+        for i, v in enumerate(vols):
+            c = ctypes.c_void_p()
+            self._check(L.lsseg_batch_volume(self._h, i, ctypes.byref(c)))
+            self._vols.append(Context._borrow(c, v.shape, v.dtype, self))
+
+    def _check(self, st: int):
+        if st != OK:
+            raise LssegError(st, lib().lsseg_batch_last_error(self._h).decode())
+
+    def __len__(self):
+        return len(self._vols)
+
+    def volume(self, i: int) -> Context:
+        return self._vols[i]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for v in self._vols:
+                v._h = None
+            lib().lsseg_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _params(params):
+        return (Params * len(params))(*[Params(*p) if not isinstance(p, Params) else p for p in params])
+
+    def iterate(self, params, iters):
+        """params: one (T, eps, alpha, dt, mode) per volume.  Returns one Stats per volume."""
+        if len(params) != len(self._vols):
+            raise ValueError("one parameter set per volume")
+        stats = (Stats * len(self._vols))()
+        self._check(lib().lsseg_batch_iterate(self._h, self._params(params), int(iters), stats))
+        return list(stats)
+
+    def checkpoint_save(self, slot: int = 0):
+        self._check(lib().lsseg_batch_checkpoint_save(self._h, slot))
+
+    def checkpoint_restore(self, slot: int = 0):
+        self._check(lib().lsseg_batch_checkpoint_restore(self._h, slot))
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        self._check(lib().lsseg_batch_get_tuning(self._h, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, update_blocks: int, compact_blocks: int):
+        t = Tuning(update_blocks, compact_blocks, 0.0, 0)
+        self._check(lib().lsseg_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, params, iters) -> Tuning:
+        t = Tuning()
+        self._check(lib().lsseg_batch_autotune(self._h, self._params(params), int(iters), ctypes.byref(t)))
+        return t

@@ -38,6 +38,7 @@ extern "C" {
 #endif
 
 typedef struct lsseg_ctx lsseg_ctx;
+typedef struct lsseg_batch lsseg_batch;
 
 typedef enum lsseg_status {
     LSSEG_OK = 0,
@@ -225,6 +226,58 @@ lsseg_status lsseg_iterate_profiled(lsseg_ctx* ctx, const lsseg_params* params, 
    that keep data resident in HBM (e.g. a torch wrapper). */
 lsseg_status lsseg_get_device_buffers(const lsseg_ctx* ctx, void** phi_read, void** phi_write,
                                       void** stream);
+
+/* ---- batched form --------------------------------------------------------
+
+   No reference counterpart: the reference segments one volume at a time.  A
+   batch advances many independent volumes of uneven size, each with its own
+   parameters, with one launch per step (update, marking, compaction) for the
+   whole batch per iteration; runs of iterations replay one captured hipGraph.
+   Each volume's results are bit-identical to a solo lsseg_iterate run of the
+   same volume.  All volumes share the batch's stream and one source type. */
+
+#define LSSEG_MAX_BATCH 1024
+
+/* Creates `count` volumes (descs[i], host_volumes[i]); all descs must share
+   one dtype.  On failure *out is NULL and lsseg_batch_last_error(NULL) says why. */
+lsseg_status lsseg_batch_create(const lsseg_volume_desc* descs, const void* const* host_volumes,
+                                uint32_t count, lsseg_batch** out);
+lsseg_status lsseg_batch_destroy(lsseg_batch* batch);
+const char* lsseg_batch_last_error(const lsseg_batch* batch);
+lsseg_status lsseg_batch_count(const lsseg_batch* batch, uint32_t* count);
+
+/* The context of volume `index`, for initialisation and readback with the
+   single-volume calls (lsseg_set_phi, lsseg_seed_sphere, lsseg_get_mask,
+   lsseg_get_active, lsseg_get_phi, lsseg_freeze, lsseg_clear, checkpoints ...).
+   It is owned by the batch: lsseg_destroy on it fails with LSSEG_ERR_STATE. */
+lsseg_status lsseg_batch_volume(lsseg_batch* batch, uint32_t index, lsseg_ctx** out);
+
+/* `iters` iterations of every volume; params[i] applies to volume i; stats
+   (may be NULL) receives one lsseg_stats per volume (device_ms is the batch's). */
+lsseg_status lsseg_batch_iterate(lsseg_batch* batch, const lsseg_params* params, int32_t iters,
+                                 lsseg_stats* stats);
+
+/* Checkpoint every volume into / from its slot `slot` (lsseg_checkpoint_*). */
+lsseg_status lsseg_batch_checkpoint_save(lsseg_batch* batch, int32_t slot);
+lsseg_status lsseg_batch_checkpoint_restore(lsseg_batch* batch, int32_t slot);
+
+/* Launch shape of the batched kernels: workgroups of the update and marking
+   launches and of the compaction launch (at most the co-resident count). */
+typedef struct lsseg_tuning {
+    uint32_t update_blocks;
+    uint32_t compact_blocks;
+    double ms;            /* autotune: device time of the chosen shape */
+    uint32_t candidates;  /* autotune: shapes timed */
+} lsseg_tuning;
+
+lsseg_status lsseg_batch_get_tuning(const lsseg_batch* batch, lsseg_tuning* out);
+lsseg_status lsseg_batch_set_tuning(lsseg_batch* batch, const lsseg_tuning* tuning);
+
+/* Times `iters` iterations of the batch from its current state for each
+   candidate launch shape, keeps the fastest and restores the state.  Uses
+   checkpoint slot LSSEG_MAX_CHECKPOINTS - 1 of every volume. */
+lsseg_status lsseg_batch_autotune(lsseg_batch* batch, const lsseg_params* params, int32_t iters,
+                                  lsseg_tuning* out);
 
 #ifdef __cplusplus
 }
