@@ -196,6 +196,72 @@ def cpu_baseline(w, gpu_mask, seconds_1core):
     }, dice, identical
 
 
+def batch_vs_sequential(steps, iterations):
+    """The batched form on the 64-volume workload (uneven uint8 volumes, per-volume
+    parameters) against 64 sequential plain runs of the same volumes on the same
+    GPU; masks must be bit-identical."""
+    from aworkefficientgpualgorithmforlevelsetsegmentation_amd import synth
+    from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import Batch, Context
+    t = time.perf_counter()
+    ws = synth.batch(iterations=iterations) if iterations else synth.batch()
+    iters = ws[0].iterations
+    log(f"batch workload: {len(ws)} volumes, {sum(w.volume.size for w in ws) / 1e6:.1f} M voxels, "
+        f"{iters} iterations, generated in {time.perf_counter() - t:.1f} s")
+    params = [(w.T, w.eps, w.alpha, w.dt, 0) for w in ws]
+    b = Batch([w.volume for w in ws])
+    for i, w in enumerate(ws):
+        b.volume(i).set_phi(w.phi0, w.band)
+    b.checkpoint_save(0)
+    tune = b.autotune(params, min(iters, 100))
+    log(f"autotuned launch shape: {tune.update_blocks} update/mark workgroups, {tune.compact_blocks} compaction "
+        f"workgroups ({tune.candidates} shapes)")
+    b.checkpoint_restore(0)
+    b.iterate(params, iters)  # warm-up (graph capture)
+    device_sync()
+    t0 = time.perf_counter()
+    upd = 0
+    for _ in range(steps):
+        b.checkpoint_restore(0)
+        st = b.iterate(params, iters)
+        upd = sum(int(x.sum_active) for x in st)
+    device_sync()
+    t_batch = (time.perf_counter() - t0) / steps
+    batch_masks = [b.volume(i).mask() for i in range(len(ws))]
+    b.close()
+    # sequential plain runs: each volume warmed up once, then timed
+    t_seq = 0.0
+    upd_seq = 0
+    same = True
+    for i, w in enumerate(ws):
+        ctx = Context(w.volume)
+        ctx.set_phi(w.phi0, w.band)
+        ctx.checkpoint_save(0)
+        ctx.iterate(*params[i][:4], iters)
+        ctx.checkpoint_restore(0)
+        device_sync()
+        t0 = time.perf_counter()
+        st = ctx.iterate(*params[i][:4], iters)
+        device_sync()
+        t_seq += time.perf_counter() - t0
+        upd_seq += int(st.sum_active)
+        same &= bool(np.array_equal(ctx.mask(), batch_masks[i]))
+        ctx.close()
+    assert upd_seq == upd, "batched and sequential runs did different work"
+    out = {
+        "workload": f"{len(ws)} uint8 volumes, edges log-uniform in [64,256], per-volume T/eps/alpha, "
+                    f"dt=0.5, {iters} iterations each",
+        "volumes": len(ws), "iterations": iters, "updates_per_step": upd,
+        "ms_per_step": 1e3 * t_batch, "value": upd / t_batch, "unit": "active-voxel updates/s",
+        "sequential_ms": 1e3 * t_seq, "sequential_value": upd / t_seq,
+        "speedup_vs_sequential": t_seq / t_batch, "masks_identical_to_sequential": same,
+        "tuning": {"update_blocks": tune.update_blocks, "compact_blocks": tune.compact_blocks},
+    }
+    log(f"batched: {1e3 * t_batch:.2f} ms/step ({upd / t_batch:.3e} updates/s); sequential plain runs: "
+        f"{1e3 * t_seq:.2f} ms ({upd / t_seq:.3e} updates/s); speedup {t_seq / t_batch:.2f}x; "
+        f"masks identical: {same}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,6 +270,8 @@ def main():
     ap.add_argument("--workload", default="phantom", choices=["ellipsoids", "phantom", "vessels"])
     ap.add_argument("--iterations", type=int, default=0, help="override the workload's iteration count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-sequential comparison")
+    ap.add_argument("--batch-iterations", type=int, default=0, help="override the batch workload's iterations")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="single-core baseline sample budget")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (HBM counter calibration)")
@@ -316,6 +384,10 @@ def main():
                                    for k, v in kernels.items())
         + f"; kernel time sum {kt.total_ms:.2f} ms vs {ms_per_step:.2f} ms/step")
 
+    batched = None
+    if not args.no_batch:
+        batched = batch_vs_sequential(max(1, min(args.steps, 3)), args.batch_iterations)
+
     cpu = None
     dice = None
     identical = None
@@ -354,6 +426,7 @@ def main():
                           "frac": path_gbs / HBM_PEAK_GBS, "bytes_per_step": path_bytes},
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "batched": batched,
         "dice_vs_oracle": dice,
         "mask_identical_to_oracle": identical,
     }
