@@ -68,10 +68,13 @@ struct lsseg_ctx {
     std::vector<hipEvent_t> prof_events;
     size_t prof_next = 0;
     std::vector<void*> allocs;
+    std::vector<lsseg_record*> records;  // live records of the differentiable form
     std::string err;
 };
 
 namespace {
+
+void detach_records(lsseg_ctx* c);
 
 lsseg_status fail(lsseg_ctx* c, lsseg_status st, const char* fmt, ...)
 {
@@ -234,7 +237,9 @@ lsseg_status sync_and_check(lsseg_ctx* c)
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (c->h_ctl->error)
         return fail(c, LSSEG_ERR_DEVICE, "device error flag 0x%x (%s)", c->h_ctl->error,
-                    (c->h_ctl->error & kErrSpin) ? "look-back wait exceeded its bound" : "list overflow");
+                    (c->h_ctl->error & kErrSpin)     ? "look-back wait exceeded its bound"
+                    : (c->h_ctl->error & kErrRecord) ? "a recording run diverged from its first pass"
+                                                     : "list overflow");
     return LSSEG_OK;
 }
 
@@ -489,6 +494,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
 void destroy_ctx(lsseg_ctx* ctx)
 {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    detach_records(ctx);
     destroy_graphs(ctx);
     for (hipEvent_t e : ctx->prof_events) (void)hipEventDestroy(e);
     ctx->prof_events.clear();
@@ -538,7 +544,7 @@ lsseg_status lsseg_set_phi(lsseg_ctx* c, const float* phi0, float band)
         (void)hipGetLastError();
         return fail(c, LSSEG_ERR_OUT_OF_MEMORY, "staging allocation failed");
     }
-    hipError_t e = hipMemcpyAsync(lin, phi0, cnt * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    hipError_t e = hipMemcpyAsync(lin, phi0, cnt * sizeof(float), hipMemcpyDefault, c->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_set_phi, dim3(dense_grid(c->a.g.N)), dim3(kBlock), 0, c->stream, c->a, lin, c->nx,
                            c->ny, c->nz, band);
@@ -731,7 +737,7 @@ lsseg_status lsseg_get_phi(const lsseg_ctx* c, int32_t which, float* out)
     }
     hipLaunchKernelGGL(k_get_phi, dim3(dense_grid(cnt)), dim3(kBlock), 0, c->stream, c->a, which == 0 ? c->q : c->q ^ 1,
                        d, c->nx, c->ny, c->nz);
-    hipError_t e = hipMemcpyAsync(out, d, cnt * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    hipError_t e = hipMemcpyAsync(out, d, cnt * sizeof(float), hipMemcpyDefault, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(cm, LSSEG_ERR_DEVICE, "get_phi failed: %s", hipGetErrorString(e));
@@ -1281,6 +1287,314 @@ lsseg_status lsseg_batch_autotune(lsseg_batch* b, const lsseg_params* params, in
     if ((st = lsseg_batch_set_tuning(b, &best)) != LSSEG_OK) return st;
     if ((st = lsseg_batch_checkpoint_restore(b, slot)) != LSSEG_OK) return st;
     if (out) *out = best;
+    return LSSEG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// differentiable form
+// ---------------------------------------------------------------------------
+struct lsseg_record {
+    lsseg_ctx* ctx = nullptr;
+    int32_t iters = 0, every = 1;
+    lsseg_params params{};
+    DevParams* d_prm = nullptr;       // the recorded run's parameters
+    DevParams* d_prm_replay = nullptr;
+    std::vector<uint32_t> n;          // n_k
+    std::vector<uint64_t> off;        // offsets of V_k in lists / dec
+    uint64_t total = 0;
+    uint32_t* lists = nullptr;
+    uint8_t* dec = nullptr;
+    std::vector<float*> ckA, ckB;     // (read, write) buffers at k = 0, every, 2 every, ...
+    float* seg = nullptr;             // read buffers A_k of one checkpoint segment
+    float* work[2] = {nullptr, nullptr};
+    float* adj[2] = {nullptr, nullptr};
+    float* lin = nullptr;             // unpadded staging
+    double* d_theta = nullptr;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+constexpr int32_t kRecordSlot = LSSEG_MAX_CHECKPOINTS - 1;
+
+// A destroyed context leaves its records detached: they keep their memory
+// until lsseg_record_destroy and refuse every other call.
+void detach_records(lsseg_ctx* c)
+{
+    for (lsseg_record* r : c->records) r->ctx = nullptr;
+    c->records.clear();
+}
+
+lsseg_status ralloc(lsseg_ctx* c, lsseg_record* r, void** p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(c, LSSEG_ERR_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    r->allocs.push_back(*p);
+    return LSSEG_OK;
+}
+
+#define LSSEG_DTYPE_KERNEL(name)                                                          \
+    const void* name##_kernel(int32_t dtype)                                              \
+    {                                                                                     \
+        switch (dtype) {                                                                  \
+        case LSSEG_U8: return (const void*)&name<uint8_t, false>;                         \
+        case LSSEG_I8: return (const void*)&name<int8_t, false>;                          \
+        case LSSEG_U16: return (const void*)&name<uint16_t, false>;                       \
+        case LSSEG_I16: return (const void*)&name<int16_t, false>;                        \
+        case LSSEG_U32: return (const void*)&name<uint32_t, false>;                       \
+        case LSSEG_I32: return (const void*)&name<int32_t, false>;                        \
+        default: return (const void*)&name<float, true>;                                  \
+        }                                                                                 \
+    }
+LSSEG_DTYPE_KERNEL(k_record)
+LSSEG_DTYPE_KERNEL(k_replay)
+LSSEG_DTYPE_KERNEL(k_adjoint)
+#undef LSSEG_DTYPE_KERNEL
+
+uint32_t list_grid(uint64_t n) { return dense_grid(std::max<uint64_t>(n, 1)); }
+
+void launch_replay(lsseg_ctx* c, lsseg_record* r, const KArgs& ra, int k, const float* R, float* W)
+{
+    const uint32_t* V = r->lists + r->off[k];
+    const uint8_t* D = r->dec + r->off[k];
+    uint32_t n = r->n[k];
+    void* args[] = {(void*)&ra, (void*)&V, (void*)&D, (void*)&n, (void*)&R, (void*)&W};
+    (void)hipLaunchKernel(k_replay_kernel(c->dtype), dim3(list_grid(n)), dim3(kBlock), args, 0, c->stream);
+}
+
+// the context's geometry and source with the record's parameters
+KArgs record_args(lsseg_ctx* c, const DevParams* prm)
+{
+    KArgs ra = c->a;
+    ra.prm = prm;
+    return ra;
+}
+
+lsseg_status check_record(lsseg_ctx* c, const lsseg_record* r)
+{
+    if (!c || !r) return LSSEG_ERR_INVALID_ARG;
+    if (r->ctx != c) return fail(c, LSSEG_ERR_STATE, "the record belongs to another (or a destroyed) context");
+    return LSSEG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+lsseg_status lsseg_record_destroy(lsseg_record* r)
+{
+    if (!r) return LSSEG_ERR_INVALID_ARG;
+    if (r->ctx) {
+        if (r->ctx->stream) (void)hipStreamSynchronize(r->ctx->stream);
+        auto& v = r->ctx->records;
+        v.erase(std::remove(v.begin(), v.end(), r), v.end());
+    }
+    for (void* p : r->allocs) (void)hipFree(p);
+    delete r;
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t iters, int32_t every,
+                              lsseg_record** out)
+{
+    if (!c || !prm || !out) return LSSEG_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (iters <= 0 || every <= 0) return fail(c, LSSEG_ERR_INVALID_ARG, "iterations and checkpoint interval must be positive");
+    DevParams P;
+    lsseg_status st = make_params(c, prm, &P);
+    if (st != LSSEG_OK) return st;
+    const uint64_t N = c->a.g.N;
+    // pass 1: the plain run gives n_k, hence the record's size
+    if ((st = lsseg_checkpoint_save(c, kRecordSlot)) != LSSEG_OK) return st;
+    lsseg_stats s1;
+    if ((st = lsseg_iterate(c, prm, iters, &s1)) != LSSEG_OK) return st;
+    lsseg_record* r = new (std::nothrow) lsseg_record();
+    if (!r) return fail(c, LSSEG_ERR_OUT_OF_MEMORY, "host allocation failed");
+    auto bail = [&](lsseg_status e) {
+        lsseg_record_destroy(r);
+        return e;
+    };
+    r->ctx = c;
+    r->iters = iters;
+    r->every = every;
+    r->params = *prm;
+    r->n.resize(iters);
+    HIP_TRY(c, hipMemcpy(r->n.data(), c->a.n_hist, iters * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    r->off.resize(iters + 1);
+    r->off[0] = 0;
+    for (int k = 0; k < iters; ++k) r->off[k + 1] = r->off[k] + r->n[k];
+    r->total = r->off[iters];
+    void* p = nullptr;
+    const int nck = (iters + every - 1) / every;
+    const int seg = std::min(every, iters);
+    if ((st = ralloc(c, r, &p, sizeof(DevParams))) != LSSEG_OK) return bail(st);
+    r->d_prm = (DevParams*)p;
+    if ((st = ralloc(c, r, &p, sizeof(DevParams))) != LSSEG_OK) return bail(st);
+    r->d_prm_replay = (DevParams*)p;
+    if ((st = ralloc(c, r, &p, r->total * sizeof(uint32_t))) != LSSEG_OK) return bail(st);
+    r->lists = (uint32_t*)p;
+    if ((st = ralloc(c, r, &p, r->total)) != LSSEG_OK) return bail(st);
+    r->dec = (uint8_t*)p;
+    for (int i = 0; i < nck; ++i) {
+        if ((st = ralloc(c, r, &p, N * sizeof(float))) != LSSEG_OK) return bail(st);
+        r->ckA.push_back((float*)p);
+        if ((st = ralloc(c, r, &p, N * sizeof(float))) != LSSEG_OK) return bail(st);
+        r->ckB.push_back((float*)p);
+    }
+    if ((st = ralloc(c, r, &p, (size_t)seg * N * sizeof(float))) != LSSEG_OK) return bail(st);
+    r->seg = (float*)p;
+    for (int i = 0; i < 2; ++i) {
+        if ((st = ralloc(c, r, &p, N * sizeof(float))) != LSSEG_OK) return bail(st);
+        r->work[i] = (float*)p;
+        if ((st = ralloc(c, r, &p, N * sizeof(float))) != LSSEG_OK) return bail(st);
+        r->adj[i] = (float*)p;
+    }
+    if ((st = ralloc(c, r, &p, (size_t)c->nx * c->ny * c->nz * sizeof(float))) != LSSEG_OK) return bail(st);
+    r->lin = (float*)p;
+    if ((st = ralloc(c, r, &p, 3 * sizeof(double))) != LSSEG_OK) return bail(st);
+    r->d_theta = (double*)p;
+    HIP_TRY(c, hipMemcpyAsync(r->d_prm, &P, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
+    // pass 2 from the same start: record lists and decisions, checkpoint phi
+    if ((st = lsseg_checkpoint_restore(c, kRecordSlot)) != LSSEG_OK) return bail(st);
+    if ((st = upload_params(c, P)) != LSSEG_OK) return bail(st);
+    HIP_TRY(c, hipMemcpyAsync(c->h_ctl, c->a.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->h_ctl->iter = 0;
+    c->h_ctl->stats_cap = c->stats_cap;
+    c->h_ctl->error = 0;
+    c->h_ctl->c_acc = 0;
+    c->h_ctl->few_iters = 0;
+    c->h_ctl->many_iters = 0;
+    HIP_TRY(c, hipMemcpyAsync(c->a.ctl, c->h_ctl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
+    int q = c->q;
+    for (int k = 0; k < iters; ++k) {
+        if (k % every == 0) {
+            HIP_TRY(c, hipMemcpyAsync(r->ckA[k / every], c->a.phi[q], N * sizeof(float), hipMemcpyDeviceToDevice,
+                                      c->stream));
+            HIP_TRY(c, hipMemcpyAsync(r->ckB[k / every], c->a.phi[q ^ 1], N * sizeof(float),
+                                      hipMemcpyDeviceToDevice, c->stream));
+        }
+        uint32_t* L = r->lists + r->off[k];
+        uint8_t* D = r->dec + r->off[k];
+        uint32_t n = r->n[k];
+        void* args[] = {(void*)&c->a, (void*)&q, (void*)&n, (void*)&L, (void*)&D};
+        (void)hipLaunchKernel(k_record_kernel(c->dtype), dim3(list_grid(n)), dim3(kBlock), args, 0, c->stream);
+        launch_iteration(c, q);
+        q ^= 1;
+    }
+    HIP_TRY(c, hipGetLastError());
+    if ((st = sync_and_check(c)) != LSSEG_OK) return bail(st);
+    c->q = q;
+    c->last_iters = (uint64_t)iters;
+    c->records.push_back(r);
+    *out = r;
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_record_info(const lsseg_record* r, int32_t* iters, int32_t* every, uint64_t* total_active)
+{
+    if (!r) return LSSEG_ERR_INVALID_ARG;
+    if (iters) *iters = r->iters;
+    if (every) *every = r->every;
+    if (total_active) *total_active = r->total;
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_record_get(const lsseg_record* r, uint32_t* n_per_iter, uint32_t* lists, uint8_t* decisions)
+{
+    if (!r || !r->ctx) return LSSEG_ERR_INVALID_ARG;
+    lsseg_ctx* c = r->ctx;
+    if (n_per_iter) memcpy(n_per_iter, r->n.data(), r->n.size() * sizeof(uint32_t));
+    if (lists && r->total) HIP_TRY(c, hipMemcpy(lists, r->lists, r->total * sizeof(uint32_t), hipMemcpyDefault));
+    if (decisions && r->total) HIP_TRY(c, hipMemcpy(decisions, r->dec, r->total, hipMemcpyDefault));
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_fwd_replay(lsseg_ctx* c, lsseg_record* r, const float* phi0, const lsseg_params* prm,
+                              float* phi_out)
+{
+    lsseg_status st = check_record(c, r);
+    if (st != LSSEG_OK) return st;
+    if (!phi0 || !prm || !phi_out) return fail(c, LSSEG_ERR_INVALID_ARG, "null argument");
+    DevParams P;
+    if ((st = make_params(c, prm, &P)) != LSSEG_OK) return st;
+    const uint64_t N = c->a.g.N, cnt = (uint64_t)c->nx * c->ny * c->nz;
+    HIP_TRY(c, hipMemcpyAsync(r->d_prm_replay, &P, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(r->lin, phi0, cnt * sizeof(float), hipMemcpyDefault, c->stream));
+    hipLaunchKernelGGL(k_lin_to_tiled, dim3(dense_grid(N)), dim3(kBlock), 0, c->stream, c->a.g, r->lin, r->work[0],
+                       c->nx, c->ny, c->nz, 1.0f);
+    HIP_TRY(c, hipMemcpyAsync(r->work[1], r->work[0], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    const KArgs ra = record_args(c, r->d_prm_replay);
+    int rd = 0;
+    for (int k = 0; k < r->iters; ++k) {
+        launch_replay(c, r, ra, k, r->work[rd], r->work[rd ^ 1]);
+        rd ^= 1;
+    }
+    const float* none = nullptr;
+    hipLaunchKernelGGL(k_tiled_sum_to_lin, dim3(dense_grid(cnt)), dim3(kBlock), 0, c->stream, c->a.g, r->work[rd],
+                       none, r->lin, c->nx, c->ny, c->nz);
+    HIP_TRY(c, hipMemcpyAsync(phi_out, r->lin, cnt * sizeof(float), hipMemcpyDefault, c->stream));
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_bwd(lsseg_ctx* c, lsseg_record* r, const float* dphi, float* dphi0, double* dtheta)
+{
+    lsseg_status st = check_record(c, r);
+    if (st != LSSEG_OK) return st;
+    if (!dphi || !dphi0) return fail(c, LSSEG_ERR_INVALID_ARG, "null argument");
+    const uint64_t N = c->a.g.N, cnt = (uint64_t)c->nx * c->ny * c->nz;
+    const uint32_t dg = dense_grid(N);
+    // Abar_K = dL/dphi (padding 0), Bbar_K = 0
+    HIP_TRY(c, hipMemcpyAsync(r->lin, dphi, cnt * sizeof(float), hipMemcpyDefault, c->stream));
+    hipLaunchKernelGGL(k_lin_to_tiled, dim3(dg), dim3(kBlock), 0, c->stream, c->a.g, r->lin, r->adj[0], c->nx, c->ny,
+                       c->nz, 0.0f);
+    hipLaunchKernelGGL(k_fill_f32, dim3(dg), dim3(kBlock), 0, c->stream, r->adj[1], N, 0.0f);
+    HIP_TRY(c, hipMemsetAsync(r->d_theta, 0, 3 * sizeof(double), c->stream));
+    const KArgs ra = record_args(c, r->d_prm);
+    int ab = 0;  // adj[ab] = Abar, adj[ab ^ 1] = Bbar
+    const int nck = (int)r->ckA.size();
+    for (int sgi = nck - 1; sgi >= 0; --sgi) {
+        const int k0 = sgi * r->every, k1 = std::min(r->iters, k0 + r->every);
+        // recompute the read buffers A_k of the segment from its checkpoint
+        HIP_TRY(c, hipMemcpyAsync(r->work[0], r->ckA[sgi], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(r->work[1], r->ckB[sgi], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        int rd = 0;
+        for (int k = k0; k < k1; ++k) {
+            HIP_TRY(c, hipMemcpyAsync(r->seg + (size_t)(k - k0) * N, r->work[rd], N * sizeof(float),
+                                      hipMemcpyDeviceToDevice, c->stream));
+            if (k + 1 < k1) launch_replay(c, r, ra, k, r->work[rd], r->work[rd ^ 1]);
+            rd ^= 1;
+        }
+        for (int k = k1 - 1; k >= k0; --k) {
+            const uint32_t* V = r->lists + r->off[k];
+            const uint8_t* D = r->dec + r->off[k];
+            uint32_t n = r->n[k];
+            const float* A = r->seg + (size_t)(k - k0) * N;
+            float* X = r->adj[ab];
+            float* Y = r->adj[ab ^ 1];
+            double* th = r->d_theta;
+            void* args[] = {(void*)&ra, (void*)&V, (void*)&D, (void*)&n, (void*)&A, (void*)&X, (void*)&Y, (void*)&th};
+            (void)hipLaunchKernel(k_adjoint_kernel(c->dtype), dim3(list_grid(n)), dim3(kBlock), args, 0, c->stream);
+            ab ^= 1;
+        }
+    }
+    // phi0 fed both buffers: its adjoint is Abar_0 + Bbar_0
+    hipLaunchKernelGGL(k_tiled_sum_to_lin, dim3(dense_grid(cnt)), dim3(kBlock), 0, c->stream, c->a.g, r->adj[0],
+                       r->adj[1], r->lin, c->nx, c->ny, c->nz);
+    HIP_TRY(c, hipMemcpyAsync(dphi0, r->lin, cnt * sizeof(float), hipMemcpyDefault, c->stream));
+    double th[3] = {0, 0, 0};
+    HIP_TRY(c, hipMemcpyAsync(th, r->d_theta, sizeof(th), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (dtheta) memcpy(dtheta, th, sizeof(th));
     return LSSEG_OK;
 }
 

@@ -42,7 +42,7 @@ constexpr int kItems = 8;                     // compaction slots per thread
 constexpr int kScanTile = kBlock * kItems;    // slots per look-back tile
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded look-back wait
 
-enum : uint32_t { kErrSpin = 1u, kErrOverflow = 2u };
+enum : uint32_t { kErrSpin = 1u, kErrOverflow = 2u, kErrRecord = 4u };
 
 enum KMode : int {
     kUpdate = 0,   // update of the current list
@@ -308,9 +308,26 @@ __device__ inline float density_factor(const KArgs& a, const DevParams& P, uint3
     }
 }
 
-// the update of one voxel from its 7-point neighbourhood and density factor f
+// Decisions of one voxel update, recorded by the differentiable form so that a
+// replay takes the same branches: the two clamps, the connectivity gate, a
+// vanishing gradient and the side of the window's absolute value.
+enum : uint32_t {
+    kDecLo1 = 1u,     // u4 + delta <= -1: first clamp at -1
+    kDecHi1 = 2u,     // u4 + delta >= 1: first clamp at +1
+    kDecLo2 = 4u,     // rescaled value <= -1: second clamp at -1
+    kDecHi2 = 8u,     // rescaled value >= 1: second clamp at +1
+    kDecConn = 16u,   // connectivity term applied
+    kDecFlat = 32u,   // |grad phi| == 0
+    kDecBelow = 64u   // float sources: I < T (the branch of |I - T|)
+};
+
+enum DecideMode : int { kDecideNone = 0, kDecideRecord = 1, kDecideReplay = 2 };
+
+// the update of one voxel from its 7-point neighbourhood and density factor f;
+// kDecideRecord stores the decisions in *dec, kDecideReplay takes them from *dec
+template <int kDecide = kDecideNone>
 __device__ inline float update_core(const DevParams& P, float unz, float u1, float u3, float u4, float u5, float u7,
-                                    float upz, float f)
+                                    float upz, float f, uint32_t* dec = nullptr)
 {
     const float dcx = (u5 - u3) / 2.0f;
     const float dcy = (u7 - u1) / 2.0f;
@@ -319,20 +336,44 @@ __device__ inline float update_core(const DevParams& P, float unz, float u1, flo
     const float bx = u4 - u3, by = u4 - u1, bz = u4 - unz;
     const float sdx = fx - bx, sdy = fy - by, sdz = fz - bz;
     float conn = 0.0f;
-    if (P.alphaPositive) {
-        if (fabsf(upz - unz) <= kConnTol && fabsf(unz - u1) <= kConnTol && fabsf(u1 - u3) <= kConnTol &&
-            fabsf(u3 - u5) <= kConnTol && fabsf(u5 - u7) <= kConnTol && fabsf(u7 - upz) <= kConnTol) {
-            const float avg = (upz + u1 + u3 + u5 + u7 + unz) / 6.0f;
-            conn = avg - u4;
-        }
+    bool gate;
+    if constexpr (kDecide == kDecideReplay) {
+        gate = (*dec & kDecConn) != 0;
+    } else {
+        gate = P.alphaPositive && fabsf(upz - unz) <= kConnTol && fabsf(unz - u1) <= kConnTol &&
+               fabsf(u1 - u3) <= kConnTol && fabsf(u3 - u5) <= kConnTol && fabsf(u5 - u7) <= kConnTol &&
+               fabsf(u7 - upz) <= kConnTol;
+    }
+    if (gate) {
+        const float avg = (upz + u1 + u3 + u5 + u7 + unz) / 6.0f;
+        conn = avg - u4;
     }
     const float curv = sdx + sdy + sdz;
     const float dens = P.flipDensity ? f : -f;
     const float speed = (P.oneMinusAlpha * dens) + (P.alpha * curv);
     const float g = __builtin_sqrtf(dcx * dcx + dcy * dcy + dcz * dcz);
     const float delta = ((P.dtAbs * speed * g)) + conn;
-    float nv = fmin_ref(1.0f, fmax_ref(-1.0f, u4 + delta));
-    nv = fmin_ref(1.0f, fmax_ref(-1.0f, ((1.0f + kRescale) * nv) + kRescale));
+    const float x1 = u4 + delta;
+    float c1, nv;
+    if constexpr (kDecide == kDecideReplay) {
+        c1 = (*dec & kDecLo1) ? -1.0f : ((*dec & kDecHi1) ? 1.0f : x1);
+        const float x2 = ((1.0f + kRescale) * c1) + kRescale;
+        nv = (*dec & kDecLo2) ? -1.0f : ((*dec & kDecHi2) ? 1.0f : x2);
+    } else {
+        c1 = fmin_ref(1.0f, fmax_ref(-1.0f, x1));
+        const float x2 = ((1.0f + kRescale) * c1) + kRescale;
+        nv = fmin_ref(1.0f, fmax_ref(-1.0f, x2));
+        if constexpr (kDecide == kDecideRecord) {
+            uint32_t d = 0;
+            if (!(x1 > -1.0f)) d |= kDecLo1;
+            else if (!(x1 < 1.0f)) d |= kDecHi1;
+            if (!(x2 > -1.0f)) d |= kDecLo2;
+            else if (!(x2 < 1.0f)) d |= kDecHi2;
+            if (gate) d |= kDecConn;
+            if (g == 0.0f) d |= kDecFlat;
+            *dec = d;
+        }
+    }
     return nv;
 }
 
@@ -732,6 +773,185 @@ __global__ __launch_bounds__(kBlock) void k_compact_batch(BatchArgs b, int q)
             if (s_pre[v + 1] == s_pre[v])
                 compact_finish<kCompact>(b.vols[v], q, compact_plan<kCompact>(b.vols[v].g, b.ctls[v].n[q]), 0,
                                          b.ctls[v].epoch);
+}
+
+#define LSSEG_FOR_EACH(p, count)                                                           \
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < (count); p += gridDim.x * blockDim.x)
+
+// ---------------------------------------------------------------------------
+// differentiable form.  A recording run stores, per iteration k, the active
+// list V_k and the decisions of every update; a replay runs the update over the
+// recorded lists with the recorded decisions (the same values as the recorded
+// run for the same inputs, and a smooth function of phi0, T, eps and alpha for
+// perturbed ones); the adjoint walks the iterations backwards.
+//
+// One iteration maps the buffer pair (A, B) (read, write) to
+//   A'(v) = f_v(A) for v in V_k,  A'(v) = B(v) otherwise,  B' = A,
+// so with adjoints (Abar', Bbar') of the outputs
+//   Abar = Bbar' + sum_{v in V_k} J_v^T Abar'(v),  Bbar = Abar' outside V_k (0 on V_k).
+// k_adjoint reads and clears X = Abar' on V_k (leaving Bbar) and scatters the
+// J_v^T terms into Y = Bbar' (leaving Abar).
+// ---------------------------------------------------------------------------
+template <typename SrcT, bool kFloatSrc>
+__global__ __launch_bounds__(kBlock) void k_record(KArgs a, int q, uint32_t n, uint32_t* __restrict__ lists,
+                                                   uint8_t* __restrict__ dec)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.ctl->n[q] != n) atomicOr(&a.ctl->error, kErrRecord);
+    const DevParams P = *a.prm;
+    const uint32_t* __restrict__ V = a.list[q];
+    const float* __restrict__ R = a.phi[q];
+    LSSEG_FOR_EACH(p, n)
+    {
+        const uint32_t u = V[p];
+        uint32_t nb[7], inb, d = 0;
+        neighbors(a.g, u, nb, &inb);
+        (void)update_core<kDecideRecord>(P, R[nb[0]], R[nb[1]], R[nb[2]], R[u], R[nb[4]], R[nb[5]], R[nb[6]],
+                                         density_factor<SrcT, kFloatSrc>(a, P, u), &d);
+        if constexpr (kFloatSrc)
+            if (static_cast<const float*>(a.src)[u] - P.Tf < 0.0f) d |= kDecBelow;
+        lists[p] = u;
+        dec[p] = (uint8_t)d;
+    }
+}
+
+template <typename SrcT, bool kFloatSrc>
+__global__ __launch_bounds__(kBlock) void k_replay(KArgs a, const uint32_t* __restrict__ V,
+                                                   const uint8_t* __restrict__ dec, uint32_t n,
+                                                   const float* __restrict__ R, float* __restrict__ W)
+{
+    const DevParams P = *a.prm;
+    LSSEG_FOR_EACH(p, n)
+    {
+        const uint32_t u = V[p];
+        uint32_t nb[7], inb, d = dec[p];
+        neighbors(a.g, u, nb, &inb);
+        W[u] = update_core<kDecideReplay>(P, R[nb[0]], R[nb[1]], R[nb[2]], R[u], R[nb[4]], R[nb[5]], R[nb[6]],
+                                          density_replay<SrcT, kFloatSrc>(a, P, u, d), &d);
+    }
+}
+
+// density factor with the recorded side of |I - T| (float sources); the same
+// value as density_factor at the recorded inputs, since T - I == -(I - T)
+template <typename SrcT, bool kFloatSrc>
+__device__ inline float density_replay(const KArgs& a, const DevParams& P, uint32_t u, uint32_t d)
+{
+    if constexpr (kFloatSrc) {
+        const float di = static_cast<const float*>(a.src)[u] - P.Tf;
+        return P.negInv * ((d & kDecBelow) ? -di : di) + 1.0f;
+    } else {
+        return density_factor<SrcT, kFloatSrc>(a, P, u);
+    }
+}
+
+// block-wide sum of three values into theta[0..2] (double atomics); every thread calls it
+__device__ inline void block_add3(double* theta, double v0, double v1, double v2, double* s_red)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v0 += __shfl_xor(v0, o, 64);
+        v1 += __shfl_xor(v1, o, 64);
+        v2 += __shfl_xor(v2, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_red[(threadIdx.x >> 6) * 3 + 0] = v0;
+        s_red[(threadIdx.x >> 6) * 3 + 1] = v1;
+        s_red[(threadIdx.x >> 6) * 3 + 2] = v2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) t += s_red[w * 3 + threadIdx.x];
+        if (t != 0.0) atomicAdd(&theta[threadIdx.x], t);
+    }
+}
+
+// Adjoint of iteration k: A holds the read buffer A_k; X = Abar_{k+1} (cleared
+// on V_k), Y = Bbar_{k+1} (receives the J^T terms); theta[0..2] accumulates
+// dL/dT, dL/deps, dL/dalpha (T and eps only for float sources: integer sources
+// truncate them, Segmenter.cpp:817-818, and have zero derivative).
+template <typename SrcT, bool kFloatSrc>
+__global__ __launch_bounds__(kBlock) void k_adjoint(KArgs a, const uint32_t* __restrict__ V,
+                                                    const uint8_t* __restrict__ dec, uint32_t n,
+                                                    const float* __restrict__ A, float* __restrict__ X,
+                                                    float* __restrict__ Y, double* theta)
+{
+    __shared__ double s_red[3 * kBlock / 64];
+    const DevParams P = *a.prm;
+    const float dd = P.flipDensity ? 1.0f : -1.0f;
+    float tT = 0.0f, tE = 0.0f, tA = 0.0f;
+    LSSEG_FOR_EACH(p, n)
+    {
+        const uint32_t u = V[p];
+        const uint32_t d = dec[p];
+        const float gbar = X[u];
+        X[u] = 0.0f;
+        if ((d & (kDecLo1 | kDecHi1 | kDecLo2 | kDecHi2)) || gbar == 0.0f) continue;
+        const float s = gbar * (1.0f + kRescale);  // through both clamps, inside
+        uint32_t nb[7], inb;
+        neighbors(a.g, u, nb, &inb);
+        const float unz = A[nb[0]], u1 = A[nb[1]], u3 = A[nb[2]], u4 = A[u], u5 = A[nb[4]], u7 = A[nb[5]],
+                    upz = A[nb[6]];
+        const float dcx = (u5 - u3) / 2.0f, dcy = (u7 - u1) / 2.0f, dcz = (upz - unz) / 2.0f;
+        const float curv = ((u5 - u4) - (u4 - u3)) + ((u7 - u4) - (u4 - u1)) + ((upz - u4) - (u4 - unz));
+        const float g = __builtin_sqrtf(dcx * dcx + dcy * dcy + dcz * dcz);
+        const float f = density_replay<SrcT, kFloatSrc>(a, P, u, d);
+        const float dens = dd * f;
+        const float speed = P.oneMinusAlpha * dens + P.alpha * curv;
+        const float sp = s * P.dtAbs * g;      // dL/dspeed
+        const float gs = s * P.dtAbs * speed;  // dL/d|grad phi|
+        const float gc = sp * P.alpha;         // dL/dcurvature
+        const float cn = (d & kDecConn) ? s / 6.0f : 0.0f;
+        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        if (!(d & kDecFlat) && g > 0.0f) {
+            const float k = 0.5f * gs / g;
+            gx = k * dcx;
+            gy = k * dcy;
+            gz = k * dcz;
+        }
+        atomicAdd(&Y[nb[0]], gc + cn - gz);
+        atomicAdd(&Y[nb[1]], gc + cn - gy);
+        atomicAdd(&Y[nb[2]], gc + cn - gx);
+        atomicAdd(&Y[u], s - 6.0f * gc - ((d & kDecConn) ? s : 0.0f));
+        atomicAdd(&Y[nb[4]], gc + cn + gx);
+        atomicAdd(&Y[nb[5]], gc + cn + gy);
+        atomicAdd(&Y[nb[6]], gc + cn + gz);
+        tA += sp * (curv - dens);
+        if constexpr (kFloatSrc) {
+            // |I - T| on its recorded side: sgn (I - T)
+            const float di = static_cast<const float*>(a.src)[u] - P.Tf;
+            const float sgn = (d & kDecBelow) ? -1.0f : 1.0f;
+            const float dspeed = sp * P.oneMinusAlpha * dd;
+            tT += dspeed * (-P.negInv * sgn);
+            tE += dspeed * (P.negInv * P.negInv * (sgn * di));
+        }
+    }
+    block_add3(theta, tT, tE, tA, s_red);
+}
+
+// linear unpadded <-> tiled padded
+__global__ void k_lin_to_tiled(Geom g, const float* __restrict__ lin, float* __restrict__ tiled, uint32_t nx,
+                               uint32_t ny, uint32_t nz, float pad)
+{
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < g.N;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t x, y, z;
+        decode_xyz(g, (uint32_t)t, &x, &y, &z);
+        tiled[t] = (x < nx && y < ny && z < nz) ? lin[((uint64_t)z * ny + y) * nx + x] : pad;
+    }
+}
+
+__global__ void k_tiled_sum_to_lin(Geom g, const float* __restrict__ a, const float* __restrict__ b,
+                                   float* __restrict__ lin, uint32_t nx, uint32_t ny, uint32_t nz)
+{
+    const uint64_t cnt = (uint64_t)nx * ny * nz;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cnt;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)(i % nx);
+        const uint32_t y = (uint32_t)((i / nx) % ny);
+        const uint32_t z = (uint32_t)(i / ((uint64_t)nx * ny));
+        const uint32_t t = tau_of(g, x, y, z);
+        lin[i] = b ? a[t] + b[t] : a[t];
+    }
 }
 
 // ---------------------------------------------------------------------------

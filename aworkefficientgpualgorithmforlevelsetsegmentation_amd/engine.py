@@ -34,6 +34,8 @@ EXPORTS = (
     "lsseg_batch_create", "lsseg_batch_destroy", "lsseg_batch_last_error", "lsseg_batch_count",
     "lsseg_batch_volume", "lsseg_batch_iterate", "lsseg_batch_checkpoint_save", "lsseg_batch_checkpoint_restore",
     "lsseg_batch_get_tuning", "lsseg_batch_set_tuning", "lsseg_batch_autotune",
+    "lsseg_fwd_record", "lsseg_record_destroy", "lsseg_record_info", "lsseg_record_get", "lsseg_fwd_replay",
+    "lsseg_bwd",
 )
 
 
@@ -80,6 +82,26 @@ class KernelTimes(ctypes.Structure):
 
 
 _lib = None
+_live = None
+
+
+def _track(obj):
+    """Closes engine objects still alive at interpreter exit, before the HIP
+    runtime shuts down."""
+    global _live
+    if _live is None:
+        import atexit
+        import weakref
+        _live = weakref.WeakSet()
+
+        def _close_all():
+            for o in list(_live):
+                try:
+                    o.close()
+                except Exception:
+                    pass
+        atexit.register(_close_all)
+    _live.add(obj)
 
 
 def library_path() -> str:
@@ -127,6 +149,12 @@ def lib():
             "lsseg_batch_get_tuning": (i32, [vp, P(Tuning)]),
             "lsseg_batch_set_tuning": (i32, [vp, P(Tuning)]),
             "lsseg_batch_autotune": (i32, [vp, P(Params), i32, P(Tuning)]),
+            "lsseg_fwd_record": (i32, [vp, P(Params), i32, i32, P(vp)]),
+            "lsseg_record_destroy": (i32, [vp]),
+            "lsseg_record_info": (i32, [vp, P(i32), P(i32), P(u64)]),
+            "lsseg_record_get": (i32, [vp, vp, vp, vp]),
+            "lsseg_fwd_replay": (i32, [vp, vp, vp, P(Params), vp]),
+            "lsseg_bwd": (i32, [vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -169,6 +197,7 @@ class Context:
         self._h = h
         self._owned = True
         self._dims()
+        _track(self)
 
     @classmethod
     def _borrow(cls, handle, shape_zyx, dtype, owner):
@@ -217,6 +246,21 @@ class Context:
         if phi.shape != (self.nz, self.ny, self.nx):
             raise ValueError("phi0 shape mismatch")
         self._check(lib().lsseg_set_phi(self._h, _p(phi), band))
+
+    def set_phi_ptr(self, address: int, band: float = 1.0):
+        """set_phi from raw memory (host or device) holding nz*ny*nx float32."""
+        self._check(lib().lsseg_set_phi(self._h, ctypes.c_void_p(address), band))
+
+    def phi_into_ptr(self, address: int, which: int = 0):
+        """phi (unpadded float32) into raw memory (host or device)."""
+        self._check(lib().lsseg_get_phi(self._h, which, ctypes.c_void_p(address)))
+
+    def fwd_record(self, T, eps, alpha, dt, iters, checkpoint_every=20, mode=MODE_REF) -> "Record":
+        prm = Params(T, eps, alpha, dt, mode)
+        h = ctypes.c_void_p()
+        self._check(lib().lsseg_fwd_record(self._h, ctypes.byref(prm), int(iters), int(checkpoint_every),
+                                           ctypes.byref(h)))
+        return Record(self, h)
 
     def seed_sphere(self, cx: float, cy: float, cz: float, radius: int):
         self._check(lib().lsseg_seed_sphere(self._h, cx, cy, cz, int(radius)))
@@ -337,6 +381,7 @@ class Batch:
             raise LssegError(st, L.lsseg_batch_last_error(None).decode())
         self._h = h
         self._vols = []
+        _track(self)
         for i, v in enumerate(vols):
             c = ctypes.c_void_p()
             self._check(L.lsseg_batch_volume(self._h, i, ctypes.byref(c)))
@@ -402,3 +447,58 @@ class Batch:
         t = Tuning()
         self._check(lib().lsseg_batch_autotune(self._h, self._params(params), int(iters), ctypes.byref(t)))
         return t
+
+
+class Record:
+    """A recorded run of the differentiable form (``lsseg_record``)."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self._h = handle
+        _track(self)
+        it, ev, tot = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_uint64()
+        ctx._check(lib().lsseg_record_info(self._h, ctypes.byref(it), ctypes.byref(ev), ctypes.byref(tot)))
+        self.iters, self.checkpoint_every, self.total_active = it.value, ev.value, tot.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lsseg_record_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def lists(self):
+        """(n per iteration, concatenated lists, decision bytes) on the host."""
+        n = np.zeros(self.iters, np.uint32)
+        lst = np.zeros(max(self.total_active, 1), np.uint32)
+        dec = np.zeros(max(self.total_active, 1), np.uint8)
+        self.ctx._check(lib().lsseg_record_get(self._h, _p(n), _p(lst), _p(dec)))
+        return n, lst[:self.total_active], dec[:self.total_active]
+
+    def replay_ptr(self, phi0_address: int, T, eps, alpha, dt, out_address: int, mode=MODE_REF):
+        prm = Params(T, eps, alpha, dt, mode)
+        self.ctx._check(lib().lsseg_fwd_replay(self.ctx._h, self._h, ctypes.c_void_p(phi0_address),
+                                               ctypes.byref(prm), ctypes.c_void_p(out_address)))
+
+    def replay(self, phi0_zyx: np.ndarray, T, eps, alpha, dt, mode=MODE_REF) -> np.ndarray:
+        phi = np.ascontiguousarray(phi0_zyx, dtype=np.float32)
+        out = np.zeros_like(phi)
+        self.replay_ptr(phi.ctypes.data, T, eps, alpha, dt, out.ctypes.data, mode)
+        return out
+
+    def bwd_ptr(self, dphi_address: int, dphi0_address: int) -> np.ndarray:
+        th = np.zeros(3, np.float64)
+        self.ctx._check(lib().lsseg_bwd(self.ctx._h, self._h, ctypes.c_void_p(dphi_address),
+                                        ctypes.c_void_p(dphi0_address), _p(th)))
+        return th
+
+    def bwd(self, dphi_zyx: np.ndarray):
+        """(dL/dphi0, [dL/dT, dL/deps, dL/dalpha]) for dL/dphi of the final phi."""
+        g = np.ascontiguousarray(dphi_zyx, dtype=np.float32)
+        out = np.zeros_like(g)
+        th = self.bwd_ptr(g.ctypes.data, out.ctypes.data)
+        return out, th

@@ -39,6 +39,7 @@ extern "C" {
 
 typedef struct lsseg_ctx lsseg_ctx;
 typedef struct lsseg_batch lsseg_batch;
+typedef struct lsseg_record lsseg_record;
 
 typedef enum lsseg_status {
     LSSEG_OK = 0,
@@ -121,7 +122,7 @@ lsseg_status lsseg_get_dims(const lsseg_ctx* ctx, uint32_t* X, uint32_t* Y, uint
    (InitializeActiveElementsVolume.cu:37-74) in ascending tiled order. */
 lsseg_status lsseg_seed_sphere(lsseg_ctx* ctx, float cx, float cy, float cz, uint32_t radius);
 
-/* Injects phi0 (unpadded, linear) into both phi buffers, export := floor(127
+/* Injects phi0 (unpadded, linear; host or device memory) into both phi buffers, export := floor(127
    phi0) saturated, padded voxels := 1.0; active list := {|phi0| < band} in
    ascending tiled order.  No reference counterpart: the benchmark
    configurations start from a signed-distance phi0. */
@@ -163,8 +164,8 @@ lsseg_status lsseg_get_active_count(const lsseg_ctx* ctx, uint64_t* n);
    Returns LSSEG_ERR_CAPACITY (after copying cap entries) if cap < n. */
 lsseg_status lsseg_get_active(const lsseg_ctx* ctx, uint32_t* out, uint64_t cap, uint64_t* n);
 
-/* phi over the unpadded box; which = 0: the read buffer (latest values at
-   active voxels), which = 1: the other buffer. */
+/* phi over the unpadded box into out (host or device memory); which = 0: the
+   read buffer (latest values at active voxels), which = 1: the other buffer. */
 lsseg_status lsseg_get_phi(const lsseg_ctx* ctx, int32_t which, float* out);
 
 /* Replaces Segmenter::GetSaveableSegmentation (Segmenter.cpp:1216-1286) plus
@@ -278,6 +279,50 @@ lsseg_status lsseg_batch_set_tuning(lsseg_batch* batch, const lsseg_tuning* tuni
    checkpoint slot LSSEG_MAX_CHECKPOINTS - 1 of every volume. */
 lsseg_status lsseg_batch_autotune(lsseg_batch* batch, const lsseg_params* params, int32_t iters,
                                   lsseg_tuning* out);
+
+/* ---- differentiable form -------------------------------------------------
+
+   No reference counterpart.  The adjoint of the update lets a caller tune T,
+   eps, alpha and phi0 against a loss on the final phi.  The active set and the
+   branch decisions of every update (the two clamps, the connectivity gate, a
+   vanishing gradient, the side of |I - T|) are recorded per iteration and held
+   fixed: the map from
+   (phi0, T, eps, alpha) to the final phi is then smooth, lsseg_fwd_replay
+   evaluates it for any inputs, and lsseg_bwd is its exact adjoint (up to FP32
+   rounding).  phi is checkpointed every `checkpoint_every` iterations and
+   recomputed segment by segment in reverse.  For integer sources T and eps are
+   truncated (Segmenter.cpp:817-818), so their derivatives are zero.
+
+   Pointers marked "host or device" may point to either (unified addressing);
+   volumes are linear, x fastest, unpadded. */
+
+/* Runs `iters` iterations of lsseg_iterate from the current state (which
+   lsseg_set_phi normally made: both buffers = phi0) and records them.  The
+   context ends in the same state as after lsseg_iterate.  Uses checkpoint slot
+   LSSEG_MAX_CHECKPOINTS - 1. */
+lsseg_status lsseg_fwd_record(lsseg_ctx* ctx, const lsseg_params* params, int32_t iters,
+                              int32_t checkpoint_every, lsseg_record** out);
+/* A record may outlive its context: destroying the context detaches the
+   record, which then refuses every call but lsseg_record_destroy. */
+lsseg_status lsseg_record_destroy(lsseg_record* rec);
+lsseg_status lsseg_record_info(const lsseg_record* rec, int32_t* iters, int32_t* checkpoint_every,
+                               uint64_t* total_active);
+/* Recorded n_k (iters entries), the concatenated lists V_0..V_{iters-1} and one
+   decision byte per list entry (total_active entries each; host or device). */
+lsseg_status lsseg_record_get(const lsseg_record* rec, uint32_t* n_per_iter, uint32_t* lists,
+                              uint8_t* decisions);
+
+/* The recorded map at other inputs: both buffers := phi0 (host or device),
+   `iters` updates over the recorded lists with the recorded decisions and
+   `params`; the final phi goes to phi_out (host or device).  The context's own
+   state is not touched. */
+lsseg_status lsseg_fwd_replay(lsseg_ctx* ctx, lsseg_record* rec, const float* phi0,
+                              const lsseg_params* params, float* phi_out);
+
+/* Adjoint: given dL/dphi of the final phi (host or device), writes dL/dphi0
+   (host or device) and dtheta[0..2] = dL/dT, dL/deps, dL/dalpha (host). */
+lsseg_status lsseg_bwd(lsseg_ctx* ctx, lsseg_record* rec, const float* dphi, float* dphi0,
+                       double* dtheta);
 
 #ifdef __cplusplus
 }

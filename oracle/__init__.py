@@ -278,3 +278,99 @@ def add_export(result: np.ndarray, add: np.ndarray) -> np.ndarray:
     a = np.ascontiguousarray(add, dtype=np.int8)
     lib().lso_add_export(_p(r), _p(a), r.size)
     return r
+
+
+# ---- differentiable form (oracle/lsoracle_adjoint.c) -------------------------
+def _adjoint_lib():
+    L = lib()
+    if not getattr(L, "_adjoint_declared", False):
+        vp, c_i32, c_i64, c_f, c_d = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_float,
+                                      ctypes.c_double)
+        L.lso_decisions_f32.restype = None
+        L.lso_decisions_f32.argtypes = [c_i32, c_i32, c_i32, c_i32, vp, vp, c_f, c_f, c_f, c_f, vp, c_i64, vp]
+        L.lso_replay_f64.restype = None
+        L.lso_replay_f64.argtypes = [c_i32, c_i32, c_i32, c_i32, vp, c_d, c_d, c_d, c_d, c_i32, vp, vp, vp, vp,
+                                     vp, vp]
+        L.lso_adjoint_f64.restype = None
+        L.lso_adjoint_f64.argtypes = [c_i32, c_i32, c_i32, c_i32, vp, c_d, c_d, c_d, c_d, c_i32, vp, vp, vp, vp,
+                                      vp, vp, vp]
+        L._adjoint_declared = True
+    return L
+
+
+def padded_tiled_source(vol_zyx: np.ndarray) -> np.ndarray:
+    """The source padded to multiples of 16 (signed integer types pad with bytes
+    0x81, everything else with zero bytes, Segmenter.cpp:1131-1204) and tiled."""
+    vol = np.ascontiguousarray(vol_zyx)
+    nz, ny, nx = vol.shape
+    X, Y, Z = padded_dims(nx, ny, nz)
+    signed_int = vol.dtype in (np.dtype(np.int8), np.dtype(np.int16), np.dtype(np.int32))
+    raw = np.full((Z, Y, X * vol.itemsize), 0x81 if signed_int else 0, np.uint8)
+    raw[:nz, :ny, :nx * vol.itemsize] = vol.view(np.uint8).reshape(nz, ny, nx * vol.itemsize)
+    return tile_volume(raw.view(vol.dtype).reshape(Z, Y, X))
+
+
+def pad_tile_phi(phi_zyx: np.ndarray, pad: float, dtype=np.float64) -> np.ndarray:
+    nz, ny, nx = phi_zyx.shape
+    X, Y, Z = padded_dims(nx, ny, nz)
+    full = np.full((Z, Y, X), pad, dtype)
+    full[:nz, :ny, :nx] = phi_zyx
+    return tile_volume(full)
+
+
+def unpad_untile(flat: np.ndarray, nx: int, ny: int, nz: int) -> np.ndarray:
+    X, Y, Z = padded_dims(nx, ny, nz)
+    return untile_volume(flat, X, Y, Z)[:nz, :ny, :nx]
+
+
+def decisions_f32(X, Y, Z, src_tiled, R, T, eps, alpha, dt, V):
+    L = _adjoint_lib()
+    V = np.ascontiguousarray(V, np.uint32)
+    dec = np.zeros(max(len(V), 1), np.uint8)
+    L.lso_decisions_f32(X, Y, Z, DTYPE_CODE[src_tiled.dtype], _p(src_tiled), _p(np.ascontiguousarray(R, np.float32)),
+                        T, eps, alpha, dt, _p(V), len(V), _p(dec))
+    return dec[:len(V)]
+
+
+def record_f32(vol_zyx, phi0_zyx, band, T, eps, alpha, dt, iters, mode=MODE_REF):
+    """The oracle's FP32 run with, per iteration, the list and the decisions of
+    every update: (n per iteration, concatenated lists, decision bytes, final State)."""
+    st = State(vol_zyx)
+    st.set_phi(phi0_zyx, band)
+    src = padded_tiled_source(vol_zyx)
+    ns, lists, decs = [], [], []
+    for _ in range(iters):
+        V = st.active()
+        R = st.state()[0]
+        ns.append(len(V))
+        lists.append(V)
+        decs.append(decisions_f32(st.X, st.Y, st.Z, src, R, T, eps, alpha, dt, V))
+        st.iterate(T, eps, alpha, dt, 1, mode=mode)
+    cat = (lambda a, t: np.concatenate(a).astype(t) if a else np.zeros(0, t))
+    return np.array(ns, np.uint32), cat(lists, np.uint32), cat(decs, np.uint8), st
+
+
+def replay_f64(X, Y, Z, src_tiled, T, eps, alpha, dt, n, lists, dec, phi0_tiled, keep_history=False):
+    L = _adjoint_lib()
+    N = X * Y * Z
+    n = np.ascontiguousarray(n, np.uint32)
+    lists = np.ascontiguousarray(lists, np.uint32)
+    dec = np.ascontiguousarray(dec, np.uint8)
+    phi0 = np.ascontiguousarray(phi0_tiled, np.float64)
+    out = np.zeros(N, np.float64)
+    hist = np.zeros((len(n), N), np.float64) if keep_history else None
+    L.lso_replay_f64(X, Y, Z, DTYPE_CODE[src_tiled.dtype], _p(src_tiled), T, eps, alpha, dt, len(n), _p(n),
+                     _p(lists), _p(dec), _p(phi0), _p(out), _p(hist) if keep_history else None)
+    return out, hist
+
+
+def adjoint_f64(X, Y, Z, src_tiled, T, eps, alpha, dt, n, lists, dec, A_hist, dphiK_tiled):
+    L = _adjoint_lib()
+    N = X * Y * Z
+    dphi0 = np.zeros(N, np.float64)
+    th = np.zeros(3, np.float64)
+    L.lso_adjoint_f64(X, Y, Z, DTYPE_CODE[src_tiled.dtype], _p(src_tiled), T, eps, alpha, dt, len(n),
+                      _p(np.ascontiguousarray(n, np.uint32)), _p(np.ascontiguousarray(lists, np.uint32)),
+                      _p(np.ascontiguousarray(dec, np.uint8)), _p(np.ascontiguousarray(A_hist, np.float64)),
+                      _p(np.ascontiguousarray(dphiK_tiled, np.float64)), _p(dphi0), _p(th))
+    return dphi0, th

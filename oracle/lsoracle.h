@@ -103,6 +103,16 @@ void       lso_clear(lso_state* s, int32_t all);
 /* union of two export volumes (InitializeLevelSetVolume.cu:151-169) */
 void       lso_add_export(int8_t* result, const int8_t* add, int64_t N);
 
+/* ---- differentiable form (lsoracle_adjoint.c) ----------------------------- */
+void lso_decisions_f32(int32_t X, int32_t Y, int32_t Z, int32_t dtype, const void* src, const float* R,
+                       float T, float eps, float alpha, float dt, const uint32_t* V, int64_t n, uint8_t* dec);
+void lso_replay_f64(int32_t X, int32_t Y, int32_t Z, int32_t dtype, const void* src, double T, double eps,
+                    double alpha, double dt, int32_t iters, const uint32_t* n, const uint32_t* lists,
+                    const uint8_t* dec, const double* phi0, double* phi_out, double* A_hist);
+void lso_adjoint_f64(int32_t X, int32_t Y, int32_t Z, int32_t dtype, const void* src, double T, double eps,
+                     double alpha, double dt, int32_t iters, const uint32_t* n, const uint32_t* lists,
+                     const uint8_t* dec, const double* A_hist, const double* dphiK, double* dphi0, double* dtheta);
+
 #ifdef __cplusplus
 }
 #endif
