@@ -262,6 +262,122 @@ def batch_vs_sequential(steps, iterations):
     return out
 
 
+def _soft_dice_grad(phi, mask, temp=0.5):
+    """1 - softDice(sigmoid(-phi / temp), mask) and its gradient in phi (float64, host)."""
+    phi = phi.astype(np.float64)
+    m = mask.astype(np.float64)
+    s = 1.0 / (1.0 + np.exp(phi / temp))
+    a = (s * m).sum()
+    b = s.sum() + m.sum()
+    return 1.0 - 2.0 * a / b, (-2.0 * (m * b - a) / (b * b)) * (-s * (1.0 - s) / temp)
+
+
+def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
+    """The differentiable form on the 16-volume 96^3 float workload: forward
+    record + adjoint against the plain forward, gradients against central
+    finite differences of the recorded map on every volume (five-point stencil
+    at h = 1e-3, and its Richardson extrapolation from h and h/2), and against
+    the FP64 host adjoint on one 64^3 instance.  Over 200 iterations the map is
+    nonlinear enough that the h = 1e-3 difference carries ~1e-2 truncation
+    error even in exact arithmetic; the extrapolation removes most of it."""
+    from aworkefficientgpualgorithmforlevelsetsegmentation_amd import synth
+    from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import Context
+    ws = synth.adjoint(iterations=iterations) if iterations else synth.adjoint()
+    K = ws[0].iterations
+    t_plain = t_adj = 0.0
+    upd = 0
+    worst = worst_dir = worst_r = worst_dir_r = 0.0
+    for i, w in enumerate(ws):
+        T, eps, alpha = w.T, w.eps, w.alpha
+        ctx = Context(w.volume)
+        ctx.set_phi(w.phi0, w.band)
+        ctx.checkpoint_save(0)
+        ctx.iterate(T, eps, alpha, w.dt, K)  # warm-up
+        ctx.checkpoint_restore(0)
+        device_sync()
+        t0 = time.perf_counter()
+        st = ctx.iterate(T, eps, alpha, w.dt, K)
+        device_sync()
+        t_plain += time.perf_counter() - t0
+        upd += int(st.sum_active)
+        ctx.checkpoint_restore(0)
+        device_sync()
+        t0 = time.perf_counter()
+        rec = ctx.fwd_record(T, eps, alpha, w.dt, K, checkpoint_every=every)
+        loss, dphi = _soft_dice_grad(ctx.phi(0), w.truth)
+        g0, th = rec.bwd(dphi.astype(np.float32))
+        device_sync()
+        t_adj += time.perf_counter() - t0
+
+        def L(p, t, e, a):
+            return _soft_dice_grad(rec.replay(p, t, e, a, w.dt), w.truth)[0]
+
+        def central(f, h):
+            return (-f(2 * h) + 8 * f(h) - 8 * f(-h) + f(-2 * h)) / (12 * h)
+
+        def rel(a, b):
+            return abs(a - b) / max(abs(b), 1e-12)
+
+        base = [T, eps, alpha]
+        for k in range(3):
+            def f(step, k=k):
+                v = list(base)
+                v[k] += step
+                return L(w.phi0, *v)
+            d1, d2 = central(f, fd_h), central(f, fd_h / 2)
+            worst = max(worst, rel(th[k], d1))
+            worst_r = max(worst_r, rel(th[k], (16 * d2 - d1) / 15))
+        d = (g0 / max(np.linalg.norm(g0), 1e-30)).astype(np.float32)
+        adj_d = float((g0.astype(np.float64) * d).sum())
+        fdir = (lambda step: L(w.phi0 + step * d, *base))
+        d1, d2 = central(fdir, fd_h), central(fdir, fd_h / 2)
+        worst_dir = max(worst_dir, rel(adj_d, d1))
+        worst_dir_r = max(worst_dir_r, rel(adj_d, (16 * d2 - d1) / 15))
+        rec.close()
+        ctx.close()
+    # FP64 host adjoint of the same recorded map on one 64^3 instance
+    import oracle as O
+    w = synth.adjoint(count=1, edge=64, iterations=K)[0]
+    ctx = Context(w.volume)
+    ctx.set_phi(w.phi0, w.band)
+    rec = ctx.fwd_record(w.T, w.eps, w.alpha, w.dt, K, checkpoint_every=every)
+    _, dphi = _soft_dice_grad(ctx.phi(0), w.truth)
+    g0, th = rec.bwd(dphi.astype(np.float32))
+    n, lists, dec = rec.lists()
+    nz, ny, nx = w.volume.shape
+    X, Y, Z = O.padded_dims(nx, ny, nz)
+    src = O.padded_tiled_source(w.volume)
+    out64, hist = O.replay_f64(X, Y, Z, src, w.T, w.eps, w.alpha, w.dt, n, lists, dec, O.pad_tile_phi(w.phi0, 1.0),
+                               keep_history=True)
+    _, dphi64 = _soft_dice_grad(O.unpad_untile(out64, nx, ny, nz), w.truth)
+    g64, th64 = O.adjoint_f64(X, Y, Z, src, w.T, w.eps, w.alpha, w.dt, n, lists, dec, hist,
+                              O.pad_tile_phi(dphi64, 0.0))
+    g64 = O.unpad_untile(g64, nx, ny, nz)
+    fp64_theta = float(max(abs(th[k] - th64[k]) / max(abs(th64[k]), 1e-12) for k in range(3)))
+    fp64_phi0 = float(np.linalg.norm(g0 - g64) / max(np.linalg.norm(g64), 1e-30))
+    rec.close()
+    ctx.close()
+    del hist
+    out = {
+        "workload": f"{len(ws)} float32 volumes 96^3 (ellipsoids / 255), K={K} iterations, dt=0.5, T=0.6 eps=0.15 "
+                    f"alpha=0.1, phi checkpointed every {every}; loss 1 - softDice(sigmoid(-phi/0.5), mask)",
+        "plain_forward_ms": 1e3 * t_plain, "forward_record_plus_adjoint_ms": 1e3 * t_adj,
+        "adjoint_over_plain": t_adj / t_plain if t_plain > 0 else None,
+        "updates": upd, "value": 2 * upd / t_adj if t_adj > 0 else 0.0,
+        "unit": "active-voxel updates/s (forward + adjoint, each counted once)",
+        "fd_max_rel_err_theta": worst, "fd_max_rel_err_phi0_direction": worst_dir,
+        "fd_richardson_max_rel_err_theta": worst_r, "fd_richardson_max_rel_err_phi0_direction": worst_dir_r,
+        "fd": "central five-point stencil at h = 1e-3, and (16 D(h/2) - D(h)) / 15, on the recorded map, "
+              "all volumes; phi0 along the normalised gradient",
+        "fp64_64cubed_max_rel_err_theta": fp64_theta, "fp64_64cubed_rel_l2_err_phi0": fp64_phi0,
+    }
+    log(f"adjoint: plain forward {1e3 * t_plain:.1f} ms, forward record + adjoint {1e3 * t_adj:.1f} ms "
+        f"({out['adjoint_over_plain']:.2f}x); FD (h=1e-3) max rel err theta {worst:.2e}, phi0 dir {worst_dir:.2e}; "
+        f"extrapolated: theta {worst_r:.2e}, phi0 dir {worst_dir_r:.2e}; "
+        f"FP64 64^3: theta {fp64_theta:.2e}, phi0 {fp64_phi0:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +388,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-sequential comparison")
     ap.add_argument("--batch-iterations", type=int, default=0, help="override the batch workload's iterations")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-form comparison")
+    ap.add_argument("--adjoint-iterations", type=int, default=0, help="override the adjoint workload's iterations")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="single-core baseline sample budget")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (HBM counter calibration)")
@@ -387,6 +505,9 @@ def main():
     batched = None
     if not args.no_batch:
         batched = batch_vs_sequential(max(1, min(args.steps, 3)), args.batch_iterations)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_vs_plain(args.adjoint_iterations)
 
     cpu = None
     dice = None
@@ -427,6 +548,7 @@ def main():
         "kernels": kernels,
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
         "dice_vs_oracle": dice,
         "mask_identical_to_oracle": identical,
     }
