@@ -1,8 +1,9 @@
 """Build recipes for the native parts of the package (in-tree outputs).
 
 * ``liblsseg_hip.so`` -- the HIP engine and its C ABI (``csrc/lsseg.hip``) plus
-  the C++ ``lsseg::Segmenter`` host class (``csrc/segmenter.cpp``), compiled
-  by ``hipcc`` for gfx950;
+  the C++ ``lsseg::Segmenter`` host class (``csrc/segmenter.cpp``) and the
+  reference's file formats (``csrc/formats.cpp``), compiled by ``hipcc`` for
+  gfx950;
 * ``_synth.so`` -- the seeded synthetic-volume generators (``csrc/synth.c``),
   compiled by ``gcc``;
 * ``segmenter_demo`` -- a small C++ program that drives ``lsseg::Segmenter``
@@ -58,7 +59,8 @@ def _atomic_out(path: str) -> str:
 
 def build_hip() -> str:
     out = _atomic_out(HIP_LIB)
-    srcs = [os.path.join(CSRC, "lsseg.hip"), os.path.join(CSRC, "segmenter.cpp")]  # + lsseg_device.hpp
+    srcs = [os.path.join(CSRC, "lsseg.hip"), os.path.join(CSRC, "segmenter.cpp"),
+            os.path.join(CSRC, "formats.cpp")]  # + lsseg_device.hpp
     _run([_hipcc(), *HIPCC_FLAGS, "-shared", f"-I{INCLUDE}", "-o", out, *srcs])
     os.replace(out, HIP_LIB)
     return HIP_LIB

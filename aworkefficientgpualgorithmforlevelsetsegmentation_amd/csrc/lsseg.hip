@@ -762,6 +762,47 @@ lsseg_status lsseg_get_mask(const lsseg_ctx* c, uint8_t* out)
     return LSSEG_OK;
 }
 
+lsseg_status lsseg_auto_parameters(lsseg_ctx* c, float* T, float* eps, uint64_t* count)
+{
+    if (!c || !T || !eps) return LSSEG_ERR_INVALID_ARG;
+    const void* fn;
+    switch (c->dtype) {
+    case LSSEG_U8: fn = (const void*)&k_inside_moments<uint8_t, false>; break;
+    case LSSEG_I8: fn = (const void*)&k_inside_moments<int8_t, false>; break;
+    case LSSEG_U16: fn = (const void*)&k_inside_moments<uint16_t, false>; break;
+    case LSSEG_I16: fn = (const void*)&k_inside_moments<int16_t, false>; break;
+    case LSSEG_U32: fn = (const void*)&k_inside_moments<uint32_t, false>; break;
+    case LSSEG_I32: fn = (const void*)&k_inside_moments<int32_t, false>; break;
+    default: fn = (const void*)&k_inside_moments<float, true>; break;
+    }
+    double* d_acc = nullptr;
+    lsseg_status st;
+    if ((st = dalloc(c, (void**)&d_acc, 2 * sizeof(double) + sizeof(unsigned long long))) != LSSEG_OK) return st;
+    unsigned long long* d_cnt = (unsigned long long*)(d_acc + 2);
+    struct Out {
+        double acc[2];
+        unsigned long long cnt;
+    } h{};
+    hipError_t e = hipMemsetAsync(d_acc, 0, sizeof(Out), c->stream);
+    const uint32_t dg = dense_grid(c->a.g.N);
+    double mean = 0.0;
+    for (int mode = 0; mode < 2 && e == hipSuccess; ++mode) {
+        double* acc = d_acc + mode;
+        void* args[] = {(void*)&c->a, (void*)&mode, (void*)&mean, (void*)&acc, (void*)&d_cnt};
+        e = hipLaunchKernel(fn, dim3(dg), dim3(kBlock), args, 0, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&h, d_acc, sizeof(Out), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (mode == 0 && h.cnt) mean = h.acc[0] / (double)h.cnt;
+    }
+    dfree(c, d_acc);
+    if (e != hipSuccess) return fail(c, LSSEG_ERR_DEVICE, "auto parameters failed: %s", hipGetErrorString(e));
+    if (count) *count = h.cnt;
+    if (h.cnt < 2) return fail(c, LSSEG_ERR_STATE, "fewer than two voxels inside the segmentation");
+    *T = (float)mean;
+    *eps = (float)std::sqrt(h.acc[1] / (double)(h.cnt - 1));
+    return LSSEG_OK;
+}
+
 lsseg_status lsseg_freeze(lsseg_ctx* c)
 {
     if (!c) return LSSEG_ERR_INVALID_ARG;

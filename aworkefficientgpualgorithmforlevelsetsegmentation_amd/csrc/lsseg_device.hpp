@@ -1089,6 +1089,50 @@ __device__ inline int8_t add_export(int8_t ra, int8_t rb)
     return (int8_t)(int32_t)floorf(r * 127.0f);
 }
 
+// CollectCurrentLevelSetVoxels / CalculateSegmentationParameters
+// (Segmenter.cpp:357-478): source values of the voxels whose export byte is
+// <= 0.  mode 0 accumulates the count and the sum, mode 1 the squared
+// deviations from `mean`; in double, so only the summation order differs from
+// the reference's sequential float sums.
+template <typename SrcT, bool kFloatSrc>
+__global__ __launch_bounds__(kBlock) void k_inside_moments(KArgs a, int mode, double mean, double* acc,
+                                                           unsigned long long* count)
+{
+    __shared__ double s_red[kBlock / 64];
+    __shared__ unsigned long long s_cnt[kBlock / 64];
+    double v = 0.0;
+    unsigned long long c = 0;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < a.g.N;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        if (a.E[t] <= 0) {
+            const double x = kFloatSrc ? (double)static_cast<const float*>(a.src)[t]
+                                       : (double)(float)static_cast<const SrcT*>(a.src)[t];
+            v += mode == 0 ? x : (x - mean) * (x - mean);
+            ++c;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v += __shfl_xor(v, o, 64);
+        c += __shfl_xor(c, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_red[threadIdx.x >> 6] = v;
+        s_cnt[threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tv = 0.0;
+        unsigned long long tc = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            tv += s_red[w];
+            tc += s_cnt[w];
+        }
+        if (tv != 0.0) atomicAdd(acc, tv);
+        if (mode == 0 && tc) atomicAdd(count, tc);
+    }
+}
+
 __global__ void k_freeze(KArgs a)
 {
     LSSEG_GRID_STRIDE(t, a.g.N) a.frozen[t] = add_export(a.frozen[t], a.E[t]);

@@ -146,6 +146,15 @@ void Segmenter::FreezeCurrentSegmentation()
     check(lsseg_freeze(ctx_), "FreezeCurrentSegmentation");
 }
 
+void Segmenter::CalculateSegmentationParameters()
+{
+    if (!ctx_) throw Error(LSSEG_ERR_STATE, "CalculateSegmentationParameters: no volume loaded");
+    float T = 0.0f, eps = 0.0f;
+    check(lsseg_auto_parameters(ctx_, &T, &eps, nullptr), "CalculateSegmentationParameters");
+    params_.growLevel = T;
+    params_.growWindow = eps;
+}
+
 bool Segmenter::ShouldInitializeActiveElements() const
 {
     // Segmenter.cpp:1371-1374

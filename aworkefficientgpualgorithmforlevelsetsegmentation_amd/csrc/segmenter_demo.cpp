@@ -5,14 +5,98 @@
 //
 // usage: segmenter_demo <raw> <nx> <ny> <nz> <bytes> <signed> <cx> <cy> <cz> <radius>
 //                       <T> <eps> <dt> <alpha> <substeps> <max_requests> <mask_out>
+//        segmenter_demo --parse <project.gproj>
+//            prints the project and its parameters (no device work)
+//        segmenter_demo --project <project.gproj> <cx> <cy> <cz> <radius> <max_requests> <mask_out> <auto>
+//            opens a reference project (MainWindow.cpp:692-725, Engine.cpp:179-257);
+//            auto = 1 runs CalculateSegmentationParameters after the seed
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
+#include "lsseg/formats.hpp"
 #include "lsseg/segmenter.hpp"
+
+namespace {
+
+void print_parameters(const lsseg::Parameters& p)
+{
+    std::printf("growLevel %.9g\ngrowWindow %.9g\ntimeStep %.9g\ncurvatureInfluence %.9g\nnumSubSteps %d\n"
+                "convergenceThreshold %.9g\n",
+                p.growLevel, p.growWindow, p.timeStep, p.curvatureInfluence, p.numSubSteps, p.convergenceThreshold);
+}
+
+lsseg::Parameters project_parameters(const lsseg::Project& pr, const lsseg::VolumeDesc& vol)
+{
+    // Engine.cpp:196-226: the project's file, else the default for the type
+    return pr.parametersFile.empty() ? lsseg::DefaultParameters(vol.maxValue)
+                                     : lsseg::LoadParameters(pr.parametersFile);
+}
+
+int run_project(int argc, char** argv)
+{
+    const bool parse_only = std::strcmp(argv[1], "--parse") == 0;
+    if ((parse_only && argc != 3) || (!parse_only && argc != 10)) {
+        std::fprintf(stderr, "usage: %s --parse gproj | --project gproj cx cy cz r max_requests mask_out auto\n",
+                     argv[0]);
+        return 2;
+    }
+    try {
+        const lsseg::Project pr = lsseg::LoadProject(argv[2]);
+        lsseg::VolumeDesc vol = lsseg::LoadVolumeRaw(pr.volume);
+        lsseg::Parameters params = project_parameters(pr, vol);
+        if (parse_only) {
+            std::printf("dataFile %s\nparametersFile %s\nnumVoxels %u %u %u\nnumBytesPerVoxel %d\nisSigned %d\n"
+                        "zAnisotropy %.9g\nupDirection %d %d %d\nminValue %.9g\nmaxValue %.9g\n",
+                        pr.volume.fileName.c_str(), pr.parametersFile.c_str(), pr.volume.numVoxelsX,
+                        pr.volume.numVoxelsY, pr.volume.numVoxelsZ, pr.volume.numBytesPerVoxel,
+                        pr.volume.isSigned ? 1 : 0, pr.volume.zAnisotropy, pr.volume.upDirection[0],
+                        pr.volume.upDirection[1], pr.volume.upDirection[2], vol.minValue, vol.maxValue);
+            print_parameters(params);
+            delete[] static_cast<unsigned char*>(vol.volumeData);
+            return 0;
+        }
+        const size_t count = (size_t)vol.numVoxelsX * vol.numVoxelsY * vol.numVoxelsZ;
+        lsseg::Segmenter seg;
+        seg.LoadVolume(vol);
+        delete[] static_cast<unsigned char*>(vol.volumeData);
+        seg.GetParameters() = params;
+        const float seed[3] = {(float)std::atof(argv[3]), (float)std::atof(argv[4]), (float)std::atof(argv[5])};
+        seg.InitializeSeed(seed, (unsigned)std::atoi(argv[6]));
+        if (std::atoi(argv[9])) seg.CalculateSegmentationParameters();
+        print_parameters(seg.GetParameters());
+        const int max_requests = std::atoi(argv[7]);
+        int requests = 0;
+        while (seg.IsSegmentationInProgress() && requests < max_requests) {
+            seg.RequestUpdateSegmentation(0.0);
+            ++requests;
+        }
+        lsseg::VolumeDesc mask = seg.GetSaveableSegmentation();
+        const signed char* m = (const signed char*)mask.volumeData;
+        FILE* o = std::fopen(argv[8], "wb");
+        const bool ok = o && std::fwrite(m, 1, count, o) == count;
+        if (o) std::fclose(o);
+        delete[] m;
+        if (!ok) {
+            std::fprintf(stderr, "cannot write %s\n", argv[8]);
+            return 2;
+        }
+        std::printf("requests %d iterations %u active %zu\n", requests, seg.GetNumIterations(),
+                    seg.GetActiveElementCount());
+    } catch (const lsseg::Error& e) {
+        std::fprintf(stderr, "lsseg error %d: %s\n", (int)e.status(), e.what());
+        return 1;
+    }
+    return 0;
+}
+
+}  // namespace
 
 int main(int argc, char** argv)
 {
+    if (argc > 1 && (std::strcmp(argv[1], "--parse") == 0 || std::strcmp(argv[1], "--project") == 0))
+        return run_project(argc, argv);
     if (argc != 18) {
         std::fprintf(stderr, "usage: %s raw nx ny nz bytes signed cx cy cz r T eps dt alpha substeps max_requests mask_out\n",
                      argv[0]);

@@ -35,7 +35,7 @@ EXPORTS = (
     "lsseg_batch_volume", "lsseg_batch_iterate", "lsseg_batch_checkpoint_save", "lsseg_batch_checkpoint_restore",
     "lsseg_batch_get_tuning", "lsseg_batch_set_tuning", "lsseg_batch_autotune",
     "lsseg_fwd_record", "lsseg_record_destroy", "lsseg_record_info", "lsseg_record_get", "lsseg_fwd_replay",
-    "lsseg_bwd",
+    "lsseg_bwd", "lsseg_auto_parameters",
 )
 
 
@@ -155,6 +155,7 @@ def lib():
             "lsseg_record_get": (i32, [vp, vp, vp, vp]),
             "lsseg_fwd_replay": (i32, [vp, vp, vp, P(Params), vp]),
             "lsseg_bwd": (i32, [vp, vp, vp, vp, vp]),
+            "lsseg_auto_parameters": (i32, [vp, P(f32), P(f32), P(u64)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -329,6 +330,13 @@ class Context:
 
     def freeze(self):
         self._check(lib().lsseg_freeze(self._h))
+
+    def auto_parameters(self):
+        """(T, eps, count): mean and sample standard deviation of the source inside
+        the current segmentation (Segmenter.cpp:357-478)."""
+        T, eps, n = ctypes.c_float(), ctypes.c_float(), ctypes.c_uint64()
+        self._check(lib().lsseg_auto_parameters(self._h, ctypes.byref(T), ctypes.byref(eps), ctypes.byref(n)))
+        return T.value, eps.value, n.value
 
     def clear(self, all_segmentations: bool = False):
         self._check(lib().lsseg_clear(self._h, 1 if all_segmentations else 0))

@@ -91,6 +91,13 @@ class Segmenter:
     def FreezeCurrentSegmentation(self):
         self.context.freeze()
 
+    def CalculateSegmentationParameters(self):
+        """Segmenter.cpp:357-478: growLevel := mean, growWindow := sample standard
+        deviation of the source inside the current segmentation."""
+        T, eps, _ = self.context.auto_parameters()
+        self.parameters.growLevel = T
+        self.parameters.growWindow = eps
+
     def _should_init_active(self) -> bool:
         return self._initialized and self._previous != self._snapshot()  # Segmenter.cpp:1371-1374
 

@@ -179,6 +179,14 @@ lsseg_status lsseg_get_mask(const lsseg_ctx* ctx, uint8_t* out);
    frozen := union(frozen, export). */
 lsseg_status lsseg_freeze(lsseg_ctx* ctx);
 
+/* Replaces Segmenter::CalculateSegmentationParameters (Segmenter.cpp:357-478):
+   over the voxels whose export byte is <= 0 (the current segmentation, without
+   the frozen one), *T := mean and *eps := sample standard deviation of the
+   source values; *count (may be NULL) := the number of such voxels.  The sums
+   run in double; the reference sums sequentially in float, so results differ
+   by that rounding only.  Fails with LSSEG_ERR_STATE below two voxels. */
+lsseg_status lsseg_auto_parameters(lsseg_ctx* ctx, float* T, float* eps, uint64_t* count);
+
 /* Replaces Segmenter::ClearCurrentSegmentation (Segmenter.cpp:597-640): phi := 1,
    export := 127, empty active list.  all != 0 also clears the frozen
    segmentation (ClearAllSegmentations, :641-657). */

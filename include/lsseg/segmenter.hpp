@@ -78,6 +78,9 @@ public:
     void ClearCurrentSegmentation();    // :597-640
     void ClearAllSegmentations();       // :641-657
     void FreezeCurrentSegmentation();   // :659-679
+    // :357-478: growLevel := mean, growWindow := sample standard deviation of
+    // the source inside the current segmentation (export byte <= 0)
+    void CalculateSegmentationParameters();
 
     // Segmenter.cpp:759-1051: runs numSubSteps iterations if the segmentation
     // is in progress; a change of growLevel, growWindow or curvatureInfluence

@@ -96,6 +96,8 @@ def lib():
         L.lso_iterate_reinit.argtypes = [vp, c_f, c_f, c_f, c_f]
         L.lso_freeze.restype = None
         L.lso_freeze.argtypes = [vp]
+        L.lso_auto_parameters.restype = c_i64
+        L.lso_auto_parameters.argtypes = [vp, vp, vp]
         L.lso_clear.restype = None
         L.lso_clear.argtypes = [vp, c_i32]
         L.lso_add_export.restype = None
@@ -221,6 +223,13 @@ class State:
 
     def seed_sphere(self, cx, cy, cz, radius):
         lib().lso_seed_sphere(self._h, cx, cy, cz, radius)
+
+    def auto_parameters(self):
+        """(T, eps, count): CalculateSegmentationParameters in the reference's float arithmetic."""
+        T = ctypes.c_float()
+        eps = ctypes.c_float()
+        n = lib().lso_auto_parameters(self._h, ctypes.byref(T), ctypes.byref(eps))
+        return T.value, eps.value, int(n)
 
     def iterate(self, T, eps, alpha, dt, iters, mode=MODE_REF, threads=1):
         n = np.zeros(max(iters, 1), np.uint32)

@@ -490,10 +490,59 @@ void lso_get_mask(const lso_state* s, uint8_t* out_linear)
             }
 }
 
+/* a source voxel as the reference reads it for the statistics: static_cast<float> */
+static float source_value(const lso_state* s, uint32_t t)
+{
+    switch (s->dtype) {
+    case LSO_U8: return (float)((const uint8_t*)s->src)[t];
+    case LSO_I8: return (float)((const int8_t*)s->src)[t];
+    case LSO_U16: return (float)((const uint16_t*)s->src)[t];
+    case LSO_I16: return (float)((const int16_t*)s->src)[t];
+    case LSO_U32: return (float)((const uint32_t*)s->src)[t];
+    case LSO_I32: return (float)((const int32_t*)s->src)[t];
+    default: return ((const float*)s->src)[t];
+    }
+}
+
 /* FreezeCurrentSegmentation (Segmenter.cpp:659-679): frozen := union(frozen, export) */
 void lso_freeze(lso_state* s)
 {
     for (uint64_t t = 0; t < s->N; ++t) s->frozen[t] = add_export_byte(s->frozen[t], s->E[t]);
+}
+
+/* CollectCurrentLevelSetVoxels + CalculateSegmentationParameters
+   (Segmenter.cpp:357-478): in the reference's order (the padded volume, x
+   fastest) and arithmetic (sequential float sums), the mean and the sample
+   standard deviation of the source over the voxels whose export byte is <= 0.
+   Returns the number of such voxels. */
+int64_t lso_auto_parameters(const lso_state* s, float* T, float* eps)
+{
+    float total = 0.0f;
+    int64_t n = 0;
+    for (int32_t z = 0; z < s->Z; ++z)
+        for (int32_t y = 0; y < s->Y; ++y)
+            for (int32_t x = 0; x < s->X; ++x) {
+                const uint32_t t = lso_tau(s->X, s->Y, x, y, z);
+                if (s->E[t] <= 0) {
+                    total += source_value(s, t);
+                    ++n;
+                }
+            }
+    const float count = (float)n;
+    const float mean = total / count;
+    float sd = 0.0f;
+    for (int32_t z = 0; z < s->Z; ++z)
+        for (int32_t y = 0; y < s->Y; ++y)
+            for (int32_t x = 0; x < s->X; ++x) {
+                const uint32_t t = lso_tau(s->X, s->Y, x, y, z);
+                if (s->E[t] <= 0) {
+                    const float d = source_value(s, t) - mean;
+                    sd += d * d;
+                }
+            }
+    *T = mean;
+    *eps = (float)sqrt(sd / (count - 1.0f));
+    return n;
 }
 
 /* ClearCurrentSegmentation (Segmenter.cpp:597-640): phi := 1, export := 127,

@@ -99,6 +99,7 @@ void       lso_get_mask(const lso_state* s, uint8_t* out_linear);
 void       lso_reinit_active(lso_state* s);
 void       lso_iterate_reinit(lso_state* s, float T, float eps, float alpha, float dt);
 void       lso_freeze(lso_state* s);
+int64_t    lso_auto_parameters(const lso_state* s, float* T, float* eps);
 void       lso_clear(lso_state* s, int32_t all);
 /* union of two export volumes (InitializeLevelSetVolume.cu:151-169) */
 void       lso_add_export(int8_t* result, const int8_t* add, int64_t N);
