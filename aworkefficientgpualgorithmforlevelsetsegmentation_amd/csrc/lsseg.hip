@@ -45,6 +45,8 @@ struct lsseg_ctx {
     uint32_t nx = 0, ny = 0, nz = 0;
     int32_t dtype = 0;
     uint32_t S = 1;
+    bool phi8 = false;                // fixed-point (int8) phi storage
+    size_t P = 4;                     // bytes per phi value
     float z_aniso = 1.0f;
     int q = 0;                        // parity of the current list and phi read buffer
     uint32_t grid = 1024;             // co-resident workgroups of the compaction kernel
@@ -58,7 +60,7 @@ struct lsseg_ctx {
     uint64_t last_iters = 0;
     hipGraphExec_t graphs[2] = {nullptr, nullptr};
     struct Checkpoint {
-        float* phi[2] = {nullptr, nullptr};
+        void* phi[2] = {nullptr, nullptr};
         int8_t* E = nullptr;
         uint32_t* list = nullptr;
         uint32_t n = 0;
@@ -135,18 +137,50 @@ int env_int(const char* name, int dflt)
     return v && *v ? atoi(v) : dflt;
 }
 
-// k_update instantiation for the context's source type
-const void* update_kernel(int32_t dtype)
+// kernel instantiations for a source type and a phi storage
+#define LSSEG_SRC_PHI_KERNEL(fn, name)                                                                 \
+    template <typename PhiT>                                                                          \
+    const void* fn##_t(int32_t dtype)                                                                 \
+    {                                                                                                 \
+        switch (dtype) {                                                                              \
+        case LSSEG_U8: return (const void*)&name<uint8_t, false, PhiT>;                               \
+        case LSSEG_I8: return (const void*)&name<int8_t, false, PhiT>;                                \
+        case LSSEG_U16: return (const void*)&name<uint16_t, false, PhiT>;                             \
+        case LSSEG_I16: return (const void*)&name<int16_t, false, PhiT>;                              \
+        case LSSEG_U32: return (const void*)&name<uint32_t, false, PhiT>;                             \
+        case LSSEG_I32: return (const void*)&name<int32_t, false, PhiT>;                              \
+        default: return (const void*)&name<float, true, PhiT>;                                        \
+        }                                                                                             \
+    }                                                                                                 \
+    const void* fn(int32_t dtype, bool phi8) { return phi8 ? fn##_t<int8_t>(dtype) : fn##_t<float>(dtype); }
+LSSEG_SRC_PHI_KERNEL(update_kernel, k_update)
+LSSEG_SRC_PHI_KERNEL(update_only_kernel, k_update_only)
+LSSEG_SRC_PHI_KERNEL(update_batch_kernel, k_update_batch)
+#undef LSSEG_SRC_PHI_KERNEL
+
+const void* mark_kernel(bool phi8) { return phi8 ? (const void*)&k_mark<int8_t> : (const void*)&k_mark<float>; }
+const void* mark_batch_kernel(bool phi8)
 {
-    switch (dtype) {
-    case LSSEG_U8: return (const void*)&k_update<uint8_t, false>;
-    case LSSEG_I8: return (const void*)&k_update<int8_t, false>;
-    case LSSEG_U16: return (const void*)&k_update<uint16_t, false>;
-    case LSSEG_I16: return (const void*)&k_update<int16_t, false>;
-    case LSSEG_U32: return (const void*)&k_update<uint32_t, false>;
-    case LSSEG_I32: return (const void*)&k_update<int32_t, false>;
-    default: return (const void*)&k_update<float, true>;
-    }
+    return phi8 ? (const void*)&k_mark_batch<int8_t> : (const void*)&k_mark_batch<float>;
+}
+
+// a single-volume kernel of the phi storage, launched on the context's stream
+template <typename... Args>
+hipError_t launch_phi(lsseg_ctx* c, void (*f32)(Args...), void (*i8)(Args...), uint32_t grid, Args... args)
+{
+    void* packed[] = {(void*)&args...};
+    return hipLaunchKernel(c->phi8 ? (const void*)i8 : (const void*)f32, dim3(grid), dim3(kBlock), packed, 0,
+                           c->stream);
+}
+
+void fill_phi_one(lsseg_ctx* c, void* buf)
+{
+    const uint64_t N = c->a.g.N;
+    if (c->phi8)
+        hipLaunchKernelGGL(k_fill_u8, dim3(dense_grid(N)), dim3(kBlock), 0, c->stream, (uint8_t*)buf, N,
+                           (uint8_t)127);
+    else
+        hipLaunchKernelGGL(k_fill_f32, dim3(dense_grid(N)), dim3(kBlock), 0, c->stream, (float*)buf, N, 1.0f);
 }
 
 void launch(lsseg_ctx* c, const void* fn, uint32_t grid, int q)
@@ -165,24 +199,15 @@ void launch(lsseg_ctx* c, const void* fn, uint32_t grid, int q)
 // one iteration over list[q]: update, marking, compaction into list[q ^ 1]
 void launch_iteration(lsseg_ctx* c, int q)
 {
-    launch(c, update_kernel(c->dtype), c->sgrid, q);
-    launch(c, (const void*)&k_mark, c->sgrid, q);
+    launch(c, update_kernel(c->dtype, c->phi8), c->sgrid, q);
+    launch(c, mark_kernel(c->phi8), c->sgrid, q);
     launch(c, (const void*)&k_compact<kCompact>, c->grid, q);
 }
 
 void launch_update_only(lsseg_ctx* c, int op)
 {
     void* args[] = {(void*)&c->a, (void*)&op};
-    const void* fn;
-    switch (c->dtype) {
-    case LSSEG_U8: fn = (const void*)&k_update_only<uint8_t, false>; break;
-    case LSSEG_I8: fn = (const void*)&k_update_only<int8_t, false>; break;
-    case LSSEG_U16: fn = (const void*)&k_update_only<uint16_t, false>; break;
-    case LSSEG_I16: fn = (const void*)&k_update_only<int16_t, false>; break;
-    case LSSEG_U32: fn = (const void*)&k_update_only<uint32_t, false>; break;
-    case LSSEG_I32: fn = (const void*)&k_update_only<int32_t, false>; break;
-    default: fn = (const void*)&k_update_only<float, true>; break;
-    }
+    const void* fn = update_only_kernel(c->dtype, c->phi8);
     (void)hipLaunchKernel(fn, dim3(c->grid), dim3(kBlock), args, 0, c->stream);
 }
 
@@ -319,6 +344,46 @@ lsseg_status inject_list_counts(lsseg_ctx* c, uint32_t n)
     return LSSEG_OK;
 }
 
+// a phi buffer as N floats (host memory); fixed point converts to k / 127
+lsseg_status phi_to_floats(lsseg_ctx* c, const void* buf, float* dst)
+{
+    const uint64_t N = c->a.g.N;
+    if (!c->phi8) {
+        HIP_TRY(c, hipMemcpy(dst, buf, N * sizeof(float), hipMemcpyDeviceToHost));
+        return LSSEG_OK;
+    }
+    float* d = nullptr;
+    lsseg_status st = dalloc(c, (void**)&d, N * sizeof(float));
+    if (st != LSSEG_OK) return st;
+    hipLaunchKernelGGL(k_phi8_to_float, dim3(dense_grid(N)), dim3(kBlock), 0, c->stream, (const int8_t*)buf, d, N);
+    hipError_t e = hipMemcpyAsync(dst, d, N * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(c, d);
+    if (e != hipSuccess) return fail(c, LSSEG_ERR_DEVICE, "phi readback failed: %s", hipGetErrorString(e));
+    return LSSEG_OK;
+}
+
+// N floats (host memory) into a phi buffer; fixed point rounds v * 127
+lsseg_status floats_to_phi(lsseg_ctx* c, const float* src, void* buf)
+{
+    const uint64_t N = c->a.g.N;
+    if (!c->phi8) {
+        HIP_TRY(c, hipMemcpy(buf, src, N * sizeof(float), hipMemcpyHostToDevice));
+        return LSSEG_OK;
+    }
+    float* d = nullptr;
+    lsseg_status st = dalloc(c, (void**)&d, N * sizeof(float));
+    if (st != LSSEG_OK) return st;
+    hipError_t e = hipMemcpyAsync(d, src, N * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        hipLaunchKernelGGL(k_float_to_phi8, dim3(dense_grid(N)), dim3(kBlock), 0, c->stream, (const float*)d,
+                           (int8_t*)buf, N);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(c, d);
+    if (e != hipSuccess) return fail(c, LSSEG_ERR_DEVICE, "phi upload failed: %s", hipGetErrorString(e));
+    return LSSEG_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -347,6 +412,8 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
         return fail(nullptr, LSSEG_ERR_INVALID_ARG, "zero dimension");
     if (desc->dtype < LSSEG_U8 || desc->dtype > LSSEG_F32)
         return fail(nullptr, LSSEG_ERR_UNSUPPORTED, "unsupported dtype %d", desc->dtype);
+    if (desc->phi_storage != LSSEG_PHI_F32 && desc->phi_storage != LSSEG_PHI_I8)
+        return fail(nullptr, LSSEG_ERR_UNSUPPORTED, "unsupported phi storage %d", desc->phi_storage);
     const uint64_t X = (desc->nx + 15ull) / 16 * 16, Y = (desc->ny + 15ull) / 16 * 16,
                    Z = (desc->nz + 15ull) / 16 * 16;
     const uint64_t N = X * Y * Z;
@@ -367,6 +434,8 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     c->dtype = desc->dtype;
     c->S = (desc->dtype <= LSSEG_I8) ? 1 : (desc->dtype <= LSSEG_I16 ? 2 : 4);
     c->z_aniso = desc->z_aniso;
+    c->phi8 = desc->phi_storage == LSSEG_PHI_I8;
+    c->P = c->phi8 ? 1 : sizeof(float);
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) {
         c->err = std::string("no HIP device: ") + hipGetErrorString(e);
@@ -434,8 +503,11 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     if ((st = dalloc(c, &p, (size_t)(count) * sizeof(type))) != LSSEG_OK) \
         return bail(st);                                                   \
     c->a.field = (type*)p;
-    ALLOC(phi[0], float, N);
-    ALLOC(phi[1], float, N);
+    ALLOC(phi[0], uint8_t, N * c->P);
+    ALLOC(phi[1], uint8_t, N * c->P);
+    if (c->phi8) {
+        ALLOC(flag, uint8_t, N);
+    }
     ALLOC(E, int8_t, N);
     ALLOC(frozen, int8_t, N);
     ALLOC(C, uint64_t, N);
@@ -474,10 +546,11 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     }
     // Segmenter.cpp:150-154, 265-266: phi := 1, export := 127, scratch := 0
     const uint32_t dg = dense_grid(N);
-    hipLaunchKernelGGL(k_fill_f32, dim3(dg), dim3(kBlock), 0, c->stream, c->a.phi[0], N, 1.0f);
-    hipLaunchKernelGGL(k_fill_f32, dim3(dg), dim3(kBlock), 0, c->stream, c->a.phi[1], N, 1.0f);
+    fill_phi_one(c, c->a.phi[0]);
+    fill_phi_one(c, c->a.phi[1]);
     hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, (uint8_t*)c->a.E, N, (uint8_t)127);
     hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, (uint8_t*)c->a.frozen, N, (uint8_t)127);
+    if (c->phi8) hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, c->a.flag, N, (uint8_t)0);
     fill_u64(c, c->a.C, N, 0);
     fill_u64(c, c->a.tstate, ntiles, 0);
     e = hipMemsetAsync(c->a.ctl, 0, sizeof(DevCtl), c->stream);
@@ -546,9 +619,9 @@ lsseg_status lsseg_set_phi(lsseg_ctx* c, const float* phi0, float band)
     }
     hipError_t e = hipMemcpyAsync(lin, phi0, cnt * sizeof(float), hipMemcpyDefault, c->stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_set_phi, dim3(dense_grid(c->a.g.N)), dim3(kBlock), 0, c->stream, c->a, lin, c->nx,
-                           c->ny, c->nz, band);
-        e = hipStreamSynchronize(c->stream);
+        e = launch_phi(c, &k_set_phi<float>, &k_set_phi<int8_t>, dense_grid(c->a.g.N), c->a, (const float*)lin,
+                       c->nx, c->ny, c->nz, band);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     }
     (void)hipFree(lin);
     if (e != hipSuccess) return fail(c, LSSEG_ERR_DEVICE, "set_phi failed: %s", hipGetErrorString(e));
@@ -566,11 +639,11 @@ lsseg_status lsseg_seed_sphere(lsseg_ctx* c, float cx, float cy, float cz, uint3
     const uint32_t dg = dense_grid(c->a.g.N);
     int rd = c->q;
     for (int pass = 0; pass < 2; ++pass) {  // Segmenter.cpp:525-531
-        hipLaunchKernelGGL(k_seed_pass, dim3(dg), dim3(kBlock), 0, c->stream, c->a, rd, sx, sy, sz,
-                           (int32_t)radius, c->z_aniso);
+        HIP_TRY(c, launch_phi(c, &k_seed_pass<float>, &k_seed_pass<int8_t>, dg, c->a, rd, sx, sy, sz,
+                              (int32_t)radius, c->z_aniso));
         rd ^= 1;
     }
-    hipLaunchKernelGGL(k_keep_flags, dim3(dg), dim3(kBlock), 0, c->stream, c->a, rd);
+    HIP_TRY(c, launch_phi(c, &k_keep_flags<float>, &k_keep_flags<int8_t>, dg, c->a, rd));
     HIP_TRY(c, hipGetLastError());
     // two swaps leave the read parity unchanged: the list goes to that parity
     lsseg_status st = dense_init_list(c, rd);
@@ -587,7 +660,7 @@ lsseg_status lsseg_iterate_reinit(lsseg_ctx* c, const lsseg_params* prm)
     if ((st = upload_params(c, P)) != LSSEG_OK) return st;
     launch_update_only(c, c->q);
     const int rd = c->q ^ 1;
-    hipLaunchKernelGGL(k_keep_flags, dim3(dense_grid(c->a.g.N)), dim3(kBlock), 0, c->stream, c->a, rd);
+    HIP_TRY(c, launch_phi(c, &k_keep_flags<float>, &k_keep_flags<int8_t>, dense_grid(c->a.g.N), c->a, rd));
     HIP_TRY(c, hipGetLastError());
     if ((st = dense_init_list(c, rd)) != LSSEG_OK) return st;
     return sync_and_check(c);
@@ -735,9 +808,9 @@ lsseg_status lsseg_get_phi(const lsseg_ctx* c, int32_t which, float* out)
         (void)hipGetLastError();
         return fail(cm, LSSEG_ERR_OUT_OF_MEMORY, "staging allocation failed");
     }
-    hipLaunchKernelGGL(k_get_phi, dim3(dense_grid(cnt)), dim3(kBlock), 0, c->stream, c->a, which == 0 ? c->q : c->q ^ 1,
-                       d, c->nx, c->ny, c->nz);
-    hipError_t e = hipMemcpyAsync(out, d, cnt * sizeof(float), hipMemcpyDefault, c->stream);
+    hipError_t e = launch_phi(cm, &k_get_phi<float>, &k_get_phi<int8_t>, dense_grid(cnt), c->a,
+                              which == 0 ? c->q : c->q ^ 1, d, c->nx, c->ny, c->nz);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, cnt * sizeof(float), hipMemcpyDefault, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(cm, LSSEG_ERR_DEVICE, "get_phi failed: %s", hipGetErrorString(e));
@@ -817,8 +890,8 @@ lsseg_status lsseg_clear(lsseg_ctx* c, int32_t all)
     if (!c) return LSSEG_ERR_INVALID_ARG;
     const uint64_t N = c->a.g.N;
     const uint32_t dg = dense_grid(N);
-    hipLaunchKernelGGL(k_fill_f32, dim3(dg), dim3(kBlock), 0, c->stream, c->a.phi[0], N, 1.0f);
-    hipLaunchKernelGGL(k_fill_f32, dim3(dg), dim3(kBlock), 0, c->stream, c->a.phi[1], N, 1.0f);
+    fill_phi_one(c, c->a.phi[0]);
+    fill_phi_one(c, c->a.phi[1]);
     hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, (uint8_t*)c->a.E, N, (uint8_t)127);
     if (all)
         hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, (uint8_t*)c->a.frozen, N, (uint8_t)127);
@@ -833,8 +906,12 @@ lsseg_status lsseg_get_state(const lsseg_ctx* c, float* phi_read, float* phi_wri
     if (!c) return LSSEG_ERR_INVALID_ARG;
     lsseg_ctx* cm = const_cast<lsseg_ctx*>(c);
     const uint64_t N = c->a.g.N;
-    if (phi_read) HIP_TRY(cm, hipMemcpy(phi_read, c->a.phi[c->q], N * sizeof(float), hipMemcpyDeviceToHost));
-    if (phi_write) HIP_TRY(cm, hipMemcpy(phi_write, c->a.phi[c->q ^ 1], N * sizeof(float), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 2; ++i) {
+        float* dst = i == 0 ? phi_read : phi_write;
+        if (!dst) continue;
+        lsseg_status st = phi_to_floats(cm, c->a.phi[c->q ^ i], dst);
+        if (st != LSSEG_OK) return st;
+    }
     if (E) HIP_TRY(cm, hipMemcpy(E, c->a.E, N, hipMemcpyDeviceToHost));
     if (n || list) {
         uint64_t cnt = 0;
@@ -857,8 +934,9 @@ lsseg_status lsseg_set_state(lsseg_ctx* c, const float* phi_read, const float* p
             return fail(c, LSSEG_ERR_INVALID_ARG, "list entry %llu is a duplicate", (unsigned long long)i);
         seen[list[i]] = 1;
     }
-    HIP_TRY(c, hipMemcpy(c->a.phi[0], phi_read, N * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->a.phi[1], phi_write, N * sizeof(float), hipMemcpyHostToDevice));
+    lsseg_status st0 = floats_to_phi(c, phi_read, c->a.phi[0]);
+    if (st0 != LSSEG_OK) return st0;
+    if ((st0 = floats_to_phi(c, phi_write, c->a.phi[1])) != LSSEG_OK) return st0;
     HIP_TRY(c, hipMemcpy(c->a.E, E, N, hipMemcpyHostToDevice));
     if (n) HIP_TRY(c, hipMemcpy(c->a.list[0], list, n * sizeof(uint32_t), hipMemcpyHostToDevice));
     lsseg_status st = inject_list_counts(c, (uint32_t)n);
@@ -875,10 +953,10 @@ lsseg_status lsseg_checkpoint_save(lsseg_ctx* c, int32_t slot)
     if (!k.phi[0]) {
         void* p = nullptr;
         lsseg_status st;
-        if ((st = dalloc(c, &p, N * sizeof(float))) != LSSEG_OK) return st;
-        k.phi[0] = (float*)p;
-        if ((st = dalloc(c, &p, N * sizeof(float))) != LSSEG_OK) return st;
-        k.phi[1] = (float*)p;
+        if ((st = dalloc(c, &p, N * c->P)) != LSSEG_OK) return st;
+        k.phi[0] = p;
+        if ((st = dalloc(c, &p, N * c->P)) != LSSEG_OK) return st;
+        k.phi[1] = p;
         if ((st = dalloc(c, &p, N)) != LSSEG_OK) return st;
         k.E = (int8_t*)p;
         if ((st = dalloc(c, &p, N * sizeof(uint32_t))) != LSSEG_OK) return st;
@@ -887,8 +965,8 @@ lsseg_status lsseg_checkpoint_save(lsseg_ctx* c, int32_t slot)
     uint64_t n = 0;
     lsseg_status st = lsseg_get_active_count(c, &n);
     if (st != LSSEG_OK) return st;
-    HIP_TRY(c, hipMemcpyAsync(k.phi[0], c->a.phi[c->q], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(k.phi[1], c->a.phi[c->q ^ 1], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(k.phi[0], c->a.phi[c->q], N * c->P, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(k.phi[1], c->a.phi[c->q ^ 1], N * c->P, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(k.E, c->a.E, N, hipMemcpyDeviceToDevice, c->stream));
     if (n) HIP_TRY(c, hipMemcpyAsync(k.list, c->a.list[c->q], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -904,8 +982,8 @@ lsseg_status lsseg_checkpoint_restore(lsseg_ctx* c, int32_t slot)
         return fail(c, LSSEG_ERR_STATE, "checkpoint slot %d is empty", slot);
     auto& k = c->ckpt[slot];
     const uint64_t N = c->a.g.N;
-    HIP_TRY(c, hipMemcpyAsync(c->a.phi[0], k.phi[0], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->a.phi[1], k.phi[1], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->a.phi[0], k.phi[0], N * c->P, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->a.phi[1], k.phi[1], N * c->P, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->a.E, k.E, N, hipMemcpyDeviceToDevice, c->stream));
     if (k.n) HIP_TRY(c, hipMemcpyAsync(c->a.list[0], k.list, k.n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
     lsseg_status st = inject_list_counts(c, k.n);
@@ -983,25 +1061,13 @@ void destroy_batch_graphs(lsseg_batch* b)
     }
 }
 
-const void* update_batch_kernel(int32_t dtype)
-{
-    switch (dtype) {
-    case LSSEG_U8: return (const void*)&k_update_batch<uint8_t, false>;
-    case LSSEG_I8: return (const void*)&k_update_batch<int8_t, false>;
-    case LSSEG_U16: return (const void*)&k_update_batch<uint16_t, false>;
-    case LSSEG_I16: return (const void*)&k_update_batch<int16_t, false>;
-    case LSSEG_U32: return (const void*)&k_update_batch<uint32_t, false>;
-    case LSSEG_I32: return (const void*)&k_update_batch<int32_t, false>;
-    default: return (const void*)&k_update_batch<float, true>;
-    }
-}
-
 void launch_batch_iteration(lsseg_batch* b, int q)
 {
     BatchArgs args{b->d_table, b->d_ctls, (uint32_t)b->vols.size()};
     void* kargs[] = {(void*)&args, (void*)&q};
-    (void)hipLaunchKernel(update_batch_kernel(b->dtype), dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
-    (void)hipLaunchKernel((const void*)&k_mark_batch, dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
+    const bool phi8 = b->vols[0]->phi8;
+    (void)hipLaunchKernel(update_batch_kernel(b->dtype, phi8), dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
+    (void)hipLaunchKernel(mark_batch_kernel(phi8), dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
     (void)hipLaunchKernel((const void*)&k_compact_batch, dim3(b->grid), dim3(kBlock), kargs, 0, b->stream);
 }
 
@@ -1101,8 +1167,9 @@ lsseg_status lsseg_batch_create(const lsseg_volume_desc* descs, const void* cons
     if (count == 0 || count > LSSEG_MAX_BATCH)
         return bfail(nullptr, LSSEG_ERR_INVALID_ARG, "batch size %u outside 1..%d", count, LSSEG_MAX_BATCH);
     for (uint32_t v = 0; v < count; ++v)
-        if (descs[v].dtype != descs[0].dtype)
-            return bfail(nullptr, LSSEG_ERR_UNSUPPORTED, "volume %u has another source type than volume 0", v);
+        if (descs[v].dtype != descs[0].dtype || descs[v].phi_storage != descs[0].phi_storage)
+            return bfail(nullptr, LSSEG_ERR_UNSUPPORTED,
+                         "volume %u has another source type or phi storage than volume 0", v);
     lsseg_batch* b = new (std::nothrow) lsseg_batch();
     if (!b) return bfail(nullptr, LSSEG_ERR_OUT_OF_MEMORY, "host allocation failed");
     auto bail = [&](lsseg_status st) {
@@ -1447,6 +1514,7 @@ lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t ite
     if (!c || !prm || !out) return LSSEG_ERR_INVALID_ARG;
     *out = nullptr;
     if (iters <= 0 || every <= 0) return fail(c, LSSEG_ERR_INVALID_ARG, "iterations and checkpoint interval must be positive");
+    if (c->phi8) return fail(c, LSSEG_ERR_UNSUPPORTED, "the differentiable form needs FP32 phi storage");
     DevParams P;
     lsseg_status st = make_params(c, prm, &P);
     if (st != LSSEG_OK) return st;

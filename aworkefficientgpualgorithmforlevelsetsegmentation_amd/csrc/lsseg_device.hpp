@@ -178,7 +178,8 @@ struct DevParams {
 struct KArgs {
     Geom g;
     uint32_t run_log2;    // log2 of the list positions per XCD run (0: plain grid stride)
-    float* phi[2];
+    void* phi[2];         // float, or int8_t in the fixed-point storage mode
+    uint8_t* flag;        // fixed-point storage mode: temporal-derivative flag per voxel
     const void* src;
     int8_t* E;            // export byte (tiled)
     int8_t* frozen;       // frozen export (tiled)
@@ -290,6 +291,34 @@ __device__ inline int8_t export_byte(float v)
     float q = floorf(v * 127.0f);
     q = q > 127.0f ? 127.0f : (q < -128.0f ? -128.0f : q);
     return (int8_t)(int32_t)q;
+}
+
+// phi storage.  float: as computed.  int8_t (the reference's LEVEL_SET_CHAR,
+// Config.hpp:72-80): stored as __float2int_rd(v * 127) (UpdateLevelSetVolume.cu:
+// 316-318) and read as a normalised float, s / 127 with -128 -> -1
+// (cudaReadModeNormalizedFloat, Textures.cu:12).
+template <typename PhiT>
+__device__ inline float phi_load(const PhiT* p, uint32_t i)
+{
+    if constexpr (sizeof(PhiT) == 1) {
+        const int s = p[i];
+        return s == -128 ? -1.0f : (float)s / 127.0f;
+    } else {
+        return p[i];
+    }
+}
+
+template <typename PhiT>
+__device__ inline void phi_store(PhiT* p, uint32_t i, float v)
+{
+    if constexpr (sizeof(PhiT) == 1) p[i] = export_byte(v);
+    else p[i] = v;
+}
+
+template <typename PhiT>
+__device__ inline PhiT* phi_buf(const KArgs& a, int i)
+{
+    return static_cast<PhiT*>(a.phi[i]);
 }
 
 // ComputeDensityTerm (UpdateLevelSetVolume.cu:335-342): f = (-1/eps) |I - T| + 1
@@ -423,24 +452,29 @@ struct ListWalk {
 // tile; the single-volume kernels and the batched kernels (which take each
 // volume's arguments from a table) share them.
 // ---------------------------------------------------------------------------
-template <typename SrcT, bool kFloatSrc>
+template <typename SrcT, bool kFloatSrc, typename PhiT>
 __device__ inline void update_at(const KArgs& a, const DevParams& P, const uint32_t* __restrict__ V,
-                                 const float* __restrict__ R, float* __restrict__ W, uint64_t p)
+                                 const PhiT* __restrict__ R, PhiT* __restrict__ W, uint64_t p)
 {
     const uint32_t u = V[p];
     uint32_t nb[7], inb;
     neighbors(a.g, u, nb, &inb);
-    const float nv = update_core(P, R[nb[0]], R[nb[1]], R[nb[2]], R[u], R[nb[4]], R[nb[5]], R[nb[6]],
+    const float u4 = phi_load(R, u);
+    const float nv = update_core(P, phi_load(R, nb[0]), phi_load(R, nb[1]), phi_load(R, nb[2]), u4,
+                                 phi_load(R, nb[4]), phi_load(R, nb[5]), phi_load(R, nb[6]),
                                  density_factor<SrcT, kFloatSrc>(a, P, u));
-    W[u] = nv;
+    phi_store(W, u, nv);
     a.E[u] = export_byte(nv);
     a.C[u] = 0ull;  // the record of the candidates that produced this list
+    // fixed-point storage: the flag of the unquantised value (UpdateLevelSetVolume.cu:327-331)
+    if constexpr (sizeof(PhiT) == 1) a.flag[u] = fabsf(nv - u4) > kTemporalThreshold ? 1 : 0;
 }
 
 // marking of list position p (UpdateActiveElements.cu:75-103); returns the
 // number of candidate emissions
+template <typename PhiT>
 __device__ inline uint32_t mark_at(const KArgs& a, bool ref, const uint32_t* __restrict__ V,
-                                   const float* __restrict__ R, const float* __restrict__ W,
+                                   const PhiT* __restrict__ R, const PhiT* __restrict__ W,
                                    uint8_t* __restrict__ Cb, uint64_t p)
 {
     uint32_t m = 0;
@@ -448,12 +482,16 @@ __device__ inline uint32_t mark_at(const KArgs& a, bool ref, const uint32_t* __r
         const uint32_t u = V[p];
         uint32_t nb[7], inb;
         neighbors(a.g, u, nb, &inb);
-        const float nv = W[u];
+        const float nv = phi_load(W, u);
         float wn[7];
 #pragma unroll
-        for (int e = 0; e < 7; ++e) wn[e] = W[nb[e]];
-        // temporal test: the time derivative of the update (UpdateLevelSetVolume.cu:327-331)
-        if (fabsf(nv - R[u]) > kTemporalThreshold) {
+        for (int e = 0; e < 7; ++e) wn[e] = phi_load(W, nb[e]);
+        // temporal test: the time derivative of the update (UpdateLevelSetVolume.cu:327-331);
+        // FP32 storage recovers it exactly from the two buffers, fixed point stores it
+        bool moved;
+        if constexpr (sizeof(PhiT) == 1) moved = a.flag[u] != 0;
+        else moved = fabsf(nv - phi_load(R, u)) > kTemporalThreshold;
+        if (moved) {
 #pragma unroll
             for (int e = 0; e < 7; ++e) {
                 if (e == 3 || !((inb >> e) & 1u)) continue;
@@ -621,16 +659,17 @@ __device__ inline bool compact_tile(const KArgs& a, int q, const CompactPlan& c,
 // ---------------------------------------------------------------------------
 // single-volume iteration kernels
 // ---------------------------------------------------------------------------
-template <typename SrcT, bool kFloatSrc>
+template <typename SrcT, bool kFloatSrc, typename PhiT>
 __global__ __launch_bounds__(kBlock) void k_update(KArgs a, int q)
 {
     const DevParams P = *a.prm;
     const uint32_t n = a.ctl->n[q];
     ListWalk w(a.run_log2);
     for (uint64_t p = w.pos(); p < n; w.next(), p = w.pos())
-        update_at<SrcT, kFloatSrc>(a, P, a.list[q], a.phi[q], a.phi[q ^ 1], p);
+        update_at<SrcT, kFloatSrc, PhiT>(a, P, a.list[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), p);
 }
 
+template <typename PhiT>
 __global__ __launch_bounds__(kBlock) void k_mark(KArgs a, int q)
 {
     __shared__ uint32_t s_red[kBlock / 64];
@@ -639,7 +678,7 @@ __global__ __launch_bounds__(kBlock) void k_mark(KArgs a, int q)
     uint32_t emitted = 0;
     ListWalk w(a.run_log2);
     for (uint64_t p = w.pos(); p < n; w.next(), p = w.pos())
-        emitted += mark_at(a, ref, a.list[q], a.phi[q], a.phi[q ^ 1], (uint8_t*)a.C, p);
+        emitted += mark_at<PhiT>(a, ref, a.list[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, p);
     block_add(&a.ctl->c_acc, emitted, s_red);
 }
 
@@ -713,7 +752,7 @@ __device__ inline uint32_t batch_find(const uint32_t* s_pre, uint32_t count, uin
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
 }
 
-template <typename SrcT, bool kFloatSrc>
+template <typename SrcT, bool kFloatSrc, typename PhiT>
 __global__ __launch_bounds__(kBlock) void k_update_batch(BatchArgs b, int q)
 {
     __shared__ uint32_t s_pre[kMaxBatch + 1];
@@ -726,11 +765,12 @@ __global__ __launch_bounds__(kBlock) void k_update_batch(BatchArgs b, int q)
         const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
         if (p < b.ctls[v].n[q]) {
             const DevParams P = *a.prm;
-            update_at<SrcT, kFloatSrc>(a, P, a.list[q], a.phi[q], a.phi[q ^ 1], p);
+            update_at<SrcT, kFloatSrc, PhiT>(a, P, a.list[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), p);
         }
     }
 }
 
+template <typename PhiT>
 __global__ __launch_bounds__(kBlock) void k_mark_batch(BatchArgs b, int q)
 {
     __shared__ uint32_t s_pre[kMaxBatch + 1];
@@ -744,7 +784,8 @@ __global__ __launch_bounds__(kBlock) void k_mark_batch(BatchArgs b, int q)
         const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
         uint32_t emitted = 0;
         if (p < b.ctls[v].n[q])
-            emitted = mark_at(a, a.prm->refMode != 0, a.list[q], a.phi[q], a.phi[q ^ 1], (uint8_t*)a.C, p);
+            emitted = mark_at<PhiT>(a, a.prm->refMode != 0, a.list[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1),
+                                    (uint8_t*)a.C, p);
         block_add(&b.ctls[v].c_acc, emitted, s_red);
     }
 }
@@ -799,7 +840,7 @@ __global__ __launch_bounds__(kBlock) void k_record(KArgs a, int q, uint32_t n, u
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.ctl->n[q] != n) atomicOr(&a.ctl->error, kErrRecord);
     const DevParams P = *a.prm;
     const uint32_t* __restrict__ V = a.list[q];
-    const float* __restrict__ R = a.phi[q];
+    const float* __restrict__ R = phi_buf<float>(a, q);  // FP32 storage only
     LSSEG_FOR_EACH(p, n)
     {
         const uint32_t u = V[p];
@@ -993,7 +1034,9 @@ __global__ void k_pad_tile_source(Geom g, const uint8_t* lin, uint8_t* dst, uint
     }
 }
 
-// phi0 injection: both buffers, export byte, dense band flags in C
+// phi0 injection: both buffers, export byte, dense band flags in C (the
+// band test uses phi0 itself; fixed-point storage saturates beyond [-1, 1))
+template <typename PhiT>
 __global__ void k_set_phi(KArgs a, const float* lin, uint32_t nx, uint32_t ny, uint32_t nz, float band)
 {
     LSSEG_GRID_STRIDE(t, a.g.N)
@@ -1002,18 +1045,19 @@ __global__ void k_set_phi(KArgs a, const float* lin, uint32_t nx, uint32_t ny, u
         decode_xyz(a.g, (uint32_t)t, &x, &y, &z);
         float v = 1.0f;
         if (x < nx && y < ny && z < nz) v = lin[((uint64_t)z * ny + y) * nx + x];
-        a.phi[0][t] = v;
-        a.phi[1][t] = v;
+        phi_store(phi_buf<PhiT>(a, 0), (uint32_t)t, v);
+        phi_store(phi_buf<PhiT>(a, 1), (uint32_t)t, v);
         a.E[t] = export_byte(v);
         a.C[t] = fabsf(v) < band ? 1ull : 0ull;
     }
 }
 
 // InitializeLevelSetKernel (InitializeLevelSetVolume.cu:55-98)
+template <typename PhiT>
 __global__ void k_seed_pass(KArgs a, int rd, uint32_t sx, uint32_t sy, uint32_t sz, int32_t radius, float aniso)
 {
-    const float* R = a.phi[rd];
-    float* W = a.phi[rd ^ 1];
+    const PhiT* R = phi_buf<PhiT>(a, rd);
+    PhiT* W = phi_buf<PhiT>(a, rd ^ 1);
     LSSEG_GRID_STRIDE(t, a.g.N)
     {
         uint32_t x, y, z;
@@ -1026,49 +1070,51 @@ __global__ void k_seed_pass(KArgs a, int rd, uint32_t sx, uint32_t sy, uint32_t 
         float c = 1.0f;
         if (d < (float)radius) c = -1.0f;
         else if (d < (float)radius + kSmoothDistance) c = -1.0f + ((d - (float)radius) / (kSmoothDistance / 2.0f));
-        float nv = R[t] + c - 1.0f;
+        float nv = phi_load(R, (uint32_t)t) + c - 1.0f;
         nv = nv < -1.0f ? -1.0f : (nv > 1.0f ? 1.0f : nv);
-        W[t] = nv;
+        phi_store(W, (uint32_t)t, nv);
         a.E[t] = export_byte(nv);
     }
 }
 
 // InitializeActiveElementsVolumeKernel (InitializeActiveElementsVolume.cu:37-74):
 // dense keep flags into C
+template <typename PhiT>
 __global__ void k_keep_flags(KArgs a, int rd)
 {
-    const float* R = a.phi[rd];
+    const PhiT* R = phi_buf<PhiT>(a, rd);
     LSSEG_GRID_STRIDE(t, a.g.N)
     {
         uint32_t nb[7], inb;
         neighbors(a.g, (uint32_t)t, nb, &inb);
-        const float u4 = R[t];
+        const float u4 = phi_load(R, (uint32_t)t);
         bool flat = true;
 #pragma unroll
         for (int e = 0; e < 7; ++e) {
             if (e == 3) continue;
-            if (!(fabsf(u4 - R[nb[e]]) <= kSpatialThreshold)) flat = false;
+            if (!(fabsf(u4 - phi_load(R, nb[e])) <= kSpatialThreshold)) flat = false;
         }
         a.C[t] = flat ? 0ull : 1ull;
     }
 }
 
 // update only (the parameter-change sub-step, Segmenter.cpp:811-876)
-template <typename SrcT, bool kFloatSrc>
+template <typename SrcT, bool kFloatSrc, typename PhiT>
 __global__ __launch_bounds__(kBlock) void k_update_only(KArgs a, int op)
 {
     const DevParams P = *a.prm;
     const uint32_t n = a.ctl->n[op];
-    const float* R = a.phi[op];
-    float* W = a.phi[op ^ 1];
+    const PhiT* R = phi_buf<PhiT>(a, op);
+    PhiT* W = phi_buf<PhiT>(a, op ^ 1);
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
         const uint32_t u = a.list[op][p];
         uint32_t nb[7], inb;
         neighbors(a.g, u, nb, &inb);
-        const float nv = update_core(P, R[nb[0]], R[nb[1]], R[nb[2]], R[u], R[nb[4]], R[nb[5]], R[nb[6]],
+        const float nv = update_core(P, phi_load(R, nb[0]), phi_load(R, nb[1]), phi_load(R, nb[2]), phi_load(R, u),
+                                     phi_load(R, nb[4]), phi_load(R, nb[5]), phi_load(R, nb[6]),
                                      density_factor<SrcT, kFloatSrc>(a, P, u));
-        W[u] = nv;
+        phi_store(W, u, nv);
         a.E[u] = export_byte(nv);
     }
 }
@@ -1138,6 +1184,7 @@ __global__ void k_freeze(KArgs a)
     LSSEG_GRID_STRIDE(t, a.g.N) a.frozen[t] = add_export(a.frozen[t], a.E[t]);
 }
 
+template <typename PhiT>
 __global__ void k_get_phi(KArgs a, int which, float* out, uint32_t nx, uint32_t ny, uint32_t nz)
 {
     const uint64_t cnt = (uint64_t)nx * ny * nz;
@@ -1146,7 +1193,24 @@ __global__ void k_get_phi(KArgs a, int which, float* out, uint32_t nx, uint32_t 
         const uint32_t x = (uint32_t)(i % nx);
         const uint32_t y = (uint32_t)((i / nx) % ny);
         const uint32_t z = (uint32_t)(i / ((uint64_t)nx * ny));
-        out[i] = a.phi[which][tau_of(a.g, x, y, z)];
+        out[i] = phi_load(phi_buf<PhiT>(a, which), tau_of(a.g, x, y, z));
+    }
+}
+
+// the raw state of a fixed-point phi buffer as floats (k / 127) and back; the
+// way back rounds, so a value k / 127 returns to k
+__global__ void k_phi8_to_float(const int8_t* in, float* out, uint64_t count)
+{
+    LSSEG_GRID_STRIDE(i, count) out[i] = phi_load(in, (uint32_t)i);
+}
+
+__global__ void k_float_to_phi8(const float* in, int8_t* out, uint64_t count)
+{
+    LSSEG_GRID_STRIDE(i, count)
+    {
+        float q = rintf(in[i] * 127.0f);
+        q = q > 127.0f ? 127.0f : (q < -128.0f ? -128.0f : q);
+        out[i] = (int8_t)(int32_t)q;
     }
 }
 

@@ -40,7 +40,7 @@ VolumeDesc Segmenter::LoadVolume(const VolumeDesc& volumeDesc)
     const int32_t dt = dtype_of(volumeDesc);
     if (dt < 0) throw Error(LSSEG_ERR_UNSUPPORTED, "LoadVolume: unsupported voxel type");
     lsseg_volume_desc d{volumeDesc.numVoxelsX, volumeDesc.numVoxelsY, volumeDesc.numVoxelsZ, dt,
-                        volumeDesc.zAnisotropy};
+                        volumeDesc.zAnisotropy, volumeDesc.fixedPointPhi ? LSSEG_PHI_I8 : LSSEG_PHI_F32};
     lsseg_ctx* c = nullptr;
     const lsseg_status st = lsseg_create(&d, volumeDesc.volumeData, &c);
     if (st != LSSEG_OK) throw Error(st, std::string("LoadVolume: ") + lsseg_last_error(nullptr));

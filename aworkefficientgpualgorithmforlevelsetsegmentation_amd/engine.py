@@ -18,6 +18,7 @@ OK = 0
 ERR_INVALID_ARG, ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_CAPACITY, ERR_STATE, ERR_UNSUPPORTED = range(1, 7)
 U8, I8, U16, I16, U32, I32, F32 = range(7)
 MODE_REF, MODE_CLEAN = 0, 1
+PHI_F32, PHI_I8 = 0, 1
 
 DTYPE_CODE = {np.dtype(np.uint8): U8, np.dtype(np.int8): I8, np.dtype(np.uint16): U16,
               np.dtype(np.int16): I16, np.dtype(np.uint32): U32, np.dtype(np.int32): I32,
@@ -47,7 +48,7 @@ class LssegError(RuntimeError):
 
 class VolumeDesc(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_uint32), ("ny", ctypes.c_uint32), ("nz", ctypes.c_uint32),
-                ("dtype", ctypes.c_int32), ("z_aniso", ctypes.c_float)]
+                ("dtype", ctypes.c_int32), ("z_aniso", ctypes.c_float), ("phi_storage", ctypes.c_int32)]
 
 
 class Params(ctypes.Structure):
@@ -181,7 +182,7 @@ class IterationCounts:
 class Context:
     """One segmentation context (the C ABI's ``lsseg_ctx``)."""
 
-    def __init__(self, volume_zyx: np.ndarray, z_aniso: float = 1.0):
+    def __init__(self, volume_zyx: np.ndarray, z_aniso: float = 1.0, phi_storage: int = PHI_F32):
         vol = np.ascontiguousarray(volume_zyx)
         if vol.ndim != 3:
             raise ValueError("volume must be 3-D (z, y, x)")
@@ -189,8 +190,9 @@ class Context:
             raise ValueError(f"unsupported dtype {vol.dtype}")
         self.nz, self.ny, self.nx = (int(v) for v in vol.shape)
         self.dtype = vol.dtype
+        self.phi_storage = phi_storage
         L = lib()
-        desc = VolumeDesc(self.nx, self.ny, self.nz, DTYPE_CODE[vol.dtype], z_aniso)
+        desc = VolumeDesc(self.nx, self.ny, self.nz, DTYPE_CODE[vol.dtype], z_aniso, phi_storage)
         h = ctypes.c_void_p()
         st = L.lsseg_create(ctypes.byref(desc), _p(vol), ctypes.byref(h))
         if st != OK:
@@ -201,11 +203,12 @@ class Context:
         _track(self)
 
     @classmethod
-    def _borrow(cls, handle, shape_zyx, dtype, owner):
+    def _borrow(cls, handle, shape_zyx, dtype, owner, phi_storage=PHI_F32):
         """A view of a context owned by someone else (a batch volume)."""
         self = cls.__new__(cls)
         self.nz, self.ny, self.nx = (int(v) for v in shape_zyx)
         self.dtype = np.dtype(dtype)
+        self.phi_storage = phi_storage
         self._h = handle
         self._owned = False
         self._owner = owner
@@ -372,7 +375,7 @@ class Batch:
     """Many volumes advanced together (the C ABI's ``lsseg_batch``).  ``volume(i)``
     is a ``Context`` view of volume i for initialisation and readback."""
 
-    def __init__(self, volumes_zyx, z_aniso: float = 1.0):
+    def __init__(self, volumes_zyx, z_aniso: float = 1.0, phi_storage: int = PHI_F32):
         vols = [np.ascontiguousarray(v) for v in volumes_zyx]
         if not vols:
             raise ValueError("empty batch")
@@ -381,7 +384,7 @@ class Batch:
                 raise ValueError("volumes must be 3-D arrays of a supported dtype")
         L = lib()
         descs = (VolumeDesc * len(vols))(*[VolumeDesc(v.shape[2], v.shape[1], v.shape[0], DTYPE_CODE[v.dtype],
-                                                      z_aniso) for v in vols])
+                                                      z_aniso, phi_storage) for v in vols])
         ptrs = (ctypes.c_void_p * len(vols))(*[v.ctypes.data for v in vols])
         h = ctypes.c_void_p()
         st = L.lsseg_batch_create(descs, ptrs, len(vols), ctypes.byref(h))
@@ -393,7 +396,7 @@ class Batch:
         for i, v in enumerate(vols):
             c = ctypes.c_void_p()
             self._check(L.lsseg_batch_volume(self._h, i, ctypes.byref(c)))
-            self._vols.append(Context._borrow(c, v.shape, v.dtype, self))
+            self._vols.append(Context._borrow(c, v.shape, v.dtype, self, phi_storage))
 
     def _check(self, st: int):
         if st != OK:

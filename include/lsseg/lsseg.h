@@ -58,11 +58,21 @@ typedef enum lsseg_dtype {
     LSSEG_U32 = 4, LSSEG_I32 = 5, LSSEG_F32 = 6
 } lsseg_dtype;
 
-/* Replaces VolumeDesc (VolumeDesc.hpp:12-38): dims, type, z anisotropy. */
+/* Storage of phi.  LSSEG_PHI_F32 keeps FP32 values (the parity target with the
+   FP64 oracle).  LSSEG_PHI_I8 is the reference's shipped LEVEL_SET_CHAR
+   representation (Config.hpp:72-80): stored as floor(127 phi)
+   (UpdateLevelSetVolume.cu:316-318), read as s / 127 with -128 -> -1 (the
+   normalised-float texture read, Textures.cu:12); 1 byte instead of 4 per
+   value.  The differentiable form needs LSSEG_PHI_F32. */
+typedef enum lsseg_phi_storage { LSSEG_PHI_F32 = 0, LSSEG_PHI_I8 = 1 } lsseg_phi_storage;
+
+/* Replaces VolumeDesc (VolumeDesc.hpp:12-38): dims, type, z anisotropy, plus
+   the phi storage (zero-initialised: FP32). */
 typedef struct lsseg_volume_desc {
     uint32_t nx, ny, nz;
     int32_t dtype;       /* lsseg_dtype */
     float z_aniso;       /* VolumeDesc::zAnisotropy, used by the seed sphere */
+    int32_t phi_storage; /* lsseg_phi_storage */
 } lsseg_volume_desc;
 
 typedef enum lsseg_mode {
@@ -195,7 +205,9 @@ lsseg_status lsseg_clear(lsseg_ctx* ctx, int32_t all);
 /* ---- raw state (checkpoint / resume, parity tests) ---------------------- */
 
 /* Raw padded tiled state: phi read buffer, phi write buffer, export byte
-   (N = X*Y*Z entries each; any pointer may be NULL) and the active list. */
+   (N = X*Y*Z entries each; any pointer may be NULL) and the active list.  With
+   LSSEG_PHI_I8 storage phi travels as the floats s / 127 it reads as, and
+   lsseg_set_state stores round(127 v). */
 lsseg_status lsseg_get_state(const lsseg_ctx* ctx, float* phi_read, float* phi_write,
                              int8_t* export_bytes, uint32_t* list, uint64_t list_cap,
                              uint64_t* n);
@@ -231,8 +243,9 @@ lsseg_status lsseg_iterate_profiled(lsseg_ctx* ctx, const lsseg_params* params, 
                                     lsseg_stats* stats, lsseg_kernel_times* times,
                                     float* launch_ms, uint64_t launch_cap);
 
-/* Device pointers of the phi buffers (read, write) and the stream, for callers
-   that keep data resident in HBM (e.g. a torch wrapper). */
+/* Device pointers of the phi buffers (read, write; float or int8_t by the
+   context's phi storage) and the stream, for callers that keep data resident
+   in HBM (e.g. a torch wrapper). */
 lsseg_status lsseg_get_device_buffers(const lsseg_ctx* ctx, void** phi_read, void** phi_write,
                                       void** stream);
 

@@ -42,6 +42,9 @@ struct VolumeDesc {
     unsigned char numBytesPerVoxel = 1;
     bool isSigned = false;
     bool isFloat = false;
+    // phi stored as the reference's fixed-point signed char (LEVEL_SET_CHAR,
+    // Config.hpp:72-80) instead of FP32 (an extension of VolumeDesc)
+    bool fixedPointPhi = false;
 };
 
 // The Segmenter system of a parameter file (bin/defaults/UnsignedChar.par).

@@ -74,6 +74,8 @@ def lib():
         L.lso_dims.argtypes = [vp, vp, vp, vp]
         L.lso_set_phi.restype = c_i32
         L.lso_set_phi.argtypes = [vp, vp, c_f]
+        L.lso_set_phi_storage.restype = None
+        L.lso_set_phi_storage.argtypes = [vp, c_i32]
         L.lso_seed_sphere.restype = c_i32
         L.lso_seed_sphere.argtypes = [vp, c_f, c_f, c_f, c_u32]
         L.lso_iterate.restype = c_i32
@@ -193,7 +195,8 @@ def path_is_few(n, N):
 class State:
     """Owning wrapper of lso_state (the oracle's Segmenter)."""
 
-    def __init__(self, volume_zyx: np.ndarray, z_aniso: float = 1.0):
+    def __init__(self, volume_zyx: np.ndarray, z_aniso: float = 1.0, phi_storage: int = 0):
+        """phi_storage 1: the reference's fixed-point (signed char) phi."""
         L = lib()
         vol = np.ascontiguousarray(volume_zyx)
         self.dtype_code = DTYPE_CODE[vol.dtype]
@@ -202,6 +205,7 @@ class State:
         self._h = L.lso_create(self.nx, self.ny, self.nz, self.dtype_code, _p(vol), z_aniso)
         if not self._h:
             raise ValueError("lso_create failed")
+        L.lso_set_phi_storage(self._h, 1 if phi_storage else 0)
         self.X, self.Y, self.Z = padded_dims(self.nx, self.ny, self.nz)
         self.N = self.X * self.Y * self.Z
 

@@ -213,6 +213,7 @@ struct lso_state {
     uint32_t* Vn;            /* scratch list */
     uint8_t* flags;          /* per position */
     uint8_t* masks;          /* per position */
+    int32_t phi_i8;          /* fixed-point phi storage (LEVEL_SET_CHAR, Config.hpp:72-80) */
 };
 
 static void* xcalloc(uint64_t count, uint64_t size)
@@ -297,6 +298,29 @@ static int8_t export_byte(float v)
     return (int8_t)(int32_t)q;
 }
 
+/* The value a stored phi reads back as.  FP32 storage: itself.  Fixed point
+   (LEVEL_SET_CHAR, Config.hpp:72-80): stored as __float2int_rd(v * 127)
+   (UpdateLevelSetVolume.cu:316-318), read through a normalised-float texture
+   as s / 127 with -128 -> -1 (Textures.cu:12). */
+static float phi_stored(const lso_state* s, float v)
+{
+    if (!s->phi_i8) return v;
+    const int8_t q = export_byte(v);
+    return q == -128 ? -1.0f : (float)q / 127.0f;
+}
+
+void lso_set_phi_storage(lso_state* s, int32_t fixed_point)
+{
+    s->phi_i8 = fixed_point ? 1 : 0;
+}
+
+/* fixed point: the values just written at the list voxels become what they read back as */
+static void store_list(lso_state* s, float* W)
+{
+    if (!s->phi_i8) return;
+    for (int64_t p = 0; p < s->n; ++p) W[s->V[p]] = phi_stored(s, W[s->V[p]]);
+}
+
 /* Build extension used by the benchmark configurations: both buffers hold phi0,
    the export byte is floor(127 phi0), and V0 = {|phi0| < band} in ascending tau
    (the order a dense compaction produces, Cuda.cpp:699-758). */
@@ -314,13 +338,14 @@ int32_t lso_set_phi(lso_state* s, const float* phi0_linear, float band)
                 s->phi[0][t] = phi0_linear[li];
                 s->E[t] = export_byte(phi0_linear[li]);
             }
-    memcpy(s->phi[1], s->phi[0], (size_t)(s->N * sizeof(float)));
     s->rd = 0;
     int64_t m = 0;
     for (uint64_t t = 0; t < s->N; ++t) {
         const float v = s->phi[0][t];
         if ((v < 0 ? -v : v) < band) s->V[m++] = (uint32_t)t;
+        s->phi[0][t] = phi_stored(s, v); /* the band uses phi0 itself */
     }
+    memcpy(s->phi[1], s->phi[0], (size_t)(s->N * sizeof(float)));
     s->n = m;
     return 0;
 }
@@ -352,7 +377,7 @@ int32_t lso_seed_sphere(lso_state* s, float cx, float cy, float cz, uint32_t rad
                     float nv = R[t] + c - 1.0f;
                     if (nv < -1.0f) nv = -1.0f;
                     if (nv > 1.0f) nv = 1.0f;
-                    W[t] = nv;
+                    W[t] = phi_stored(s, nv);
                     s->E[t] = export_byte(nv);
                 }
         s->rd = 1 - s->rd; /* SwapLevelSetBuffers, Segmenter.cpp:1339-1351 */
@@ -386,6 +411,7 @@ void lso_iterate_reinit(lso_state* s, float T, float eps, float alpha, float dt)
 {
     lso_update_f32(s->X, s->Y, s->Z, s->dtype, s->src, s->phi[s->rd], s->phi[1 - s->rd], s->E, s->flags,
                    s->V, s->n, T, eps, alpha, dt, 1);
+    store_list(s, s->phi[1 - s->rd]);
     s->rd = 1 - s->rd;
     lso_reinit_active(s);
 }
@@ -401,6 +427,7 @@ int32_t lso_iterate(lso_state* s, float T, float eps, float alpha, float dt, int
         float* W = s->phi[1 - s->rd];
         lso_update_f32(s->X, s->Y, s->Z, s->dtype, s->src, R, W, s->E, s->flags, s->V, s->n,
                        T, eps, alpha, dt, threads);
+        store_list(s, W);
         s->rd = 1 - s->rd;
         lso_mark(s->X, s->Y, s->Z, s->phi[s->rd], s->flags, s->V, s->n, mode, s->masks, threads);
         uint64_t c = 0;

@@ -80,6 +80,9 @@ lso_state* lso_create(int32_t nx, int32_t ny, int32_t nz, int32_t dtype, const v
 void       lso_destroy(lso_state* s);
 void       lso_dims(const lso_state* s, int32_t* X, int32_t* Y, int32_t* Z);
 /* phi0 over the unpadded box (x fastest); padded voxels get 1.0; V0 = {|phi0| < band} */
+/* fixed_point != 0: phi stored as the reference's LEVEL_SET_CHAR (call before
+   initialising phi) */
+void       lso_set_phi_storage(lso_state* s, int32_t fixed_point);
 int32_t    lso_set_phi(lso_state* s, const float* phi0_linear, float band);
 /* reference seed initialisation: two init passes + gradient keep-flags */
 int32_t    lso_seed_sphere(lso_state* s, float cx, float cy, float cz, uint32_t radius);
