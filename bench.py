@@ -66,11 +66,13 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def algorithmic_bytes(n, c, m, S):
+def algorithmic_bytes(n, c, m, S, P=4):
+    """The gather model with P bytes per phi value: update n (4 + 7P + S + P + 2),
+    mark n (5 + 7P), compaction 10 c + 5 m."""
     n = n.astype(np.float64)
     c = c.astype(np.float64)
     m = m.astype(np.float64)
-    return {"update": (38 + S) * n, "mark": 33.0 * n, "compact": 10.0 * c + 5.0 * m}
+    return {"update": (6 + 8 * P + S) * n, "mark": (5 + 7 * P) * n, "compact": 10.0 * c + 5.0 * m}
 
 
 KERNELS = ("k_update", "k_mark", "k_compact")
@@ -156,12 +158,12 @@ def describe(w, X, Y, Z):
             f"active band |phi0|<{w.band:g}")
 
 
-def cpu_baseline(w, gpu_mask, seconds_1core):
+def cpu_baseline(w, gpu_mask, seconds_1core, phi_storage=0):
     """The oracle on this host: full workload on all cores (also gives the mask
     for the Dice check), plus a bounded single-core sample."""
     import oracle as O
     cores = host_cores()
-    ref = O.State(w.volume)
+    ref = O.State(w.volume, phi_storage=phi_storage)
     ref.set_phi(w.phi0, w.band)
     t0 = time.perf_counter()
     n_tot = 0
@@ -177,7 +179,7 @@ def cpu_baseline(w, gpu_mask, seconds_1core):
     dice = synth.dice(gpu_mask, mask)
     identical = bool(np.array_equal(gpu_mask, mask))
     # single core: the first iterations of the same workload, bounded in time
-    one = O.State(w.volume)
+    one = O.State(w.volume, phi_storage=phi_storage)
     one.set_phi(w.phi0, w.band)
     t0 = time.perf_counter()
     n1 = 0
@@ -386,6 +388,8 @@ def main():
     ap.add_argument("--workload", default="phantom", choices=["ellipsoids", "phantom", "vessels"])
     ap.add_argument("--iterations", type=int, default=0, help="override the workload's iteration count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--phi", default="f32", choices=["f32", "i8"],
+                    help="phi storage: FP32 (default) or the reference's fixed-point signed char")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-sequential comparison")
     ap.add_argument("--batch-iterations", type=int, default=0, help="override the batch workload's iterations")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-form comparison")
@@ -428,15 +432,17 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import Context
+    from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import PHI_F32, PHI_I8, Context
 
     t = time.perf_counter()
     w = make_workload(args.workload, args.iterations)
     log(f"workload generated in {time.perf_counter() - t:.1f} s")
-    ctx = Context(w.volume)
+    phi_storage = PHI_I8 if args.phi == "i8" else PHI_F32
+    ctx = Context(w.volume, phi_storage=phi_storage)
     ctx.set_phi(w.phi0, w.band)
     ctx.checkpoint_save(0)
     S = w.volume.dtype.itemsize
+    P = 1 if args.phi == "i8" else 4
     log(f"context ready: {describe(w, ctx.X, ctx.Y, ctx.Z)}; n0={ctx.active_count}")
 
     for _ in range(args.warmup):
@@ -469,7 +475,7 @@ def main():
     pst, kt, per_launch = ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, w.iterations)
     pc = ctx.iteration_counts()
     assert np.array_equal(pc.n, counts.n), "profiled run diverged from the timed run"
-    B = algorithmic_bytes(pc.n, pc.c, pc.m, S)
+    B = algorithmic_bytes(pc.n, pc.c, pc.m, S, P)
     it = w.iterations
     assert per_launch.size == 3 * it, f"expected {3 * it} launches, got {per_launch.size}"
     per_launch = per_launch.astype(np.float64)
@@ -514,7 +520,7 @@ def main():
     identical = None
     if not args.no_cpu_baseline:
         log("running the CPU oracle (full workload on all cores, then a single-core sample)")
-        cpu, dice, identical = cpu_baseline(w, gpu_mask, args.cpu_seconds)
+        cpu, dice, identical = cpu_baseline(w, gpu_mask, args.cpu_seconds, 1 if args.phi == "i8" else 0)
         log(f"cpu baseline {cpu['value']:.3e} updates/s on {cpu['cores']} cores; Dice vs oracle {dice:.6f}")
 
     out = {
@@ -536,6 +542,7 @@ def main():
             "iterations": w.iterations,
             "active_voxel_updates_per_step": updates_per_step,
             "parallelism": "single GPU",
+            "phi_storage": "fixed-point int8 (LEVEL_SET_CHAR)" if args.phi == "i8" else "FP32",
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
