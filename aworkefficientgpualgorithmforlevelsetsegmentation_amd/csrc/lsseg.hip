@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <initializer_list>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -63,6 +64,7 @@ struct lsseg_ctx {
         void* phi[2] = {nullptr, nullptr};
         int8_t* E = nullptr;
         uint32_t* list = nullptr;
+        uint32_t* S = nullptr;        // the traversal order of the list
         uint32_t n = 0;
         bool valid = false;
     } ckpt[LSSEG_MAX_CHECKPOINTS];
@@ -125,6 +127,22 @@ void destroy_graphs(lsseg_ctx* c)
     }
 }
 
+// resident workgroups per CU that every listed kernel reaches (the look-back
+// scans need the whole grid resident)
+int min_occupancy(std::initializer_list<const void*> kernels)
+{
+    int best = 1 << 30;
+    for (const void* k : kernels) {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, kBlock, 0) != hipSuccess || occ < 1) {
+            (void)hipGetLastError();
+            occ = 1;
+        }
+        best = std::min(best, occ);
+    }
+    return best;
+}
+
 uint32_t dense_grid(uint64_t count)
 {
     const uint64_t b = (count + kBlock - 1) / kBlock;
@@ -154,6 +172,7 @@ int env_int(const char* name, int dflt)
     }                                                                                                 \
     const void* fn(int32_t dtype, bool phi8) { return phi8 ? fn##_t<int8_t>(dtype) : fn##_t<float>(dtype); }
 LSSEG_SRC_PHI_KERNEL(update_kernel, k_update)
+LSSEG_SRC_PHI_KERNEL(update_fused_kernel, k_update_fused)
 LSSEG_SRC_PHI_KERNEL(update_only_kernel, k_update_only)
 LSSEG_SRC_PHI_KERNEL(update_batch_kernel, k_update_batch)
 #undef LSSEG_SRC_PHI_KERNEL
@@ -196,11 +215,28 @@ void launch(lsseg_ctx* c, const void* fn, uint32_t grid, int q)
     (void)hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args, 0, c->stream);
 }
 
-// one iteration over list[q]: update, marking, compaction into list[q ^ 1]
+// The update over S[q]; `fused`: in the same launch as the compaction of
+// list[q ^ 1] into list[q] that the previous iteration left pending.
+void launch_head(lsseg_ctx* c, int q, bool fused)
+{
+    if (fused) launch(c, update_fused_kernel(c->dtype, c->phi8), c->grid, q);
+    else launch(c, update_kernel(c->dtype, c->phi8), c->sgrid, q);
+}
+
+// marking and keep step of the iteration over list[q]; afterwards the count
+// and the traversal of the next list are set, its compaction is pending
+void launch_tail(lsseg_ctx* c, int q)
+{
+    launch(c, mark_kernel(c->phi8), c->sgrid, q);
+    launch(c, (const void*)&k_keep, c->grid, q);
+}
+
+// one complete iteration over list[q] (update, marking, keep, compaction into
+// list[q ^ 1]); the context is at rest afterwards
 void launch_iteration(lsseg_ctx* c, int q)
 {
-    launch(c, update_kernel(c->dtype, c->phi8), c->sgrid, q);
-    launch(c, mark_kernel(c->phi8), c->sgrid, q);
+    launch_head(c, q, false);
+    launch_tail(c, q);
     launch(c, (const void*)&k_compact<kCompact>, c->grid, q);
 }
 
@@ -226,17 +262,33 @@ lsseg_status dense_init_list(lsseg_ctx* c, int target)
     return LSSEG_OK;
 }
 
-// `count` iterations starting from list parity q; runs of kGraphIters
-// iterations (even, so the parity returns to its start) replay a captured graph.
+// `count` iterations starting from list parity q, at rest before and after.
+// Each iteration's compaction runs in the next iteration's update launch
+// (k_update_fused), the last one on its own: 3 count + 1 launches.  Runs of
+// kGraphIters fused iterations (even, so the parity returns to its start)
+// replay a captured graph.  LSSEG_FUSE=0 issues the four launches of every
+// iteration separately instead (the same results).
 lsseg_status launch_iterations(lsseg_ctx* c, int count, int q)
 {
+    if (count <= 0) return LSSEG_OK;
+    if (!env_int("LSSEG_FUSE", 1)) {
+        for (int i = 0; i < count; ++i) launch_iteration(c, q ^ (i & 1));
+        return LSSEG_OK;
+    }
+    launch_head(c, q, false);
+    launch_tail(c, q);
+    q ^= 1;
+    --count;
     const int chunk = env_int("LSSEG_GRAPH", 1) ? kGraphIters : 0;
     while (chunk && !c->profile && count >= chunk) {
         hipGraphExec_t& ge = c->graphs[q];
         if (!ge) {
             hipGraph_t g = nullptr;
             HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < chunk; ++i) launch_iteration(c, q ^ (i & 1));
+            for (int i = 0; i < chunk; ++i) {
+                launch_head(c, q ^ (i & 1), true);
+                launch_tail(c, q ^ (i & 1));
+            }
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             if (e != hipSuccess) return fail(c, LSSEG_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
             e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
@@ -250,9 +302,11 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q)
         count -= chunk;
     }
     for (int i = 0; i < count; ++i) {
-        launch_iteration(c, q);
+        launch_head(c, q, true);
+        launch_tail(c, q);
         q ^= 1;
     }
+    launch(c, (const void*)&k_compact<kCompact>, c->grid, q ^ 1);
     return LSSEG_OK;
 }
 
@@ -264,6 +318,7 @@ lsseg_status sync_and_check(lsseg_ctx* c)
         return fail(c, LSSEG_ERR_DEVICE, "device error flag 0x%x (%s)", c->h_ctl->error,
                     (c->h_ctl->error & kErrSpin)     ? "look-back wait exceeded its bound"
                     : (c->h_ctl->error & kErrRecord) ? "a recording run diverged from its first pass"
+                    : (c->h_ctl->error & kErrCount)  ? "the compaction's count differs from the keep step's"
                                                      : "list overflow");
     return LSSEG_OK;
 }
@@ -327,16 +382,30 @@ lsseg_status ensure_stats(lsseg_ctx* c, uint64_t iters)
     return LSSEG_OK;
 }
 
-// Makes list[0] (n entries, already on the device) the current list and
-// rebuilds the membership record C.
-lsseg_status inject_list_counts(lsseg_ctx* c, uint32_t n)
+// Makes list[0] (n entries, already on the device) the current list: rebuilds
+// the membership record C, the positions and the traversal S[0], taken from
+// S_dev (device, n entries) if given, else the list sorted by tiled index.
+lsseg_status inject_list_counts(lsseg_ctx* c, uint32_t n, const uint32_t* S_dev = nullptr)
 {
     const uint64_t N = c->a.g.N;
     c->q = 0;
     fill_u64(c, c->a.C, N, 0);
-    if (n)
+    if (n) {
         hipLaunchKernelGGL(k_scatter_membership, dim3(dense_grid(n)), dim3(kBlock), 0, c->stream, c->a.list[0], n,
                            c->a.C);
+        hipLaunchKernelGGL(k_scatter_positions, dim3(dense_grid(n)), dim3(kBlock), 0, c->stream, c->a.list[0], n,
+                           c->a.pos);
+        if (S_dev) {
+            HIP_TRY(c, hipMemcpyAsync(c->a.S[0], S_dev, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+        } else {
+            std::vector<uint32_t> h(n);
+            HIP_TRY(c, hipMemcpyAsync(h.data(), c->a.list[0], n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            std::sort(h.begin(), h.end());
+            HIP_TRY(c, hipMemcpyAsync(c->a.S[0], h.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+        }
+    }
     c->h_ctl->n[0] = n;
     c->h_ctl->n[1] = 0;
     HIP_TRY(c, hipMemcpyAsync(c->a.ctl->n, c->h_ctl->n, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
@@ -444,14 +513,10 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     int cus = 0;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
     c->cus = (uint32_t)std::max(1, cus);
-    // the look-back needs every workgroup of the compaction kernel resident at once
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)&k_compact<kCompact>, kBlock, 0) !=
-            hipSuccess ||
-        occ < 1) {
-        (void)hipGetLastError();
-        occ = 1;
-    }
+    // the look-back needs every workgroup of the keep and compaction kernels
+    // (and of the update fused with a compaction) resident at once
+    const int occ = min_occupancy({(const void*)&k_compact<kCompact>, (const void*)&k_compact<kInit>,
+                                   (const void*)&k_keep, update_fused_kernel(desc->dtype, c->phi8)});
     const int want = env_int("LSSEG_GRID_BLOCKS", 0);
     c->grid = (uint32_t)occ * c->cus;
     if (want > 0) c->grid = std::min<uint32_t>(c->grid, (uint32_t)want);
@@ -497,7 +562,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     g.dTXY = make_fastdiv(g.TX * g.TY);
 
     void* p = nullptr;
-    const uint64_t ntiles = N / kScanTile + 2;
+    const uint64_t ntiles = N / std::min(kScanTile, kKeepTile) + 2;  // look-back states of both scans
     lsseg_status st;
 #define ALLOC(field, type, count)                                          \
     if ((st = dalloc(c, &p, (size_t)(count) * sizeof(type))) != LSSEG_OK) \
@@ -513,7 +578,12 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     ALLOC(C, uint64_t, N);
     ALLOC(list[0], uint32_t, N);
     ALLOC(list[1], uint32_t, N);
-    ALLOC(mlist, uint8_t, N);
+    ALLOC(S[0], uint32_t, N);
+    ALLOC(S[1], uint32_t, N);
+    ALLOC(pos, uint32_t, N);
+    ALLOC(mvox, uint8_t, N);
+    ALLOC(keep, uint8_t, N);
+    ALLOC(memb, uint8_t, N);
     ALLOC(tstate, uint64_t, ntiles);
     ALLOC(ctl, DevCtl, 1);
 #undef ALLOC
@@ -553,6 +623,13 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     if (c->phi8) hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, c->a.flag, N, (uint8_t)0);
     fill_u64(c, c->a.C, N, 0);
     fill_u64(c, c->a.tstate, ntiles, 0);
+    // positions, masks and keep bytes of voxels outside the list are never read;
+    // zero keeps every entry a valid index
+    hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, (uint8_t*)c->a.pos, N * sizeof(uint32_t),
+                       (uint8_t)0);
+    hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, c->a.mvox, N, (uint8_t)0);
+    hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, c->a.keep, N, (uint8_t)0);
+    hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, c->a.memb, N, (uint8_t)0);
     e = hipMemsetAsync(c->a.ctl, 0, sizeof(DevCtl), c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
@@ -726,7 +803,7 @@ lsseg_status lsseg_iterate_profiled(lsseg_ctx* c, const lsseg_params* prm, int32
 {
     if (!c || !times) return LSSEG_ERR_INVALID_ARG;
     if (iters < 0) return fail(c, LSSEG_ERR_INVALID_ARG, "negative iteration count");
-    const size_t need = 6 * (size_t)iters;
+    const size_t need = 8 * (size_t)iters + 2;
     while (c->prof_events.size() < need) {
         hipEvent_t e;
         HIP_TRY(c, hipEventCreate(&e));
@@ -739,8 +816,9 @@ lsseg_status lsseg_iterate_profiled(lsseg_ctx* c, const lsseg_params* prm, int32
     if (st != LSSEG_OK) return st;
     memset(times, 0, sizeof(*times));
     if (iters == 0) return LSSEG_OK;
-    // a start/stop event pair per launch: update, marking, compaction of each
-    // iteration in turn
+    // a start/stop event pair per launch, in issue order: the update, marking
+    // and keep step of each iteration (the update from the second iteration on
+    // fused with the previous compaction), then the last compaction
     const int32_t launches = (int32_t)(c->prof_next / 2);
     for (int32_t i = 0; i < launches; ++i) {
         float ms = 0;
@@ -961,6 +1039,8 @@ lsseg_status lsseg_checkpoint_save(lsseg_ctx* c, int32_t slot)
         k.E = (int8_t*)p;
         if ((st = dalloc(c, &p, N * sizeof(uint32_t))) != LSSEG_OK) return st;
         k.list = (uint32_t*)p;
+        if ((st = dalloc(c, &p, N * sizeof(uint32_t))) != LSSEG_OK) return st;
+        k.S = (uint32_t*)p;
     }
     uint64_t n = 0;
     lsseg_status st = lsseg_get_active_count(c, &n);
@@ -968,7 +1048,10 @@ lsseg_status lsseg_checkpoint_save(lsseg_ctx* c, int32_t slot)
     HIP_TRY(c, hipMemcpyAsync(k.phi[0], c->a.phi[c->q], N * c->P, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(k.phi[1], c->a.phi[c->q ^ 1], N * c->P, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(k.E, c->a.E, N, hipMemcpyDeviceToDevice, c->stream));
-    if (n) HIP_TRY(c, hipMemcpyAsync(k.list, c->a.list[c->q], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    if (n) {
+        HIP_TRY(c, hipMemcpyAsync(k.list, c->a.list[c->q], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(k.S, c->a.S[c->q], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     k.n = (uint32_t)n;
     k.valid = true;
@@ -986,7 +1069,7 @@ lsseg_status lsseg_checkpoint_restore(lsseg_ctx* c, int32_t slot)
     HIP_TRY(c, hipMemcpyAsync(c->a.phi[1], k.phi[1], N * c->P, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->a.E, k.E, N, hipMemcpyDeviceToDevice, c->stream));
     if (k.n) HIP_TRY(c, hipMemcpyAsync(c->a.list[0], k.list, k.n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
-    lsseg_status st = inject_list_counts(c, k.n);
+    lsseg_status st = inject_list_counts(c, k.n, k.S);
     if (st != LSSEG_OK) return st;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return LSSEG_OK;
@@ -1068,6 +1151,7 @@ void launch_batch_iteration(lsseg_batch* b, int q)
     const bool phi8 = b->vols[0]->phi8;
     (void)hipLaunchKernel(update_batch_kernel(b->dtype, phi8), dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
     (void)hipLaunchKernel(mark_batch_kernel(phi8), dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
+    (void)hipLaunchKernel((const void*)&k_keep_batch, dim3(b->grid), dim3(kBlock), kargs, 0, b->stream);
     (void)hipLaunchKernel((const void*)&k_compact_batch, dim3(b->grid), dim3(kBlock), kargs, 0, b->stream);
 }
 
@@ -1111,6 +1195,7 @@ lsseg_status normalise_parity(lsseg_ctx* c)
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     std::swap(c->a.phi[0], c->a.phi[1]);
     std::swap(c->a.list[0], c->a.list[1]);
+    std::swap(c->a.S[0], c->a.S[1]);
     c->q = 0;
     destroy_graphs(c);
     return LSSEG_OK;
@@ -1218,12 +1303,7 @@ lsseg_status lsseg_batch_create(const lsseg_volume_desc* descs, const void* cons
         return bail(LSSEG_ERR_DEVICE);
     }
     b->cus = b->vols[0]->cus;
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)&k_compact_batch, kBlock, 0) != hipSuccess ||
-        occ < 1) {
-        (void)hipGetLastError();
-        occ = 1;
-    }
+    const int occ = min_occupancy({(const void*)&k_compact_batch, (const void*)&k_keep_batch});
     b->max_compact = (uint32_t)occ * b->cus;
     b->grid = b->max_compact;
     b->sgrid = 8u * b->cus;

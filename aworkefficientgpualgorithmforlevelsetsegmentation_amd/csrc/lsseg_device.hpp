@@ -3,25 +3,34 @@
 // Reference paths in comments are relative to code/gpuseg/code/ of the
 // reference repository.
 //
-// Three kernel launches per iteration k (all counts in device memory):
-//   update   one lane per list position: the 7-point update of phi
-//            (UpdateLevelSetVolume.cu:222-342) from the read buffer into the
-//            write buffer, plus the export byte;
-//   mark     one lane per list position: the temporal AND spatial test on the
-//            new phi (UpdateActiveElements.cu:75-103), positions p % 4 == 3
-//            skipped (UpdateActiveElementsVolume.cu:329), candidates recorded;
-//   compact  duplicate removal + stable compaction of the candidates into the
-//            next list (Segmenter.cpp:883-1019) with a single-pass decoupled
-//            look-back scan (Merrill & Garland, "Single-pass Parallel Prefix Scan
-//            with Decoupled Look-back", 2016).
+// The active list V (the reference's order, which decides the p % 4 == 3 rule
+// and the next list's order) is kept together with S, the same voxels in a
+// spatially coherent order, and pos[u], the position of u in V.  The list order
+// scatters neighbouring voxels over the whole volume after a few hundred
+// iterations, so every per-voxel step runs over S, where consecutive lanes
+// touch neighbouring voxels, and only the compaction walks V.  Four kernel
+// launches per iteration k (all counts in device memory):
+//   update   over S: the 7-point update of phi (UpdateLevelSetVolume.cu:222-342)
+//            from the read buffer into the write buffer, plus the export byte;
+//   mark     over S: the temporal AND spatial test on the new phi
+//            (UpdateActiveElements.cu:75-103), positions p % 4 == 3 skipped
+//            (UpdateActiveElementsVolume.cu:329), candidates recorded;
+//   keep     over S: first-wins duplicate removal (FilterDuplicates.cu:42-66),
+//            one byte of kept lists per list position, and the kept voxels in
+//            S order, which become the next S;
+//   compact  over the candidate slots in list order: stable compaction into the
+//            next list (Segmenter.cpp:883-1019, Cuda.cpp:699-758) and its
+//            positions.
+// Both scans are single-pass decoupled look-back scans (Merrill & Garland,
+// "Single-pass Parallel Prefix Scan with Decoupled Look-back", 2016).
 //
 // Emission record C (8 bytes per voxel): byte e of C[u] is 1 iff u was emitted
 // into candidate list e (0:-z 1:-y 2:-x 3:self 4:+x 5:+y 6:+z,
-// UpdateActiveElementsVolume.cu:168-174).  First-wins duplicate removal over the
-// list-major concatenation (FilterDuplicates.cu:42-66) keeps candidate (e, p)
-// iff the lowest set byte of C[u] is e; the dense path keeps every u with
-// C[u] != 0 in tiled order.  The voxels with C[u] != 0 are exactly the next
-// list, so the next update clears C at the voxels it visits.
+// UpdateActiveElementsVolume.cu:168-174).  Over the list-major concatenation of
+// the seven lists, first-wins keeps candidate (e, p) iff the lowest set byte
+// of C[u] is e; the dense path keeps every u with C[u] != 0 in tiled order.
+// The voxels with C[u] != 0 are exactly the next list, so the next update
+// clears C at the voxels it visits.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -40,9 +49,11 @@ constexpr float kConnTol = 0.1f;              // UpdateLevelSetVolume.cu:282-287
 constexpr int kBlock = 256;                   // threads per workgroup (4 waves of 64)
 constexpr int kItems = 8;                     // compaction slots per thread
 constexpr int kScanTile = kBlock * kItems;    // slots per look-back tile
+constexpr int kKeepItems = 2;                 // S positions per thread in the keep step
+constexpr int kKeepTile = kBlock * kKeepItems;
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded look-back wait
 
-enum : uint32_t { kErrSpin = 1u, kErrOverflow = 2u, kErrRecord = 4u };
+enum : uint32_t { kErrSpin = 1u, kErrOverflow = 2u, kErrRecord = 4u, kErrCount = 8u };
 
 enum KMode : int {
     kUpdate = 0,   // update of the current list
@@ -155,7 +166,7 @@ __host__ __device__ inline bool path_is_few(uint64_t n, uint64_t N) { return 7u 
 struct DevCtl {
     uint32_t n[2];        // entries of list[0], list[1]
     uint32_t iter;        // statistics index within the current lsseg_iterate call
-    uint32_t epoch;       // +1 per compaction launch; stamps the look-back states
+    uint32_t epoch;       // +1 per keep and compaction pass; stamps the look-back states
     uint32_t stats_cap;
     uint32_t error;
     uint64_t c_acc;       // candidate emissions of the iteration being marked
@@ -184,8 +195,12 @@ struct KArgs {
     int8_t* E;            // export byte (tiled)
     int8_t* frozen;       // frozen export (tiled)
     uint64_t* C;          // emission record
-    uint32_t* list[2];
-    uint8_t* mlist;       // per-position emission masks
+    uint32_t* list[2];    // active lists in the reference's order
+    uint32_t* S[2];       // the same voxels in a spatially coherent order (the traversal)
+    uint32_t* pos;        // list position of every voxel of the current list
+    uint8_t* mvox;        // emission mask of every voxel of the current list
+    uint8_t* keep;        // kept candidate lists of every list position
+    uint8_t* memb;        // members of the next list, for a dense compaction (zero at rest)
     uint64_t* tstate;     // look-back tile states
     DevCtl* ctl;
     const DevParams* prm;
@@ -234,10 +249,12 @@ __device__ inline uint32_t state_flag(uint64_t s, uint32_t ep)
     return ((uint32_t)(s >> 34) == ep) ? (uint32_t)((s >> 32) & 3u) : 0u;
 }
 
-// Decoupled look-back for tile `tile` with aggregate `total`, run by all 64
-// lanes of one wave.  The count travels inside the 64-bit state word, so no
-// other memory has to be ordered with it.  Returns the exclusive prefix.
-__device__ uint32_t lookback(uint64_t* states, uint32_t tile, uint32_t total, uint32_t epoch, uint32_t* err)
+// Decoupled look-back for tile `tile` with aggregate `total`, run by one wave:
+// each round reads the states of the 64 preceding tiles at once, adds the
+// aggregates back to the nearest inclusive prefix and publishes this tile's.
+// The count travels inside the 64-bit state word, so no other memory has to be
+// ordered with it.  Returns the exclusive prefix (in every lane).
+__device__ uint32_t wave_lookback(uint64_t* states, uint32_t tile, uint32_t total, uint32_t epoch, uint32_t* err)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t ep = epoch & 0x3FFFFFFFu;
@@ -276,6 +293,19 @@ __device__ uint32_t lookback(uint64_t* states, uint32_t tile, uint32_t total, ui
     }
     if (lane == 0) st_state(&states[tile], stamp | (2ull << 32) | (uint64_t)(excl + total));
     return excl;
+}
+
+// the look-back for a whole block: wave 0 resolves it, every thread gets the
+// exclusive prefix (through *s_excl)
+__device__ inline uint32_t lookback(uint64_t* states, uint32_t tile, uint32_t total, uint32_t epoch, uint32_t* err,
+                                    uint32_t* s_excl)
+{
+    if (threadIdx.x < 64) {
+        const uint32_t excl = wave_lookback(states, tile, total, epoch, err);
+        if (threadIdx.x == 0) *s_excl = excl;
+    }
+    __syncthreads();
+    return *s_excl;
 }
 
 // ---------------------------------------------------------------------------
@@ -473,24 +503,27 @@ __device__ inline void update_at(const KArgs& a, const DevParams& P, const uint3
 // marking of list position p (UpdateActiveElements.cu:75-103); returns the
 // number of candidate emissions
 template <typename PhiT>
-__device__ inline uint32_t mark_at(const KArgs& a, bool ref, const uint32_t* __restrict__ V,
+__device__ inline uint32_t mark_at(const KArgs& a, bool ref, const uint32_t* __restrict__ S,
                                    const PhiT* __restrict__ R, const PhiT* __restrict__ W,
-                                   uint8_t* __restrict__ Cb, uint64_t p)
+                                   uint8_t* __restrict__ Cb, uint64_t i)
 {
     uint32_t m = 0;
-    if (!ref || (p & 3u) != 3u) {  // UpdateActiveElementsVolume.cu:329
-        const uint32_t u = V[p];
-        uint32_t nb[7], inb;
-        neighbors(a.g, u, nb, &inb);
-        const float nv = phi_load(W, u);
-        float wn[7];
+    const uint32_t u = S[i];
+    // the position and every operand are loaded together (all addresses are
+    // voxels of the volume); the position then decides whether u emits
+    uint32_t nb[7], inb;
+    neighbors(a.g, u, nb, &inb);
+    const uint32_t pos = a.pos[u];
+    const float nv = phi_load(W, u);
+    float wn[7];
 #pragma unroll
-        for (int e = 0; e < 7; ++e) wn[e] = phi_load(W, nb[e]);
-        // temporal test: the time derivative of the update (UpdateLevelSetVolume.cu:327-331);
-        // FP32 storage recovers it exactly from the two buffers, fixed point stores it
-        bool moved;
-        if constexpr (sizeof(PhiT) == 1) moved = a.flag[u] != 0;
-        else moved = fabsf(nv - phi_load(R, u)) > kTemporalThreshold;
+    for (int e = 0; e < 7; ++e) wn[e] = phi_load(W, nb[e]);
+    // temporal test: the time derivative of the update (UpdateLevelSetVolume.cu:327-331);
+    // FP32 storage recovers it exactly from the two buffers, fixed point stores it
+    bool moved;
+    if constexpr (sizeof(PhiT) == 1) moved = a.flag[u] != 0;
+    else moved = fabsf(nv - phi_load(R, u)) > kTemporalThreshold;
+    if (!ref || (pos & 3u) != 3u) {  // list positions p % 4 == 3: UpdateActiveElementsVolume.cu:329
         if (moved) {
 #pragma unroll
             for (int e = 0; e < 7; ++e) {
@@ -505,7 +538,7 @@ __device__ inline uint32_t mark_at(const KArgs& a, bool ref, const uint32_t* __r
                 if ((m >> e) & 1u) Cb[(uint64_t)nb[e] * 8u + (uint32_t)e] = 1;
         }
     }
-    a.mlist[p] = (uint8_t)m;
+    a.mvox[u] = (uint8_t)m;
     return (uint32_t)__popc(m);
 }
 
@@ -550,40 +583,34 @@ __device__ inline CompactPlan compact_plan(const Geom& g, uint32_t n_src)
     return c;
 }
 
-// counts and statistics once the last tile is placed (one thread)
+// once the last tile is placed (one thread).  The keep step has already set
+// the count of the next list; the compaction must arrive at the same count.
 template <int kMode>
-__device__ inline void compact_finish(const KArgs& a, int q, const CompactPlan& c, uint32_t m, uint32_t epoch)
+__device__ inline void compact_finish(const KArgs& a, int q, uint32_t m, uint32_t epoch)
 {
     DevCtl* ctl = a.ctl;
-    ctl->n[q ^ 1] = m;
     if constexpr (kMode == kInit) {
+        ctl->n[q ^ 1] = m;
         ctl->n[q] = 0;
-    } else {
-        const uint32_t k = ctl->iter;
-        if (k < ctl->stats_cap) {
-            a.n_hist[k] = c.n_src;
-            a.c_hist[k] = ctl->c_acc;
-            a.m_hist[k] = m;
-        }
-        if (c.dense) ctl->many_iters += 1;
-        else ctl->few_iters += 1;
-        ctl->c_acc = 0;
-        ctl->iter = k + 1;
+    } else if (ctl->n[q ^ 1] != m) {
+        atomicOr(&ctl->error, kErrCount);
     }
     ctl->epoch = epoch + 1;
 }
 
-// One look-back tile of the compaction of list[q] (masks in mlist, record in
-// C) into list[q ^ 1]; kInit: the voxels with C != 0 in tiled order.  Every
-// thread of the block calls it.  Returns false (with the error flag set) if the
-// next list would overflow, which a consistent record never does.
+// One look-back tile of the compaction of list[q] into list[q ^ 1] (the
+// reference's order): few path, the kept candidates of the keep bytes in
+// list-major slot order; dense path, the voxels marked in `memb` in tiled order
+// (the marks are cleared on the way); kInit, the voxels with C != 0 in tiled
+// order.  Writes the positions of the new list.  Every thread of the block
+// calls it.  Returns false (with the error flag set) if the next list would
+// overflow, which a consistent record never does.
 template <int kMode>
 __device__ inline bool compact_tile(const KArgs& a, int q, const CompactPlan& c, uint32_t tile, uint32_t epoch,
                                     CompactShared& sh)
 {
     const uint32_t* __restrict__ Vs = a.list[q];
-    const uint8_t* __restrict__ Ms = a.mlist;
-    const uint64_t* __restrict__ Ck = a.C;
+    const uint8_t* __restrict__ Ks = a.keep;
     uint32_t* __restrict__ out = a.list[q ^ 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -599,25 +626,24 @@ __device__ inline bool compact_tile(const KArgs& a, int q, const CompactPlan& c,
         bool keep = false;
         uint32_t v = 0;
         if (s < c.slots) {
-            if (c.dense) {
-                keep = Ck[s] != 0ull;
+            if constexpr (kMode == kInit) {
+                keep = a.C[s] != 0ull;
                 v = (uint32_t)s;
+            } else if (c.dense) {
+                keep = a.memb[s] != 0;
+                v = (uint32_t)s;
+                if (keep) a.memb[s] = 0;
             } else {
                 // slot s = e * n_src + p: list-major concatenation of the
-                // seven candidate lists (Cuda.cpp:699-758 over 7 A slots)
+                // seven candidate lists (Cuda.cpp:699-758 over 7 A slots);
+                // the voxel and its keep byte are loaded together
                 const uint32_t e = (uint32_t)(s >= n_src) + (uint32_t)(s >= 2ull * n_src) +
                                    (uint32_t)(s >= 3ull * n_src) + (uint32_t)(s >= 4ull * n_src) +
                                    (uint32_t)(s >= 5ull * n_src) + (uint32_t)(s >= 6ull * n_src);
                 const uint32_t p = (uint32_t)(s - (uint64_t)e * n_src);
-                const uint32_t m = Ms[p];
-                const uint32_t t = Vs[p];
-                if ((m >> e) & 1u) {
-                    const uint32_t u = neighbor_in(a.g, t, e);
-                    const uint64_t rec = Ck[u];
-                    // first-wins: the lowest list that emitted u keeps it
-                    keep = (uint32_t)((__ffsll((unsigned long long)rec) - 1) >> 3) == e;
-                    v = u;
-                }
+                const uint32_t u = Vs[p];
+                keep = ((Ks[p] >> e) & 1u) != 0;
+                v = neighbor_in(a.g, u, e);
             }
         }
         bal[j] = __ballot(keep);
@@ -629,14 +655,10 @@ __device__ inline bool compact_tile(const KArgs& a, int q, const CompactPlan& c,
         const uint32_t cc = lane < kItems * 4 ? sh.cnt[lane] : 0u;
         const uint32_t incl = wave_inclusive_scan(cc, lane);
         if (lane < kItems * 4) sh.excl[lane] = incl - cc;
-        const uint32_t total = __shfl(incl, 63, 64);
-        const uint32_t prefix = lookback(a.tstate, tile, total, epoch, &a.ctl->error);
-        if (lane == 0) {
-            sh.prefix = prefix;
-            sh.total = total;
-        }
+        if (lane == 63) sh.total = incl;
     }
     __syncthreads();
+    lookback(a.tstate, tile, sh.total, epoch, &a.ctl->error, &sh.prefix);
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         if ((bal[j] >> lane) & 1ull) {
@@ -650,23 +672,187 @@ __device__ inline bool compact_tile(const KArgs& a, int q, const CompactPlan& c,
         if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
         return false;
     }
-    for (uint32_t i = threadIdx.x; i < total; i += kBlock) out[prefix + i] = sh.stage[i];
-    if (threadIdx.x == 0 && tile == c.ntiles - 1) compact_finish<kMode>(a, q, c, prefix + total, epoch);
+    for (uint32_t i = threadIdx.x; i < total; i += kBlock) {
+        const uint32_t v = sh.stage[i];
+        out[prefix + i] = v;
+        a.pos[v] = prefix + i;
+        if constexpr (kMode == kInit) a.S[q ^ 1][prefix + i] = v;  // tiled order is coherent
+    }
+    if (threadIdx.x == 0 && tile == c.ntiles - 1) compact_finish<kMode>(a, q, prefix + total, epoch);
     __syncthreads();
     return true;
 }
 
+// all compaction tiles of list[q] held by this workgroup (every workgroup of
+// the grid calls it; the grid is resident, so every wait ends)
+template <int kMode>
+__device__ inline void compact_all(const KArgs& a, int q, CompactShared& sh)
+{
+    const uint32_t epoch = a.ctl->epoch;
+    const CompactPlan c = compact_plan<kMode>(a.g, a.ctl->n[q]);
+    for (uint32_t tile = blockIdx.x; tile < c.ntiles; tile += gridDim.x)
+        if (!compact_tile<kMode>(a, q, c, tile, epoch, sh)) break;
+    if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kMode>(a, q, 0, epoch);
+}
+
 // ---------------------------------------------------------------------------
-// single-volume iteration kernels
+// keep step.  One look-back tile covers kKeepTile positions of S (slot
+// j * kBlock + tid of the tile).  Candidate (e, u -> v) is kept iff the lowest
+// set byte of C[v] is e (first-wins over the list-major concatenation,
+// FilterDuplicates.cu:42-66); the kept lists of u go to keep[pos[u]] for the
+// compaction, and the kept voxels are appended to the next S in S order, each
+// voxel's in list order: consecutive lanes write neighbouring voxels, so the
+// next S is as coherent as this one.  Every voxel of the next list is kept
+// exactly once; when the compaction takes the dense path it is also marked in
+// `memb`.  The last tile sets the count of the next list and the statistics.
+// ---------------------------------------------------------------------------
+struct KeepShared {
+    uint32_t cnt[kKeepItems * 4];
+    uint32_t excl[kKeepItems * 4];
+    uint32_t prefix, total;
+};
+
+__device__ inline uint32_t keep_tiles(uint32_t n) { return (n + kKeepTile - 1) / kKeepTile; }
+
+// counts and statistics of the iteration from list[q] (n entries) to the next
+// list (m entries); one thread
+__device__ inline void keep_finish(const KArgs& a, int q, uint32_t n, uint32_t m, uint32_t epoch)
+{
+    DevCtl* ctl = a.ctl;
+    ctl->n[q ^ 1] = m;
+    const uint32_t k = ctl->iter;
+    if (k < ctl->stats_cap) {
+        a.n_hist[k] = n;
+        a.c_hist[k] = ctl->c_acc;
+        a.m_hist[k] = m;
+    }
+    if (path_is_few(n, a.g.N)) ctl->few_iters += 1;
+    else ctl->many_iters += 1;
+    ctl->c_acc = 0;
+    ctl->iter = k + 1;
+    ctl->epoch = epoch + 1;
+}
+
+__device__ inline bool keep_tile(const KArgs& a, int q, uint32_t n, uint32_t tile, uint32_t epoch, KeepShared& sh)
+{
+    const uint32_t* __restrict__ S = a.S[q];
+    uint32_t* __restrict__ Sn = a.S[q ^ 1];
+    const uint64_t* __restrict__ Ck = a.C;
+    const bool dense = !path_is_few(n, a.g.N);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const uint64_t base = (uint64_t)tile * kKeepTile;
+    // loads in three rounds, each issued at once: the voxels, their masks and
+    // positions, then the records of all their emissions (inactive lanes load
+    // a valid address and ignore it)
+    uint32_t us[kKeepItems], ms[kKeepItems], ps[kKeepItems], kb[kKeepItems], ex[kKeepItems];
+#pragma unroll
+    for (int j = 0; j < kKeepItems; ++j) {
+        const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+        us[j] = S[i < n ? i : 0];
+    }
+#pragma unroll
+    for (int j = 0; j < kKeepItems; ++j) {
+        const bool valid = base + (uint64_t)j * kBlock + threadIdx.x < n;
+        ms[j] = valid ? a.mvox[us[j]] : 0u;
+        ps[j] = a.pos[us[j]];
+    }
+    uint64_t rec[kKeepItems][7];
+#pragma unroll
+    for (int j = 0; j < kKeepItems; ++j)
+#pragma unroll
+        for (int e = 0; e < 7; ++e)
+            rec[j][e] = Ck[((ms[j] >> e) & 1u) ? neighbor_in(a.g, us[j], (uint32_t)e) : us[j]];
+#pragma unroll
+    for (int j = 0; j < kKeepItems; ++j) {
+        uint32_t k = 0;
+#pragma unroll
+        for (int e = 0; e < 7; ++e)
+            if (((ms[j] >> e) & 1u) && (uint32_t)((__ffsll((unsigned long long)rec[j][e]) - 1) >> 3) == (uint32_t)e)
+                k |= 1u << e;
+        kb[j] = k;
+        if (base + (uint64_t)j * kBlock + threadIdx.x < n) a.keep[ps[j]] = (uint8_t)k;
+        const uint32_t c = (uint32_t)__popc(k);
+        const uint32_t incl = wave_inclusive_scan(c, lane);
+        ex[j] = incl - c;
+        if (lane == 63) sh.cnt[j * 4 + wave] = incl;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const uint32_t cc = lane < kKeepItems * 4 ? sh.cnt[lane] : 0u;
+        const uint32_t incl = wave_inclusive_scan(cc, lane);
+        if (lane < kKeepItems * 4) sh.excl[lane] = incl - cc;
+        if (lane == 63) sh.total = incl;
+    }
+    __syncthreads();
+    const uint32_t total = sh.total;
+    const uint32_t prefix = lookback(a.tstate, tile, total, epoch, &a.ctl->error, &sh.prefix);
+    if ((uint64_t)prefix + total > a.g.N) {
+        if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
+        return false;
+    }
+#pragma unroll
+    for (int j = 0; j < kKeepItems; ++j) {
+        uint32_t off = prefix + sh.excl[j * 4 + wave] + ex[j];
+        const uint32_t k = kb[j];
+        if (k) {
+#pragma unroll
+            for (int e = 0; e < 7; ++e)
+                if ((k >> e) & 1u) {
+                    const uint32_t v = neighbor_in(a.g, us[j], (uint32_t)e);
+                    Sn[off++] = v;
+                    if (dense) a.memb[v] = 1;
+                }
+        }
+    }
+    if (threadIdx.x == 0 && tile == keep_tiles(n) - 1) keep_finish(a, q, n, prefix + total, epoch);
+    __syncthreads();
+    return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_keep(KArgs a, int q)
+{
+    __shared__ KeepShared sh;
+    const uint32_t epoch = a.ctl->epoch;
+    const uint32_t n = a.ctl->n[q];
+    const uint32_t ntiles = keep_tiles(n);
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+        if (!keep_tile(a, q, n, tile, epoch, sh)) break;
+    if (ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) keep_finish(a, q, 0, 0, epoch);
+}
+
+// ---------------------------------------------------------------------------
+// single-volume iteration kernels.  An iteration over list[q] is the update,
+// the marking, the keep step and the compaction into list[q ^ 1].  The
+// compaction reads only the keep bytes, the membership marks and the list, and
+// writes only the next list and the positions, none of which the next
+// iteration's update touches (it walks the next S, which the keep step wrote):
+// k_update_fused runs the pending compaction and the next update in one launch.
 // ---------------------------------------------------------------------------
 template <typename SrcT, bool kFloatSrc, typename PhiT>
-__global__ __launch_bounds__(kBlock) void k_update(KArgs a, int q)
+__device__ inline void update_walk(const KArgs& a, int q)
 {
     const DevParams P = *a.prm;
     const uint32_t n = a.ctl->n[q];
     ListWalk w(a.run_log2);
     for (uint64_t p = w.pos(); p < n; w.next(), p = w.pos())
-        update_at<SrcT, kFloatSrc, PhiT>(a, P, a.list[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), p);
+        update_at<SrcT, kFloatSrc, PhiT>(a, P, a.S[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), p);
+}
+
+template <typename SrcT, bool kFloatSrc, typename PhiT>
+__global__ __launch_bounds__(kBlock) void k_update(KArgs a, int q)
+{
+    update_walk<SrcT, kFloatSrc, PhiT>(a, q);
+}
+
+// the compaction of list[q ^ 1] into list[q], then the update over S[q]; the
+// grid is resident (the compaction's look-back waits on other workgroups)
+template <typename SrcT, bool kFloatSrc, typename PhiT>
+__global__ __launch_bounds__(kBlock) void k_update_fused(KArgs a, int q)
+{
+    __shared__ CompactShared sh;
+    compact_all<kCompact>(a, q ^ 1, sh);
+    update_walk<SrcT, kFloatSrc, PhiT>(a, q);
 }
 
 template <typename PhiT>
@@ -678,7 +864,7 @@ __global__ __launch_bounds__(kBlock) void k_mark(KArgs a, int q)
     uint32_t emitted = 0;
     ListWalk w(a.run_log2);
     for (uint64_t p = w.pos(); p < n; w.next(), p = w.pos())
-        emitted += mark_at<PhiT>(a, ref, a.list[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, p);
+        emitted += mark_at<PhiT>(a, ref, a.S[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, p);
     block_add(&a.ctl->c_acc, emitted, s_red);
 }
 
@@ -686,11 +872,7 @@ template <int kMode>
 __global__ __launch_bounds__(kBlock) void k_compact(KArgs a, int q)
 {
     __shared__ CompactShared sh;
-    const uint32_t epoch = a.ctl->epoch;
-    const CompactPlan c = compact_plan<kMode>(a.g, a.ctl->n[q]);
-    for (uint32_t tile = blockIdx.x; tile < c.ntiles; tile += gridDim.x)
-        if (!compact_tile<kMode>(a, q, c, tile, epoch, sh)) break;
-    if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kMode>(a, q, c, 0, epoch);
+    compact_all<kMode>(a, q, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -765,7 +947,7 @@ __global__ __launch_bounds__(kBlock) void k_update_batch(BatchArgs b, int q)
         const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
         if (p < b.ctls[v].n[q]) {
             const DevParams P = *a.prm;
-            update_at<SrcT, kFloatSrc, PhiT>(a, P, a.list[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), p);
+            update_at<SrcT, kFloatSrc, PhiT>(a, P, a.S[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), p);
         }
     }
 }
@@ -784,10 +966,27 @@ __global__ __launch_bounds__(kBlock) void k_mark_batch(BatchArgs b, int q)
         const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
         uint32_t emitted = 0;
         if (p < b.ctls[v].n[q])
-            emitted = mark_at<PhiT>(a, a.prm->refMode != 0, a.list[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1),
+            emitted = mark_at<PhiT>(a, a.prm->refMode != 0, a.S[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1),
                                     (uint8_t*)a.C, p);
         block_add(&b.ctls[v].c_acc, emitted, s_red);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_keep_batch(BatchArgs b, int q)
+{
+    __shared__ KeepShared sh;
+    __shared__ uint32_t s_pre[kMaxBatch + 1];
+    __shared__ uint32_t s_tmp[kBlock / 64];
+    batch_prefix(b.count, [&](uint32_t v) { return keep_tiles(b.ctls[v].n[q]); }, s_pre, s_tmp);
+    const uint32_t items = s_pre[b.count];
+    for (uint32_t j = blockIdx.x; j < items; j += gridDim.x) {
+        const uint32_t v = batch_find(s_pre, b.count, j);
+        if (!keep_tile(b.vols[v], q, b.ctls[v].n[q], j - s_pre[v], b.ctls[v].epoch, sh)) break;
+    }
+    // volumes with an empty list still advance their counts
+    if (threadIdx.x == 0)
+        for (uint32_t v = blockIdx.x; v < b.count; v += gridDim.x)
+            if (s_pre[v + 1] == s_pre[v]) keep_finish(b.vols[v], q, 0, 0, b.ctls[v].epoch);
 }
 
 // Every workgroup of the grid is resident (occupancy-sized grid), so a tile
@@ -808,12 +1007,10 @@ __global__ __launch_bounds__(kBlock) void k_compact_batch(BatchArgs b, int q)
         const CompactPlan c = compact_plan<kCompact>(a.g, b.ctls[v].n[q]);
         if (!compact_tile<kCompact>(a, q, c, j - s_pre[v], b.ctls[v].epoch, sh)) break;
     }
-    // volumes with nothing to compact still advance their counts
+    // volumes with nothing to compact still advance their epochs
     if (threadIdx.x == 0)
         for (uint32_t v = blockIdx.x; v < b.count; v += gridDim.x)
-            if (s_pre[v + 1] == s_pre[v])
-                compact_finish<kCompact>(b.vols[v], q, compact_plan<kCompact>(b.vols[v].g, b.ctls[v].n[q]), 0,
-                                         b.ctls[v].epoch);
+            if (s_pre[v + 1] == s_pre[v]) compact_finish<kCompact>(b.vols[v], q, 0, b.ctls[v].epoch);
 }
 
 #define LSSEG_FOR_EACH(p, count)                                                           \
@@ -1123,6 +1320,12 @@ __global__ __launch_bounds__(kBlock) void k_update_only(KArgs a, int op)
 __global__ void k_scatter_membership(const uint32_t* list, uint32_t n, uint64_t* C)
 {
     LSSEG_GRID_STRIDE(p, n) C[list[p]] = 1ull;
+}
+
+// list positions of an injected list
+__global__ void k_scatter_positions(const uint32_t* list, uint32_t n, uint32_t* pos)
+{
+    LSSEG_GRID_STRIDE(p, n) pos[list[p]] = (uint32_t)p;
 }
 
 // AddLevelSetKernel (InitializeLevelSetVolume.cu:151-169)

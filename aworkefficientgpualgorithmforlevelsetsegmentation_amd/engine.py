@@ -278,7 +278,7 @@ class Context:
 
     def iterate_profiled(self, T, eps, alpha, dt, iters, mode=MODE_REF):
         """Returns (Stats, KernelTimes, per-launch milliseconds in launch order:
-        update, marking, compaction of iteration 0, then of iteration 1, ...)."""
+        update, marking, keep, compaction of iteration 0, then of iteration 1, ...)."""
         prm = Params(T, eps, alpha, dt, mode)
         st = Stats()
         kt = KernelTimes()

@@ -16,9 +16,14 @@ equations compiled as host C, timed on this machine's cores).
 Algorithmic bytes per iteration k, FP32 phi (4 bytes), source S bytes/voxel:
   update   (38 + S) n_k   coordinate, 7 phi taps, source, phi write, flag, mask byte
   mark     33 n_k         coordinate, flag, 7 phi taps
-  compact  10 c_k + 5 m_k candidate write/read-back, tag probe/set; list write, clear
-with n_k active voxels, c_k candidate emissions and m_k new active voxels.  Each
-iteration is three launches: k_update, k_mark and k_compact.
+  keep     5 c_k          candidate read-back, tag probe
+  compact  5 c_k + 5 m_k  candidate write, tag set; list write, clear
+with n_k active voxels, c_k candidate emissions and m_k new active voxels (the
+gather model's 10 c_k + 5 m_k for duplicate removal and compaction, split over
+the two kernels that do that work).  Each iteration is three launches: the
+update, which from the second iteration on runs in k_update_fused together with
+the previous iteration's compaction, k_mark and k_keep; the run ends with the
+last iteration's compaction (k_compact).  3 K + 1 launches for K iterations.
 """
 from __future__ import annotations
 
@@ -72,10 +77,10 @@ def algorithmic_bytes(n, c, m, S, P=4):
     n = n.astype(np.float64)
     c = c.astype(np.float64)
     m = m.astype(np.float64)
-    return {"update": (6 + 8 * P + S) * n, "mark": (5 + 7 * P) * n, "compact": 10.0 * c + 5.0 * m}
+    return {"update": (6 + 8 * P + S) * n, "mark": (5 + 7 * P) * n, "keep": 5.0 * c, "compact": 5.0 * c + 5.0 * m}
 
 
-KERNELS = ("k_update", "k_mark", "k_compact")
+KERNELS = ("k_update_fused", "k_mark", "k_keep", "k_update", "k_compact")
 
 
 def _pmc_means(path):
@@ -116,8 +121,8 @@ def summarize_pmc(root, workload):
     out = {"workload": workload, "copy_kernel": copy_name, "copy_bytes": copy_bytes,
            "fetch_scale": f_scale, "write_scale": w_scale, "kernels": {}}
     for k in KERNELS:
-        fk = [v for n, v in fe.items() if n.startswith("lsseg::" + k)]
-        wk = [v for n, v in wr.items() if n.startswith("lsseg::" + k)]
+        fk = [v for n, v in fe.items() if n == "lsseg::" + k or n.startswith("lsseg::" + k + "<")]
+        wk = [v for n, v in wr.items() if n == "lsseg::" + k or n.startswith("lsseg::" + k + "<")]
         if not fk or not wk:
             continue
         fk = max(fk, key=lambda v: v[1])
@@ -469,29 +474,39 @@ def main():
         ctx.close()
         return
 
-    # per-launch device times (HIP events on the engine's stream): update,
-    # marking and compaction of each iteration in turn.
+    # per-launch device times (HIP events on the engine's stream), in issue
+    # order: update, marking and keep of each iteration (the update fused with
+    # the previous compaction from the second iteration on), then the last
+    # compaction.
     ctx.checkpoint_restore(0)
     pst, kt, per_launch = ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, w.iterations)
     pc = ctx.iteration_counts()
     assert np.array_equal(pc.n, counts.n), "profiled run diverged from the timed run"
     B = algorithmic_bytes(pc.n, pc.c, pc.m, S, P)
     it = w.iterations
-    assert per_launch.size == 3 * it, f"expected {3 * it} launches, got {per_launch.size}"
+    assert it >= 2 and per_launch.size == 3 * it + 1, f"expected {3 * it + 1} launches, got {per_launch.size}"
     per_launch = per_launch.astype(np.float64)
+    head = per_launch[0:3 * it:3]
     parts = {
-        "update": {"kernel": "k_update", "bytes": B["update"], "ms": per_launch[0::3]},
-        "mark": {"kernel": "k_mark", "bytes": B["mark"], "ms": per_launch[1::3]},
-        "compact": {"kernel": "k_compact", "bytes": B["compact"], "ms": per_launch[2::3]},
+        # launch k >= 1: the update of iteration k and the compaction of k - 1
+        "update_fused": {"kernel": "k_update_fused", "bytes": B["update"][1:] + B["compact"][:-1], "ms": head[1:]},
+        "mark": {"kernel": "k_mark", "bytes": B["mark"], "ms": per_launch[1:3 * it:3]},
+        "keep": {"kernel": "k_keep", "bytes": B["keep"], "ms": per_launch[2:3 * it:3]},
+        # the first update and the last compaction run on their own
+        "ends": {"kernel": "k_update + k_compact", "bytes": np.array([B["update"][0], B["compact"][-1]]),
+                 "ms": np.array([head[0], per_launch[-1]])},
     }
     kernels = {}
     for k, v in parts.items():
         tot_ms = float(v["ms"].sum())
-        avg_us = 1e3 * tot_ms / it
-        bpl = float(v["bytes"].sum()) / it
+        launches = v["ms"].size
+        avg_us = 1e3 * tot_ms / launches
+        bpl = float(v["bytes"].sum()) / launches
         kernels[k] = {"kernel": v["kernel"], "avg_launch_us": avg_us, "bytes_per_launch": bpl,
                       "achieved_GBs": bpl / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0, "total_ms": tot_ms}
-        big = pc.n >= np.percentile(pc.n, 90)
+        kernels[k]["launches"] = launches
+        nk = pc.n[1:] if k == "update_fused" else (pc.n if v["ms"].size == it else pc.n[[0, -1]])
+        big = nk >= np.percentile(nk, 90)
         kernels[k]["busiest_decile_GBs"] = (float(v["bytes"][big].sum()) / (float(v["ms"][big].sum()) * 1e-3) / 1e9
                                             if big.any() and v["ms"][big].sum() > 0 else 0.0)
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])

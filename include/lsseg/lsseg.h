@@ -227,13 +227,17 @@ lsseg_status lsseg_checkpoint_restore(lsseg_ctx* ctx, int32_t slot);
 /* Device time of the iteration kernels.  Every launch carries its own start
    and stop HIP events on the context's stream (hipExtLaunchKernel), which take
    the kernel's start and end timestamps.  An lsseg_iterate call of `iters`
-   iterations makes 3 * iters launches: update, marking, and duplicate removal +
-   compaction of each iteration in turn. */
+   (>= 1) iterations makes 3 * iters + 1 launches: the update, marking and keep
+   step (duplicate removal) of each iteration in turn, where the update of
+   every iteration after the first also runs the previous iteration's
+   compaction, then the last iteration's compaction.  Under the environment
+   setting LSSEG_FUSE=0 every iteration is four launches (update, marking,
+   keep, compaction) instead, with identical results. */
 typedef struct lsseg_kernel_times {
     double total_ms;     /* sum over all iteration launches */
     double first_ms;     /* the first launch */
     double last_ms;      /* the last launch (the final compaction) */
-    uint64_t launches;   /* 3 * iters */
+    uint64_t launches;   /* 3 * iters + 1 */
 } lsseg_kernel_times;
 
 /* lsseg_iterate with per-launch event timing (launches issued one by one, no

@@ -173,6 +173,44 @@ def test_many_path_multi_iteration():
     ctx2.close()
 
 
+@pytest.mark.parametrize("wide", [False, True], ids=["few", "dense"])
+def test_fused_and_separate_launches_agree(monkeypatch, wide):
+    """Compactions run inside the next update launch (the default) or as launches
+    of their own (LSSEG_FUSE=0): identical states, counts and statistics; the
+    profiled run issues 3 K + 1 launches, or 4 K."""
+    _say(f"fused vs separate launches, wide band {wide}")
+    if wide:
+        n = 32
+        z, y, x = np.mgrid[0:n, 0:n, 0:n]
+        vol = ((x + y + z) % 7 * 30).astype(np.uint8)
+        phi0 = np.clip((x - 15.5) / 3.0, -4, 4).astype(np.float32)
+        w = synth.Workload("dense", vol, phi0, 100.0, 60.0, 0.2, 0.5, 70, band=3.5)
+    else:
+        w = synth.ellipsoids(edge=64, iterations=70)
+    out = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("LSSEG_FUSE", fuse)
+        ctx = Context(w.volume)
+        ctx.set_phi(w.phi0, w.band)
+        st = ctx.iterate(w.T, w.eps, w.alpha, w.dt, w.iterations)
+        cnt = ctx.iteration_counts()
+        state = ctx.state()
+        ctx.set_phi(w.phi0, w.band)
+        _, kt, per = ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, 5)
+        out[fuse] = (st, cnt, state, int(kt.launches), ctx.state())
+        ctx.close()
+    a, b = out["1"], out["0"]
+    assert (a[0].few_path_iters, a[0].many_path_iters, a[0].sum_active) == \
+        (b[0].few_path_iters, b[0].many_path_iters, b[0].sum_active)
+    if wide:
+        assert a[0].many_path_iters > 0
+    for f in ("n", "c", "m"):
+        assert np.array_equal(getattr(a[1], f), getattr(b[1], f)), f
+    for x, y in zip(a[2] + a[4], b[2] + b[4]):
+        assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
+    assert a[3] == 3 * 5 + 1 and b[3] == 4 * 5
+
+
 def test_fp64_tolerance_per_iteration():
     _say("fp64 tolerance along the ellipsoids run")
     w = synth.ellipsoids(edge=64)
