@@ -57,6 +57,8 @@ struct lsseg_ctx {
     DevParams* d_prm = nullptr;
     DevParams h_prm{};
     DevCtl* h_ctl = nullptr;          // pinned mirror
+    uint32_t* h_watch = nullptr;      // pinned: list counts after the last two graph chunks
+    hipEvent_t watch_ev[2] = {nullptr, nullptr};
     uint32_t stats_cap = 0;
     uint64_t last_iters = 0;
     hipGraphExec_t graphs[2] = {nullptr, nullptr};
@@ -268,7 +270,7 @@ lsseg_status dense_init_list(lsseg_ctx* c, int target)
 // kGraphIters fused iterations (even, so the parity returns to its start)
 // replay a captured graph.  LSSEG_FUSE=0 issues the four launches of every
 // iteration separately instead (the same results).
-lsseg_status launch_iterations(lsseg_ctx* c, int count, int q)
+lsseg_status launch_iterations(lsseg_ctx* c, int count, int q, uint32_t* skipped_out = nullptr)
 {
     if (count <= 0) return LSSEG_OK;
     if (!env_int("LSSEG_FUSE", 1)) {
@@ -280,7 +282,23 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q)
     q ^= 1;
     --count;
     const int chunk = env_int("LSSEG_GRAPH", 1) ? kGraphIters : 0;
+    // An iteration over an empty list changes nothing but the counters, so once
+    // a chunk ends with an empty list the remaining iterations are not issued:
+    // k_empty_iterations writes their statistics and the parity follows from
+    // the count.  The host reads the count of the chunk two before the one it
+    // is about to issue, so the device always has a chunk queued meanwhile.
+    int chunks = 0;
+    uint32_t skipped = 0;
     while (chunk && !c->profile && count >= chunk) {
+        const int slot = chunks & 1;
+        if (chunks >= 2) {
+            HIP_TRY(c, hipEventSynchronize(c->watch_ev[slot]));
+            if (c->h_watch[slot] == 0) {
+                skipped = (uint32_t)count;
+                count = 0;
+                break;
+            }
+        }
         hipGraphExec_t& ge = c->graphs[q];
         if (!ge) {
             hipGraph_t g = nullptr;
@@ -299,6 +317,10 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q)
             }
         }
         HIP_TRY(c, hipGraphLaunch(ge, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(&c->h_watch[slot], &c->a.ctl->n[q], sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                  c->stream));
+        HIP_TRY(c, hipEventRecord(c->watch_ev[slot], c->stream));
+        ++chunks;
         count -= chunk;
     }
     for (int i = 0; i < count; ++i) {
@@ -307,6 +329,8 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q)
         q ^= 1;
     }
     launch(c, (const void*)&k_compact<kCompact>, c->grid, q ^ 1);
+    if (skipped) hipLaunchKernelGGL(k_empty_iterations, dim3(1), dim3(kBlock), 0, c->stream, c->a, skipped);
+    if (skipped_out) *skipped_out = skipped;
     return LSSEG_OK;
 }
 
@@ -520,6 +544,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     const int want = env_int("LSSEG_GRID_BLOCKS", 0);
     c->grid = (uint32_t)occ * c->cus;
     if (want > 0) c->grid = std::min<uint32_t>(c->grid, (uint32_t)want);
+    c->grid = std::max<uint32_t>(16u, c->grid & ~7u);  // k_update_fused splits it in multiples of 8
     // update and marking: eight workgroups per CU (a multiple of the eight
     // XCDs), list positions dealt to the XCDs in runs of 4096
     const int swant = env_int("LSSEG_STREAM_BLOCKS", 0);
@@ -542,10 +567,16 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
         c->err = std::string("stream/event creation failed: ") + hipGetErrorString(e);
         return bail(LSSEG_ERR_DEVICE);
     }
-    if (hipHostMalloc((void**)&c->h_ctl, sizeof(DevCtl), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void**)&c->h_ctl, sizeof(DevCtl), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_watch, 2 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
         c->err = "pinned allocation failed";
         return bail(LSSEG_ERR_OUT_OF_MEMORY);
     }
+    for (hipEvent_t& ev : c->watch_ev)
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
+            c->err = std::string("event creation failed: ") + hipGetErrorString(e);
+            return bail(LSSEG_ERR_DEVICE);
+        }
     memset(c->h_ctl, 0, sizeof(DevCtl));
 
     Geom& g = c->a.g;
@@ -651,6 +682,9 @@ void destroy_ctx(lsseg_ctx* ctx)
     for (void* p : ctx->allocs) (void)hipFree(p);
     ctx->allocs.clear();
     if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
+    if (ctx->h_watch) (void)hipHostFree(ctx->h_watch);
+    for (hipEvent_t ev : ctx->watch_ev)
+        if (ev) (void)hipEventDestroy(ev);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream && ctx->owns_stream) (void)hipStreamDestroy(ctx->stream);
@@ -764,9 +798,10 @@ lsseg_status lsseg_iterate(lsseg_ctx* c, const lsseg_params* prm, int32_t iters,
     HIP_TRY(c, hipMemcpyAsync(c->a.ctl, c->h_ctl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
 
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    uint32_t skipped = 0;
     if (iters > 0) {
         const int q0 = c->q;
-        if ((st = launch_iterations(c, iters, q0)) != LSSEG_OK) return st;
+        if ((st = launch_iterations(c, iters, q0, &skipped)) != LSSEG_OK) return st;
         c->q = q0 ^ (iters & 1);
     }
     HIP_TRY(c, hipGetLastError());
@@ -782,6 +817,7 @@ lsseg_status lsseg_iterate(lsseg_ctx* c, const lsseg_params* prm, int32_t iters,
         stats->active_final = c->h_ctl->n[c->q];
         stats->few_path_iters = c->h_ctl->few_iters;
         stats->many_path_iters = c->h_ctl->many_iters;
+        stats->empty_skipped = skipped;
         if (iters > 0) {
             std::vector<uint32_t> hn(iters), hm(iters);
             std::vector<uint64_t> hc(iters);

@@ -448,21 +448,24 @@ __device__ inline uint32_t tau_of(const Geom& g, uint32_t x, uint32_t y, uint32_
 // to the eight XCDs round-robin (workgroup b on XCD b % 8) and each XCD has its
 // own L2, so list positions are dealt to XCDs in runs of 2^run_log2 (>= the
 // workgroup size): run j goes to XCD j % 8 and neighbouring list entries, which
-// gather from the same cache lines, share one L2.  The grid is a multiple of 8.
+// gather from the same cache lines, share one L2.  The walk runs over the
+// `count` workgroups from `first` (both multiples of 8; by default the grid).
 // ---------------------------------------------------------------------------
 struct ListWalk {
     uint32_t vt, step, xcd, lr;
-    __device__ ListWalk(uint32_t run_log2)
+    __device__ ListWalk(uint32_t run_log2, uint32_t first = 0, uint32_t count = 0)
     {
         lr = run_log2;
+        const uint32_t b = blockIdx.x - first;
+        const uint32_t wgs = count ? count : gridDim.x;
         if (lr) {
-            xcd = blockIdx.x & 7u;
-            vt = (blockIdx.x >> 3) * kBlock + threadIdx.x;
-            step = (gridDim.x >> 3) * kBlock;
+            xcd = b & 7u;
+            vt = (b >> 3) * kBlock + threadIdx.x;
+            step = (wgs >> 3) * kBlock;
         } else {
             xcd = 0;
-            vt = blockIdx.x * kBlock + threadIdx.x;
-            step = gridDim.x * kBlock;
+            vt = b * kBlock + threadIdx.x;
+            step = wgs * kBlock;
         }
     }
     // 64-bit so that no position near 2^32 wraps back into the list
@@ -757,17 +760,18 @@ __device__ inline bool keep_tile(const KArgs& a, int q, uint32_t n, uint32_t til
         ms[j] = valid ? a.mvox[us[j]] : 0u;
         ps[j] = a.pos[us[j]];
     }
+    // a -z candidate (e = 0) comes first in any record, so only e >= 1 loads one
     uint64_t rec[kKeepItems][7];
 #pragma unroll
     for (int j = 0; j < kKeepItems; ++j)
 #pragma unroll
-        for (int e = 0; e < 7; ++e)
+        for (int e = 1; e < 7; ++e)
             rec[j][e] = Ck[((ms[j] >> e) & 1u) ? neighbor_in(a.g, us[j], (uint32_t)e) : us[j]];
 #pragma unroll
     for (int j = 0; j < kKeepItems; ++j) {
-        uint32_t k = 0;
+        uint32_t k = ms[j] & 1u;
 #pragma unroll
-        for (int e = 0; e < 7; ++e)
+        for (int e = 1; e < 7; ++e)
             if (((ms[j] >> e) & 1u) && (uint32_t)((__ffsll((unsigned long long)rec[j][e]) - 1) >> 3) == (uint32_t)e)
                 k |= 1u << e;
         kb[j] = k;
@@ -830,11 +834,11 @@ __global__ __launch_bounds__(kBlock) void k_keep(KArgs a, int q)
 // k_update_fused runs the pending compaction and the next update in one launch.
 // ---------------------------------------------------------------------------
 template <typename SrcT, bool kFloatSrc, typename PhiT>
-__device__ inline void update_walk(const KArgs& a, int q)
+__device__ inline void update_walk(const KArgs& a, int q, uint32_t first = 0, uint32_t count = 0)
 {
     const DevParams P = *a.prm;
     const uint32_t n = a.ctl->n[q];
-    ListWalk w(a.run_log2);
+    ListWalk w(a.run_log2, first, count);
     for (uint64_t p = w.pos(); p < n; w.next(), p = w.pos())
         update_at<SrcT, kFloatSrc, PhiT>(a, P, a.S[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), p);
 }
@@ -845,14 +849,48 @@ __global__ __launch_bounds__(kBlock) void k_update(KArgs a, int q)
     update_walk<SrcT, kFloatSrc, PhiT>(a, q);
 }
 
-// the compaction of list[q ^ 1] into list[q], then the update over S[q]; the
-// grid is resident (the compaction's look-back waits on other workgroups)
+// the compaction of list[q ^ 1] into list[q] and the update over S[q].  The
+// first workgroups (a multiple of 8, at most half the grid and no more than
+// the compaction has tiles) take the compaction tiles, the others the update,
+// so neither waits behind the other.  The grid is resident (the compaction's
+// look-back waits on other workgroups), a multiple of 8 and at least 16.
 template <typename SrcT, bool kFloatSrc, typename PhiT>
 __global__ __launch_bounds__(kBlock) void k_update_fused(KArgs a, int q)
 {
     __shared__ CompactShared sh;
-    compact_all<kCompact>(a, q ^ 1, sh);
-    update_walk<SrcT, kFloatSrc, PhiT>(a, q);
+    const uint32_t epoch = a.ctl->epoch;
+    const CompactPlan c = compact_plan<kCompact>(a.g, a.ctl->n[q ^ 1]);
+    const uint32_t gc = min((c.ntiles + 7u) & ~7u, (gridDim.x / 2u) & ~7u);
+    if (blockIdx.x < gc) {
+        for (uint32_t tile = blockIdx.x; tile < c.ntiles; tile += gc)
+            if (!compact_tile<kCompact>(a, q ^ 1, c, tile, epoch, sh)) break;
+    } else {
+        if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kCompact>(a, q ^ 1, 0, epoch);
+        update_walk<SrcT, kFloatSrc, PhiT>(a, q, gc, gridDim.x - gc);
+    }
+}
+
+// The counts and statistics of `count` iterations over an empty list, which
+// change nothing else (one workgroup): both lists stay empty, the iterations
+// take the few path.
+__global__ __launch_bounds__(kBlock) void k_empty_iterations(KArgs a, uint32_t count)
+{
+    DevCtl* ctl = a.ctl;
+    const uint32_t k0 = ctl->iter, cap = ctl->stats_cap;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < count; i += kBlock)
+        if (k0 + i < cap) {
+            a.n_hist[k0 + i] = 0;
+            a.c_hist[k0 + i] = 0;
+            a.m_hist[k0 + i] = 0;
+        }
+    if (threadIdx.x == 0) {
+        ctl->n[0] = 0;
+        ctl->n[1] = 0;
+        ctl->c_acc = 0;
+        ctl->few_iters += count;
+        ctl->iter = k0 + count;
+    }
 }
 
 template <typename PhiT>

@@ -60,7 +60,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_uint64), ("sum_active", ctypes.c_uint64),
                 ("sum_candidates", ctypes.c_uint64), ("sum_new_active", ctypes.c_uint64),
                 ("active_final", ctypes.c_uint64), ("few_path_iters", ctypes.c_uint64),
-                ("many_path_iters", ctypes.c_uint64), ("device_ms", ctypes.c_double)]
+                ("many_path_iters", ctypes.c_uint64), ("device_ms", ctypes.c_double),
+                ("empty_skipped", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -103,6 +103,9 @@ typedef struct lsseg_stats {
     uint64_t few_path_iters;   /* iterations that took the sparse ("few") path */
     uint64_t many_path_iters;  /* iterations that took the dense ("many") path */
     double   device_ms;        /* device time of the call (HIP events) */
+    uint64_t empty_skipped;    /* trailing iterations over an empty list that were not
+                                  launched: they change nothing but the counters, which
+                                  are set as if they had run */
 } lsseg_stats;
 
 /* ---- lifetime ----------------------------------------------------------- */

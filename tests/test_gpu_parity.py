@@ -211,6 +211,39 @@ def test_fused_and_separate_launches_agree(monkeypatch, wide):
     assert a[3] == 3 * 5 + 1 and b[3] == 4 * 5
 
 
+def test_empty_list_ends_the_run_early(monkeypatch):
+    """vessels 64^3 with 4 tubes empties its list at iteration 101; of 400
+    iterations the trailing ones over the empty list are not launched.  Counts,
+    statistics, both phi buffers (and which one is current), export and mask
+    equal the oracle's full run and a run that launches every iteration."""
+    _say("vessels 64^3, 400 iterations, list empty from iteration 101")
+    w = synth.vessels(edge=64, n_tubes=4)
+    ref = O.State(w.volume)
+    ref.set_phi(w.phi0, w.band)
+    nh, ch, mh = ref.iterate(w.T, w.eps, w.alpha, w.dt, 400, threads=8)
+    assert nh[-1] == 0 and int(np.argmax(np.asarray(nh) == 0)) < 150
+    runs = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("LSSEG_FUSE", fuse)
+        ctx = Context(w.volume)
+        ctx.set_phi(w.phi0, w.band)
+        st = ctx.iterate(w.T, w.eps, w.alpha, w.dt, 400)
+        cnt = ctx.iteration_counts()
+        assert np.array_equal(cnt.n, nh) and np.array_equal(cnt.c, ch) and np.array_equal(cnt.m, mh), fuse
+        _assert_same_state(ctx, ref, f"LSSEG_FUSE={fuse}")
+        assert np.array_equal(ctx.phi(), ref.phi()), fuse
+        assert np.array_equal(ctx.mask(), ref.mask()), fuse
+        runs[fuse] = st
+        # iterating on from the empty list is still exact
+        ctx.iterate(w.T, w.eps, w.alpha, w.dt, 3)
+        ctx.close()
+    a, b = runs["1"], runs["0"]
+    assert a.empty_skipped > 150 and b.empty_skipped == 0
+    many = sum(1 for n in nh if not O.path_is_few(int(n), ctx.N))
+    assert (a.few_path_iters, a.many_path_iters, a.sum_active, a.active_final) == \
+        (b.few_path_iters, b.many_path_iters, b.sum_active, b.active_final) == (400 - many, many, sum(nh), 0)
+
+
 def test_fp64_tolerance_per_iteration():
     _say("fp64 tolerance along the ellipsoids run")
     w = synth.ellipsoids(edge=64)
