@@ -129,8 +129,12 @@ void destroy_graphs(lsseg_ctx* c)
     }
 }
 
-// resident workgroups per CU that every listed kernel reaches (the look-back
-// scans need the whole grid resident)
+// Resident workgroups per CU that every listed kernel reaches.  The look-back
+// scans hand their tiles out in order, so they finish with any grid; a grid of
+// this size keeps every workgroup running from the start.  The runtime's
+// figure is capped by one computed from the kernel's own resources with a
+// 64 KiB LDS budget per CU and the 512-entry register file of a SIMD (one wave
+// of each kBlock workgroup per SIMD, at most 8 waves per SIMD).
 int min_occupancy(std::initializer_list<const void*> kernels)
 {
     int best = 1 << 30;
@@ -139,6 +143,15 @@ int min_occupancy(std::initializer_list<const void*> kernels)
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, kBlock, 0) != hipSuccess || occ < 1) {
             (void)hipGetLastError();
             occ = 1;
+        }
+        hipFuncAttributes fa{};
+        if (hipFuncGetAttributes(&fa, k) == hipSuccess) {
+            const int regs = std::max(8, (fa.numRegs + 7) / 8 * 8);
+            const int by_regs = 512 / regs;
+            const int by_lds = fa.sharedSizeBytes ? (int)(65536 / fa.sharedSizeBytes) : 8;
+            occ = std::min(occ, std::max(1, std::min(8, std::min(by_regs, by_lds))));
+        } else {
+            (void)hipGetLastError();
         }
         best = std::min(best, occ);
     }
@@ -537,8 +550,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     int cus = 0;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
     c->cus = (uint32_t)std::max(1, cus);
-    // the look-back needs every workgroup of the keep and compaction kernels
-    // (and of the update fused with a compaction) resident at once
+    // the keep and compaction kernels (and the update fused with a compaction)
     const int occ = min_occupancy({(const void*)&k_compact<kCompact>, (const void*)&k_compact<kInit>,
                                    (const void*)&k_keep, update_fused_kernel(desc->dtype, c->phi8)});
     const int want = env_int("LSSEG_GRID_BLOCKS", 0);
@@ -615,6 +627,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     ALLOC(mvox, uint8_t, N);
     ALLOC(keep, uint8_t, N);
     ALLOC(memb, uint8_t, N);
+    ALLOC(kept, uint8_t, N);
     ALLOC(tstate, uint64_t, ntiles);
     ALLOC(ctl, DevCtl, 1);
 #undef ALLOC
@@ -1131,7 +1144,7 @@ struct lsseg_batch {
     std::vector<lsseg_ctx*> vols;
     int32_t dtype = 0;
     uint32_t cus = 256;
-    uint32_t max_compact = 1024;      // co-resident workgroups of k_compact_batch
+    uint32_t max_compact = 1024;      // resident workgroups of k_compact_batch (and above the volume count)
     uint32_t grid = 1024;             // compaction workgroups
     uint32_t sgrid = 2048;            // update and marking workgroups
     KArgs* d_table = nullptr;
@@ -1340,7 +1353,9 @@ lsseg_status lsseg_batch_create(const lsseg_volume_desc* descs, const void* cons
     }
     b->cus = b->vols[0]->cus;
     const int occ = min_occupancy({(const void*)&k_compact_batch, (const void*)&k_keep_batch});
-    b->max_compact = (uint32_t)occ * b->cus;
+    // the batched scans take one look-back chunk per workgroup and need more
+    // workgroups than volumes
+    b->max_compact = std::max<uint32_t>((uint32_t)occ * b->cus, count + 8u);
     b->grid = b->max_compact;
     b->sgrid = 8u * b->cus;
     *out = b;
@@ -1463,9 +1478,10 @@ lsseg_status lsseg_batch_get_tuning(const lsseg_batch* b, lsseg_tuning* out)
 lsseg_status lsseg_batch_set_tuning(lsseg_batch* b, const lsseg_tuning* t)
 {
     if (!b || !t) return LSSEG_ERR_INVALID_ARG;
-    if (t->update_blocks == 0 || t->compact_blocks == 0 || t->compact_blocks > b->max_compact)
-        return bfail(b, LSSEG_ERR_INVALID_ARG, "launch shape %u/%u outside 1.. / 1..%u", t->update_blocks,
-                     t->compact_blocks, b->max_compact);
+    const uint32_t min_compact = (uint32_t)b->vols.size() + 1u;
+    if (t->update_blocks == 0 || t->compact_blocks < min_compact || t->compact_blocks > b->max_compact)
+        return bfail(b, LSSEG_ERR_INVALID_ARG, "launch shape %u/%u outside 1.. / %u..%u", t->update_blocks,
+                     t->compact_blocks, min_compact, b->max_compact);
     if (t->update_blocks != b->sgrid || t->compact_blocks != b->grid) destroy_batch_graphs(b);
     b->sgrid = t->update_blocks;
     b->grid = t->compact_blocks;
@@ -1485,7 +1501,7 @@ lsseg_status lsseg_batch_autotune(lsseg_batch* b, const lsseg_params* params, in
         for (uint32_t cd : {occ, (occ + 1) / 2, 1u}) {
             lsseg_tuning t{};
             t.update_blocks = su * b->cus;
-            t.compact_blocks = std::max(1u, cd) * b->cus;
+            t.compact_blocks = std::max<uint32_t>(std::max(1u, cd) * b->cus, (uint32_t)b->vols.size() + 8u);
             bool dup = false;
             for (auto& o : cands) dup |= o.update_blocks == t.update_blocks && o.compact_blocks == t.compact_blocks;
             if (!dup) cands.push_back(t);

@@ -201,6 +201,7 @@ struct KArgs {
     uint8_t* mvox;        // emission mask of every voxel of the current list
     uint8_t* keep;        // kept candidate lists of every list position
     uint8_t* memb;        // members of the next list, for a dense compaction (zero at rest)
+    uint8_t* kept;        // kept directions per traversal position (keep chunks of several sub-tiles)
     uint64_t* tstate;     // look-back tile states
     DevCtl* ctl;
     const DevParams* prm;
@@ -306,6 +307,29 @@ __device__ inline uint32_t lookback(uint64_t* states, uint32_t tile, uint32_t to
     }
     __syncthreads();
     return *s_excl;
+}
+
+// ---------------------------------------------------------------------------
+// Look-back scans with one look-back tile per workgroup.  A scan of `items`
+// sub-tiles over `participants` workgroups groups them into chunks of r
+// consecutive sub-tiles, r the smallest that leaves at most `participants`
+// chunks, and workgroup b takes chunk b.  A chunk waits only on chunks of lower
+// workgroups, which are dispatched before it and publish their aggregates
+// without waiting, so every wait ends however many workgroups are resident at
+// once.  With r > 1 a chunk counts its sub-tiles, resolves its prefix, then
+// places them in a second pass over their inputs.
+// ---------------------------------------------------------------------------
+struct ChunkPlan {
+    uint32_t r;       // sub-tiles per chunk
+    uint32_t chunks;  // look-back tiles
+};
+
+__device__ inline ChunkPlan chunk_plan(uint32_t items, uint32_t participants)
+{
+    ChunkPlan p;
+    p.r = items > participants ? (items + participants - 1) / participants : 1u;
+    p.chunks = (items + p.r - 1) / p.r;
+    return p;
 }
 
 // ---------------------------------------------------------------------------
@@ -486,10 +510,9 @@ struct ListWalk {
 // volume's arguments from a table) share them.
 // ---------------------------------------------------------------------------
 template <typename SrcT, bool kFloatSrc, typename PhiT>
-__device__ inline void update_at(const KArgs& a, const DevParams& P, const uint32_t* __restrict__ V,
-                                 const PhiT* __restrict__ R, PhiT* __restrict__ W, uint64_t p)
+__device__ inline void update_at(const KArgs& a, const DevParams& P, const PhiT* __restrict__ R,
+                                 PhiT* __restrict__ W, uint32_t u)
 {
-    const uint32_t u = V[p];
     uint32_t nb[7], inb;
     neighbors(a.g, u, nb, &inb);
     const float u4 = phi_load(R, u);
@@ -503,15 +526,13 @@ __device__ inline void update_at(const KArgs& a, const DevParams& P, const uint3
     if constexpr (sizeof(PhiT) == 1) a.flag[u] = fabsf(nv - u4) > kTemporalThreshold ? 1 : 0;
 }
 
-// marking of list position p (UpdateActiveElements.cu:75-103); returns the
+// marking of active voxel u (UpdateActiveElements.cu:75-103); returns the
 // number of candidate emissions
 template <typename PhiT>
-__device__ inline uint32_t mark_at(const KArgs& a, bool ref, const uint32_t* __restrict__ S,
-                                   const PhiT* __restrict__ R, const PhiT* __restrict__ W,
-                                   uint8_t* __restrict__ Cb, uint64_t i)
+__device__ inline uint32_t mark_at(const KArgs& a, bool ref, const PhiT* __restrict__ R,
+                                   const PhiT* __restrict__ W, uint8_t* __restrict__ Cb, uint32_t u)
 {
     uint32_t m = 0;
-    const uint32_t u = S[i];
     // the position and every operand are loaded together (all addresses are
     // voxels of the volume); the position then decides whether u emits
     uint32_t nb[7], inb;
@@ -572,7 +593,7 @@ struct CompactPlan {
     uint32_t n_src;
     bool dense;
     uint64_t slots;
-    uint32_t ntiles;
+    uint32_t ntiles;  // sub-tiles of kScanTile slots
 };
 
 template <int kMode>
@@ -586,7 +607,7 @@ __device__ inline CompactPlan compact_plan(const Geom& g, uint32_t n_src)
     return c;
 }
 
-// once the last tile is placed (one thread).  The keep step has already set
+// once the last chunk is placed (one thread).  The keep step has already set
 // the count of the next list; the compaction must arrive at the same count.
 template <int kMode>
 __device__ inline void compact_finish(const KArgs& a, int q, uint32_t m, uint32_t epoch)
@@ -601,28 +622,27 @@ __device__ inline void compact_finish(const KArgs& a, int q, uint32_t m, uint32_
     ctl->epoch = epoch + 1;
 }
 
-// One look-back tile of the compaction of list[q] into list[q ^ 1] (the
-// reference's order): few path, the kept candidates of the keep bytes in
-// list-major slot order; dense path, the voxels marked in `memb` in tiled order
-// (the marks are cleared on the way); kInit, the voxels with C != 0 in tiled
-// order.  Writes the positions of the new list.  Every thread of the block
-// calls it.  Returns false (with the error flag set) if the next list would
-// overflow, which a consistent record never does.
-template <int kMode>
-__device__ inline bool compact_tile(const KArgs& a, int q, const CompactPlan& c, uint32_t tile, uint32_t epoch,
-                                    CompactShared& sh)
+// The compaction of list[q] into list[q ^ 1] (the reference's order): few
+// path, the kept candidates of the keep bytes in list-major slot order; dense
+// path, the voxels marked in `memb` in tiled order (the marks are cleared when
+// the voxels are placed); kInit, the voxels with C != 0 in tiled order.  The
+// positions of the new list are written too.  Every thread of the block runs
+// these.
+//
+// compact_load: the keep flags (ballots) of sub-tile `sub` (slot
+// sub * kScanTile + j * kBlock + tid) and per-wave counts into sh.cnt; with
+// kPlace also the target voxels (and the dense path's marks are cleared).
+template <int kMode, bool kPlace>
+__device__ __attribute__((always_inline)) inline void compact_load(const KArgs& a, int q, const CompactPlan& c,
+                                                                   uint32_t sub, uint64_t* bal, uint32_t* val,
+                                                                   CompactShared& sh)
 {
     const uint32_t* __restrict__ Vs = a.list[q];
     const uint8_t* __restrict__ Ks = a.keep;
-    uint32_t* __restrict__ out = a.list[q ^ 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const uint64_t lt = lanemask_lt(lane);
     const uint32_t n_src = c.n_src;
-
-    uint64_t bal[kItems];
-    uint32_t val[kItems];
-    const uint64_t base = (uint64_t)tile * kScanTile;
+    const uint64_t base = (uint64_t)sub * kScanTile;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const uint64_t s = base + (uint64_t)j * kBlock + threadIdx.x;
@@ -635,7 +655,7 @@ __device__ inline bool compact_tile(const KArgs& a, int q, const CompactPlan& c,
             } else if (c.dense) {
                 keep = a.memb[s] != 0;
                 v = (uint32_t)s;
-                if (keep) a.memb[s] = 0;
+                if (kPlace && keep) a.memb[s] = 0;
             } else {
                 // slot s = e * n_src + p: list-major concatenation of the
                 // seven candidate lists (Cuda.cpp:699-758 over 7 A slots);
@@ -644,24 +664,43 @@ __device__ inline bool compact_tile(const KArgs& a, int q, const CompactPlan& c,
                                    (uint32_t)(s >= 3ull * n_src) + (uint32_t)(s >= 4ull * n_src) +
                                    (uint32_t)(s >= 5ull * n_src) + (uint32_t)(s >= 6ull * n_src);
                 const uint32_t p = (uint32_t)(s - (uint64_t)e * n_src);
-                const uint32_t u = Vs[p];
                 keep = ((Ks[p] >> e) & 1u) != 0;
-                v = neighbor_in(a.g, u, e);
+                if constexpr (kPlace) v = neighbor_in(a.g, Vs[p], e);
             }
         }
         bal[j] = __ballot(keep);
         val[j] = v;
         if (lane == 0) sh.cnt[j * 4 + wave] = (uint32_t)__popcll(bal[j]);
     }
+}
+
+// block scan of a loaded sub-tile's wave counts (sh.cnt -> sh.excl); returns
+// the sub-tile's total in every thread
+__device__ __attribute__((always_inline)) inline uint32_t compact_counts(CompactShared& sh)
+{
+    const int lane = threadIdx.x & 63;
     __syncthreads();
-    if (wave == 0) {
+    if (threadIdx.x < 64) {
         const uint32_t cc = lane < kItems * 4 ? sh.cnt[lane] : 0u;
         const uint32_t incl = wave_inclusive_scan(cc, lane);
         if (lane < kItems * 4) sh.excl[lane] = incl - cc;
         if (lane == 63) sh.total = incl;
     }
     __syncthreads();
-    lookback(a.tstate, tile, sh.total, epoch, &a.ctl->error, &sh.prefix);
+    return sh.total;
+}
+
+// places the kept voxels of a loaded and counted sub-tile from list position
+// `prefix` on, through the LDS stage so that the stores are contiguous
+template <int kMode>
+__device__ __attribute__((always_inline)) inline void compact_place(const KArgs& a, int q, const uint64_t* bal,
+                                                                    const uint32_t* val, uint32_t prefix,
+                                                                    uint32_t total, CompactShared& sh)
+{
+    uint32_t* __restrict__ out = a.list[q ^ 1];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt(lane);
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         if ((bal[j] >> lane) & 1ull) {
@@ -670,44 +709,77 @@ __device__ inline bool compact_tile(const KArgs& a, int q, const CompactPlan& c,
         }
     }
     __syncthreads();
-    const uint32_t total = sh.total, prefix = sh.prefix;
-    if ((uint64_t)prefix + total > a.g.N) {
-        if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
-        return false;
-    }
     for (uint32_t i = threadIdx.x; i < total; i += kBlock) {
         const uint32_t v = sh.stage[i];
         out[prefix + i] = v;
         a.pos[v] = prefix + i;
         if constexpr (kMode == kInit) a.S[q ^ 1][prefix + i] = v;  // tiled order is coherent
     }
-    if (threadIdx.x == 0 && tile == c.ntiles - 1) compact_finish<kMode>(a, q, prefix + total, epoch);
+    __syncthreads();
+}
+
+// Look-back chunk `chunk` of the compaction.  Returns false (with the error
+// flag set) if the next list would overflow, which a consistent record never
+// does.
+template <int kMode>
+__device__ inline bool compact_chunk(const KArgs& a, int q, const CompactPlan& c, const ChunkPlan& cp, uint32_t chunk,
+                                     uint32_t epoch, CompactShared& sh)
+{
+    const uint32_t s0 = chunk * cp.r, s1 = min(s0 + cp.r, c.ntiles);
+    uint64_t bal[kItems];
+    uint32_t val[kItems];
+    uint32_t total = 0;
+    if (cp.r == 1) {
+        compact_load<kMode, true>(a, q, c, s0, bal, val, sh);
+        total = compact_counts(sh);
+    } else {
+        for (uint32_t sub = s0; sub < s1; ++sub) {
+            compact_load<kMode, false>(a, q, c, sub, bal, val, sh);
+            total += compact_counts(sh);
+        }
+    }
+    const uint32_t prefix = lookback(a.tstate, chunk, total, epoch, &a.ctl->error, &sh.prefix);
+    if ((uint64_t)prefix + total > a.g.N) {
+        if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
+        return false;
+    }
+    if (cp.r == 1) {
+        compact_place<kMode>(a, q, bal, val, prefix, total, sh);
+    } else {
+        uint32_t run = prefix;
+        for (uint32_t sub = s0; sub < s1; ++sub) {
+            compact_load<kMode, true>(a, q, c, sub, bal, val, sh);
+            const uint32_t t = compact_counts(sh);
+            compact_place<kMode>(a, q, bal, val, run, t, sh);
+            run += t;
+        }
+    }
+    if (threadIdx.x == 0 && chunk == cp.chunks - 1) compact_finish<kMode>(a, q, prefix + total, epoch);
     __syncthreads();
     return true;
 }
 
-// all compaction tiles of list[q] held by this workgroup (every workgroup of
-// the grid calls it; the grid is resident, so every wait ends)
+// the whole compaction of list[q] over the first `participants` workgroups
+// (every one of them calls it)
 template <int kMode>
-__device__ inline void compact_all(const KArgs& a, int q, CompactShared& sh)
+__device__ inline void compact_all(const KArgs& a, int q, const CompactPlan& c, uint32_t epoch, CompactShared& sh,
+                                   uint32_t participants)
 {
-    const uint32_t epoch = a.ctl->epoch;
-    const CompactPlan c = compact_plan<kMode>(a.g, a.ctl->n[q]);
-    for (uint32_t tile = blockIdx.x; tile < c.ntiles; tile += gridDim.x)
-        if (!compact_tile<kMode>(a, q, c, tile, epoch, sh)) break;
+    const ChunkPlan cp = chunk_plan(c.ntiles, participants);
+    if (blockIdx.x < cp.chunks) (void)compact_chunk<kMode>(a, q, c, cp, blockIdx.x, epoch, sh);
     if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kMode>(a, q, 0, epoch);
 }
 
 // ---------------------------------------------------------------------------
-// keep step.  One look-back tile covers kKeepTile positions of S (slot
-// j * kBlock + tid of the tile).  Candidate (e, u -> v) is kept iff the lowest
-// set byte of C[v] is e (first-wins over the list-major concatenation,
+// keep step.  A sub-tile covers kKeepTile positions of S (slot j * kBlock + tid
+// of the sub-tile).  Candidate (e, u -> v) is kept iff the lowest set byte of
+// C[v] is e (first-wins over the list-major concatenation,
 // FilterDuplicates.cu:42-66); the kept lists of u go to keep[pos[u]] for the
 // compaction, and the kept voxels are appended to the next S in S order, each
 // voxel's in list order: consecutive lanes write neighbouring voxels, so the
 // next S is as coherent as this one.  Every voxel of the next list is kept
 // exactly once; when the compaction takes the dense path it is also marked in
-// `memb`.  The last tile sets the count of the next list and the statistics.
+// `memb`.  The last chunk sets the count of the next list and the statistics.
 // ---------------------------------------------------------------------------
 struct KeepShared {
     uint32_t cnt[kKeepItems * 4];
@@ -736,65 +808,93 @@ __device__ inline void keep_finish(const KArgs& a, int q, uint32_t n, uint32_t m
     ctl->epoch = epoch + 1;
 }
 
-__device__ inline bool keep_tile(const KArgs& a, int q, uint32_t n, uint32_t tile, uint32_t epoch, KeepShared& sh)
+// the traversal entries of keep sub-tile `sub` of list q; positions are
+// bounded by N (every one a valid address), and keep_decide discards the
+// entries at or beyond the count
+__device__ __attribute__((always_inline)) inline void keep_voxels(const KArgs& a, int q, uint32_t sub, uint32_t* us)
 {
-    const uint32_t* __restrict__ S = a.S[q];
-    uint32_t* __restrict__ Sn = a.S[q ^ 1];
-    const uint64_t* __restrict__ Ck = a.C;
-    const bool dense = !path_is_few(n, a.g.N);
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const uint64_t base = (uint64_t)tile * kKeepTile;
-    // loads in three rounds, each issued at once: the voxels, their masks and
-    // positions, then the records of all their emissions (inactive lanes load
-    // a valid address and ignore it)
-    uint32_t us[kKeepItems], ms[kKeepItems], ps[kKeepItems], kb[kKeepItems], ex[kKeepItems];
+    const uint64_t base = (uint64_t)sub * kKeepTile;
 #pragma unroll
     for (int j = 0; j < kKeepItems; ++j) {
         const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-        us[j] = S[i < n ? i : 0];
+        us[j] = a.S[q][i < a.g.N ? i : 0];
+    }
+}
+
+// the kept directions kb of the voxels us of keep sub-tile `sub`, written to
+// keep[pos[u]] as well; entries past the count become voxel 0 with kb = 0
+__device__ __attribute__((always_inline)) inline void keep_decide(const KArgs& a, uint32_t n, uint32_t sub,
+                                                                  uint32_t* us, uint32_t* kb)
+{
+    const uint64_t* __restrict__ Ck = a.C;
+    const uint64_t base = (uint64_t)sub * kKeepTile;
+    // the masks and positions, then the records of the emitted candidates
+    // (other lanes load the voxel's own record and ignore it)
+    uint32_t ms[kKeepItems], ps[kKeepItems];
+#pragma unroll
+    for (int j = 0; j < kKeepItems; ++j)
+        if (base + (uint64_t)j * kBlock + threadIdx.x >= n) us[j] = 0;
+#pragma unroll
+    for (int j = 0; j < kKeepItems; ++j) {
+        ms[j] = a.mvox[us[j]];
+        ps[j] = a.pos[us[j]];
+    }
+    // a -z candidate (e = 0) comes first in any record, so only e >= 1 needs one
+    uint64_t rec[kKeepItems][7];
+#pragma unroll
+    for (int j = 0; j < kKeepItems; ++j) {
+        const bool valid = base + (uint64_t)j * kBlock + threadIdx.x < n;
+        const uint32_t m = valid ? ms[j] : 0u;
+#pragma unroll
+        for (int e = 1; e < 7; ++e) rec[j][e] = Ck[((m >> e) & 1u) ? neighbor_in(a.g, us[j], (uint32_t)e) : us[j]];
     }
 #pragma unroll
     for (int j = 0; j < kKeepItems; ++j) {
         const bool valid = base + (uint64_t)j * kBlock + threadIdx.x < n;
-        ms[j] = valid ? a.mvox[us[j]] : 0u;
-        ps[j] = a.pos[us[j]];
-    }
-    // a -z candidate (e = 0) comes first in any record, so only e >= 1 loads one
-    uint64_t rec[kKeepItems][7];
-#pragma unroll
-    for (int j = 0; j < kKeepItems; ++j)
+        const uint32_t m = valid ? ms[j] : 0u;
+        uint32_t k = m & 1u;
 #pragma unroll
         for (int e = 1; e < 7; ++e)
-            rec[j][e] = Ck[((ms[j] >> e) & 1u) ? neighbor_in(a.g, us[j], (uint32_t)e) : us[j]];
-#pragma unroll
-    for (int j = 0; j < kKeepItems; ++j) {
-        uint32_t k = ms[j] & 1u;
-#pragma unroll
-        for (int e = 1; e < 7; ++e)
-            if (((ms[j] >> e) & 1u) && (uint32_t)((__ffsll((unsigned long long)rec[j][e]) - 1) >> 3) == (uint32_t)e)
+            if (((m >> e) & 1u) && (uint32_t)((__ffsll((unsigned long long)rec[j][e]) - 1) >> 3) == (uint32_t)e)
                 k |= 1u << e;
         kb[j] = k;
-        if (base + (uint64_t)j * kBlock + threadIdx.x < n) a.keep[ps[j]] = (uint8_t)k;
-        const uint32_t c = (uint32_t)__popc(k);
+        if (valid) a.keep[ps[j]] = (uint8_t)k;
+    }
+}
+
+// offsets of the kept voxels of a sub-tile: ex within the wave, sh.excl per
+// wave and item; returns the sub-tile's total in every thread
+__device__ __attribute__((always_inline)) inline uint32_t keep_counts(const uint32_t* kb, uint32_t* ex,
+                                                                      KeepShared& sh)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < kKeepItems; ++j) {
+        const uint32_t c = (uint32_t)__popc(kb[j]);
         const uint32_t incl = wave_inclusive_scan(c, lane);
         ex[j] = incl - c;
         if (lane == 63) sh.cnt[j * 4 + wave] = incl;
     }
     __syncthreads();
-    if (wave == 0) {
+    if (threadIdx.x < 64) {
         const uint32_t cc = lane < kKeepItems * 4 ? sh.cnt[lane] : 0u;
         const uint32_t incl = wave_inclusive_scan(cc, lane);
         if (lane < kKeepItems * 4) sh.excl[lane] = incl - cc;
         if (lane == 63) sh.total = incl;
     }
     __syncthreads();
-    const uint32_t total = sh.total;
-    const uint32_t prefix = lookback(a.tstate, tile, total, epoch, &a.ctl->error, &sh.prefix);
-    if ((uint64_t)prefix + total > a.g.N) {
-        if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
-        return false;
-    }
+    return sh.total;
+}
+
+// appends the kept voxels of a counted sub-tile to the next S from `prefix` on
+__device__ __attribute__((always_inline)) inline void keep_append(const KArgs& a, int q, bool dense,
+                                                                  const uint32_t* us, const uint32_t* kb,
+                                                                  const uint32_t* ex, uint32_t prefix,
+                                                                  KeepShared& sh)
+{
+    uint32_t* __restrict__ Sn = a.S[q ^ 1];
+    const int wave = threadIdx.x >> 6;
 #pragma unroll
     for (int j = 0; j < kKeepItems; ++j) {
         uint32_t off = prefix + sh.excl[j * 4 + wave] + ex[j];
@@ -809,7 +909,60 @@ __device__ inline bool keep_tile(const KArgs& a, int q, uint32_t n, uint32_t til
                 }
         }
     }
-    if (threadIdx.x == 0 && tile == keep_tiles(n) - 1) keep_finish(a, q, n, prefix + total, epoch);
+    __syncthreads();
+}
+
+// Look-back chunk `chunk` of the keep step; `us` holds the entries of its
+// first sub-tile.  With several sub-tiles the kept directions of the first
+// pass are kept per traversal position in `kept` for the second.  Returns
+// false (with the error flag set) if the next list would overflow.
+__device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a, int q, uint32_t n,
+                                                                 const ChunkPlan& cp, uint32_t chunk,
+                                                                 uint32_t epoch, KeepShared& sh, uint32_t* us)
+{
+    const bool dense = !path_is_few(n, a.g.N);
+    const uint32_t s0 = chunk * cp.r, s1 = min(s0 + cp.r, keep_tiles(n));
+    uint32_t kb[kKeepItems], ex[kKeepItems];
+    uint32_t total = 0;
+    if (cp.r == 1) {
+        keep_decide(a, n, s0, us, kb);
+        total = keep_counts(kb, ex, sh);
+    } else {
+        for (uint32_t sub = s0; sub < s1; ++sub) {
+            keep_voxels(a, q, sub, us);
+            keep_decide(a, n, sub, us, kb);
+            const uint64_t base = (uint64_t)sub * kKeepTile;
+#pragma unroll
+            for (int j = 0; j < kKeepItems; ++j) {
+                const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+                if (i < n) a.kept[i] = (uint8_t)kb[j];
+            }
+            total += keep_counts(kb, ex, sh);
+        }
+    }
+    const uint32_t prefix = lookback(a.tstate, chunk, total, epoch, &a.ctl->error, &sh.prefix);
+    if ((uint64_t)prefix + total > a.g.N) {
+        if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
+        return false;
+    }
+    if (cp.r == 1) {
+        keep_append(a, q, dense, us, kb, ex, prefix, sh);
+    } else {
+        uint32_t run = prefix;
+        for (uint32_t sub = s0; sub < s1; ++sub) {
+            keep_voxels(a, q, sub, us);
+            const uint64_t base = (uint64_t)sub * kKeepTile;
+#pragma unroll
+            for (int j = 0; j < kKeepItems; ++j) {
+                const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+                kb[j] = i < n ? a.kept[i] : 0u;
+            }
+            const uint32_t t = keep_counts(kb, ex, sh);
+            keep_append(a, q, dense, us, kb, ex, run, sh);
+            run += t;
+        }
+    }
+    if (threadIdx.x == 0 && chunk == cp.chunks - 1) keep_finish(a, q, n, prefix + total, epoch);
     __syncthreads();
     return true;
 }
@@ -817,12 +970,13 @@ __device__ inline bool keep_tile(const KArgs& a, int q, uint32_t n, uint32_t til
 __global__ __launch_bounds__(kBlock) void k_keep(KArgs a, int q)
 {
     __shared__ KeepShared sh;
+    uint32_t us[kKeepItems];
+    keep_voxels(a, q, blockIdx.x, us);  // with the count: this workgroup's sub-tile when chunks are single
     const uint32_t epoch = a.ctl->epoch;
     const uint32_t n = a.ctl->n[q];
-    const uint32_t ntiles = keep_tiles(n);
-    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-        if (!keep_tile(a, q, n, tile, epoch, sh)) break;
-    if (ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) keep_finish(a, q, 0, 0, epoch);
+    const ChunkPlan cp = chunk_plan(keep_tiles(n), gridDim.x);
+    if (blockIdx.x < cp.chunks) (void)keep_chunk(a, q, n, cp, blockIdx.x, epoch, sh, us);
+    if (cp.chunks == 0 && blockIdx.x == 0 && threadIdx.x == 0) keep_finish(a, q, 0, 0, epoch);
 }
 
 // ---------------------------------------------------------------------------
@@ -833,14 +987,32 @@ __global__ __launch_bounds__(kBlock) void k_keep(KArgs a, int q)
 // iteration's update touches (it walks the next S, which the keep step wrote):
 // k_update_fused runs the pending compaction and the next update in one launch.
 // ---------------------------------------------------------------------------
+// The voxels S[p] of list positions p < n, first voxel included: it is loaded
+// together with the count n, not after it (S has N entries, so a position
+// below N is a valid address whatever n is).
+template <typename F>
+__device__ inline void walk_list(const KArgs& a, int q, uint32_t first, uint32_t count, F f)
+{
+    const uint32_t* __restrict__ S = a.S[q];
+    ListWalk w(a.run_log2, first, count);
+    uint64_t p = w.pos();
+    uint32_t u = S[p < a.g.N ? p : 0];
+    const uint32_t n = a.ctl->n[q];
+    while (p < n) {
+        f(u);
+        w.next();
+        p = w.pos();
+        if (p < n) u = S[p];
+    }
+}
+
 template <typename SrcT, bool kFloatSrc, typename PhiT>
 __device__ inline void update_walk(const KArgs& a, int q, uint32_t first = 0, uint32_t count = 0)
 {
     const DevParams P = *a.prm;
-    const uint32_t n = a.ctl->n[q];
-    ListWalk w(a.run_log2, first, count);
-    for (uint64_t p = w.pos(); p < n; w.next(), p = w.pos())
-        update_at<SrcT, kFloatSrc, PhiT>(a, P, a.S[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), p);
+    walk_list(a, q, first, count, [&](uint32_t u) {
+        update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), u);
+    });
 }
 
 template <typename SrcT, bool kFloatSrc, typename PhiT>
@@ -850,20 +1022,19 @@ __global__ __launch_bounds__(kBlock) void k_update(KArgs a, int q)
 }
 
 // the compaction of list[q ^ 1] into list[q] and the update over S[q].  The
-// first workgroups (a multiple of 8, at most half the grid and no more than
-// the compaction has tiles) take the compaction tiles, the others the update,
-// so neither waits behind the other.  The grid is resident (the compaction's
-// look-back waits on other workgroups), a multiple of 8 and at least 16.
+// first workgroups (a multiple of 8, at most three quarters of the grid and
+// no more than the compaction has sub-tiles) take the compaction, the others
+// the update, so neither waits behind the other.  The grid is a multiple of 8
+// and at least 16.
 template <typename SrcT, bool kFloatSrc, typename PhiT>
 __global__ __launch_bounds__(kBlock) void k_update_fused(KArgs a, int q)
 {
     __shared__ CompactShared sh;
     const uint32_t epoch = a.ctl->epoch;
     const CompactPlan c = compact_plan<kCompact>(a.g, a.ctl->n[q ^ 1]);
-    const uint32_t gc = min((c.ntiles + 7u) & ~7u, (gridDim.x / 2u) & ~7u);
+    const uint32_t gc = min((c.ntiles + 7u) & ~7u, (gridDim.x / 4u * 3u) & ~7u);
     if (blockIdx.x < gc) {
-        for (uint32_t tile = blockIdx.x; tile < c.ntiles; tile += gc)
-            if (!compact_tile<kCompact>(a, q ^ 1, c, tile, epoch, sh)) break;
+        compact_all<kCompact>(a, q ^ 1, c, epoch, sh, gc);
     } else {
         if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kCompact>(a, q ^ 1, 0, epoch);
         update_walk<SrcT, kFloatSrc, PhiT>(a, q, gc, gridDim.x - gc);
@@ -898,11 +1069,10 @@ __global__ __launch_bounds__(kBlock) void k_mark(KArgs a, int q)
 {
     __shared__ uint32_t s_red[kBlock / 64];
     const bool ref = a.prm->refMode != 0;
-    const uint32_t n = a.ctl->n[q];
     uint32_t emitted = 0;
-    ListWalk w(a.run_log2);
-    for (uint64_t p = w.pos(); p < n; w.next(), p = w.pos())
-        emitted += mark_at<PhiT>(a, ref, a.S[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, p);
+    walk_list(a, q, 0, 0, [&](uint32_t u) {
+        emitted += mark_at<PhiT>(a, ref, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, u);
+    });
     block_add(&a.ctl->c_acc, emitted, s_red);
 }
 
@@ -910,7 +1080,8 @@ template <int kMode>
 __global__ __launch_bounds__(kBlock) void k_compact(KArgs a, int q)
 {
     __shared__ CompactShared sh;
-    compact_all<kMode>(a, q, sh);
+    const uint32_t epoch = a.ctl->epoch;
+    compact_all<kMode>(a, q, compact_plan<kMode>(a.g, a.ctl->n[q]), epoch, sh, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -985,7 +1156,7 @@ __global__ __launch_bounds__(kBlock) void k_update_batch(BatchArgs b, int q)
         const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
         if (p < b.ctls[v].n[q]) {
             const DevParams P = *a.prm;
-            update_at<SrcT, kFloatSrc, PhiT>(a, P, a.S[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), p);
+            update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), a.S[q][p]);
         }
     }
 }
@@ -1004,10 +1175,22 @@ __global__ __launch_bounds__(kBlock) void k_mark_batch(BatchArgs b, int q)
         const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
         uint32_t emitted = 0;
         if (p < b.ctls[v].n[q])
-            emitted = mark_at<PhiT>(a, a.prm->refMode != 0, a.S[q], phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1),
-                                    (uint8_t*)a.C, p);
+            emitted = mark_at<PhiT>(a, a.prm->refMode != 0, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1),
+                                    (uint8_t*)a.C, a.S[q][p]);
         block_add(&b.ctls[v].c_acc, emitted, s_red);
     }
+}
+
+// Sub-tiles per look-back chunk of a batched scan: with t_v sub-tiles in
+// volume v and G workgroups, r = 1 when the sum T of the t_v is at most G, else
+// ceil(T / (G - count)), which leaves at most T / r + count <= G chunks (the
+// host keeps G above the volume count).  Workgroup j takes chunk j of the
+// volume-major numbering.
+__device__ inline uint32_t batch_chunk_r(uint32_t total, uint32_t count, uint32_t G)
+{
+    if (total <= G) return 1u;
+    const uint32_t room = G > count ? G - count : 1u;
+    return (total + room - 1) / room;
 }
 
 __global__ __launch_bounds__(kBlock) void k_keep_batch(BatchArgs b, int q)
@@ -1016,10 +1199,19 @@ __global__ __launch_bounds__(kBlock) void k_keep_batch(BatchArgs b, int q)
     __shared__ uint32_t s_pre[kMaxBatch + 1];
     __shared__ uint32_t s_tmp[kBlock / 64];
     batch_prefix(b.count, [&](uint32_t v) { return keep_tiles(b.ctls[v].n[q]); }, s_pre, s_tmp);
-    const uint32_t items = s_pre[b.count];
-    for (uint32_t j = blockIdx.x; j < items; j += gridDim.x) {
+    const uint32_t r = batch_chunk_r(s_pre[b.count], b.count, gridDim.x);
+    __syncthreads();
+    batch_prefix(b.count, [&](uint32_t v) { return (keep_tiles(b.ctls[v].n[q]) + r - 1) / r; }, s_pre, s_tmp);
+    const uint32_t j = blockIdx.x;
+    if (j < s_pre[b.count]) {
         const uint32_t v = batch_find(s_pre, b.count, j);
-        if (!keep_tile(b.vols[v], q, b.ctls[v].n[q], j - s_pre[v], b.ctls[v].epoch, sh)) break;
+        const KArgs& a = b.vols[v];
+        const uint32_t n = b.ctls[v].n[q];
+        const ChunkPlan cp{r, (keep_tiles(n) + r - 1) / r};
+        const uint32_t chunk = j - s_pre[v];
+        uint32_t us[kKeepItems];
+        keep_voxels(a, q, chunk * r, us);
+        (void)keep_chunk(a, q, n, cp, chunk, b.ctls[v].epoch, sh, us);
     }
     // volumes with an empty list still advance their counts
     if (threadIdx.x == 0)
@@ -1027,10 +1219,9 @@ __global__ __launch_bounds__(kBlock) void k_keep_batch(BatchArgs b, int q)
             if (s_pre[v + 1] == s_pre[v]) keep_finish(b.vols[v], q, 0, 0, b.ctls[v].epoch);
 }
 
-// Every workgroup of the grid is resident (occupancy-sized grid), so a tile
-// may wait on a tile of the same volume held by any workgroup.  A volume's
-// epoch is read at each of its tiles: its last tile, which advances the epoch,
-// completes only after every other tile of the volume has published.
+// A volume's epoch is read at each of its chunks: its last chunk, which
+// advances the epoch, completes only after every other chunk of the volume has
+// published.
 __global__ __launch_bounds__(kBlock) void k_compact_batch(BatchArgs b, int q)
 {
     __shared__ CompactShared sh;
@@ -1038,12 +1229,19 @@ __global__ __launch_bounds__(kBlock) void k_compact_batch(BatchArgs b, int q)
     __shared__ uint32_t s_tmp[kBlock / 64];
     batch_prefix(b.count, [&](uint32_t v) { return compact_plan<kCompact>(b.vols[v].g, b.ctls[v].n[q]).ntiles; },
                  s_pre, s_tmp);
-    const uint32_t items = s_pre[b.count];
-    for (uint32_t j = blockIdx.x; j < items; j += gridDim.x) {
+    const uint32_t r = batch_chunk_r(s_pre[b.count], b.count, gridDim.x);
+    __syncthreads();
+    batch_prefix(
+        b.count,
+        [&](uint32_t v) { return (compact_plan<kCompact>(b.vols[v].g, b.ctls[v].n[q]).ntiles + r - 1) / r; },
+        s_pre, s_tmp);
+    const uint32_t j = blockIdx.x;
+    if (j < s_pre[b.count]) {
         const uint32_t v = batch_find(s_pre, b.count, j);
         const KArgs& a = b.vols[v];
         const CompactPlan c = compact_plan<kCompact>(a.g, b.ctls[v].n[q]);
-        if (!compact_tile<kCompact>(a, q, c, j - s_pre[v], b.ctls[v].epoch, sh)) break;
+        const ChunkPlan cp{r, (c.ntiles + r - 1) / r};
+        (void)compact_chunk<kCompact>(a, q, c, cp, j - s_pre[v], b.ctls[v].epoch, sh);
     }
     // volumes with nothing to compact still advance their epochs
     if (threadIdx.x == 0)

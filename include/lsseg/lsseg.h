@@ -291,7 +291,8 @@ lsseg_status lsseg_batch_checkpoint_save(lsseg_batch* batch, int32_t slot);
 lsseg_status lsseg_batch_checkpoint_restore(lsseg_batch* batch, int32_t slot);
 
 /* Launch shape of the batched kernels: workgroups of the update and marking
-   launches and of the compaction launch (at most the co-resident count). */
+   launches and of the keep and compaction launches (more than the volume
+   count, at most the resident count). */
 typedef struct lsseg_tuning {
     uint32_t update_blocks;
     uint32_t compact_blocks;
