@@ -624,10 +624,9 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     ALLOC(S[0], uint32_t, N);
     ALLOC(S[1], uint32_t, N);
     ALLOC(pos, uint32_t, N);
-    ALLOC(mvox, uint8_t, N);
+    ALLOC(emit, uint8_t, N);
     ALLOC(keep, uint8_t, N);
     ALLOC(memb, uint8_t, N);
-    ALLOC(kept, uint8_t, N);
     ALLOC(tstate, uint64_t, ntiles);
     ALLOC(ctl, DevCtl, 1);
 #undef ALLOC
@@ -671,7 +670,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     // zero keeps every entry a valid index
     hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, (uint8_t*)c->a.pos, N * sizeof(uint32_t),
                        (uint8_t)0);
-    hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, c->a.mvox, N, (uint8_t)0);
+    hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, c->a.emit, N, (uint8_t)0);
     hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, c->a.keep, N, (uint8_t)0);
     hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, c->a.memb, N, (uint8_t)0);
     e = hipMemsetAsync(c->a.ctl, 0, sizeof(DevCtl), c->stream);

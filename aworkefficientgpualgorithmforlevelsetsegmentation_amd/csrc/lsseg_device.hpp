@@ -198,10 +198,9 @@ struct KArgs {
     uint32_t* list[2];    // active lists in the reference's order
     uint32_t* S[2];       // the same voxels in a spatially coherent order (the traversal)
     uint32_t* pos;        // list position of every voxel of the current list
-    uint8_t* mvox;        // emission mask of every voxel of the current list
+    uint8_t* emit;        // emission mask per traversal position of the current list
     uint8_t* keep;        // kept candidate lists of every list position
     uint8_t* memb;        // members of the next list, for a dense compaction (zero at rest)
-    uint8_t* kept;        // kept directions per traversal position (keep chunks of several sub-tiles)
     uint64_t* tstate;     // look-back tile states
     DevCtl* ctl;
     const DevParams* prm;
@@ -526,8 +525,8 @@ __device__ inline void update_at(const KArgs& a, const DevParams& P, const PhiT*
     if constexpr (sizeof(PhiT) == 1) a.flag[u] = fabsf(nv - u4) > kTemporalThreshold ? 1 : 0;
 }
 
-// marking of active voxel u (UpdateActiveElements.cu:75-103); returns the
-// number of candidate emissions
+// marking of active voxel u (UpdateActiveElements.cu:75-103); returns its
+// emission mask
 template <typename PhiT>
 __device__ inline uint32_t mark_at(const KArgs& a, bool ref, const PhiT* __restrict__ R,
                                    const PhiT* __restrict__ W, uint8_t* __restrict__ Cb, uint32_t u)
@@ -562,8 +561,7 @@ __device__ inline uint32_t mark_at(const KArgs& a, bool ref, const PhiT* __restr
                 if ((m >> e) & 1u) Cb[(uint64_t)nb[e] * 8u + (uint32_t)e] = 1;
         }
     }
-    a.mvox[u] = (uint8_t)m;
-    return (uint32_t)__popc(m);
+    return m;
 }
 
 // block-wide sum of c, added to *acc by one thread; every thread calls it
@@ -808,37 +806,38 @@ __device__ inline void keep_finish(const KArgs& a, int q, uint32_t n, uint32_t m
     ctl->epoch = epoch + 1;
 }
 
-// the traversal entries of keep sub-tile `sub` of list q; positions are
-// bounded by N (every one a valid address), and keep_decide discards the
-// entries at or beyond the count
-__device__ __attribute__((always_inline)) inline void keep_voxels(const KArgs& a, int q, uint32_t sub, uint32_t* us)
+// the traversal entries of keep sub-tile `sub` of list q and their emission
+// masks; positions are bounded by N (every one a valid address), and
+// keep_decide discards the entries at or beyond the count
+__device__ __attribute__((always_inline)) inline void keep_voxels(const KArgs& a, int q, uint32_t sub, uint32_t* us,
+                                                                  uint32_t* ms)
 {
     const uint64_t base = (uint64_t)sub * kKeepTile;
 #pragma unroll
     for (int j = 0; j < kKeepItems; ++j) {
         const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-        us[j] = a.S[q][i < a.g.N ? i : 0];
+        const uint64_t ic = i < a.g.N ? i : 0;
+        us[j] = a.S[q][ic];
+        ms[j] = a.emit[ic];
     }
 }
 
-// the kept directions kb of the voxels us of keep sub-tile `sub`, written to
-// keep[pos[u]] as well; entries past the count become voxel 0 with kb = 0
+// the kept directions kb of the voxels us (emission masks ms) of keep sub-tile
+// `sub`, written to keep[pos[u]] as well; entries past the count become voxel
+// 0 with kb = 0
 __device__ __attribute__((always_inline)) inline void keep_decide(const KArgs& a, uint32_t n, uint32_t sub,
-                                                                  uint32_t* us, uint32_t* kb)
+                                                                  uint32_t* us, const uint32_t* ms, uint32_t* kb)
 {
     const uint64_t* __restrict__ Ck = a.C;
     const uint64_t base = (uint64_t)sub * kKeepTile;
-    // the masks and positions, then the records of the emitted candidates
-    // (other lanes load the voxel's own record and ignore it)
-    uint32_t ms[kKeepItems], ps[kKeepItems];
+    // the positions and the records of the emitted candidates together (other
+    // lanes load the voxel's own record and ignore it)
+    uint32_t ps[kKeepItems];
 #pragma unroll
     for (int j = 0; j < kKeepItems; ++j)
         if (base + (uint64_t)j * kBlock + threadIdx.x >= n) us[j] = 0;
 #pragma unroll
-    for (int j = 0; j < kKeepItems; ++j) {
-        ms[j] = a.mvox[us[j]];
-        ps[j] = a.pos[us[j]];
-    }
+    for (int j = 0; j < kKeepItems; ++j) ps[j] = a.pos[us[j]];
     // a -z candidate (e = 0) comes first in any record, so only e >= 1 needs one
     uint64_t rec[kKeepItems][7];
 #pragma unroll
@@ -912,30 +911,31 @@ __device__ __attribute__((always_inline)) inline void keep_append(const KArgs& a
     __syncthreads();
 }
 
-// Look-back chunk `chunk` of the keep step; `us` holds the entries of its
-// first sub-tile.  With several sub-tiles the kept directions of the first
-// pass are kept per traversal position in `kept` for the second.  Returns
+// Look-back chunk `chunk` of the keep step; `us` and `ms` hold the entries and
+// emission masks of its first sub-tile.  With several sub-tiles the first pass
+// replaces each emission mask by the kept directions for the second.  Returns
 // false (with the error flag set) if the next list would overflow.
 __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a, int q, uint32_t n,
                                                                  const ChunkPlan& cp, uint32_t chunk,
-                                                                 uint32_t epoch, KeepShared& sh, uint32_t* us)
+                                                                 uint32_t epoch, KeepShared& sh, uint32_t* us,
+                                                                 uint32_t* ms)
 {
     const bool dense = !path_is_few(n, a.g.N);
     const uint32_t s0 = chunk * cp.r, s1 = min(s0 + cp.r, keep_tiles(n));
     uint32_t kb[kKeepItems], ex[kKeepItems];
     uint32_t total = 0;
     if (cp.r == 1) {
-        keep_decide(a, n, s0, us, kb);
+        keep_decide(a, n, s0, us, ms, kb);
         total = keep_counts(kb, ex, sh);
     } else {
         for (uint32_t sub = s0; sub < s1; ++sub) {
-            keep_voxels(a, q, sub, us);
-            keep_decide(a, n, sub, us, kb);
+            keep_voxels(a, q, sub, us, ms);
+            keep_decide(a, n, sub, us, ms, kb);
             const uint64_t base = (uint64_t)sub * kKeepTile;
 #pragma unroll
             for (int j = 0; j < kKeepItems; ++j) {
                 const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-                if (i < n) a.kept[i] = (uint8_t)kb[j];
+                if (i < n) a.emit[i] = (uint8_t)kb[j];  // read back by the second pass
             }
             total += keep_counts(kb, ex, sh);
         }
@@ -950,13 +950,10 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
     } else {
         uint32_t run = prefix;
         for (uint32_t sub = s0; sub < s1; ++sub) {
-            keep_voxels(a, q, sub, us);
+            keep_voxels(a, q, sub, us, ms);  // ms: the kept directions of the first pass
             const uint64_t base = (uint64_t)sub * kKeepTile;
 #pragma unroll
-            for (int j = 0; j < kKeepItems; ++j) {
-                const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-                kb[j] = i < n ? a.kept[i] : 0u;
-            }
+            for (int j = 0; j < kKeepItems; ++j) kb[j] = base + (uint64_t)j * kBlock + threadIdx.x < n ? ms[j] : 0u;
             const uint32_t t = keep_counts(kb, ex, sh);
             keep_append(a, q, dense, us, kb, ex, run, sh);
             run += t;
@@ -970,12 +967,12 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
 __global__ __launch_bounds__(kBlock) void k_keep(KArgs a, int q)
 {
     __shared__ KeepShared sh;
-    uint32_t us[kKeepItems];
-    keep_voxels(a, q, blockIdx.x, us);  // with the count: this workgroup's sub-tile when chunks are single
+    uint32_t us[kKeepItems], ms[kKeepItems];
+    keep_voxels(a, q, blockIdx.x, us, ms);  // with the count: this workgroup's sub-tile when chunks are single
     const uint32_t epoch = a.ctl->epoch;
     const uint32_t n = a.ctl->n[q];
     const ChunkPlan cp = chunk_plan(keep_tiles(n), gridDim.x);
-    if (blockIdx.x < cp.chunks) (void)keep_chunk(a, q, n, cp, blockIdx.x, epoch, sh, us);
+    if (blockIdx.x < cp.chunks) (void)keep_chunk(a, q, n, cp, blockIdx.x, epoch, sh, us, ms);
     if (cp.chunks == 0 && blockIdx.x == 0 && threadIdx.x == 0) keep_finish(a, q, 0, 0, epoch);
 }
 
@@ -999,7 +996,7 @@ __device__ inline void walk_list(const KArgs& a, int q, uint32_t first, uint32_t
     uint32_t u = S[p < a.g.N ? p : 0];
     const uint32_t n = a.ctl->n[q];
     while (p < n) {
-        f(u);
+        f(u, p);
         w.next();
         p = w.pos();
         if (p < n) u = S[p];
@@ -1010,7 +1007,7 @@ template <typename SrcT, bool kFloatSrc, typename PhiT>
 __device__ inline void update_walk(const KArgs& a, int q, uint32_t first = 0, uint32_t count = 0)
 {
     const DevParams P = *a.prm;
-    walk_list(a, q, first, count, [&](uint32_t u) {
+    walk_list(a, q, first, count, [&](uint32_t u, uint64_t) {
         update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), u);
     });
 }
@@ -1070,8 +1067,10 @@ __global__ __launch_bounds__(kBlock) void k_mark(KArgs a, int q)
     __shared__ uint32_t s_red[kBlock / 64];
     const bool ref = a.prm->refMode != 0;
     uint32_t emitted = 0;
-    walk_list(a, q, 0, 0, [&](uint32_t u) {
-        emitted += mark_at<PhiT>(a, ref, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, u);
+    walk_list(a, q, 0, 0, [&](uint32_t u, uint64_t p) {
+        const uint32_t m = mark_at<PhiT>(a, ref, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, u);
+        a.emit[p] = (uint8_t)m;
+        emitted += (uint32_t)__popc(m);
     });
     block_add(&a.ctl->c_acc, emitted, s_red);
 }
@@ -1174,9 +1173,12 @@ __global__ __launch_bounds__(kBlock) void k_mark_batch(BatchArgs b, int q)
         const KArgs& a = b.vols[v];
         const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
         uint32_t emitted = 0;
-        if (p < b.ctls[v].n[q])
-            emitted = mark_at<PhiT>(a, a.prm->refMode != 0, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1),
-                                    (uint8_t*)a.C, a.S[q][p]);
+        if (p < b.ctls[v].n[q]) {
+            const uint32_t m = mark_at<PhiT>(a, a.prm->refMode != 0, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1),
+                                             (uint8_t*)a.C, a.S[q][p]);
+            a.emit[p] = (uint8_t)m;
+            emitted = (uint32_t)__popc(m);
+        }
         block_add(&b.ctls[v].c_acc, emitted, s_red);
     }
 }
@@ -1209,9 +1211,9 @@ __global__ __launch_bounds__(kBlock) void k_keep_batch(BatchArgs b, int q)
         const uint32_t n = b.ctls[v].n[q];
         const ChunkPlan cp{r, (keep_tiles(n) + r - 1) / r};
         const uint32_t chunk = j - s_pre[v];
-        uint32_t us[kKeepItems];
-        keep_voxels(a, q, chunk * r, us);
-        (void)keep_chunk(a, q, n, cp, chunk, b.ctls[v].epoch, sh, us);
+        uint32_t us[kKeepItems], ms[kKeepItems];
+        keep_voxels(a, q, chunk * r, us, ms);
+        (void)keep_chunk(a, q, n, cp, chunk, b.ctls[v].epoch, sh, us, ms);
     }
     // volumes with an empty list still advance their counts
     if (threadIdx.x == 0)
