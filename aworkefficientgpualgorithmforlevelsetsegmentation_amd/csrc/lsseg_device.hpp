@@ -8,29 +8,33 @@
 // spatially coherent order, and pos[u], the position of u in V.  The list order
 // scatters neighbouring voxels over the whole volume after a few hundred
 // iterations, so every per-voxel step runs over S, where consecutive lanes
-// touch neighbouring voxels, and only the compaction walks V.  Four kernel
-// launches per iteration k (all counts in device memory):
+// touch neighbouring voxels, and only the compaction walks V.  The steps of
+// iteration k (all counts in device memory):
 //   update   over S: the 7-point update of phi (UpdateLevelSetVolume.cu:222-342)
 //            from the read buffer into the write buffer, plus the export byte;
 //   mark     over S: the temporal AND spatial test on the new phi
 //            (UpdateActiveElements.cu:75-103), positions p % 4 == 3 skipped
-//            (UpdateActiveElementsVolume.cu:329), candidates recorded;
+//            (UpdateActiveElementsVolume.cu:329), candidates recorded, the
+//            emission mask kept per S position;
 //   keep     over S: first-wins duplicate removal (FilterDuplicates.cu:42-66),
-//            one byte of kept lists per list position, and the kept voxels in
-//            S order, which become the next S;
+//            one byte of kept lists per list position, the kept voxels in S
+//            order (the next S), the next count and the statistics;
 //   compact  over the candidate slots in list order: stable compaction into the
 //            next list (Segmenter.cpp:883-1019, Cuda.cpp:699-758) and its
 //            positions.
-// Both scans are single-pass decoupled look-back scans (Merrill & Garland,
-// "Single-pass Parallel Prefix Scan with Decoupled Look-back", 2016).
+// The compaction of iteration k touches nothing that the update of k + 1
+// reads or writes, so both run in one launch (k_update_fused): three launches
+// per iteration.  Keep and compaction are single-pass decoupled look-back
+// scans (Merrill & Garland, "Single-pass Parallel Prefix Scan with Decoupled
+// Look-back", 2016) with one look-back tile per workgroup.
 //
 // Emission record C (8 bytes per voxel): byte e of C[u] is 1 iff u was emitted
 // into candidate list e (0:-z 1:-y 2:-x 3:self 4:+x 5:+y 6:+z,
 // UpdateActiveElementsVolume.cu:168-174).  Over the list-major concatenation of
 // the seven lists, first-wins keeps candidate (e, p) iff the lowest set byte
-// of C[u] is e; the dense path keeps every u with C[u] != 0 in tiled order.
-// The voxels with C[u] != 0 are exactly the next list, so the next update
-// clears C at the voxels it visits.
+// of C[u] is e; the dense path keeps the kept voxels (marked in memb by the
+// keep step) in tiled order.  The voxels with C[u] != 0 are exactly the next
+// list, so the next update clears C at the voxels it visits.
 #pragma once
 
 #include <hip/hip_runtime.h>

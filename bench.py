@@ -474,6 +474,22 @@ def main():
         ctx.close()
         return
 
+    # the same run from and to host arrays: context creation (upload, pad and
+    # tile the source), phi0 upload, the iterations and the mask download
+    device_sync()
+    t0 = time.perf_counter()
+    hctx = Context(w.volume, phi_storage=phi_storage)
+    hctx.set_phi(w.phi0, w.band)
+    hst = hctx.iterate(w.T, w.eps, w.alpha, w.dt, w.iterations)
+    hmask = hctx.mask()
+    host_ms = 1e3 * (time.perf_counter() - t0)
+    assert np.array_equal(hmask, gpu_mask), "host-inclusive run diverged from the timed run"
+    hctx.close()
+    host_inclusive = {"ms": host_ms, "value": int(hst.sum_active) / (host_ms * 1e-3), "unit": "active-voxel updates/s",
+                      "what": "lsseg_create (source upload, pad, tile) + lsseg_set_phi + lsseg_iterate + "
+                              "lsseg_get_mask, host arrays in and out, one run"}
+    log(f"host-inclusive run: {host_ms:.2f} ms ({host_inclusive['value']:.3e} updates/s)")
+
     # per-launch device times (HIP events on the engine's stream), in issue
     # order: update, marking and keep of each iteration (the update fused with
     # the previous compaction from the second iteration on), then the last
@@ -569,6 +585,7 @@ def main():
                           "frac": path_gbs / HBM_PEAK_GBS, "bytes_per_step": path_bytes},
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "host_inclusive": host_inclusive,
         "batched": batched,
         "adjoint": adjoint,
         "dice_vs_oracle": dice,
