@@ -628,6 +628,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     ALLOC(keep, uint8_t, N);
     ALLOC(memb, uint8_t, N);
     ALLOC(tstate, uint64_t, ntiles);
+    ALLOC(cpart, uint64_t, kCountSlots * kCountStride);
     ALLOC(ctl, DevCtl, 1);
 #undef ALLOC
     if ((st = dalloc(c, &p, (size_t)N * c->S)) != LSSEG_OK) return bail(st);
@@ -666,6 +667,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     if (c->phi8) hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, c->a.flag, N, (uint8_t)0);
     fill_u64(c, c->a.C, N, 0);
     fill_u64(c, c->a.tstate, ntiles, 0);
+    fill_u64(c, c->a.cpart, kCountSlots * kCountStride, 0);
     // positions, masks and keep bytes of voxels outside the list are never read;
     // zero keeps every entry a valid index
     hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, (uint8_t*)c->a.pos, N * sizeof(uint32_t),
@@ -804,7 +806,6 @@ lsseg_status lsseg_iterate(lsseg_ctx* c, const lsseg_params* prm, int32_t iters,
     c->h_ctl->iter = 0;
     c->h_ctl->stats_cap = c->stats_cap;
     c->h_ctl->error = 0;
-    c->h_ctl->c_acc = 0;
     c->h_ctl->few_iters = 0;
     c->h_ctl->many_iters = 0;
     HIP_TRY(c, hipMemcpyAsync(c->a.ctl, c->h_ctl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
@@ -1401,7 +1402,6 @@ lsseg_status lsseg_batch_iterate(lsseg_batch* b, const lsseg_params* params, int
         h.iter = 0;
         h.stats_cap = b->vols[v]->stats_cap;
         h.error = 0;
-        h.c_acc = 0;
         h.few_iters = 0;
         h.many_iters = 0;
     }
@@ -1708,7 +1708,6 @@ lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t ite
     c->h_ctl->iter = 0;
     c->h_ctl->stats_cap = c->stats_cap;
     c->h_ctl->error = 0;
-    c->h_ctl->c_acc = 0;
     c->h_ctl->few_iters = 0;
     c->h_ctl->many_iters = 0;
     HIP_TRY(c, hipMemcpyAsync(c->a.ctl, c->h_ctl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));

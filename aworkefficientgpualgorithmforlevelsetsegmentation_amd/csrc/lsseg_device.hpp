@@ -173,7 +173,6 @@ struct DevCtl {
     uint32_t epoch;       // +1 per keep and compaction pass; stamps the look-back states
     uint32_t stats_cap;
     uint32_t error;
-    uint64_t c_acc;       // candidate emissions of the iteration being marked
     uint32_t few_iters, many_iters;
 };
 
@@ -189,6 +188,14 @@ struct DevParams {
     int32_t refMode;      // skip positions p % 4 == 3
     int32_t pad_;
 };
+
+// Candidate emissions of the iteration being marked are counted in
+// kCountSlots counters kCountStride words (128 bytes) apart, workgroup b adding
+// to counter b % kCountSlots: device-scope atomics on one address serialise
+// (about 11 ns each on MI355X), so a single counter would cost microseconds
+// per launch.  The keep step sums and clears them.
+constexpr int kCountSlots = 64;
+constexpr int kCountStride = 16;
 
 struct KArgs {
     Geom g;
@@ -206,6 +213,7 @@ struct KArgs {
     uint8_t* keep;        // kept candidate lists of every list position
     uint8_t* memb;        // members of the next list, for a dense compaction (zero at rest)
     uint64_t* tstate;     // look-back tile states
+    uint64_t* cpart;      // candidate-emission counters (kCountSlots, zero between iterations)
     DevCtl* ctl;
     const DevParams* prm;
     uint32_t* n_hist;
@@ -792,20 +800,29 @@ struct KeepShared {
 __device__ inline uint32_t keep_tiles(uint32_t n) { return (n + kKeepTile - 1) / kKeepTile; }
 
 // counts and statistics of the iteration from list[q] (n entries) to the next
-// list (m entries); one thread
+// list (m entries), the candidate counters summed and cleared; run by the 64
+// lanes of one wave
 __device__ inline void keep_finish(const KArgs& a, int q, uint32_t n, uint32_t m, uint32_t epoch)
 {
+    const int lane = threadIdx.x & 63;
+    uint64_t c = 0;
+    for (int i = lane; i < kCountSlots; i += 64) {
+        c += a.cpart[i * kCountStride];
+        a.cpart[i * kCountStride] = 0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += (uint64_t)__shfl_xor((unsigned long long)c, o, 64);
+    if (lane != 0) return;
     DevCtl* ctl = a.ctl;
     ctl->n[q ^ 1] = m;
     const uint32_t k = ctl->iter;
     if (k < ctl->stats_cap) {
         a.n_hist[k] = n;
-        a.c_hist[k] = ctl->c_acc;
+        a.c_hist[k] = c;
         a.m_hist[k] = m;
     }
     if (path_is_few(n, a.g.N)) ctl->few_iters += 1;
     else ctl->many_iters += 1;
-    ctl->c_acc = 0;
     ctl->iter = k + 1;
     ctl->epoch = epoch + 1;
 }
@@ -963,7 +980,7 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
             run += t;
         }
     }
-    if (threadIdx.x == 0 && chunk == cp.chunks - 1) keep_finish(a, q, n, prefix + total, epoch);
+    if (threadIdx.x < 64 && chunk == cp.chunks - 1) keep_finish(a, q, n, prefix + total, epoch);
     __syncthreads();
     return true;
 }
@@ -977,7 +994,7 @@ __global__ __launch_bounds__(kBlock) void k_keep(KArgs a, int q)
     const uint32_t n = a.ctl->n[q];
     const ChunkPlan cp = chunk_plan(keep_tiles(n), gridDim.x);
     if (blockIdx.x < cp.chunks) (void)keep_chunk(a, q, n, cp, blockIdx.x, epoch, sh, us, ms);
-    if (cp.chunks == 0 && blockIdx.x == 0 && threadIdx.x == 0) keep_finish(a, q, 0, 0, epoch);
+    if (cp.chunks == 0 && blockIdx.x == 0 && threadIdx.x < 64) keep_finish(a, q, 0, 0, epoch);
 }
 
 // ---------------------------------------------------------------------------
@@ -1059,7 +1076,6 @@ __global__ __launch_bounds__(kBlock) void k_empty_iterations(KArgs a, uint32_t c
     if (threadIdx.x == 0) {
         ctl->n[0] = 0;
         ctl->n[1] = 0;
-        ctl->c_acc = 0;
         ctl->few_iters += count;
         ctl->iter = k0 + count;
     }
@@ -1076,7 +1092,7 @@ __global__ __launch_bounds__(kBlock) void k_mark(KArgs a, int q)
         a.emit[p] = (uint8_t)m;
         emitted += (uint32_t)__popc(m);
     });
-    block_add(&a.ctl->c_acc, emitted, s_red);
+    block_add(&a.cpart[(blockIdx.x % kCountSlots) * kCountStride], emitted, s_red);
 }
 
 template <int kMode>
@@ -1183,7 +1199,7 @@ __global__ __launch_bounds__(kBlock) void k_mark_batch(BatchArgs b, int q)
             a.emit[p] = (uint8_t)m;
             emitted = (uint32_t)__popc(m);
         }
-        block_add(&b.ctls[v].c_acc, emitted, s_red);
+        block_add(&a.cpart[(blockIdx.x % kCountSlots) * kCountStride], emitted, s_red);
     }
 }
 
@@ -1220,7 +1236,7 @@ __global__ __launch_bounds__(kBlock) void k_keep_batch(BatchArgs b, int q)
         (void)keep_chunk(a, q, n, cp, chunk, b.ctls[v].epoch, sh, us, ms);
     }
     // volumes with an empty list still advance their counts
-    if (threadIdx.x == 0)
+    if (threadIdx.x < 64)
         for (uint32_t v = blockIdx.x; v < b.count; v += gridDim.x)
             if (s_pre[v + 1] == s_pre[v]) keep_finish(b.vols[v], q, 0, 0, b.ctls[v].epoch);
 }
