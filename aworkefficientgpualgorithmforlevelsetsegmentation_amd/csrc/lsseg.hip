@@ -1697,8 +1697,8 @@ lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t ite
     }
     if ((st = ralloc(c, r, &p, (size_t)c->nx * c->ny * c->nz * sizeof(float))) != LSSEG_OK) return bail(st);
     r->lin = (float*)p;
-    if ((st = ralloc(c, r, &p, 3 * sizeof(double))) != LSSEG_OK) return bail(st);
-    r->d_theta = (double*)p;
+    if ((st = ralloc(c, r, &p, kCountSlots * kCountStride * sizeof(double))) != LSSEG_OK) return bail(st);
+    r->d_theta = (double*)p;  // per-workgroup slots of dL/dT, dL/deps, dL/dalpha
     HIP_TRY(c, hipMemcpyAsync(r->d_prm, &P, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
     // pass 2 from the same start: record lists and decisions, checkpoint phi
     if ((st = lsseg_checkpoint_restore(c, kRecordSlot)) != LSSEG_OK) return bail(st);
@@ -1796,7 +1796,7 @@ lsseg_status lsseg_bwd(lsseg_ctx* c, lsseg_record* r, const float* dphi, float* 
     hipLaunchKernelGGL(k_lin_to_tiled, dim3(dg), dim3(kBlock), 0, c->stream, c->a.g, r->lin, r->adj[0], c->nx, c->ny,
                        c->nz, 0.0f);
     hipLaunchKernelGGL(k_fill_f32, dim3(dg), dim3(kBlock), 0, c->stream, r->adj[1], N, 0.0f);
-    HIP_TRY(c, hipMemsetAsync(r->d_theta, 0, 3 * sizeof(double), c->stream));
+    HIP_TRY(c, hipMemsetAsync(r->d_theta, 0, kCountSlots * kCountStride * sizeof(double), c->stream));
     const KArgs ra = record_args(c, r->d_prm);
     int ab = 0;  // adj[ab] = Abar, adj[ab ^ 1] = Bbar
     const int nck = (int)r->ckA.size();
@@ -1829,10 +1829,14 @@ lsseg_status lsseg_bwd(lsseg_ctx* c, lsseg_record* r, const float* dphi, float* 
     hipLaunchKernelGGL(k_tiled_sum_to_lin, dim3(dense_grid(cnt)), dim3(kBlock), 0, c->stream, c->a.g, r->adj[0],
                        r->adj[1], r->lin, c->nx, c->ny, c->nz);
     HIP_TRY(c, hipMemcpyAsync(dphi0, r->lin, cnt * sizeof(float), hipMemcpyDefault, c->stream));
-    double th[3] = {0, 0, 0};
-    HIP_TRY(c, hipMemcpyAsync(th, r->d_theta, sizeof(th), hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> slots((size_t)kCountSlots * kCountStride);
+    HIP_TRY(c, hipMemcpyAsync(slots.data(), r->d_theta, slots.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    double th[3] = {0, 0, 0};
+    for (int i = 0; i < kCountSlots; ++i)
+        for (int j = 0; j < 3; ++j) th[j] += slots[(size_t)i * kCountStride + j];
     if (dtheta) memcpy(dtheta, th, sizeof(th));
     return LSSEG_OK;
 }

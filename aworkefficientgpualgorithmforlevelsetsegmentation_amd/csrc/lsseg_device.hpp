@@ -1362,7 +1362,9 @@ __device__ inline float density_replay(const KArgs& a, const DevParams& P, uint3
     }
 }
 
-// block-wide sum of three values into theta[0..2] (double atomics); every thread calls it
+// block-wide sum of three values into theta[0..2] (double atomics); every
+// thread calls it.  theta is the workgroup's slot of kCountSlots slots
+// kCountStride doubles apart (see kCountSlots), which the host adds up.
 __device__ inline void block_add3(double* theta, double v0, double v1, double v2, double* s_red)
 {
 #pragma unroll
@@ -1444,7 +1446,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(KArgs a, const uint32_t* __r
             tE += dspeed * (P.negInv * P.negInv * (sgn * di));
         }
     }
-    block_add3(theta, tT, tE, tA, s_red);
+    block_add3(theta + (blockIdx.x % kCountSlots) * kCountStride, tT, tE, tA, s_red);
 }
 
 // linear unpadded <-> tiled padded
