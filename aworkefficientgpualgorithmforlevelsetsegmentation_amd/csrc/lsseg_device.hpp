@@ -617,6 +617,7 @@ struct CompactShared {
 // in list-major order, dense path over all N voxels in tiled order
 struct CompactPlan {
     uint32_t n_src;
+    uint32_t n_dst;   // the next list's count, set by the keep step (loaded early)
     bool dense;
     uint64_t slots;
     uint32_t ntiles;  // sub-tiles of kScanTile slots
@@ -627,6 +628,7 @@ __device__ inline CompactPlan compact_plan(const Geom& g, uint32_t n_src)
 {
     CompactPlan c;
     c.n_src = n_src;
+    c.n_dst = 0;
     c.dense = (kMode == kInit) || !path_is_few(n_src, g.N);
     c.slots = c.dense ? g.N : 7ull * n_src;
     c.ntiles = (uint32_t)((c.slots + kScanTile - 1) / kScanTile);
@@ -634,15 +636,15 @@ __device__ inline CompactPlan compact_plan(const Geom& g, uint32_t n_src)
 }
 
 // once the last chunk is placed (one thread).  The keep step has already set
-// the count of the next list; the compaction must arrive at the same count.
+// the count of the next list (c.n_dst); the compaction must arrive at it.
 template <int kMode>
-__device__ inline void compact_finish(const KArgs& a, int q, uint32_t m, uint32_t epoch)
+__device__ inline void compact_finish(const KArgs& a, int q, const CompactPlan& c, uint32_t m, uint32_t epoch)
 {
     DevCtl* ctl = a.ctl;
     if constexpr (kMode == kInit) {
         ctl->n[q ^ 1] = m;
         ctl->n[q] = 0;
-    } else if (ctl->n[q ^ 1] != m) {
+    } else if (c.n_dst != m) {
         atomicOr(&ctl->error, kErrCount);
     }
     ctl->epoch = epoch + 1;
@@ -780,7 +782,7 @@ __device__ inline bool compact_chunk(const KArgs& a, int q, const CompactPlan& c
             run += t;
         }
     }
-    if (threadIdx.x == 0 && chunk == cp.chunks - 1) compact_finish<kMode>(a, q, prefix + total, epoch);
+    if (threadIdx.x == 0 && chunk == cp.chunks - 1) compact_finish<kMode>(a, q, c, prefix + total, epoch);
     __syncthreads();
     return true;
 }
@@ -793,7 +795,7 @@ __device__ inline void compact_all(const KArgs& a, int q, const CompactPlan& c, 
 {
     const ChunkPlan cp = chunk_plan(c.ntiles, participants);
     if (blockIdx.x < cp.chunks) (void)compact_chunk<kMode>(a, q, c, cp, blockIdx.x, epoch, sh);
-    if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kMode>(a, q, 0, epoch);
+    if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kMode>(a, q, c, 0, epoch);
 }
 
 // ---------------------------------------------------------------------------
@@ -823,10 +825,23 @@ __device__ inline uint64_t candidate_share(const KArgs& a)
     return c;
 }
 
+// the control fields that keep_finish updates, read before the chunk's work
+// (only keep_finish changes them during the keep step)
+struct CtlSnapshot {
+    uint32_t iter, stats_cap, few, many;
+};
+
+__device__ inline CtlSnapshot ctl_snapshot(const DevCtl* ctl)
+{
+    return {ctl->iter, ctl->stats_cap, ctl->few_iters, ctl->many_iters};
+}
+
 // counts and statistics of the iteration from list[q] (n entries) to the next
-// list (m entries); c: each lane's candidate_share, loaded earlier; the
-// counters are cleared.  Run by the 64 lanes of one wave.
-__device__ inline void keep_finish(const KArgs& a, int q, uint32_t n, uint32_t m, uint32_t epoch, uint64_t c)
+// list (m entries); c: each lane's candidate_share and st: the control fields,
+// both loaded earlier; the counters are cleared.  Run by the 64 lanes of one
+// wave.
+__device__ inline void keep_finish(const KArgs& a, int q, uint32_t n, uint32_t m, uint32_t epoch, uint64_t c,
+                                   const CtlSnapshot& st)
 {
     const int lane = threadIdx.x & 63;
     for (int i = lane; i < kCountSlots; i += 64) a.cpart[i * kCountStride] = 0;
@@ -835,14 +850,14 @@ __device__ inline void keep_finish(const KArgs& a, int q, uint32_t n, uint32_t m
     if (lane != 0) return;
     DevCtl* ctl = a.ctl;
     ctl->n[q ^ 1] = m;
-    const uint32_t k = ctl->iter;
-    if (k < ctl->stats_cap) {
+    const uint32_t k = st.iter;
+    if (k < st.stats_cap) {
         a.n_hist[k] = n;
         a.c_hist[k] = c;
         a.m_hist[k] = m;
     }
-    if (path_is_few(n, a.g.N)) ctl->few_iters += 1;
-    else ctl->many_iters += 1;
+    if (path_is_few(n, a.g.N)) ctl->few_iters = st.few + 1;
+    else ctl->many_iters = st.many + 1;
     ctl->iter = k + 1;
     ctl->epoch = epoch + 1;
 }
@@ -964,8 +979,9 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
     const bool dense = !path_is_few(n, a.g.N);
     const uint32_t s0 = chunk * cp.r, s1 = min(s0 + cp.r, keep_tiles(n));
     const bool last = chunk == cp.chunks - 1;
-    // the last chunk loads the candidate counters now; they are summed at the end
+    // the last chunk loads the candidate counters and the control fields now
     const uint64_t cshare = (last && threadIdx.x < 64) ? candidate_share(a) : 0ull;
+    const CtlSnapshot snap = last ? ctl_snapshot(a.ctl) : CtlSnapshot{0, 0, 0, 0};
     uint32_t kb[kKeepItems], ex[kKeepItems];
     uint32_t total = 0;
     if (cp.r == 1) {
@@ -1003,7 +1019,7 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
             run += t;
         }
     }
-    if (threadIdx.x < 64 && last) keep_finish(a, q, n, prefix + total, epoch, cshare);
+    if (threadIdx.x < 64 && last) keep_finish(a, q, n, prefix + total, epoch, cshare, snap);
     __syncthreads();
     return true;
 }
@@ -1017,7 +1033,8 @@ __global__ __launch_bounds__(kBlock) void k_keep(KArgs a, int q)
     const uint32_t n = a.ctl->n[q];
     const ChunkPlan cp = chunk_plan(keep_tiles(n), gridDim.x);
     if (blockIdx.x < cp.chunks) (void)keep_chunk(a, q, n, cp, blockIdx.x, epoch, sh, us, ms);
-    if (cp.chunks == 0 && blockIdx.x == 0 && threadIdx.x < 64) keep_finish(a, q, 0, 0, epoch, candidate_share(a));
+    if (cp.chunks == 0 && blockIdx.x == 0 && threadIdx.x < 64)
+        keep_finish(a, q, 0, 0, epoch, candidate_share(a), ctl_snapshot(a.ctl));
 }
 
 // ---------------------------------------------------------------------------
@@ -1072,12 +1089,13 @@ __global__ __launch_bounds__(kBlock) void k_update_fused(KArgs a, int q)
 {
     __shared__ CompactShared sh;
     const uint32_t epoch = a.ctl->epoch;
-    const CompactPlan c = compact_plan<kCompact>(a.g, a.ctl->n[q ^ 1]);
+    CompactPlan c = compact_plan<kCompact>(a.g, a.ctl->n[q ^ 1]);
+    c.n_dst = a.ctl->n[q];
     const uint32_t gc = min((c.ntiles + 7u) & ~7u, (gridDim.x / 4u * 3u) & ~7u);
     if (blockIdx.x < gc) {
         compact_all<kCompact>(a, q ^ 1, c, epoch, sh, gc);
     } else {
-        if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kCompact>(a, q ^ 1, 0, epoch);
+        if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kCompact>(a, q ^ 1, c, 0, epoch);
         update_walk<SrcT, kFloatSrc, PhiT>(a, q, gc, gridDim.x - gc);
     }
 }
@@ -1123,7 +1141,9 @@ __global__ __launch_bounds__(kBlock) void k_compact(KArgs a, int q)
 {
     __shared__ CompactShared sh;
     const uint32_t epoch = a.ctl->epoch;
-    compact_all<kMode>(a, q, compact_plan<kMode>(a.g, a.ctl->n[q]), epoch, sh, gridDim.x);
+    CompactPlan c = compact_plan<kMode>(a.g, a.ctl->n[q]);
+    c.n_dst = a.ctl->n[q ^ 1];
+    compact_all<kMode>(a, q, c, epoch, sh, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1261,7 +1281,8 @@ __global__ __launch_bounds__(kBlock) void k_keep_batch(BatchArgs b, int q)
     // volumes with an empty list still advance their counts
     if (threadIdx.x < 64)
         for (uint32_t v = blockIdx.x; v < b.count; v += gridDim.x)
-            if (s_pre[v + 1] == s_pre[v]) keep_finish(b.vols[v], q, 0, 0, b.ctls[v].epoch, candidate_share(b.vols[v]));
+            if (s_pre[v + 1] == s_pre[v])
+                keep_finish(b.vols[v], q, 0, 0, b.ctls[v].epoch, candidate_share(b.vols[v]), ctl_snapshot(&b.ctls[v]));
 }
 
 // A volume's epoch is read at each of its chunks: its last chunk, which
@@ -1284,14 +1305,19 @@ __global__ __launch_bounds__(kBlock) void k_compact_batch(BatchArgs b, int q)
     if (j < s_pre[b.count]) {
         const uint32_t v = batch_find(s_pre, b.count, j);
         const KArgs& a = b.vols[v];
-        const CompactPlan c = compact_plan<kCompact>(a.g, b.ctls[v].n[q]);
+        CompactPlan c = compact_plan<kCompact>(a.g, b.ctls[v].n[q]);
+        c.n_dst = b.ctls[v].n[q ^ 1];
         const ChunkPlan cp{r, (c.ntiles + r - 1) / r};
         (void)compact_chunk<kCompact>(a, q, c, cp, j - s_pre[v], b.ctls[v].epoch, sh);
     }
     // volumes with nothing to compact still advance their epochs
     if (threadIdx.x == 0)
         for (uint32_t v = blockIdx.x; v < b.count; v += gridDim.x)
-            if (s_pre[v + 1] == s_pre[v]) compact_finish<kCompact>(b.vols[v], q, 0, b.ctls[v].epoch);
+            if (s_pre[v + 1] == s_pre[v]) {
+                CompactPlan c = compact_plan<kCompact>(b.vols[v].g, 0);
+                c.n_dst = b.ctls[v].n[q ^ 1];
+                compact_finish<kCompact>(b.vols[v], q, c, 0, b.ctls[v].epoch);
+            }
 }
 
 #define LSSEG_FOR_EACH(p, count)                                                           \
