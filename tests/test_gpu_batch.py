@@ -96,6 +96,33 @@ def test_batch_matches_solo_runs_and_oracle():
     batch.close()
 
 
+def test_batch_with_the_smallest_scan_grid():
+    """One more scan workgroup than volumes: the batched keep and compaction
+    take several sub-tiles per look-back chunk; every volume still equals its
+    solo run (default grid)."""
+    specs = _volumes()
+    _say(f"batch of {len(specs)} volumes, {len(specs) + 1} scan workgroups")
+    batch = Batch([s[0] for s in specs])
+    batch.set_tuning(256, len(specs) + 1)
+    solos = [Context(s[0]) for s in specs]
+    for i, (vol, phi0, band, prm) in enumerate(specs):
+        batch.volume(i).set_phi(phi0, band)
+        solos[i].set_phi(phi0, band)
+    params = [s[3] for s in specs]
+    batch.iterate(params, 45)
+    for i, s in enumerate(solos):
+        T, eps, alpha, dt, mode = params[i]
+        s.iterate(T, eps, alpha, dt, 45, mode=mode)
+        cb, cs = batch.volume(i).iteration_counts(), s.iteration_counts()
+        assert np.array_equal(cb.n, cs.n) and np.array_equal(cb.c, cs.c) and np.array_equal(cb.m, cs.m)
+        _assert_same(batch.volume(i), s, f"volume {i}")
+    with pytest.raises(LssegError):
+        batch.set_tuning(256, len(specs))  # not above the volume count
+    for s in solos:
+        s.close()
+    batch.close()
+
+
 def test_batch_checkpoint_and_autotune_keep_results():
     specs = _volumes()[:4]
     batch = Batch([s[0] for s in specs])

@@ -211,6 +211,32 @@ def test_fused_and_separate_launches_agree(monkeypatch, wide):
     assert a[3] == 3 * 5 + 1 and b[3] == 4 * 5
 
 
+@pytest.mark.parametrize("wide", [False, True], ids=["few", "dense"])
+def test_chunks_of_several_subtiles(monkeypatch, wide):
+    """With a 16-workgroup grid the keep step, the compaction (8 workgroups in
+    the fused launch) and the dense initial compaction each take several
+    sub-tiles per look-back chunk (counted first, placed after the prefix):
+    states and counts still equal the oracle's every chunk."""
+    _say(f"16-workgroup grid, wide band {wide}")
+    monkeypatch.setenv("LSSEG_GRID_BLOCKS", "16")
+    if wide:
+        n = 32
+        z, y, x = np.mgrid[0:n, 0:n, 0:n]
+        vol = ((x + y + z) % 7 * 30).astype(np.uint8)
+        phi0 = np.clip((x - 15.5) / 3.0, -4, 4).astype(np.float32)
+        w = synth.Workload("dense", vol, phi0, 100.0, 60.0, 0.2, 0.5, 40, band=3.5)
+        iters, chunk, need = 40, 13, 8 * 2048 // 7  # several sub-tiles per compaction chunk
+    else:
+        w = synth.ellipsoids(edge=64)
+        iters, chunk, need = 120, 41, 16 * 512  # several sub-tiles per keep chunk
+    probe = O.State(w.volume)
+    probe.set_phi(w.phi0, w.band)
+    assert max(probe.iterate(w.T, w.eps, w.alpha, w.dt, iters, threads=8)[0]) > need
+    ctx, ref = _run_pair(w, iters, chunk)
+    assert np.array_equal(ctx.mask(), ref.mask())
+    ctx.close()
+
+
 def test_empty_list_ends_the_run_early(monkeypatch):
     """vessels 64^3 with 4 tubes empties its list at iteration 101; of 400
     iterations the trailing ones over the empty list are not launched.  Counts,
