@@ -190,6 +190,7 @@ LSSEG_SRC_PHI_KERNEL(update_kernel, k_update)
 LSSEG_SRC_PHI_KERNEL(update_fused_kernel, k_update_fused)
 LSSEG_SRC_PHI_KERNEL(update_only_kernel, k_update_only)
 LSSEG_SRC_PHI_KERNEL(update_batch_kernel, k_update_batch)
+LSSEG_SRC_PHI_KERNEL(update_fused_batch_kernel, k_update_fused_batch)
 #undef LSSEG_SRC_PHI_KERNEL
 
 const void* mark_kernel(bool phi8) { return phi8 ? (const void*)&k_mark<int8_t> : (const void*)&k_mark<float>; }
@@ -1193,26 +1194,70 @@ void destroy_batch_graphs(lsseg_batch* b)
     }
 }
 
-void launch_batch_iteration(lsseg_batch* b, int q)
+// the batched counterparts of launch_head / launch_tail / launch_iteration: the
+// update (with `fused`, in one launch with every volume's pending compaction:
+// b->grid compaction workgroups, then b->sgrid update workgroups), marking and
+// keep step, compaction
+void launch_batch_head(lsseg_batch* b, int q, bool fused)
+{
+    BatchArgs args{b->d_table, b->d_ctls, (uint32_t)b->vols.size()};
+    const bool phi8 = b->vols[0]->phi8;
+    if (fused) {
+        uint32_t gc = b->grid;
+        void* kargs[] = {(void*)&args, (void*)&q, (void*)&gc};
+        (void)hipLaunchKernel(update_fused_batch_kernel(b->dtype, phi8), dim3(b->grid + b->sgrid), dim3(kBlock), kargs,
+                              0, b->stream);
+    } else {
+        void* kargs[] = {(void*)&args, (void*)&q};
+        (void)hipLaunchKernel(update_batch_kernel(b->dtype, phi8), dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
+    }
+}
+
+void launch_batch_tail(lsseg_batch* b, int q)
 {
     BatchArgs args{b->d_table, b->d_ctls, (uint32_t)b->vols.size()};
     void* kargs[] = {(void*)&args, (void*)&q};
-    const bool phi8 = b->vols[0]->phi8;
-    (void)hipLaunchKernel(update_batch_kernel(b->dtype, phi8), dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
-    (void)hipLaunchKernel(mark_batch_kernel(phi8), dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
+    (void)hipLaunchKernel(mark_batch_kernel(b->vols[0]->phi8), dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
     (void)hipLaunchKernel((const void*)&k_keep_batch, dim3(b->grid), dim3(kBlock), kargs, 0, b->stream);
+}
+
+void launch_batch_compact(lsseg_batch* b, int q)
+{
+    BatchArgs args{b->d_table, b->d_ctls, (uint32_t)b->vols.size()};
+    void* kargs[] = {(void*)&args, (void*)&q};
     (void)hipLaunchKernel((const void*)&k_compact_batch, dim3(b->grid), dim3(kBlock), kargs, 0, b->stream);
 }
 
+void launch_batch_iteration(lsseg_batch* b, int q)
+{
+    launch_batch_head(b, q, false);
+    launch_batch_tail(b, q);
+    launch_batch_compact(b, q);
+}
+
+// as launch_iterations: 3 count + 1 launches, runs of kGraphIters fused
+// iterations from a captured graph; LSSEG_FUSE=0 issues four per iteration
 lsseg_status launch_batch_iterations(lsseg_batch* b, int count, int q)
 {
+    if (count <= 0) return LSSEG_OK;
+    if (!env_int("LSSEG_FUSE", 1)) {
+        for (int i = 0; i < count; ++i) launch_batch_iteration(b, q ^ (i & 1));
+        return LSSEG_OK;
+    }
+    launch_batch_head(b, q, false);
+    launch_batch_tail(b, q);
+    q ^= 1;
+    --count;
     const int chunk = env_int("LSSEG_GRAPH", 1) ? kGraphIters : 0;
     while (chunk && count >= chunk) {
         hipGraphExec_t& ge = b->graphs[q];
         if (!ge) {
             hipGraph_t g = nullptr;
             BHIP_TRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < chunk; ++i) launch_batch_iteration(b, q ^ (i & 1));
+            for (int i = 0; i < chunk; ++i) {
+                launch_batch_head(b, q ^ (i & 1), true);
+                launch_batch_tail(b, q ^ (i & 1));
+            }
             hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (e != hipSuccess) return bfail(b, LSSEG_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
             e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
@@ -1226,9 +1271,11 @@ lsseg_status launch_batch_iterations(lsseg_batch* b, int count, int q)
         count -= chunk;
     }
     for (int i = 0; i < count; ++i) {
-        launch_batch_iteration(b, q);
+        launch_batch_head(b, q, true);
+        launch_batch_tail(b, q);
         q ^= 1;
     }
+    launch_batch_compact(b, q ^ 1);
     return LSSEG_OK;
 }
 

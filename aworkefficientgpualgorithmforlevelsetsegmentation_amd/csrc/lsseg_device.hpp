@@ -1205,14 +1205,15 @@ __device__ inline uint32_t batch_find(const uint32_t* s_pre, uint32_t count, uin
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
 }
 
+// the update of every volume over its S[q]: work items of kBlock positions,
+// volume after volume, item j taken by slot j mod nslots
 template <typename SrcT, bool kFloatSrc, typename PhiT>
-__global__ __launch_bounds__(kBlock) void k_update_batch(BatchArgs b, int q)
+__device__ inline void update_batch_items(const BatchArgs& b, int q, uint32_t slot, uint32_t nslots, uint32_t* s_pre,
+                                          uint32_t* s_tmp)
 {
-    __shared__ uint32_t s_pre[kMaxBatch + 1];
-    __shared__ uint32_t s_tmp[kBlock / 64];
     batch_prefix(b.count, [&](uint32_t v) { return (b.ctls[v].n[q] + kBlock - 1) / kBlock; }, s_pre, s_tmp);
     const uint32_t items = s_pre[b.count];
-    for (uint32_t j = blockIdx.x; j < items; j += gridDim.x) {
+    for (uint32_t j = slot; j < items; j += nslots) {
         const uint32_t v = batch_find(s_pre, b.count, j);
         const KArgs& a = b.vols[v];
         const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
@@ -1221,6 +1222,14 @@ __global__ __launch_bounds__(kBlock) void k_update_batch(BatchArgs b, int q)
             update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), a.S[q][p]);
         }
     }
+}
+
+template <typename SrcT, bool kFloatSrc, typename PhiT>
+__global__ __launch_bounds__(kBlock) void k_update_batch(BatchArgs b, int q)
+{
+    __shared__ uint32_t s_pre[kMaxBatch + 1];
+    __shared__ uint32_t s_tmp[kBlock / 64];
+    update_batch_items<SrcT, kFloatSrc, PhiT>(b, q, blockIdx.x, gridDim.x, s_pre, s_tmp);
 }
 
 template <typename PhiT>
@@ -1285,17 +1294,16 @@ __global__ __launch_bounds__(kBlock) void k_keep_batch(BatchArgs b, int q)
                 keep_finish(b.vols[v], q, 0, 0, b.ctls[v].epoch, candidate_share(b.vols[v]), ctl_snapshot(&b.ctls[v]));
 }
 
-// A volume's epoch is read at each of its chunks: its last chunk, which
-// advances the epoch, completes only after every other chunk of the volume has
-// published.
-__global__ __launch_bounds__(kBlock) void k_compact_batch(BatchArgs b, int q)
+// The compaction of list[q] of every volume over the workgroups of slots
+// [0, participants) (each calls it with its slot, blockIdx.x).  A volume's
+// epoch is read at each of its chunks: its last chunk, which advances the
+// epoch, completes only after every other chunk of the volume has published.
+__device__ inline void compact_batch_chunks(const BatchArgs& b, int q, uint32_t participants, CompactShared& sh,
+                                            uint32_t* s_pre, uint32_t* s_tmp)
 {
-    __shared__ CompactShared sh;
-    __shared__ uint32_t s_pre[kMaxBatch + 1];
-    __shared__ uint32_t s_tmp[kBlock / 64];
     batch_prefix(b.count, [&](uint32_t v) { return compact_plan<kCompact>(b.vols[v].g, b.ctls[v].n[q]).ntiles; },
                  s_pre, s_tmp);
-    const uint32_t r = batch_chunk_r(s_pre[b.count], b.count, gridDim.x);
+    const uint32_t r = batch_chunk_r(s_pre[b.count], b.count, participants);
     __syncthreads();
     batch_prefix(
         b.count,
@@ -1312,12 +1320,32 @@ __global__ __launch_bounds__(kBlock) void k_compact_batch(BatchArgs b, int q)
     }
     // volumes with nothing to compact still advance their epochs
     if (threadIdx.x == 0)
-        for (uint32_t v = blockIdx.x; v < b.count; v += gridDim.x)
+        for (uint32_t v = blockIdx.x; v < b.count; v += participants)
             if (s_pre[v + 1] == s_pre[v]) {
                 CompactPlan c = compact_plan<kCompact>(b.vols[v].g, 0);
                 c.n_dst = b.ctls[v].n[q ^ 1];
                 compact_finish<kCompact>(b.vols[v], q, c, 0, b.ctls[v].epoch);
             }
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact_batch(BatchArgs b, int q)
+{
+    __shared__ CompactShared sh;
+    __shared__ uint32_t s_pre[kMaxBatch + 1];
+    __shared__ uint32_t s_tmp[kBlock / 64];
+    compact_batch_chunks(b, q, gridDim.x, sh, s_pre, s_tmp);
+}
+
+// the batched counterpart of k_update_fused: workgroups [0, gc) compact every
+// volume's list[q ^ 1] into list[q], the others run the update over S[q]
+template <typename SrcT, bool kFloatSrc, typename PhiT>
+__global__ __launch_bounds__(kBlock) void k_update_fused_batch(BatchArgs b, int q, uint32_t gc)
+{
+    __shared__ CompactShared sh;
+    __shared__ uint32_t s_pre[kMaxBatch + 1];
+    __shared__ uint32_t s_tmp[kBlock / 64];
+    if (blockIdx.x < gc) compact_batch_chunks(b, q ^ 1, gc, sh, s_pre, s_tmp);
+    else update_batch_items<SrcT, kFloatSrc, PhiT>(b, q, blockIdx.x - gc, gridDim.x - gc, s_pre, s_tmp);
 }
 
 #define LSSEG_FOR_EACH(p, count)                                                           \
