@@ -1758,6 +1758,10 @@ lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t ite
     c->h_ctl->few_iters = 0;
     c->h_ctl->many_iters = 0;
     HIP_TRY(c, hipMemcpyAsync(c->a.ctl, c->h_ctl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
+    // the launches of lsseg_iterate (each compaction in the next update's
+    // launch), with the recording of iteration k after its update launch: it
+    // reads list[q], which that launch completes, and the read buffer, which
+    // the update leaves alone
     int q = c->q;
     for (int k = 0; k < iters; ++k) {
         if (k % every == 0) {
@@ -1766,14 +1770,16 @@ lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t ite
             HIP_TRY(c, hipMemcpyAsync(r->ckB[k / every], c->a.phi[q ^ 1], N * sizeof(float),
                                       hipMemcpyDeviceToDevice, c->stream));
         }
+        launch_head(c, q, k > 0);
         uint32_t* L = r->lists + r->off[k];
         uint8_t* D = r->dec + r->off[k];
         uint32_t n = r->n[k];
         void* args[] = {(void*)&c->a, (void*)&q, (void*)&n, (void*)&L, (void*)&D};
         (void)hipLaunchKernel(k_record_kernel(c->dtype), dim3(list_grid(n)), dim3(kBlock), args, 0, c->stream);
-        launch_iteration(c, q);
+        launch_tail(c, q);
         q ^= 1;
     }
+    if (iters > 0) launch(c, (const void*)&k_compact<kCompact>, c->grid, q ^ 1);
     HIP_TRY(c, hipGetLastError());
     if ((st = sync_and_check(c)) != LSSEG_OK) return bail(st);
     c->q = q;

@@ -48,8 +48,11 @@ def log(msg):
 
 
 def device_sync():
-    hip = ctypes.CDLL("libamdhip64.so")
-    rc = hip.hipDeviceSynchronize()
+    """hipDeviceSynchronize of the HIP runtime the engine library is linked
+    against (resolved through the engine's own dependencies, so that with torch
+    loaded it is torch's runtime, the one the engine uses)."""
+    from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import lib
+    rc = lib().hipDeviceSynchronize()
     if rc != 0:
         raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
 
@@ -280,20 +283,24 @@ def _soft_dice_grad(phi, mask, temp=0.5):
 
 
 def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
-    """The differentiable form on the 16-volume 96^3 float workload: forward
-    record + adjoint against the plain forward, gradients against central
-    finite differences of the recorded map on every volume (five-point stencil
-    at h = 1e-3, and its Richardson extrapolation from h and h/2), and against
-    the FP64 host adjoint on one 64^3 instance.  Over 200 iterations the map is
-    nonlinear enough that the h = 1e-3 difference carries ~1e-2 truncation
-    error even in exact arithmetic; the extrapolation removes most of it."""
-    from aworkefficientgpualgorithmforlevelsetsegmentation_amd import synth
+    """The differentiable form on the 16-volume 96^3 float workload.  Timed:
+    the plain forward (lsseg_iterate) against forward + loss + backward through
+    the torch.autograd.Function (adjoint.segment, soft-Dice loss on GPU
+    tensors, loss.backward()), volume after volume.  Checked: the gradients
+    against central finite differences of the recorded map on every volume
+    (five-point stencil at h = 1e-3, and its Richardson extrapolation from h
+    and h/2), and against the FP64 host adjoint on one 64^3 instance.  Over
+    200 iterations the map is nonlinear enough that the h = 1e-3 difference
+    carries ~1e-2 truncation error even in exact arithmetic; the extrapolation
+    removes most of it."""
+    import torch
+    from aworkefficientgpualgorithmforlevelsetsegmentation_amd import adjoint, synth
     from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import Context
     ws = synth.adjoint(iterations=iterations) if iterations else synth.adjoint()
     K = ws[0].iterations
     t_plain = t_adj = 0.0
     upd = 0
-    worst = worst_dir = worst_r = worst_dir_r = 0.0
+    worst = worst_dir = worst_r = worst_dir_r = worst_torch = 0.0
     for i, w in enumerate(ws):
         T, eps, alpha = w.T, w.eps, w.alpha
         ctx = Context(w.volume)
@@ -307,14 +314,29 @@ def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
         device_sync()
         t_plain += time.perf_counter() - t0
         upd += int(st.sum_active)
-        ctx.checkpoint_restore(0)
-        device_sync()
+        # the torch path, timed: phi0 and the mask already on the GPU
+        phi0_t = torch.from_numpy(np.ascontiguousarray(w.phi0)).cuda().requires_grad_()
+        mask_t = torch.from_numpy(np.ascontiguousarray(w.truth.astype(np.float32))).cuda()
+        theta_t = torch.tensor([T, eps, alpha], dtype=torch.float64, requires_grad=True)
+        if i == 0:  # torch's own first-call set-up stays out of the timing, as the plain warm-up does
+            adjoint.soft_dice_loss(adjoint.segment(ctx, phi0_t, theta_t, dt=w.dt, iters=K, band=w.band,
+                                                   checkpoint_every=every), mask_t).backward()
+            phi0_t.grad = None
+            theta_t.grad = None
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
+        phi_t = adjoint.segment(ctx, phi0_t, theta_t, dt=w.dt, iters=K, band=w.band, checkpoint_every=every)
+        adjoint.soft_dice_loss(phi_t, mask_t).backward()
+        torch.cuda.synchronize()
+        t_adj += time.perf_counter() - t0
+        del phi_t
+        # the same gradients through the host path, for the checks
+        ctx.checkpoint_restore(0)
         rec = ctx.fwd_record(T, eps, alpha, w.dt, K, checkpoint_every=every)
         loss, dphi = _soft_dice_grad(ctx.phi(0), w.truth)
         g0, th = rec.bwd(dphi.astype(np.float32))
-        device_sync()
-        t_adj += time.perf_counter() - t0
+        torch_vs_host = max(abs(float(theta_t.grad[k]) - th[k]) / max(abs(th[k]), 1e-12) for k in range(3))
+        worst_torch = max(worst_torch, torch_vs_host)
 
         def L(p, t, e, a):
             return _soft_dice_grad(rec.replay(p, t, e, a, w.dt), w.truth)[0]
@@ -369,6 +391,8 @@ def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
         "workload": f"{len(ws)} float32 volumes 96^3 (ellipsoids / 255), K={K} iterations, dt=0.5, T=0.6 eps=0.15 "
                     f"alpha=0.1, phi checkpointed every {every}; loss 1 - softDice(sigmoid(-phi/0.5), mask)",
         "plain_forward_ms": 1e3 * t_plain, "forward_record_plus_adjoint_ms": 1e3 * t_adj,
+        "timed_path": "adjoint.segment (torch.autograd.Function) + soft-Dice loss + backward, GPU tensors",
+        "torch_vs_host_max_rel_err_theta": worst_torch,
         "adjoint_over_plain": t_adj / t_plain if t_plain > 0 else None,
         "updates": upd, "value": 2 * upd / t_adj if t_adj > 0 else 0.0,
         "unit": "active-voxel updates/s (forward + adjoint, each counted once)",
@@ -378,8 +402,8 @@ def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
               "all volumes; phi0 along the normalised gradient",
         "fp64_64cubed_max_rel_err_theta": fp64_theta, "fp64_64cubed_rel_l2_err_phi0": fp64_phi0,
     }
-    log(f"adjoint: plain forward {1e3 * t_plain:.1f} ms, forward record + adjoint {1e3 * t_adj:.1f} ms "
-        f"({out['adjoint_over_plain']:.2f}x); FD (h=1e-3) max rel err theta {worst:.2e}, phi0 dir {worst_dir:.2e}; "
+    log(f"adjoint: plain forward {1e3 * t_plain:.1f} ms, forward record + loss + adjoint (torch) {1e3 * t_adj:.1f} ms "
+        f"({out['adjoint_over_plain']:.2f}x; torch vs host theta {worst_torch:.1e}); FD (h=1e-3) max rel err theta {worst:.2e}, phi0 dir {worst_dir:.2e}; "
         f"extrapolated: theta {worst_r:.2e}, phi0 dir {worst_dir_r:.2e}; "
         f"FP64 64^3: theta {fp64_theta:.2e}, phi0 {fp64_phi0:.2e}")
     return out
@@ -408,6 +432,10 @@ def main():
                     help="only summarize rocprofv3 --pmc runs under DIR (fetch/, write/, copy_fetch/, copy_write/, "
                          "copy_fetch.json) into profiles/hbm_traffic_<workload>.json")
     args = ap.parse_args()
+    if not (args.no_adjoint or args.summarize_pmc or args.calibrate_copy):
+        # the differentiable form runs through torch, whose HIP runtime must be
+        # the process's: imported before the engine library is loaded
+        import torch  # noqa: F401
 
     if args.summarize_pmc:
         out = summarize_pmc(args.summarize_pmc, args.workload)
