@@ -614,9 +614,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     c->a.field = (type*)p;
     ALLOC(phi[0], uint8_t, N * c->P);
     ALLOC(phi[1], uint8_t, N * c->P);
-    if (c->phi8) {
-        ALLOC(flag, uint8_t, N);
-    }
+    ALLOC(moved, uint8_t, N);
     ALLOC(E, int8_t, N);
     ALLOC(frozen, int8_t, N);
     ALLOC(C, uint64_t, N);
@@ -665,7 +663,6 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     fill_phi_one(c, c->a.phi[1]);
     hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, (uint8_t*)c->a.E, N, (uint8_t)127);
     hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, (uint8_t*)c->a.frozen, N, (uint8_t)127);
-    if (c->phi8) hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, c->a.flag, N, (uint8_t)0);
     fill_u64(c, c->a.C, N, 0);
     fill_u64(c, c->a.tstate, ntiles, 0);
     fill_u64(c, c->a.cpart, kCountSlots * kCountStride, 0);

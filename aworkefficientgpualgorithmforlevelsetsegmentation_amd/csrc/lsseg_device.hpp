@@ -201,7 +201,7 @@ struct KArgs {
     Geom g;
     uint32_t run_log2;    // log2 of the list positions per XCD run (0: plain grid stride)
     void* phi[2];         // float, or int8_t in the fixed-point storage mode
-    uint8_t* flag;        // fixed-point storage mode: temporal-derivative flag per voxel
+    uint8_t* moved;       // temporal-derivative flag per traversal position, written by the update
     const void* src;
     int8_t* E;            // export byte (tiled)
     int8_t* frozen;       // frozen export (tiled)
@@ -536,9 +536,10 @@ struct ListWalk {
 // tile; the single-volume kernels and the batched kernels (which take each
 // volume's arguments from a table) share them.
 // ---------------------------------------------------------------------------
+// the update of active voxel u at traversal position p
 template <typename SrcT, bool kFloatSrc, typename PhiT>
 __device__ inline void update_at(const KArgs& a, const DevParams& P, const PhiT* __restrict__ R,
-                                 PhiT* __restrict__ W, uint32_t u)
+                                 PhiT* __restrict__ W, uint32_t u, uint64_t p)
 {
     uint32_t nb[7], inb;
     neighbors(a.g, u, nb, &inb);
@@ -549,15 +550,17 @@ __device__ inline void update_at(const KArgs& a, const DevParams& P, const PhiT*
     phi_store(W, u, nv);
     a.E[u] = export_byte(nv);
     a.C[u] = 0ull;  // the record of the candidates that produced this list
-    // fixed-point storage: the flag of the unquantised value (UpdateLevelSetVolume.cu:327-331)
-    if constexpr (sizeof(PhiT) == 1) a.flag[u] = fabsf(nv - u4) > kTemporalThreshold ? 1 : 0;
+    // the temporal flag |phi_W - phi_R| > 0.008 of the value as computed (before
+    // a fixed-point store quantises it; UpdateLevelSetVolume.cu:327-331), in
+    // traversal order for the marking
+    a.moved[p] = fabsf(nv - u4) > kTemporalThreshold ? 1 : 0;
 }
 
-// marking of active voxel u (UpdateActiveElements.cu:75-103); returns its
-// emission mask
+// marking of active voxel u (UpdateActiveElements.cu:75-103) whose update set
+// the temporal flag `moved`; returns its emission mask
 template <typename PhiT>
-__device__ inline uint32_t mark_at(const KArgs& a, bool ref, const PhiT* __restrict__ R,
-                                   const PhiT* __restrict__ W, uint8_t* __restrict__ Cb, uint32_t u)
+__device__ inline uint32_t mark_at(const KArgs& a, bool ref, bool moved, const PhiT* __restrict__ W,
+                                   uint8_t* __restrict__ Cb, uint32_t u)
 {
     uint32_t m = 0;
     // the position and every operand are loaded together (all addresses are
@@ -569,11 +572,6 @@ __device__ inline uint32_t mark_at(const KArgs& a, bool ref, const PhiT* __restr
     float wn[7];
 #pragma unroll
     for (int e = 0; e < 7; ++e) wn[e] = phi_load(W, nb[e]);
-    // temporal test: the time derivative of the update (UpdateLevelSetVolume.cu:327-331);
-    // FP32 storage recovers it exactly from the two buffers, fixed point stores it
-    bool moved;
-    if constexpr (sizeof(PhiT) == 1) moved = a.flag[u] != 0;
-    else moved = fabsf(nv - phi_load(R, u)) > kTemporalThreshold;
     if (!ref || (pos & 3u) != 3u) {  // list positions p % 4 == 3: UpdateActiveElementsVolume.cu:329
         if (moved) {
 #pragma unroll
@@ -1068,8 +1066,8 @@ template <typename SrcT, bool kFloatSrc, typename PhiT>
 __device__ inline void update_walk(const KArgs& a, int q, uint32_t first = 0, uint32_t count = 0)
 {
     const DevParams P = *a.prm;
-    walk_list(a, q, first, count, [&](uint32_t u, uint64_t) {
-        update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), u);
+    walk_list(a, q, first, count, [&](uint32_t u, uint64_t p) {
+        update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), u, p);
     });
 }
 
@@ -1129,7 +1127,7 @@ __global__ __launch_bounds__(kBlock) void k_mark(KArgs a, int q)
     const bool ref = a.prm->refMode != 0;
     uint32_t emitted = 0;
     walk_list(a, q, 0, 0, [&](uint32_t u, uint64_t p) {
-        const uint32_t m = mark_at<PhiT>(a, ref, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, u);
+        const uint32_t m = mark_at<PhiT>(a, ref, a.moved[p] != 0, phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, u);
         a.emit[p] = (uint8_t)m;
         emitted += (uint32_t)__popc(m);
     });
@@ -1219,7 +1217,7 @@ __device__ inline void update_batch_items(const BatchArgs& b, int q, uint32_t sl
         const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
         if (p < b.ctls[v].n[q]) {
             const DevParams P = *a.prm;
-            update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), a.S[q][p]);
+            update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), a.S[q][p], p);
         }
     }
 }
@@ -1246,7 +1244,7 @@ __global__ __launch_bounds__(kBlock) void k_mark_batch(BatchArgs b, int q)
         const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
         uint32_t emitted = 0;
         if (p < b.ctls[v].n[q]) {
-            const uint32_t m = mark_at<PhiT>(a, a.prm->refMode != 0, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1),
+            const uint32_t m = mark_at<PhiT>(a, a.prm->refMode != 0, a.moved[p] != 0, phi_buf<PhiT>(a, q ^ 1),
                                              (uint8_t*)a.C, a.S[q][p]);
             a.emit[p] = (uint8_t)m;
             emitted = (uint32_t)__popc(m);
