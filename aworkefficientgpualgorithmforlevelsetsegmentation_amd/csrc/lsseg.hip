@@ -75,6 +75,8 @@ struct lsseg_ctx {
     size_t prof_next = 0;
     std::vector<void*> allocs;
     std::vector<lsseg_record*> records;  // live records of the differentiable form
+    uint64_t record_total = 0;        // sum_k n_k of the last record, the next record's first guess
+    int32_t record_iters = 0;
     std::string err;
 };
 
@@ -354,10 +356,9 @@ lsseg_status sync_and_check(lsseg_ctx* c)
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (c->h_ctl->error)
         return fail(c, LSSEG_ERR_DEVICE, "device error flag 0x%x (%s)", c->h_ctl->error,
-                    (c->h_ctl->error & kErrSpin)     ? "look-back wait exceeded its bound"
-                    : (c->h_ctl->error & kErrRecord) ? "a recording run diverged from its first pass"
-                    : (c->h_ctl->error & kErrCount)  ? "the compaction's count differs from the keep step's"
-                                                     : "list overflow");
+                    (c->h_ctl->error & kErrSpin)    ? "look-back wait exceeded its bound"
+                    : (c->h_ctl->error & kErrCount) ? "the compaction's count differs from the keep step's"
+                                                    : "list overflow");
     return LSSEG_OK;
 }
 
@@ -1587,6 +1588,9 @@ struct lsseg_record {
     std::vector<uint32_t> n;          // n_k
     std::vector<uint64_t> off;        // offsets of V_k in lists / dec
     uint64_t total = 0;
+    uint64_t* offs = nullptr;         // the same offsets, written by the recording run
+    uint64_t cap = 0;                 // entries that lists / dec hold
+    void* lists_block = nullptr;      // lists and dec
     uint32_t* lists = nullptr;
     uint8_t* dec = nullptr;
     std::vector<float*> ckA, ckB;     // (read, write) buffers at k = 0, every, 2 every, ...
@@ -1666,6 +1670,85 @@ lsseg_status check_record(lsseg_ctx* c, const lsseg_record* r)
     return LSSEG_OK;
 }
 
+// offsets of the record's buffers in its one allocation, 256-byte aligned
+struct RecordLayout {
+    size_t end = 0;
+    size_t take(size_t bytes)
+    {
+        const size_t o = end;
+        end += (bytes + 255) / 256 * 256;
+        return o;
+    }
+};
+
+// bound of the first guess of sum_k n_k without an earlier record (5 bytes each)
+constexpr uint64_t kRecordFirstGuess = 1ull << 27;
+
+// lists and decisions of `cap` entries, replacing any earlier pair
+lsseg_status record_lists_alloc(lsseg_ctx* c, lsseg_record* r, uint64_t cap)
+{
+    if (r->lists_block) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(r->lists_block);
+        r->allocs.erase(std::remove(r->allocs.begin(), r->allocs.end(), r->lists_block), r->allocs.end());
+        r->lists_block = nullptr;
+    }
+    const size_t lb = (size_t)(cap * sizeof(uint32_t) + 255) / 256 * 256;
+    void* p = nullptr;
+    lsseg_status st = ralloc(c, r, &p, lb + cap);
+    if (st != LSSEG_OK) return st;
+    r->lists_block = p;
+    r->lists = (uint32_t*)p;
+    r->dec = (uint8_t*)p + lb;
+    r->cap = cap;
+    return LSSEG_OK;
+}
+
+// One recording run of `iters` iterations from the context's state: the
+// launches of lsseg_iterate (each compaction in the next update's launch),
+// phi checkpointed every `every` iterations, and the recording of iteration
+// k after its update launch: it reads list[q], which that launch completes,
+// and the read buffer, which the update leaves alone.  V_k and its decisions
+// go to the device offsets r->offs.
+lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
+{
+    const uint64_t N = c->a.g.N;
+    HIP_TRY(c, hipMemsetAsync(r->offs, 0, sizeof(uint64_t), c->stream));
+    // reset the per-call counters (keep n[], epoch)
+    HIP_TRY(c, hipMemcpyAsync(c->h_ctl, c->a.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->h_ctl->iter = 0;
+    c->h_ctl->stats_cap = c->stats_cap;
+    c->h_ctl->error = 0;
+    c->h_ctl->few_iters = 0;
+    c->h_ctl->many_iters = 0;
+    HIP_TRY(c, hipMemcpyAsync(c->a.ctl, c->h_ctl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
+    int q = c->q;
+    const uint32_t grid = std::min<uint32_t>(c->sgrid, list_grid(N));
+    for (int k = 0; k < iters; ++k) {
+        if (k % r->every == 0) {
+            HIP_TRY(c, hipMemcpyAsync(r->ckA[k / r->every], c->a.phi[q], N * sizeof(float), hipMemcpyDeviceToDevice,
+                                      c->stream));
+            HIP_TRY(c, hipMemcpyAsync(r->ckB[k / r->every], c->a.phi[q ^ 1], N * sizeof(float),
+                                      hipMemcpyDeviceToDevice, c->stream));
+        }
+        launch_head(c, q, k > 0);
+        uint32_t kk = (uint32_t)k;
+        void* args[] = {(void*)&c->a, (void*)&q, (void*)&kk, (void*)&r->offs, (void*)&r->cap,
+                        (void*)&r->lists, (void*)&r->dec};
+        (void)hipLaunchKernel(k_record_kernel(c->dtype), dim3(grid), dim3(kBlock), args, 0, c->stream);
+        launch_tail(c, q);
+        q ^= 1;
+    }
+    if (iters > 0) launch(c, (const void*)&k_compact<kCompact>, c->grid, q ^ 1);
+    HIP_TRY(c, hipGetLastError());
+    lsseg_status st = sync_and_check(c);
+    if (st != LSSEG_OK) return st;
+    c->q = q;
+    c->last_iters = (uint64_t)iters;
+    return LSSEG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1694,10 +1777,10 @@ lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t ite
     lsseg_status st = make_params(c, prm, &P);
     if (st != LSSEG_OK) return st;
     const uint64_t N = c->a.g.N;
-    // pass 1: the plain run gives n_k, hence the record's size
+    if ((st = ensure_stats(c, (uint64_t)iters)) != LSSEG_OK) return st;
+    if ((st = upload_params(c, P)) != LSSEG_OK) return st;
+    // the start state, for a second run if the first guess of the size was short
     if ((st = lsseg_checkpoint_save(c, kRecordSlot)) != LSSEG_OK) return st;
-    lsseg_stats s1;
-    if ((st = lsseg_iterate(c, prm, iters, &s1)) != LSSEG_OK) return st;
     lsseg_record* r = new (std::nothrow) lsseg_record();
     if (!r) return fail(c, LSSEG_ERR_OUT_OF_MEMORY, "host allocation failed");
     auto bail = [&](lsseg_status e) {
@@ -1708,79 +1791,59 @@ lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t ite
     r->iters = iters;
     r->every = every;
     r->params = *prm;
-    r->n.resize(iters);
-    HIP_TRY(c, hipMemcpy(r->n.data(), c->a.n_hist, iters * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    r->off.resize(iters + 1);
-    r->off[0] = 0;
-    for (int k = 0; k < iters; ++k) r->off[k + 1] = r->off[k] + r->n[k];
-    r->total = r->off[iters];
-    void* p = nullptr;
+    // everything but the lists in one allocation
     const int nck = (iters + every - 1) / every;
     const int seg = std::min(every, iters);
-    if ((st = ralloc(c, r, &p, sizeof(DevParams))) != LSSEG_OK) return bail(st);
-    r->d_prm = (DevParams*)p;
-    if ((st = ralloc(c, r, &p, sizeof(DevParams))) != LSSEG_OK) return bail(st);
-    r->d_prm_replay = (DevParams*)p;
-    if ((st = ralloc(c, r, &p, r->total * sizeof(uint32_t))) != LSSEG_OK) return bail(st);
-    r->lists = (uint32_t*)p;
-    if ((st = ralloc(c, r, &p, r->total)) != LSSEG_OK) return bail(st);
-    r->dec = (uint8_t*)p;
+    const size_t NF = N * sizeof(float);
+    RecordLayout lay;
+    const size_t o_prm = lay.take(sizeof(DevParams)), o_prm_replay = lay.take(sizeof(DevParams));
+    const size_t o_theta = lay.take(kCountSlots * kCountStride * sizeof(double));
+    const size_t o_offs = lay.take(((size_t)iters + 1) * sizeof(uint64_t));
+    const size_t o_ck = lay.take(2 * (size_t)nck * NF), o_seg = lay.take((size_t)seg * NF);
+    const size_t o_work = lay.take(4 * NF), o_lin = lay.take((size_t)c->nx * c->ny * c->nz * sizeof(float));
+    void* p = nullptr;
+    if ((st = ralloc(c, r, &p, lay.end)) != LSSEG_OK) return bail(st);
+    char* base = static_cast<char*>(p);
+    r->d_prm = (DevParams*)(base + o_prm);
+    r->d_prm_replay = (DevParams*)(base + o_prm_replay);
+    r->d_theta = (double*)(base + o_theta);  // per-workgroup slots of dL/dT, dL/deps, dL/dalpha
+    r->offs = (uint64_t*)(base + o_offs);
     for (int i = 0; i < nck; ++i) {
-        if ((st = ralloc(c, r, &p, N * sizeof(float))) != LSSEG_OK) return bail(st);
-        r->ckA.push_back((float*)p);
-        if ((st = ralloc(c, r, &p, N * sizeof(float))) != LSSEG_OK) return bail(st);
-        r->ckB.push_back((float*)p);
+        r->ckA.push_back((float*)(base + o_ck + (size_t)(2 * i) * NF));
+        r->ckB.push_back((float*)(base + o_ck + (size_t)(2 * i + 1) * NF));
     }
-    if ((st = ralloc(c, r, &p, (size_t)seg * N * sizeof(float))) != LSSEG_OK) return bail(st);
-    r->seg = (float*)p;
+    r->seg = (float*)(base + o_seg);
     for (int i = 0; i < 2; ++i) {
-        if ((st = ralloc(c, r, &p, N * sizeof(float))) != LSSEG_OK) return bail(st);
-        r->work[i] = (float*)p;
-        if ((st = ralloc(c, r, &p, N * sizeof(float))) != LSSEG_OK) return bail(st);
-        r->adj[i] = (float*)p;
+        r->work[i] = (float*)(base + o_work + (size_t)i * NF);
+        r->adj[i] = (float*)(base + o_work + (size_t)(2 + i) * NF);
     }
-    if ((st = ralloc(c, r, &p, (size_t)c->nx * c->ny * c->nz * sizeof(float))) != LSSEG_OK) return bail(st);
-    r->lin = (float*)p;
-    if ((st = ralloc(c, r, &p, kCountSlots * kCountStride * sizeof(double))) != LSSEG_OK) return bail(st);
-    r->d_theta = (double*)p;  // per-workgroup slots of dL/dT, dL/deps, dL/dalpha
+    r->lin = (float*)(base + o_lin);
     HIP_TRY(c, hipMemcpyAsync(r->d_prm, &P, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
-    // pass 2 from the same start: record lists and decisions, checkpoint phi
-    if ((st = lsseg_checkpoint_restore(c, kRecordSlot)) != LSSEG_OK) return bail(st);
-    if ((st = upload_params(c, P)) != LSSEG_OK) return bail(st);
-    HIP_TRY(c, hipMemcpyAsync(c->h_ctl, c->a.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    c->h_ctl->iter = 0;
-    c->h_ctl->stats_cap = c->stats_cap;
-    c->h_ctl->error = 0;
-    c->h_ctl->few_iters = 0;
-    c->h_ctl->many_iters = 0;
-    HIP_TRY(c, hipMemcpyAsync(c->a.ctl, c->h_ctl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
-    // the launches of lsseg_iterate (each compaction in the next update's
-    // launch), with the recording of iteration k after its update launch: it
-    // reads list[q], which that launch completes, and the read buffer, which
-    // the update leaves alone
-    int q = c->q;
-    for (int k = 0; k < iters; ++k) {
-        if (k % every == 0) {
-            HIP_TRY(c, hipMemcpyAsync(r->ckA[k / every], c->a.phi[q], N * sizeof(float), hipMemcpyDeviceToDevice,
-                                      c->stream));
-            HIP_TRY(c, hipMemcpyAsync(r->ckB[k / every], c->a.phi[q ^ 1], N * sizeof(float),
-                                      hipMemcpyDeviceToDevice, c->stream));
-        }
-        launch_head(c, q, k > 0);
-        uint32_t* L = r->lists + r->off[k];
-        uint8_t* D = r->dec + r->off[k];
-        uint32_t n = r->n[k];
-        void* args[] = {(void*)&c->a, (void*)&q, (void*)&n, (void*)&L, (void*)&D};
-        (void)hipLaunchKernel(k_record_kernel(c->dtype), dim3(list_grid(n)), dim3(kBlock), args, 0, c->stream);
-        launch_tail(c, q);
-        q ^= 1;
+    // first guess of sum_k n_k: the last record of this context, scaled to
+    // `iters`, else an eighth of the volume per iteration (bounded)
+    uint64_t cap = c->record_iters > 0
+                       ? c->record_total * (uint64_t)iters / (uint64_t)c->record_iters * 5 / 4 + 4096
+                       : std::min<uint64_t>((uint64_t)iters * (N / 8 + 1), kRecordFirstGuess);
+    const int force = env_int("LSSEG_RECORD_CAPACITY", 0);
+    if (force > 0) cap = (uint64_t)force;
+    if ((st = record_lists_alloc(c, r, cap)) != LSSEG_OK) return bail(st);
+    if ((st = record_run(c, r, iters)) != LSSEG_OK) return bail(st);
+    r->off.resize((size_t)iters + 1);
+    HIP_TRY(c, hipMemcpy(r->off.data(), r->offs, r->off.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (r->off[iters] > r->cap) {
+        // short: record again from the start with the exact size
+        if ((st = record_lists_alloc(c, r, r->off[iters])) != LSSEG_OK) return bail(st);
+        if ((st = lsseg_checkpoint_restore(c, kRecordSlot)) != LSSEG_OK) return bail(st);
+        if ((st = record_run(c, r, iters)) != LSSEG_OK) return bail(st);
+        HIP_TRY(c, hipMemcpy(r->off.data(), r->offs, r->off.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        if (r->off[iters] > r->cap)
+            return bail(fail(c, LSSEG_ERR_DEVICE, "the second recording run differs from the first"));
     }
-    if (iters > 0) launch(c, (const void*)&k_compact<kCompact>, c->grid, q ^ 1);
-    HIP_TRY(c, hipGetLastError());
-    if ((st = sync_and_check(c)) != LSSEG_OK) return bail(st);
-    c->q = q;
-    c->last_iters = (uint64_t)iters;
+    r->n.resize(iters);
+    for (int k = 0; k < iters; ++k) r->n[k] = (uint32_t)(r->off[k + 1] - r->off[k]);
+    r->total = r->off[iters];
+    c->record_total = r->total;
+    c->record_iters = iters;
     c->records.push_back(r);
     *out = r;
     return LSSEG_OK;

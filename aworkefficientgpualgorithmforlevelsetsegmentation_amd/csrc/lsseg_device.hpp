@@ -57,7 +57,7 @@ constexpr int kKeepItems = 2;                 // S positions per thread in the k
 constexpr int kKeepTile = kBlock * kKeepItems;
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded look-back wait
 
-enum : uint32_t { kErrSpin = 1u, kErrOverflow = 2u, kErrRecord = 4u, kErrCount = 8u };
+enum : uint32_t { kErrSpin = 1u, kErrOverflow = 2u, kErrCount = 8u };
 
 enum KMode : int {
     kUpdate = 0,   // update of the current list
@@ -1363,14 +1363,24 @@ __global__ __launch_bounds__(kBlock) void k_update_fused_batch(BatchArgs b, int 
 // k_adjoint reads and clears X = Abar' on V_k (leaving Bbar) and scatters the
 // J_v^T terms into Y = Bbar' (leaving Abar).
 // ---------------------------------------------------------------------------
+// Iteration k of a recording run appends V_k at offs[k] and sets offs[k + 1]
+// (offs[0] = 0): the sizes come from the device counts, so one run records.
+// Entries beyond `cap` are not written; the host then sizes the record from
+// offs and records again.
 template <typename SrcT, bool kFloatSrc>
-__global__ __launch_bounds__(kBlock) void k_record(KArgs a, int q, uint32_t n, uint32_t* __restrict__ lists,
+__global__ __launch_bounds__(kBlock) void k_record(KArgs a, int q, uint32_t k, uint64_t* __restrict__ offs,
+                                                   uint64_t cap, uint32_t* __restrict__ lists,
                                                    uint8_t* __restrict__ dec)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0 && a.ctl->n[q] != n) atomicOr(&a.ctl->error, kErrRecord);
+    const uint32_t n = a.ctl->n[q];
+    const uint64_t base = offs[k];
+    if (blockIdx.x == 0 && threadIdx.x == 0) offs[k + 1] = base + n;
+    if (base + n > cap) return;
     const DevParams P = *a.prm;
     const uint32_t* __restrict__ V = a.list[q];
     const float* __restrict__ R = phi_buf<float>(a, q);  // FP32 storage only
+    lists += base;
+    dec += base;
     LSSEG_FOR_EACH(p, n)
     {
         const uint32_t u = V[p];

@@ -76,6 +76,36 @@ def test_record_and_replay_match_oracle_and_forward():
     plain.close()
 
 
+@pytest.mark.parametrize("capacity", ["1", ""], ids=["short-first-guess", "guess-from-last-record"])
+def test_record_sizes_itself(monkeypatch, capacity):
+    """One recording run sizes the record from device offsets.  With a first
+    guess that is too short (forced to one entry) the record runs again from
+    the start with the exact size; without it, the second record of a context
+    starts from the first one's size.  Either way the lists and decisions are
+    the oracle's and the context ends in the plain run's state."""
+    vol, _, phi0, (T, eps, alpha), K = _case(edge=(30, 26, 22), seed=777, iters=30)
+    if capacity:
+        monkeypatch.setenv("LSSEG_RECORD_CAPACITY", capacity)
+    _say(f"record sizing {vol.shape} K={K} capacity={capacity or 'auto'}")
+    on, olists, odec, _ = O.record_f32(vol, phi0, 1.0, T, eps, alpha, DT, K)
+    plain = Context(vol)
+    plain.set_phi(phi0, 1.0)
+    plain.iterate(T, eps, alpha, DT, K)
+    ctx = Context(vol)
+    for rep in range(2):
+        ctx.set_phi(phi0, 1.0)
+        rec = ctx.fwd_record(T, eps, alpha, DT, K, checkpoint_every=4)
+        n, lists, dec = rec.lists()
+        assert np.array_equal(n, on) and np.array_equal(lists, olists) and np.array_equal(dec, odec), rep
+        for a, b in zip(ctx.state(), plain.state()):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), rep
+        out = rec.replay(phi0, T, eps, alpha, DT)
+        assert np.array_equal(out.view(np.uint32), ctx.phi(0).view(np.uint32)), rep
+        rec.close()
+    ctx.close()
+    plain.close()
+
+
 def test_adjoint_matches_fp64_oracle_and_finite_differences():
     vol, gt, phi0, (T, eps, alpha), K = _case(edge=(40, 36, 30), iters=40)
     nz, ny, nx = vol.shape
