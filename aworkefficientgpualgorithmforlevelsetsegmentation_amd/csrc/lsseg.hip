@@ -607,7 +607,9 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     g.dTXY = make_fastdiv(g.TX * g.TY);
 
     void* p = nullptr;
-    const uint64_t ntiles = N / std::min(kScanTile, kKeepTile) + 2;  // look-back states of both scans
+    // look-back states: one per keep or dense-compaction tile, seven per
+    // few-path compaction tile
+    const uint64_t ntiles = std::max<uint64_t>(N / std::min(kScanTile, kKeepTile) + 2, 7 * (N / kPosTile + 2));
     lsseg_status st;
 #define ALLOC(field, type, count)                                          \
     if ((st = dalloc(c, &p, (size_t)(count) * sizeof(type))) != LSSEG_OK) \
@@ -629,6 +631,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     ALLOC(memb, uint8_t, N);
     ALLOC(tstate, uint64_t, ntiles);
     ALLOC(cpart, uint64_t, kCountSlots * kCountStride);
+    ALLOC(dirc, uint32_t, kDirCounters);
     ALLOC(ctl, DevCtl, 1);
 #undef ALLOC
     if ((st = dalloc(c, &p, (size_t)N * c->S)) != LSSEG_OK) return bail(st);
@@ -667,6 +670,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     fill_u64(c, c->a.C, N, 0);
     fill_u64(c, c->a.tstate, ntiles, 0);
     fill_u64(c, c->a.cpart, kCountSlots * kCountStride, 0);
+    fill_u64(c, (uint64_t*)c->a.dirc, kDirCounters / 2, 0);
     // positions, masks and keep bytes of voxels outside the list are never read;
     // zero keeps every entry a valid index
     hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, (uint8_t*)c->a.pos, N * sizeof(uint32_t),

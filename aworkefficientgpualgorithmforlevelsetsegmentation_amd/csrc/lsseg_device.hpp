@@ -55,6 +55,9 @@ constexpr int kItems = 8;                     // compaction slots per thread
 constexpr int kScanTile = kBlock * kItems;    // slots per look-back tile
 constexpr int kKeepItems = 2;                 // S positions per thread in the keep step
 constexpr int kKeepTile = kBlock * kKeepItems;
+constexpr int kPosItems = 4;                  // list positions per thread in a few-path compaction
+constexpr int kPosTile = kBlock * kPosItems;
+constexpr uint32_t kSpecTiles = 64;           // workgroups that load their compaction tile with the counts
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded look-back wait
 
 enum : uint32_t { kErrSpin = 1u, kErrOverflow = 2u, kErrCount = 8u };
@@ -196,6 +199,7 @@ struct DevParams {
 // per launch.  The keep step sums and clears them.
 constexpr int kCountSlots = 64;
 constexpr int kCountStride = 16;
+constexpr size_t kDirCounters = 2 * 7 * kCountSlots * kCountStride;  // words of KArgs::dirc
 
 struct KArgs {
     Geom g;
@@ -214,6 +218,8 @@ struct KArgs {
     uint8_t* memb;        // members of the next list, for a dense compaction (zero at rest)
     uint64_t* tstate;     // look-back tile states
     uint64_t* cpart;      // candidate-emission counters (kCountSlots, zero between iterations)
+    uint32_t* dirc;       // kept candidates per direction of list 0 and list 1 (7 x kCountSlots
+                          // counters each, written by the keep step, cleared by the compaction)
     DevCtl* ctl;
     const DevParams* prm;
     uint32_t* n_hist;
@@ -226,21 +232,38 @@ struct KArgs {
 // ---------------------------------------------------------------------------
 __device__ inline uint64_t lanemask_lt(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
 
-__device__ inline uint32_t wave_sum(uint32_t v)
+// Scans through DPP lane moves, which stay in the vector ALU (a shuffle goes
+// through the LDS crossbar, ~10x the latency per step).  Every lane of the
+// wave must be active.
+//
+// inclusive prefix within each row of 16 lanes: partial sums of 4 from the
+// original value (row_shr 1, 2, 3), then of 8 and 16 (row_shr 4 on banks 1-3,
+// row_shr 8 on banks 2-3)
+__device__ inline uint32_t row_inclusive_scan(uint32_t v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    const int x = (int)v;
+    int s = x + __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);
+    s += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);
+    s += __builtin_amdgcn_update_dpp(0, x, 0x113, 0xf, 0xf, true);
+    s += __builtin_amdgcn_update_dpp(0, s, 0x114, 0xf, 0xe, false);
+    s += __builtin_amdgcn_update_dpp(0, s, 0x118, 0xf, 0xc, false);
+    return (uint32_t)s;
 }
 
-__device__ inline uint32_t wave_inclusive_scan(uint32_t v, int lane)
+// inclusive prefix over the wave: the row scans, then row 0's total into
+// rows 1 and 3 (row_bcast 15) and rows 0-1's into rows 2 and 3 (row_bcast 31)
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v, int /*lane*/ = 0)
 {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t t = __shfl_up(v, d, 64);
-        if (lane >= d) v += t;
-    }
-    return v;
+    int s = (int)row_inclusive_scan(v);
+    s += __builtin_amdgcn_update_dpp(0, s, 0x142, 0xa, 0xf, false);
+    s += __builtin_amdgcn_update_dpp(0, s, 0x143, 0xc, 0xf, false);
+    return (uint32_t)s;
+}
+
+// the wave's total, in every lane
+__device__ inline uint32_t wave_sum(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_inclusive_scan(v), 63);
 }
 
 __device__ inline uint64_t ld_state(const uint64_t* p)
@@ -334,6 +357,78 @@ __device__ inline uint32_t lookback(uint64_t* states, uint32_t tile, uint32_t to
     }
     __syncthreads();
     return *s_excl;
+}
+
+// Seven look-back chains at once, one per candidate direction e, over the same
+// tiles (state tile * 7 + e), run by one wave: lane e < 7 publishes tile
+// `tile`'s aggregate tot[e] first and its inclusive prefix last; each round
+// loads the seven states of the 64 preceding tiles (lane l: tile look - l) and
+// walks every unresolved chain back to its nearest inclusive prefix.  The
+// exclusive prefixes go to pre[0..6].
+__device__ void wave_lookback7(uint64_t* states, uint32_t tile, const uint32_t* tot, uint32_t epoch, uint32_t* err,
+                               uint32_t* pre)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t ep = epoch & 0x3FFFFFFFu;
+    const uint64_t stamp = (uint64_t)ep << 34;
+    const uint64_t none = stamp | (2ull << 32);  // before tile 0: inclusive prefixes of 0
+    const uint32_t mine = lane < 7 ? tot[lane] : 0u;
+    uint64_t* own = states + (uint64_t)tile * 7u;
+    if (tile == 0) {
+        if (lane < 7) {
+            st_state(&own[lane], stamp | (2ull << 32) | mine);
+            pre[lane] = 0;
+        }
+        return;
+    }
+    if (lane < 7) st_state(&own[lane], stamp | (1ull << 32) | mine);
+    uint32_t excl[7] = {0, 0, 0, 0, 0, 0, 0};
+    uint32_t done = 0;  // bit e: chain e resolved (uniform)
+    int64_t look = (int64_t)tile - 1;
+    uint32_t spins = 0;
+    while (done != 0x7Fu) {
+        const int64_t idx = look - lane;
+        uint64_t s[7];
+#pragma unroll
+        for (int e = 0; e < 7; ++e)
+            s[e] = (idx >= 0 && !((done >> e) & 1u)) ? ld_state(&states[(uint64_t)idx * 7u + e]) : none;
+        for (;;) {
+            bool pending = false;
+#pragma unroll
+            for (int e = 0; e < 7; ++e) pending |= state_flag(s[e], ep) == 0;
+            if (__ballot(pending) == 0) break;
+            if (++spins > kSpinLimit) {
+                if (lane == 0) atomicOr(err, kErrSpin);
+#pragma unroll
+                for (int e = 0; e < 7; ++e) s[e] = none;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+            for (int e = 0; e < 7; ++e)
+                if (state_flag(s[e], ep) == 0) s[e] = ld_state(&states[(uint64_t)idx * 7u + e]);
+        }
+#pragma unroll
+        for (int e = 0; e < 7; ++e) {
+            if ((done >> e) & 1u) continue;
+            const uint64_t pm = __ballot(state_flag(s[e], ep) == 2u);
+            const uint32_t v = (uint32_t)s[e];
+            if (pm) {
+                const int k = __ffsll((unsigned long long)pm) - 1;
+                excl[e] += wave_sum(lane <= k ? v : 0u);
+                done |= 1u << e;
+            } else {
+                excl[e] += wave_sum(v);
+            }
+        }
+        look -= 64;
+    }
+#pragma unroll
+    for (int e = 0; e < 7; ++e)
+        if (lane == e) {
+            st_state(&own[e], stamp | (2ull << 32) | (uint64_t)(excl[e] + mine));
+            pre[e] = excl[e];
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -604,21 +699,34 @@ __device__ inline void block_add(uint64_t* acc, uint32_t c, uint32_t* s_red)
     __syncthreads();
 }
 
+// per-tile state of a few-path compaction (by list positions)
+struct PosShared {
+    uint32_t cnt[7 * kPosItems * 4];   // kept candidates per direction, item and wave
+    uint32_t excl[7 * kPosItems * 4];  // their exclusive prefix within the tile, per direction
+    uint32_t tot[7];                   // the tile's kept candidates per direction
+    uint32_t chain[7];                 // the chunk's, summed over its tiles
+    uint32_t pre[7];                   // kept candidates of direction e in the chunks before
+    uint32_t run[7];                   // list offset of the next tile's direction-e candidates
+    uint32_t base[8];                  // kept candidates of the directions before e; base[7] = all
+};
+
 struct CompactShared {
     uint32_t stage[kScanTile];
     uint32_t cnt[kItems * 4];
     uint32_t excl[kItems * 4];
     uint32_t prefix, total;
+    PosShared pos;
 };
 
-// one compaction launch over one volume: few path over the 7 n candidate slots
-// in list-major order, dense path over all N voxels in tiled order
+// one compaction launch over one volume: few path over the n list positions
+// (seven candidate slots each, taken in list-major order), dense path over all
+// N voxels in tiled order
 struct CompactPlan {
     uint32_t n_src;
     uint32_t n_dst;   // the next list's count, set by the keep step (loaded early)
     bool dense;
-    uint64_t slots;
-    uint32_t ntiles;  // sub-tiles of kScanTile slots
+    uint64_t slots;   // list positions (few path) or voxels (dense path)
+    uint32_t ntiles;  // sub-tiles of kPosTile positions or kScanTile voxels
 };
 
 template <int kMode>
@@ -628,8 +736,9 @@ __device__ inline CompactPlan compact_plan(const Geom& g, uint32_t n_src)
     c.n_src = n_src;
     c.n_dst = 0;
     c.dense = (kMode == kInit) || !path_is_few(n_src, g.N);
-    c.slots = c.dense ? g.N : 7ull * n_src;
-    c.ntiles = (uint32_t)((c.slots + kScanTile - 1) / kScanTile);
+    c.slots = c.dense ? g.N : (uint64_t)n_src;
+    const uint32_t tile = c.dense ? kScanTile : kPosTile;
+    c.ntiles = (uint32_t)((c.slots + tile - 1) / tile);
     return c;
 }
 
@@ -649,25 +758,22 @@ __device__ inline void compact_finish(const KArgs& a, int q, const CompactPlan& 
 }
 
 // The compaction of list[q] into list[q ^ 1] (the reference's order): few
-// path, the kept candidates of the keep bytes in list-major slot order; dense
-// path, the voxels marked in `memb` in tiled order (the marks are cleared when
-// the voxels are placed); kInit, the voxels with C != 0 in tiled order.  The
-// positions of the new list are written too.  Every thread of the block runs
-// these.
+// path, the kept candidates of the keep bytes in list-major order (by list
+// positions, compact_pos_*); dense path, the voxels marked in `memb` in tiled
+// order (the marks are cleared when the voxels are placed); kInit, the voxels
+// with C != 0 in tiled order.  The positions of the new list are written too.
+// Every thread of the block runs these.
 //
-// compact_load: the keep flags (ballots) of sub-tile `sub` (slot
-// sub * kScanTile + j * kBlock + tid) and per-wave counts into sh.cnt; with
-// kPlace also the target voxels (and the dense path's marks are cleared).
+// compact_load (dense path and kInit): the flags (ballots) of sub-tile `sub`
+// (voxel sub * kScanTile + j * kBlock + tid) and per-wave counts into sh.cnt;
+// with kPlace also the target voxels (and the dense path's marks are cleared).
 template <int kMode, bool kPlace>
 __device__ __attribute__((always_inline)) inline void compact_load(const KArgs& a, int q, const CompactPlan& c,
                                                                    uint32_t sub, uint64_t* bal, uint32_t* val,
                                                                    CompactShared& sh)
 {
-    const uint32_t* __restrict__ Vs = a.list[q];
-    const uint8_t* __restrict__ Ks = a.keep;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const uint32_t n_src = c.n_src;
     const uint64_t base = (uint64_t)sub * kScanTile;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
@@ -678,20 +784,10 @@ __device__ __attribute__((always_inline)) inline void compact_load(const KArgs& 
             if constexpr (kMode == kInit) {
                 keep = a.C[s] != 0ull;
                 v = (uint32_t)s;
-            } else if (c.dense) {
+            } else {
                 keep = a.memb[s] != 0;
                 v = (uint32_t)s;
                 if (kPlace && keep) a.memb[s] = 0;
-            } else {
-                // slot s = e * n_src + p: list-major concatenation of the
-                // seven candidate lists (Cuda.cpp:699-758 over 7 A slots);
-                // the voxel and its keep byte are loaded together
-                const uint32_t e = (uint32_t)(s >= n_src) + (uint32_t)(s >= 2ull * n_src) +
-                                   (uint32_t)(s >= 3ull * n_src) + (uint32_t)(s >= 4ull * n_src) +
-                                   (uint32_t)(s >= 5ull * n_src) + (uint32_t)(s >= 6ull * n_src);
-                const uint32_t p = (uint32_t)(s - (uint64_t)e * n_src);
-                keep = ((Ks[p] >> e) & 1u) != 0;
-                if constexpr (kPlace) v = neighbor_in(a.g, Vs[p], e);
             }
         }
         bal[j] = __ballot(keep);
@@ -744,13 +840,212 @@ __device__ __attribute__((always_inline)) inline void compact_place(const KArgs&
     __syncthreads();
 }
 
-// Look-back chunk `chunk` of the compaction.  Returns false (with the error
-// flag set) if the next list would overflow, which a consistent record never
-// does.
+// ---------------------------------------------------------------------------
+// few-path compaction by list positions.  The next list is the concatenation
+// over directions e of the kept candidates (e, p) in position order
+// (Cuda.cpp:699-758 over the seven candidate lists), so candidate (e, p) goes
+// to base_e + (kept e-candidates at positions before p), base_e being the kept
+// candidates of the directions before e, which the keep step counts.  A tile
+// of kPosTile positions reads each keep byte and list entry once and runs
+// seven look-back chains, one per direction (wave_lookback7).  Its inputs do
+// not depend on the count, so the first workgroups load them with it.
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t* dir_counter(const KArgs& a, int q, int e, uint32_t slot)
+{
+    return a.dirc + ((uint32_t)(q * 7 + e) * kCountSlots + slot) * kCountStride;
+}
+
+// this lane's direction counters of list q (one per direction; run by a wave)
+__device__ inline void dir_load(const KArgs& a, int q, uint32_t* d)
+{
+    static_assert(kCountSlots == 64, "one counter slot per lane");
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (int e = 0; e < 7; ++e) d[e] = *dir_counter(a, q, e, lane);
+}
+
+__device__ inline void dir_clear(const KArgs& a, int q)
+{
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (int e = 0; e < 7; ++e) *dir_counter(a, q, e, lane) = 0;
+}
+
+// keep bytes and list entries of position tile `tile` of list[q]; positions
+// are bounded by N (every one a valid address), and compact_pos_mask drops the
+// ones at or beyond the count
+__device__ __attribute__((always_inline)) inline void compact_pos_load(const KArgs& a, int q, uint32_t tile,
+                                                                       uint32_t* kb, uint32_t* vx)
+{
+    const uint64_t base = (uint64_t)tile * kPosTile;
+#pragma unroll
+    for (int j = 0; j < kPosItems; ++j) {
+        const uint64_t p = base + (uint64_t)j * kBlock + threadIdx.x;
+        const uint64_t pc = p < a.g.N ? p : 0;
+        kb[j] = a.keep[pc];
+        vx[j] = a.list[q][pc];
+    }
+}
+
+__device__ __attribute__((always_inline)) inline void compact_pos_mask(uint32_t n, uint32_t tile, uint32_t* kb)
+{
+    const uint64_t base = (uint64_t)tile * kPosTile;
+#pragma unroll
+    for (int j = 0; j < kPosItems; ++j)
+        if (base + (uint64_t)j * kBlock + threadIdx.x >= n) kb[j] = 0;
+}
+
+// base_e from the direction counters d (each lane's, see dir_load), into
+// ps.base (run by wave 0)
+__device__ inline void compact_pos_bases(const uint32_t* d, PosShared& ps)
+{
+    uint32_t run = 0;
+#pragma unroll
+    for (int e = 0; e < 7; ++e) {
+        const uint32_t t = wave_sum(d[e]);
+        if (threadIdx.x == 0) ps.base[e] = run;
+        run += t;
+    }
+    if (threadIdx.x == 0) ps.base[7] = run;
+}
+
+// per-direction counts of a loaded tile in position order (item, wave, lane):
+// ps.excl, ps.tot
+__device__ __attribute__((always_inline)) inline void compact_pos_counts(const uint32_t* kb, PosShared& ps)
+{
+    constexpr int kSeg = kPosItems * 4;  // counts per direction
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int e = 0; e < 7; ++e)
+#pragma unroll
+        for (int j = 0; j < kPosItems; ++j) {
+            const uint32_t c = (uint32_t)__popcll(__ballot((kb[j] >> e) & 1u));
+            if (lane == 0) ps.cnt[(e * kPosItems + j) * 4 + wave] = c;
+        }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+#pragma unroll
+        for (int r = 0; r < (7 * kSeg + 63) / 64; ++r) {
+            const int i = r * 64 + lane;
+            const uint32_t cc = i < 7 * kSeg ? ps.cnt[i] : 0u;
+            static_assert(kSeg == 16, "one direction per row of 16 lanes");
+            const uint32_t v = row_inclusive_scan(cc);
+            if (i < 7 * kSeg) {
+                ps.excl[i] = v - cc;
+                if (i % kSeg == kSeg - 1) ps.tot[i / kSeg] = v;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// places the kept candidates of a loaded and counted tile, direction e from
+// list offset ps.run[e] on
+__device__ __attribute__((always_inline)) inline void compact_pos_place(const KArgs& a, int q, const uint32_t* kb,
+                                                                        const uint32_t* vx, const PosShared& ps)
+{
+    uint32_t* __restrict__ out = a.list[q ^ 1];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt(lane);
+#pragma unroll
+    for (int e = 0; e < 7; ++e) {
+        const uint32_t r0 = ps.run[e];
+#pragma unroll
+        for (int j = 0; j < kPosItems; ++j) {
+            const bool k = ((kb[j] >> e) & 1u) != 0;
+            const uint64_t bal = __ballot(k);
+            if (k) {
+                const uint32_t at = r0 + ps.excl[(e * kPosItems + j) * 4 + wave] + (uint32_t)__popcll(bal & lt);
+                const uint32_t v = neighbor_in(a.g, vx[j], (uint32_t)e);
+                out[at] = v;
+                a.pos[v] = at;
+            }
+        }
+    }
+}
+
+// Look-back chunk `chunk` of a few-path compaction.  When `loaded`, kb and vx
+// hold its first tile's inputs and (wave 0) d its direction counters.  Every
+// chunk reads the counters before it publishes its aggregates, so the last
+// chunk, whose prefixes need every other chunk's aggregates, clears them.
+// Returns false (with the error flag set) if the next list would overflow.
+template <int kMode>
+__device__ inline bool compact_pos_chunk(const KArgs& a, int q, const CompactPlan& c, const ChunkPlan& cp,
+                                         uint32_t chunk, uint32_t epoch, CompactShared& sh, uint32_t* kb,
+                                         uint32_t* vx, uint32_t* d, bool loaded)
+{
+    PosShared& ps = sh.pos;
+    const uint32_t s0 = chunk * cp.r, s1 = min(s0 + cp.r, c.ntiles);
+    const bool last = chunk == cp.chunks - 1;
+    if (!loaded) {
+        if (threadIdx.x < 64) dir_load(a, q, d);
+        if (cp.r == 1) compact_pos_load(a, q, s0, kb, vx);
+    }
+    if (threadIdx.x < 64) compact_pos_bases(d, ps);  // before the barriers below
+    if (cp.r == 1) {
+        compact_pos_mask(c.n_src, s0, kb);
+        compact_pos_counts(kb, ps);
+        if (threadIdx.x < 7) ps.chain[threadIdx.x] = ps.tot[threadIdx.x];
+    } else {
+        uint32_t acc = 0;
+        for (uint32_t sub = s0; sub < s1; ++sub) {
+            compact_pos_load(a, q, sub, kb, vx);
+            compact_pos_mask(c.n_src, sub, kb);
+            compact_pos_counts(kb, ps);
+            if (threadIdx.x < 7) acc += ps.tot[threadIdx.x];
+        }
+        if (threadIdx.x < 7) ps.chain[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) wave_lookback7(a.tstate, chunk, ps.chain, epoch, &a.ctl->error, ps.pre);
+    __syncthreads();
+    bool over = false;
+#pragma unroll
+    for (int e = 0; e < 7; ++e) over |= (uint64_t)ps.base[e] + ps.pre[e] + ps.chain[e] > a.g.N;
+    if (over) {
+        if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
+        return false;
+    }
+    if (threadIdx.x < 7) ps.run[threadIdx.x] = ps.base[threadIdx.x] + ps.pre[threadIdx.x];
+    __syncthreads();
+    if (cp.r == 1) {
+        compact_pos_place(a, q, kb, vx, ps);
+    } else {
+        for (uint32_t sub = s0; sub < s1; ++sub) {
+            compact_pos_load(a, q, sub, kb, vx);
+            compact_pos_mask(c.n_src, sub, kb);
+            compact_pos_counts(kb, ps);
+            compact_pos_place(a, q, kb, vx, ps);
+            __syncthreads();
+            if (threadIdx.x < 7) ps.run[threadIdx.x] += ps.tot[threadIdx.x];
+            __syncthreads();
+        }
+    }
+    if (last && threadIdx.x < 64) {
+        // every chain ends at its direction's count from the keep step
+        bool bad = false;
+#pragma unroll
+        for (int e = 0; e < 7; ++e) bad |= ps.pre[e] + ps.chain[e] != ps.base[e + 1] - ps.base[e];
+        if (threadIdx.x == 0 && bad) atomicOr(&a.ctl->error, kErrCount);
+        dir_clear(a, q);
+        if (threadIdx.x == 0) compact_finish<kMode>(a, q, c, ps.base[7], epoch);
+    }
+    __syncthreads();
+    return true;
+}
+
+// Look-back chunk `chunk` of the compaction (a few-path chunk takes the
+// loaded inputs, see compact_pos_chunk).  Returns false (with the error flag
+// set) if the next list would overflow, which a consistent record never does.
 template <int kMode>
 __device__ inline bool compact_chunk(const KArgs& a, int q, const CompactPlan& c, const ChunkPlan& cp, uint32_t chunk,
-                                     uint32_t epoch, CompactShared& sh)
+                                     uint32_t epoch, CompactShared& sh, uint32_t* kb, uint32_t* vx, uint32_t* d,
+                                     bool loaded)
 {
+    if constexpr (kMode == kCompact)
+        if (!c.dense) return compact_pos_chunk<kMode>(a, q, c, cp, chunk, epoch, sh, kb, vx, d, loaded);
     const uint32_t s0 = chunk * cp.r, s1 = min(s0 + cp.r, c.ntiles);
     uint64_t bal[kItems];
     uint32_t val[kItems];
@@ -780,20 +1075,37 @@ __device__ inline bool compact_chunk(const KArgs& a, int q, const CompactPlan& c
             run += t;
         }
     }
-    if (threadIdx.x == 0 && chunk == cp.chunks - 1) compact_finish<kMode>(a, q, c, prefix + total, epoch);
+    if (chunk == cp.chunks - 1) {
+        // the keep step's direction counters are not needed on the dense path
+        if (kMode == kCompact && threadIdx.x < 64) dir_clear(a, q);
+        if (threadIdx.x == 0) compact_finish<kMode>(a, q, c, prefix + total, epoch);
+    }
     __syncthreads();
     return true;
 }
 
 // the whole compaction of list[q] over the first `participants` workgroups
-// (every one of them calls it)
+// (every one of them calls it); `loaded` as in compact_pos_chunk, for tile
+// blockIdx.x
 template <int kMode>
 __device__ inline void compact_all(const KArgs& a, int q, const CompactPlan& c, uint32_t epoch, CompactShared& sh,
-                                   uint32_t participants)
+                                   uint32_t participants, uint32_t* kb, uint32_t* vx, uint32_t* d, bool loaded)
 {
     const ChunkPlan cp = chunk_plan(c.ntiles, participants);
-    if (blockIdx.x < cp.chunks) (void)compact_chunk<kMode>(a, q, c, cp, blockIdx.x, epoch, sh);
+    if (blockIdx.x < cp.chunks)
+        (void)compact_chunk<kMode>(a, q, c, cp, blockIdx.x, epoch, sh, kb, vx, d, loaded && cp.r == 1);
     if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kMode>(a, q, c, 0, epoch);
+}
+
+// the first workgroups' inputs of a few-path compaction of list[q], loaded
+// before the counts are known (see compact_pos_chunk)
+__device__ __attribute__((always_inline)) inline bool compact_preload(const KArgs& a, int q, uint32_t* kb,
+                                                                      uint32_t* vx, uint32_t* d)
+{
+    if (blockIdx.x >= kSpecTiles) return false;
+    compact_pos_load(a, q, blockIdx.x, kb, vx);
+    if (threadIdx.x < 64) dir_load(a, q, d);
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -810,6 +1122,7 @@ __device__ inline void compact_all(const KArgs& a, int q, const CompactPlan& c, 
 struct KeepShared {
     uint32_t cnt[kKeepItems * 4];
     uint32_t excl[kKeepItems * 4];
+    uint32_t dir[2][7 * 4];  // kept candidates per direction and wave, by sub-tile parity
     uint32_t prefix, total;
 };
 
@@ -940,6 +1253,33 @@ __device__ __attribute__((always_inline)) inline uint32_t keep_counts(const uint
     return sh.total;
 }
 
+// The kept candidates of a sub-tile per direction go to this workgroup's
+// direction counters of list q (the compaction's list-major bases):
+// keep_dirs_count before keep_counts (whose barriers publish sh.dir),
+// keep_dirs_add after it.  Buffer `buf` alternates between sub-tiles: a wave
+// may count the next sub-tile while wave 0 still adds this one's.
+__device__ __attribute__((always_inline)) inline void keep_dirs_count(const uint32_t* kb, KeepShared& sh, int buf)
+{
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int e = 0; e < 7; ++e) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int j = 0; j < kKeepItems; ++j) c += (uint32_t)__popcll(__ballot((kb[j] >> e) & 1u));
+        if ((threadIdx.x & 63) == 0) sh.dir[buf][e * 4 + wave] = c;
+    }
+}
+
+__device__ __attribute__((always_inline)) inline void keep_dirs_add(const KArgs& a, int q, const KeepShared& sh,
+                                                                    int buf)
+{
+    if (threadIdx.x < 7) {
+        const uint32_t* w = &sh.dir[buf][threadIdx.x * 4];
+        const uint32_t t = w[0] + w[1] + w[2] + w[3];
+        if (t) atomicAdd(dir_counter(a, q, (int)threadIdx.x, blockIdx.x % kCountSlots), t);
+    }
+}
+
 // appends the kept voxels of a counted sub-tile to the next S from `prefix` on
 __device__ __attribute__((always_inline)) inline void keep_append(const KArgs& a, int q, bool dense,
                                                                   const uint32_t* us, const uint32_t* kb,
@@ -984,7 +1324,9 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
     uint32_t total = 0;
     if (cp.r == 1) {
         keep_decide(a, n, s0, us, ms, kb);
+        keep_dirs_count(kb, sh, 0);
         total = keep_counts(kb, ex, sh);
+        keep_dirs_add(a, q, sh, 0);
     } else {
         for (uint32_t sub = s0; sub < s1; ++sub) {
             keep_voxels(a, q, sub, us, ms);
@@ -995,7 +1337,9 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
                 const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
                 if (i < n) a.emit[i] = (uint8_t)kb[j];  // read back by the second pass
             }
+            keep_dirs_count(kb, sh, (int)(sub & 1u));
             total += keep_counts(kb, ex, sh);
+            keep_dirs_add(a, q, sh, (int)(sub & 1u));
         }
     }
     const uint32_t prefix = lookback(a.tstate, chunk, total, epoch, &a.ctl->error, &sh.prefix);
@@ -1086,12 +1430,14 @@ template <typename SrcT, bool kFloatSrc, typename PhiT>
 __global__ __launch_bounds__(kBlock) void k_update_fused(KArgs a, int q)
 {
     __shared__ CompactShared sh;
+    uint32_t kb[kPosItems], vx[kPosItems], d[7];
+    const bool loaded = compact_preload(a, q ^ 1, kb, vx, d);  // with the counts
     const uint32_t epoch = a.ctl->epoch;
     CompactPlan c = compact_plan<kCompact>(a.g, a.ctl->n[q ^ 1]);
     c.n_dst = a.ctl->n[q];
     const uint32_t gc = min((c.ntiles + 7u) & ~7u, (gridDim.x / 4u * 3u) & ~7u);
     if (blockIdx.x < gc) {
-        compact_all<kCompact>(a, q ^ 1, c, epoch, sh, gc);
+        compact_all<kCompact>(a, q ^ 1, c, epoch, sh, gc, kb, vx, d, loaded);
     } else {
         if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kCompact>(a, q ^ 1, c, 0, epoch);
         update_walk<SrcT, kFloatSrc, PhiT>(a, q, gc, gridDim.x - gc);
@@ -1138,10 +1484,12 @@ template <int kMode>
 __global__ __launch_bounds__(kBlock) void k_compact(KArgs a, int q)
 {
     __shared__ CompactShared sh;
+    uint32_t kb[kPosItems], vx[kPosItems], d[7];
+    const bool loaded = kMode == kCompact && compact_preload(a, q, kb, vx, d);
     const uint32_t epoch = a.ctl->epoch;
     CompactPlan c = compact_plan<kMode>(a.g, a.ctl->n[q]);
     c.n_dst = a.ctl->n[q ^ 1];
-    compact_all<kMode>(a, q, c, epoch, sh, gridDim.x);
+    compact_all<kMode>(a, q, c, epoch, sh, gridDim.x, kb, vx, d, loaded);
 }
 
 // ---------------------------------------------------------------------------
@@ -1314,7 +1662,8 @@ __device__ inline void compact_batch_chunks(const BatchArgs& b, int q, uint32_t 
         CompactPlan c = compact_plan<kCompact>(a.g, b.ctls[v].n[q]);
         c.n_dst = b.ctls[v].n[q ^ 1];
         const ChunkPlan cp{r, (c.ntiles + r - 1) / r};
-        (void)compact_chunk<kCompact>(a, q, c, cp, j - s_pre[v], b.ctls[v].epoch, sh);
+        uint32_t kb[kPosItems], vx[kPosItems], d[7];
+        (void)compact_chunk<kCompact>(a, q, c, cp, j - s_pre[v], b.ctls[v].epoch, sh, kb, vx, d, false);
     }
     // volumes with nothing to compact still advance their epochs
     if (threadIdx.x == 0)
