@@ -453,6 +453,28 @@ lsseg_status inject_list_counts(lsseg_ctx* c, uint32_t n, const uint32_t* S_dev 
 }
 
 // a phi buffer as N floats (host memory); fixed point converts to k / 127
+// The C ABI speaks the reference's tiled index tau; the engine stores voxel
+// ids (Morton order inside each tile).  These convert list entries and whole
+// padded volumes (host memory).
+void taus_of_ids(uint32_t* v, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i) v[i] = tau_of_id(v[i]);
+}
+
+template <typename T>
+void tau_order(T* vol, uint64_t N)  // id-ordered -> tau-ordered, in place
+{
+    std::vector<T> tmp(vol, vol + N);
+    for (uint64_t t = 0; t < N; ++t) vol[t] = tmp[id_of_tau((uint32_t)t)];
+}
+
+template <typename T>
+void id_order(T* vol, uint64_t N)  // tau-ordered -> id-ordered, in place
+{
+    std::vector<T> tmp(vol, vol + N);
+    for (uint64_t t = 0; t < N; ++t) vol[id_of_tau((uint32_t)t)] = tmp[t];
+}
+
 lsseg_status phi_to_floats(lsseg_ctx* c, const void* buf, float* dst)
 {
     const uint64_t N = c->a.g.N;
@@ -923,6 +945,7 @@ lsseg_status lsseg_get_active(const lsseg_ctx* c, uint32_t* out, uint64_t cap, u
     *n = cnt;
     const uint64_t k = std::min(cap, cnt);
     if (k) HIP_TRY(cm, hipMemcpy(out, c->a.list[c->q], k * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    taus_of_ids(out, k);
     return cap < cnt ? fail(cm, LSSEG_ERR_CAPACITY, "list has %llu entries, capacity %llu", (unsigned long long)cnt,
                             (unsigned long long)cap)
                      : LSSEG_OK;
@@ -1041,8 +1064,12 @@ lsseg_status lsseg_get_state(const lsseg_ctx* c, float* phi_read, float* phi_wri
         if (!dst) continue;
         lsseg_status st = phi_to_floats(cm, c->a.phi[c->q ^ i], dst);
         if (st != LSSEG_OK) return st;
+        tau_order(dst, N);
     }
-    if (E) HIP_TRY(cm, hipMemcpy(E, c->a.E, N, hipMemcpyDeviceToHost));
+    if (E) {
+        HIP_TRY(cm, hipMemcpy(E, c->a.E, N, hipMemcpyDeviceToHost));
+        tau_order(E, N);
+    }
     if (n || list) {
         uint64_t cnt = 0;
         if (list && !n) return fail(cm, LSSEG_ERR_INVALID_ARG, "list requires n");
@@ -1058,17 +1085,25 @@ lsseg_status lsseg_set_state(lsseg_ctx* c, const float* phi_read, const float* p
     const uint64_t N = c->a.g.N;
     if (n > N) return fail(c, LSSEG_ERR_CAPACITY, "list of %llu entries exceeds the volume", (unsigned long long)n);
     std::vector<uint8_t> seen(N, 0);
+    std::vector<uint32_t> ids(n);
     for (uint64_t i = 0; i < n; ++i) {
         if (list[i] >= N) return fail(c, LSSEG_ERR_INVALID_ARG, "list entry %llu out of range", (unsigned long long)i);
         if (seen[list[i]])
             return fail(c, LSSEG_ERR_INVALID_ARG, "list entry %llu is a duplicate", (unsigned long long)i);
         seen[list[i]] = 1;
+        ids[i] = id_of_tau(list[i]);
     }
-    lsseg_status st0 = floats_to_phi(c, phi_read, c->a.phi[0]);
+    std::vector<float> ph(phi_read, phi_read + N);
+    id_order(ph.data(), N);
+    lsseg_status st0 = floats_to_phi(c, ph.data(), c->a.phi[0]);
     if (st0 != LSSEG_OK) return st0;
-    if ((st0 = floats_to_phi(c, phi_write, c->a.phi[1])) != LSSEG_OK) return st0;
-    HIP_TRY(c, hipMemcpy(c->a.E, E, N, hipMemcpyHostToDevice));
-    if (n) HIP_TRY(c, hipMemcpy(c->a.list[0], list, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    ph.assign(phi_write, phi_write + N);
+    id_order(ph.data(), N);
+    if ((st0 = floats_to_phi(c, ph.data(), c->a.phi[1])) != LSSEG_OK) return st0;
+    std::vector<int8_t> eh(E, E + N);
+    id_order(eh.data(), N);
+    HIP_TRY(c, hipMemcpy(c->a.E, eh.data(), N, hipMemcpyHostToDevice));
+    if (n) HIP_TRY(c, hipMemcpy(c->a.list[0], ids.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
     lsseg_status st = inject_list_counts(c, (uint32_t)n);
     if (st != LSSEG_OK) return st;
     return sync_and_check(c);
@@ -1867,7 +1902,12 @@ lsseg_status lsseg_record_get(const lsseg_record* r, uint32_t* n_per_iter, uint3
     if (!r || !r->ctx) return LSSEG_ERR_INVALID_ARG;
     lsseg_ctx* c = r->ctx;
     if (n_per_iter) memcpy(n_per_iter, r->n.data(), r->n.size() * sizeof(uint32_t));
-    if (lists && r->total) HIP_TRY(c, hipMemcpy(lists, r->lists, r->total * sizeof(uint32_t), hipMemcpyDefault));
+    if (lists && r->total) {
+        std::vector<uint32_t> h(r->total);
+        HIP_TRY(c, hipMemcpy(h.data(), r->lists, r->total * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        taus_of_ids(h.data(), r->total);
+        HIP_TRY(c, hipMemcpy(lists, h.data(), r->total * sizeof(uint32_t), hipMemcpyDefault));
+    }
     if (decisions && r->total) HIP_TRY(c, hipMemcpy(decisions, r->dec, r->total, hipMemcpyDefault));
     return LSSEG_OK;
 }

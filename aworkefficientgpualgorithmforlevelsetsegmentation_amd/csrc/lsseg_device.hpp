@@ -106,60 +106,104 @@ struct Geom {
     FastDiv dTX, dTXY;
 };
 
+// Voxel ids.  Voxel (x, y, z) of the padded volume is stored at
+//   id = tile * 4096 + m,  tile = (tz * TY + ty) * TX + tx,
+// m the Morton code of its coordinates (lx, ly, lz) inside the 16^3 tile: bit
+// 3k of m is bit k of lx, bit 3k + 1 of ly, bit 3k + 2 of lz.  A 32-byte
+// sector of a float array then holds a 2x2x2 block and a 128-byte line a
+// 4x4x2 block, so the gathers around the active surface touch fewer sectors
+// than with rows of 16 whatever the surface's orientation.  The reference's
+// tiled index tau (the same tile, then lz * 256 + ly * 16 + lx) still decides
+// the dense path's list order, and the C ABI speaks tau; tau_of_id and
+// id_of_tau convert.
+constexpr uint32_t kMx = 0x249u, kMy = 0x492u, kMz = 0x924u;  // the bits of lx, ly, lz in m
+
+__host__ __device__ inline uint32_t spread4(uint32_t v)
+{
+    v = (v | (v << 4)) & 0x0C3u;
+    return (v | (v << 2)) & 0x249u;
+}
+
+__host__ __device__ inline uint32_t compact4(uint32_t v)
+{
+    v &= 0x249u;
+    v = (v | (v >> 2)) & 0x0C3u;
+    return (v | (v >> 4)) & 0x00Fu;
+}
+
+__host__ __device__ inline uint32_t id_of_tau(uint32_t t)
+{
+    return (t & ~4095u) | spread4(t & 15u) | (spread4((t >> 4) & 15u) << 1) | (spread4((t >> 8) & 15u) << 2);
+}
+
+__host__ __device__ inline uint32_t tau_of_id(uint32_t v)
+{
+    const uint32_t m = v & 4095u;
+    return (v & ~4095u) | (compact4(m >> 2) << 8) | (compact4(m >> 1) << 4) | compact4(m);
+}
+
+// one step along the axis whose bits in m are M (the other axes' bits stay; the
+// zero gaps between M's bits carry the borrow or the carry)
+__host__ __device__ inline uint32_t morton_dec(uint32_t m, uint32_t M) { return (((m & M) - 1u) & M) | (m & ~M); }
+__host__ __device__ inline uint32_t morton_inc(uint32_t m, uint32_t M) { return (((m | ~M) + 1u) & M) | (m & ~M); }
+
 // The six face neighbours in list order; an out-of-bounds neighbour is the
 // voxel itself (OffsetCoordinatesUnsafe, Coordinates.cu:157-184) and bit e of
-// *inb tells whether neighbour e is inside the volume.
-__device__ inline void neighbors(const Geom& g, uint32_t t, uint32_t nb[7], uint32_t* inb)
+// *inb tells whether neighbour e is inside the volume.  Across a tile face the
+// axis' in-tile coordinate goes from 0 to 15 (all its bits set) or back.
+__host__ __device__ inline void neighbors(const Geom& g, uint32_t t, uint32_t nb[7], uint32_t* inb)
 {
-    const uint32_t lx = t & 15u, ly = (t >> 4) & 15u, lz = (t >> 8) & 15u;
+    const uint32_t m = t & 4095u, hi = t - m;
+    const uint32_t xm = m & kMx, ym = m & kMy, zm = m & kMz;
     const uint32_t tile = t >> 12;
     const uint32_t qy = fdiv(tile, g.dTX);
     const uint32_t tx = tile - qy * g.TX;
     const uint32_t tz = fdiv(tile, g.dTXY);
     const uint32_t ty = qy - tz * g.TY;
     uint32_t in = 0;
-    nb[0] = lz ? t - 256u : (tz ? t - g.sTZ + 3840u : t);
-    in |= (lz | tz) ? 1u : 0u;
-    nb[1] = ly ? t - 16u : (ty ? t - g.sTY + 240u : t);
-    in |= (ly | ty) ? 2u : 0u;
-    nb[2] = lx ? t - 1u : (tx ? t - 4096u + 15u : t);
-    in |= (lx | tx) ? 4u : 0u;
+    nb[0] = zm ? hi | morton_dec(m, kMz) : (tz ? t - g.sTZ + kMz : t);
+    in |= (zm | tz) ? 1u : 0u;
+    nb[1] = ym ? hi | morton_dec(m, kMy) : (ty ? t - g.sTY + kMy : t);
+    in |= (ym | ty) ? 2u : 0u;
+    nb[2] = xm ? hi | morton_dec(m, kMx) : (tx ? t - 4096u + kMx : t);
+    in |= (xm | tx) ? 4u : 0u;
     nb[3] = t;
-    nb[4] = lx < 15u ? t + 1u : (tx + 1u < g.TX ? t + 4096u - 15u : t);
-    in |= (lx < 15u || tx + 1u < g.TX) ? 16u : 0u;
-    nb[5] = ly < 15u ? t + 16u : (ty + 1u < g.TY ? t + g.sTY - 240u : t);
-    in |= (ly < 15u || ty + 1u < g.TY) ? 32u : 0u;
-    nb[6] = lz < 15u ? t + 256u : (tz + 1u < g.TZ ? t + g.sTZ - 3840u : t);
-    in |= (lz < 15u || tz + 1u < g.TZ) ? 64u : 0u;
+    nb[4] = xm != kMx ? hi | morton_inc(m, kMx) : (tx + 1u < g.TX ? t + 4096u - kMx : t);
+    in |= (xm != kMx || tx + 1u < g.TX) ? 16u : 0u;
+    nb[5] = ym != kMy ? hi | morton_inc(m, kMy) : (ty + 1u < g.TY ? t + g.sTY - kMy : t);
+    in |= (ym != kMy || ty + 1u < g.TY) ? 32u : 0u;
+    nb[6] = zm != kMz ? hi | morton_inc(m, kMz) : (tz + 1u < g.TZ ? t + g.sTZ - kMz : t);
+    in |= (zm != kMz || tz + 1u < g.TZ) ? 64u : 0u;
     *inb = in;
 }
 
 // Neighbour e of t when it is known to be inside the volume (every emitted
 // candidate is): only the in-tile coordinate decides whether a tile is crossed.
-__device__ inline uint32_t neighbor_in(const Geom& g, uint32_t t, uint32_t e)
+__host__ __device__ inline uint32_t neighbor_in(const Geom& g, uint32_t t, uint32_t e)
 {
-    const uint32_t lx = t & 15u, ly = (t >> 4) & 15u, lz = (t >> 8) & 15u;
+    const uint32_t m = t & 4095u, hi = t - m;
     switch (e) {
-    case 0: return lz ? t - 256u : t - g.sTZ + 3840u;
-    case 1: return ly ? t - 16u : t - g.sTY + 240u;
-    case 2: return lx ? t - 1u : t - 4096u + 15u;
-    case 4: return lx < 15u ? t + 1u : t + 4096u - 15u;
-    case 5: return ly < 15u ? t + 16u : t + g.sTY - 240u;
-    case 6: return lz < 15u ? t + 256u : t + g.sTZ - 3840u;
+    case 0: return (m & kMz) ? hi | morton_dec(m, kMz) : t - g.sTZ + kMz;
+    case 1: return (m & kMy) ? hi | morton_dec(m, kMy) : t - g.sTY + kMy;
+    case 2: return (m & kMx) ? hi | morton_dec(m, kMx) : t - 4096u + kMx;
+    case 4: return (m & kMx) != kMx ? hi | morton_inc(m, kMx) : t + 4096u - kMx;
+    case 5: return (m & kMy) != kMy ? hi | morton_inc(m, kMy) : t + g.sTY - kMy;
+    case 6: return (m & kMz) != kMz ? hi | morton_inc(m, kMz) : t + g.sTZ - kMz;
     default: return t;
     }
 }
 
-__device__ inline void decode_xyz(const Geom& g, uint32_t t, uint32_t* x, uint32_t* y, uint32_t* z)
+__host__ __device__ inline void decode_xyz(const Geom& g, uint32_t t, uint32_t* x, uint32_t* y, uint32_t* z)
 {
     const uint32_t tile = t >> 12;
     const uint32_t qy = fdiv(tile, g.dTX);
     const uint32_t tx = tile - qy * g.TX;
     const uint32_t tz = fdiv(tile, g.dTXY);
     const uint32_t ty = qy - tz * g.TY;
-    *x = tx * 16u + (t & 15u);
-    *y = ty * 16u + ((t >> 4) & 15u);
-    *z = tz * 16u + ((t >> 8) & 15u);
+    const uint32_t m = t & 4095u;
+    *x = tx * 16u + compact4(m);
+    *y = ty * 16u + compact4(m >> 1);
+    *z = tz * 16u + compact4(m >> 2);
 }
 
 __host__ __device__ inline uint64_t align256(uint64_t n) { return n + ((256u - (n % 256u)) % 256u); }
@@ -582,10 +626,11 @@ __device__ inline float update_core(const DevParams& P, float unz, float u1, flo
     return nv;
 }
 
-__device__ inline uint32_t tau_of(const Geom& g, uint32_t x, uint32_t y, uint32_t z)
+// the id of voxel (x, y, z) of the padded volume
+__host__ __device__ inline uint32_t id_of(const Geom& g, uint32_t x, uint32_t y, uint32_t z)
 {
-    return (((z >> 4) * g.TY + (y >> 4)) * g.TX + (x >> 4)) * 4096u + ((z & 15u) * 16u + (y & 15u)) * 16u +
-           (x & 15u);
+    return (((z >> 4) * g.TY + (y >> 4)) * g.TX + (x >> 4)) * 4096u |
+           (spread4(x & 15u) | (spread4(y & 15u) << 1) | (spread4(z & 15u) << 2));
 }
 
 
@@ -781,13 +826,12 @@ __device__ __attribute__((always_inline)) inline void compact_load(const KArgs& 
         bool keep = false;
         uint32_t v = 0;
         if (s < c.slots) {
+            v = id_of_tau((uint32_t)s);  // s runs in the reference's tiled order
             if constexpr (kMode == kInit) {
-                keep = a.C[s] != 0ull;
-                v = (uint32_t)s;
+                keep = a.C[v] != 0ull;
             } else {
-                keep = a.memb[s] != 0;
-                v = (uint32_t)s;
-                if (kPlace && keep) a.memb[s] = 0;
+                keep = a.memb[v] != 0;
+                if (kPlace && keep) a.memb[v] = 0;
             }
         }
         bal[j] = __ballot(keep);
@@ -1881,7 +1925,7 @@ __global__ void k_tiled_sum_to_lin(Geom g, const float* __restrict__ a, const fl
         const uint32_t x = (uint32_t)(i % nx);
         const uint32_t y = (uint32_t)((i / nx) % ny);
         const uint32_t z = (uint32_t)(i / ((uint64_t)nx * ny));
-        const uint32_t t = tau_of(g, x, y, z);
+        const uint32_t t = id_of(g, x, y, z);
         lin[i] = b ? a[t] + b[t] : a[t];
     }
 }
@@ -2090,7 +2134,7 @@ __global__ void k_get_phi(KArgs a, int which, float* out, uint32_t nx, uint32_t 
         const uint32_t x = (uint32_t)(i % nx);
         const uint32_t y = (uint32_t)((i / nx) % ny);
         const uint32_t z = (uint32_t)(i / ((uint64_t)nx * ny));
-        out[i] = phi_load(phi_buf<PhiT>(a, which), tau_of(a.g, x, y, z));
+        out[i] = phi_load(phi_buf<PhiT>(a, which), id_of(a.g, x, y, z));
     }
 }
 
@@ -2121,7 +2165,7 @@ __global__ void k_get_mask(KArgs a, uint8_t* out, uint32_t nx, uint32_t ny, uint
         const uint32_t x = (uint32_t)(i % nx);
         const uint32_t y = (uint32_t)((i / nx) % ny);
         const uint32_t z = (uint32_t)(i / ((uint64_t)nx * ny));
-        const uint32_t t = tau_of(a.g, x, y, z);
+        const uint32_t t = id_of(a.g, x, y, z);
         out[i] = add_export(a.E[t], a.frozen[t]) <= 0 ? 1 : 0;
     }
 }

@@ -20,11 +20,15 @@
  * Conventions
  *   - Volumes crossing the ABI are linear, x fastest, unpadded (nx*ny*nz).
  *     Internally each axis is padded to a multiple of 16
- *     (Segmenter.cpp:1131-1204) and stored in the reference's 16^3-tiled order
+ *     (Segmenter.cpp:1131-1204) and split into the reference's 16^3 tiles
  *     (Coordinates.cu:65-96).
- *   - Active-list entries returned by lsseg_get_active() are tiled indices tau of
- *     the padded volume: tau = ((tz*TY + ty)*TX + tx)*4096 + (lz*16 + ly)*16 + lx
+ *   - Active-list entries and raw padded state cross the ABI in the reference's
+ *     tiled index tau of the padded volume:
+ *     tau = ((tz*TY + ty)*TX + tx)*4096 + (lz*16 + ly)*16 + lx
  *     with (tx, lx) = (x/16, x%16) and likewise for y, z; TX = X/16, TY = Y/16.
+ *     Device memory holds the voxels of a tile in Morton order instead
+ *     (tile*4096 + m, bit 3k of m = bit k of lx, 3k+1 of ly, 3k+2 of lz); only
+ *     lsseg_get_device_buffers exposes that layout.
  *   - The caller owns host arrays; the context copies them in and out.
  */
 #ifndef LSSEG_LSSEG_H
@@ -251,7 +255,8 @@ lsseg_status lsseg_iterate_profiled(lsseg_ctx* ctx, const lsseg_params* params, 
                                     float* launch_ms, uint64_t launch_cap);
 
 /* Device pointers of the phi buffers (read, write; float or int8_t by the
-   context's phi storage) and the stream, for callers that keep data resident
+   context's phi storage; padded, tiles of 16^3 voxels in Morton order, see
+   the conventions above) and the stream, for callers that keep data resident
    in HBM (e.g. a torch wrapper). */
 lsseg_status lsseg_get_device_buffers(const lsseg_ctx* ctx, void** phi_read, void** phi_write,
                                       void** stream);
