@@ -652,6 +652,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     ALLOC(keep, uint8_t, N);
     ALLOC(memb, uint8_t, N);
     ALLOC(tstate, uint64_t, ntiles);
+    ALLOC(gstate, uint64_t, 7 * (ntiles / 64 + 2));
     ALLOC(cpart, uint64_t, kCountSlots * kCountStride);
     ALLOC(dirc, uint32_t, kDirCounters);
     ALLOC(ctl, DevCtl, 1);
@@ -691,6 +692,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, c->stream, (uint8_t*)c->a.frozen, N, (uint8_t)127);
     fill_u64(c, c->a.C, N, 0);
     fill_u64(c, c->a.tstate, ntiles, 0);
+    fill_u64(c, c->a.gstate, 7 * (ntiles / 64 + 2), 0);
     fill_u64(c, c->a.cpart, kCountSlots * kCountStride, 0);
     fill_u64(c, (uint64_t*)c->a.dirc, kDirCounters / 2, 0);
     // positions, masks and keep bytes of voxels outside the list are never read;

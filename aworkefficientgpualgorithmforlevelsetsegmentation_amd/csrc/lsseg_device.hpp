@@ -260,7 +260,8 @@ struct KArgs {
     uint8_t* emit;        // emission mask per traversal position of the current list
     uint8_t* keep;        // kept candidate lists of every list position
     uint8_t* memb;        // members of the next list, for a dense compaction (zero at rest)
-    uint64_t* tstate;     // look-back tile states
+    uint64_t* tstate;     // look-back chunk states (aggregates)
+    uint64_t* gstate;     // look-back group states (totals of 64 chunks)
     uint64_t* cpart;      // candidate-emission counters (kCountSlots, zero between iterations)
     uint32_t* dirc;       // kept candidates per direction of list 0 and list 1 (7 x kCountSlots
                           // counters each, written by the keep step, cleared by the compaction)
@@ -320,159 +321,122 @@ __device__ inline void st_state(uint64_t* p, uint64_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Look-back tile state: [epoch:30][flag:2][value:32]; flag 1 = aggregate,
-// 2 = inclusive prefix.  A state written by an earlier launch carries another
-// epoch and reads as "not yet published", so the states need no clearing.
+// Look-back state: [epoch:30][flag:2][value:32]; flag 1 = a chunk's aggregate,
+// 2 = a group's total.  A state written by an earlier launch carries another
+// epoch and reads as "not yet published", so the states need no clearing.  The
+// count travels inside the 64-bit word, so no other memory has to be ordered
+// with it.
 __device__ inline uint32_t state_flag(uint64_t s, uint32_t ep)
 {
     return ((uint32_t)(s >> 34) == ep) ? (uint32_t)((s >> 32) & 3u) : 0u;
 }
 
-// Decoupled look-back for tile `tile` with aggregate `total`, run by one wave:
-// each round loads the states of the kLookWindows * 64 preceding tiles at
-// once (lane l, window j: tile look - l - 64 j), then walks the windows
-// nearest first, adding aggregates back to the nearest inclusive prefix; only
-// the windows it reaches have to be published.  The count travels inside the
-// 64-bit state word, so no other memory has to be ordered with it.  Publishes
-// this tile's aggregate first and its inclusive prefix last.  Returns the
-// exclusive prefix (in every lane).  One window per round measured fastest on
-// the phantom and vessels workloads (two: +1.5%, four: +4%): every state load
-// is a device-scope atomic, and the extra loads cost more than the rounds
-// they save.
-constexpr int kLookWindows = 1;
-
-__device__ uint32_t wave_lookback(uint64_t* states, uint32_t tile, uint32_t total, uint32_t epoch, uint32_t* err)
+// Polls this lane's C states at words[0..C-1] (when `need`) until every lane's
+// are published.  The wait is bounded: past kSpinLimit polls it sets kErrSpin,
+// zeroes the values and returns.
+template <int C>
+__device__ __attribute__((always_inline)) inline void wait_states(const uint64_t* words, bool need, uint32_t ep,
+                                                                  uint64_t* s, uint32_t& spins, uint32_t* err)
 {
-    const int lane = threadIdx.x & 63;
-    const uint32_t ep = epoch & 0x3FFFFFFFu;
-    const uint64_t stamp = (uint64_t)ep << 34;
-    const uint64_t none = stamp | (2ull << 32);  // before tile 0: an inclusive prefix of 0
-    if (tile == 0) {
-        if (lane == 0) st_state(&states[0], stamp | (2ull << 32) | total);
-        return 0;
-    }
-    if (lane == 0) st_state(&states[tile], stamp | (1ull << 32) | total);
-    uint32_t excl = 0;
-    int64_t look = (int64_t)tile - 1;
-    uint32_t spins = 0;
-    for (;;) {
-        uint64_t s[kLookWindows];
+    // one exit (the exit test is wave-uniform): two loops with early returns
+    // in one function made the compiler keep 256 VGPRs and spill
+    bool waiting = true;
+    while (waiting) {
+        bool pending = false;
 #pragma unroll
-        for (int j = 0; j < kLookWindows; ++j) {
-            const int64_t idx = look - lane - 64 * j;
-            s[j] = idx >= 0 ? ld_state(&states[idx]) : none;
-        }
+        for (int c = 0; c < C; ++c) pending |= need && state_flag(s[c], ep) == 0;
+        if (__ballot(pending) == 0) {
+            waiting = false;
+        } else if (++spins > kSpinLimit) {
+            if ((threadIdx.x & 63) == 0) atomicOr(err, kErrSpin);
 #pragma unroll
-        for (int j = 0; j < kLookWindows; ++j) {
-            const int64_t idx = look - lane - 64 * j;
-            for (;;) {
-                const bool pending = state_flag(s[j], ep) == 0;
-                if (__ballot(pending) == 0) break;
-                if (++spins > kSpinLimit) {
-                    if (lane == 0) atomicOr(err, kErrSpin);
-                    s[j] = none;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                if (pending) s[j] = ld_state(&states[idx]);
-            }
-            const uint64_t pm = __ballot(state_flag(s[j], ep) == 2u);
-            const uint32_t v = (uint32_t)s[j];
-            if (pm) {
-                const int k = __ffsll((unsigned long long)pm) - 1;
-                excl += wave_sum(lane <= k ? v : 0u);
-                if (lane == 0) st_state(&states[tile], stamp | (2ull << 32) | (uint64_t)(excl + total));
-                return excl;
-            }
-            excl += wave_sum(v);
-        }
-        look -= 64 * kLookWindows;
-    }
-}
-
-// the look-back for a whole block: wave 0 resolves it, every thread gets the
-// exclusive prefix (through *s_excl)
-__device__ inline uint32_t lookback(uint64_t* states, uint32_t tile, uint32_t total, uint32_t epoch, uint32_t* err,
-                                    uint32_t* s_excl)
-{
-    if (threadIdx.x < 64) {
-        const uint32_t excl = wave_lookback(states, tile, total, epoch, err);
-        if (threadIdx.x == 0) *s_excl = excl;
-    }
-    __syncthreads();
-    return *s_excl;
-}
-
-// Seven look-back chains at once, one per candidate direction e, over the same
-// tiles (state tile * 7 + e), run by one wave: lane e < 7 publishes tile
-// `tile`'s aggregate tot[e] first and its inclusive prefix last; each round
-// loads the seven states of the 64 preceding tiles (lane l: tile look - l) and
-// walks every unresolved chain back to its nearest inclusive prefix.  The
-// exclusive prefixes go to pre[0..6].
-__device__ void wave_lookback7(uint64_t* states, uint32_t tile, const uint32_t* tot, uint32_t epoch, uint32_t* err,
-                               uint32_t* pre)
-{
-    const int lane = threadIdx.x & 63;
-    const uint32_t ep = epoch & 0x3FFFFFFFu;
-    const uint64_t stamp = (uint64_t)ep << 34;
-    const uint64_t none = stamp | (2ull << 32);  // before tile 0: inclusive prefixes of 0
-    const uint32_t mine = lane < 7 ? tot[lane] : 0u;
-    uint64_t* own = states + (uint64_t)tile * 7u;
-    if (tile == 0) {
-        if (lane < 7) {
-            st_state(&own[lane], stamp | (2ull << 32) | mine);
-            pre[lane] = 0;
-        }
-        return;
-    }
-    if (lane < 7) st_state(&own[lane], stamp | (1ull << 32) | mine);
-    uint32_t excl[7] = {0, 0, 0, 0, 0, 0, 0};
-    uint32_t done = 0;  // bit e: chain e resolved (uniform)
-    int64_t look = (int64_t)tile - 1;
-    uint32_t spins = 0;
-    while (done != 0x7Fu) {
-        const int64_t idx = look - lane;
-        uint64_t s[7];
-#pragma unroll
-        for (int e = 0; e < 7; ++e)
-            s[e] = (idx >= 0 && !((done >> e) & 1u)) ? ld_state(&states[(uint64_t)idx * 7u + e]) : none;
-        for (;;) {
-            bool pending = false;
-#pragma unroll
-            for (int e = 0; e < 7; ++e) pending |= state_flag(s[e], ep) == 0;
-            if (__ballot(pending) == 0) break;
-            if (++spins > kSpinLimit) {
-                if (lane == 0) atomicOr(err, kErrSpin);
-#pragma unroll
-                for (int e = 0; e < 7; ++e) s[e] = none;
-                break;
-            }
+            for (int c = 0; c < C; ++c) s[c] = 0;
+            waiting = false;
+        } else {
             __builtin_amdgcn_s_sleep(1);
 #pragma unroll
-            for (int e = 0; e < 7; ++e)
-                if (state_flag(s[e], ep) == 0) s[e] = ld_state(&states[(uint64_t)idx * 7u + e]);
+            for (int c = 0; c < C; ++c)
+                if (need && state_flag(s[c], ep) == 0) s[c] = ld_state(&words[c]);
         }
+    }
+}
+
+// Two-level look-back over C independent counts at once, run by one wave.
+// Chunk j (group g = j / 64, rank r = j % 64) publishes its aggregates tot[c]
+// (agg[j * C + c], flag 1); the last chunk of a group (r = 63) also publishes
+// the group's totals (grp[g * C + c], flag 2) as soon as it has read the
+// group's other aggregates.  Chunk j's exclusive prefix is the sum of the
+// totals of groups 0..g-2 and of the aggregates of chunks 64(g-1)..j-1
+// (reading the previous group's aggregates directly spares the chunks of
+// group 1 the wait for group 0's total), written to pre[c].  Every chunk
+// resolves after at most two rounds of publication,
+// whatever its index (a chained look-back, where inclusive prefixes travel
+// one 64-chunk window per round, left chunk 400 of a 428-chunk compaction
+// waiting ~17 us), and waits only on chunks of lower workgroups.  Values are
+// 32-bit counts.
+template <int C>
+__device__ __attribute__((always_inline)) inline void wave_group_scan(uint64_t* agg, uint64_t* grp, uint32_t chunk, const uint32_t* tot, uint32_t epoch,
+                                uint32_t* err, uint32_t* pre)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t ep = epoch & 0x3FFFFFFFu;
+    const uint64_t stamp = (uint64_t)ep << 34;
+    const uint32_t g = chunk >> 6, r = chunk & 63u;
+    uint32_t mine = 0;
 #pragma unroll
-        for (int e = 0; e < 7; ++e) {
-            if ((done >> e) & 1u) continue;
-            const uint64_t pm = __ballot(state_flag(s[e], ep) == 2u);
-            const uint32_t v = (uint32_t)s[e];
-            if (pm) {
-                const int k = __ffsll((unsigned long long)pm) - 1;
-                excl[e] += wave_sum(lane <= k ? v : 0u);
-                done |= 1u << e;
-            } else {
-                excl[e] += wave_sum(v);
-            }
+    for (int c = 0; c < C; ++c)
+        if (lane == (uint32_t)c) mine = tot[c];
+    if (lane < (uint32_t)C) st_state(&agg[(uint64_t)chunk * C + lane], stamp | (1ull << 32) | mine);
+    // this group's earlier chunks, the previous group's chunks and the totals
+    // of the 64 groups before that, loaded together
+    const bool need_own = lane < r, need_prev = g > 0, need_grp = lane + 1u < g;
+    const uint64_t* ow = agg + ((uint64_t)g * 64u + lane) * C;
+    const uint64_t* pw = g ? agg + ((uint64_t)(g - 1u) * 64u + lane) * C : agg;
+    const uint64_t* gw = grp + (uint64_t)lane * C;
+    uint64_t so[C], sp[C], sg[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        so[c] = need_own ? ld_state(ow + c) : 0ull;
+        sp[c] = need_prev ? ld_state(pw + c) : 0ull;
+        sg[c] = need_grp ? ld_state(gw + c) : 0ull;
+    }
+    uint32_t spins = 0;
+    wait_states<C>(ow, need_own, ep, so, spins, err);
+    uint32_t own[C], prior[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) own[c] = wave_sum(need_own ? (uint32_t)so[c] : 0u);
+    if (r == 63u) {
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            if (lane == (uint32_t)c) st_state(&grp[(uint64_t)g * C + c], stamp | (2ull << 32) | (own[c] + mine));
+    }
+    wait_states<C>(pw, need_prev, ep, sp, spins, err);
+#pragma unroll
+    for (int c = 0; c < C; ++c) prior[c] = wave_sum(need_prev ? (uint32_t)sp[c] : 0u);
+    // the totals of groups 0..g-2, 64 groups per round
+    for (uint32_t g0 = 0; g0 + 1u < g; g0 += 64) {
+        const bool need = g0 + lane + 1u < g;
+        const uint64_t* w = grp + ((uint64_t)g0 + lane) * C;
+        if (g0) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) sg[c] = need ? ld_state(w + c) : 0ull;
         }
-        look -= 64;
+        wait_states<C>(w, need, ep, sg, spins, err);
+#pragma unroll
+        for (int c = 0; c < C; ++c) prior[c] += wave_sum(need ? (uint32_t)sg[c] : 0u);
     }
 #pragma unroll
-    for (int e = 0; e < 7; ++e)
-        if (lane == e) {
-            st_state(&own[e], stamp | (2ull << 32) | (uint64_t)(excl[e] + mine));
-            pre[e] = excl[e];
-        }
+    for (int c = 0; c < C; ++c)
+        if (lane == (uint32_t)c) pre[c] = prior[c] + own[c];
+}
+
+// the look-back of one count for a whole block: wave 0 resolves it, every
+// thread gets the exclusive prefix (through *s_excl)
+__device__ inline uint32_t lookback(const KArgs& a, uint32_t chunk, uint32_t total, uint32_t epoch, uint32_t* s_excl)
+{
+    if (threadIdx.x < 64) wave_group_scan<1>(a.tstate, a.gstate, chunk, &total, epoch, &a.ctl->error, s_excl);
+    __syncthreads();
+    return *s_excl;
 }
 
 // ---------------------------------------------------------------------------
@@ -891,7 +855,7 @@ __device__ __attribute__((always_inline)) inline void compact_place(const KArgs&
 // to base_e + (kept e-candidates at positions before p), base_e being the kept
 // candidates of the directions before e, which the keep step counts.  A tile
 // of kPosTile positions reads each keep byte and list entry once and runs
-// seven look-back chains, one per direction (wave_lookback7).  Its inputs do
+// seven look-back counts, one per direction (wave_group_scan<7>).  Its inputs do
 // not depend on the count, so the first workgroups load them with it.
 // ---------------------------------------------------------------------------
 __device__ inline uint32_t* dir_counter(const KArgs& a, int q, int e, uint32_t slot)
@@ -1043,7 +1007,7 @@ __device__ inline bool compact_pos_chunk(const KArgs& a, int q, const CompactPla
         if (threadIdx.x < 7) ps.chain[threadIdx.x] = acc;
     }
     __syncthreads();
-    if (threadIdx.x < 64) wave_lookback7(a.tstate, chunk, ps.chain, epoch, &a.ctl->error, ps.pre);
+    if (threadIdx.x < 64) wave_group_scan<7>(a.tstate, a.gstate, chunk, ps.chain, epoch, &a.ctl->error, ps.pre);
     __syncthreads();
     bool over = false;
 #pragma unroll
@@ -1103,7 +1067,7 @@ __device__ inline bool compact_chunk(const KArgs& a, int q, const CompactPlan& c
             total += compact_counts(sh);
         }
     }
-    const uint32_t prefix = lookback(a.tstate, chunk, total, epoch, &a.ctl->error, &sh.prefix);
+    const uint32_t prefix = lookback(a, chunk, total, epoch, &sh.prefix);
     if ((uint64_t)prefix + total > a.g.N) {
         if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
         return false;
@@ -1386,7 +1350,7 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
             keep_dirs_add(a, q, sh, (int)(sub & 1u));
         }
     }
-    const uint32_t prefix = lookback(a.tstate, chunk, total, epoch, &a.ctl->error, &sh.prefix);
+    const uint32_t prefix = lookback(a, chunk, total, epoch, &sh.prefix);
     if ((uint64_t)prefix + total > a.g.N) {
         if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
         return false;
