@@ -24,9 +24,11 @@
 //            positions.
 // The compaction of iteration k touches nothing that the update of k + 1
 // reads or writes, so both run in one launch (k_update_fused): three launches
-// per iteration.  Keep and compaction are single-pass decoupled look-back
-// scans (Merrill & Garland, "Single-pass Parallel Prefix Scan with Decoupled
-// Look-back", 2016) with one look-back tile per workgroup.
+// per iteration.  Keep and compaction are single-pass scans with one chunk per
+// workgroup, after the decoupled look-back of Merrill & Garland ("Single-pass
+// Parallel Prefix Scan with Decoupled Look-back", 2016) but with two levels
+// (chunk aggregates and totals of groups of 64 chunks, wave_group_scan)
+// instead of a chain of inclusive prefixes.
 //
 // Emission record C (8 bytes per voxel): byte e of C[u] is 1 iff u was emitted
 // into candidate list e (0:-z 1:-y 2:-x 3:self 4:+x 5:+y 6:+z,
@@ -55,7 +57,8 @@ constexpr int kItems = 8;                     // compaction slots per thread
 constexpr int kScanTile = kBlock * kItems;    // slots per look-back tile
 constexpr int kKeepItems = 2;                 // S positions per thread in the keep step
 constexpr int kKeepTile = kBlock * kKeepItems;
-constexpr int kPosItems = 4;                  // list positions per thread in a few-path compaction
+constexpr int kPosItems = 4;                  // list positions per thread in a few-path compaction (<= 4;
+                                              // 2 measured 4% slower on the phantom, 9% on the vessels)
 constexpr int kPosTile = kBlock * kPosItems;
 constexpr uint32_t kSpecTiles = 64;           // workgroups that load their compaction tile with the counts
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded look-back wait
@@ -366,15 +369,18 @@ __device__ __attribute__((always_inline)) inline void wait_states(const uint64_t
 // (agg[j * C + c], flag 1); the last chunk of a group (r = 63) also publishes
 // the group's totals (grp[g * C + c], flag 2) as soon as it has read the
 // group's other aggregates.  Chunk j's exclusive prefix is the sum of the
-// totals of groups 0..g-2 and of the aggregates of chunks 64(g-1)..j-1
-// (reading the previous group's aggregates directly spares the chunks of
-// group 1 the wait for group 0's total), written to pre[c].  Every chunk
+// totals of the earlier groups and of the aggregates of chunks 64g..j-1,
+// written to pre[c].  With kPrevDirect the previous group is read through
+// its chunks' aggregates instead of its total, which spares the chunks of
+// group 1 the wait for group 0's total at the price of 64 C more loads per
+// chunk (measured: without it the phantom run is 2.4% slower, the vessels
+// run unchanged, for seven counts as for one).  Every chunk
 // resolves after at most two rounds of publication,
 // whatever its index (a chained look-back, where inclusive prefixes travel
 // one 64-chunk window per round, left chunk 400 of a 428-chunk compaction
 // waiting ~17 us), and waits only on chunks of lower workgroups.  Values are
 // 32-bit counts.
-template <int C>
+template <int C, bool kPrevDirect>
 __device__ __attribute__((always_inline)) inline void wave_group_scan(uint64_t* agg, uint64_t* grp, uint32_t chunk, const uint32_t* tot, uint32_t epoch,
                                 uint32_t* err, uint32_t* pre)
 {
@@ -389,7 +395,9 @@ __device__ __attribute__((always_inline)) inline void wave_group_scan(uint64_t* 
     if (lane < (uint32_t)C) st_state(&agg[(uint64_t)chunk * C + lane], stamp | (1ull << 32) | mine);
     // this group's earlier chunks, the previous group's chunks and the totals
     // of the 64 groups before that, loaded together
-    const bool need_own = lane < r, need_prev = g > 0, need_grp = lane + 1u < g;
+    const bool need_own = lane < r, need_prev = kPrevDirect && g > 0;
+    const uint32_t ng = kPrevDirect && g > 0 ? g - 1u : g;  // groups read through their totals
+    const bool need_grp = lane < ng;
     const uint64_t* ow = agg + ((uint64_t)g * 64u + lane) * C;
     const uint64_t* pw = g ? agg + ((uint64_t)(g - 1u) * 64u + lane) * C : agg;
     const uint64_t* gw = grp + (uint64_t)lane * C;
@@ -413,9 +421,9 @@ __device__ __attribute__((always_inline)) inline void wave_group_scan(uint64_t* 
     wait_states<C>(pw, need_prev, ep, sp, spins, err);
 #pragma unroll
     for (int c = 0; c < C; ++c) prior[c] = wave_sum(need_prev ? (uint32_t)sp[c] : 0u);
-    // the totals of groups 0..g-2, 64 groups per round
-    for (uint32_t g0 = 0; g0 + 1u < g; g0 += 64) {
-        const bool need = g0 + lane + 1u < g;
+    // the totals of groups 0..ng-1, 64 groups per round
+    for (uint32_t g0 = 0; g0 < ng; g0 += 64) {
+        const bool need = g0 + lane < ng;
         const uint64_t* w = grp + ((uint64_t)g0 + lane) * C;
         if (g0) {
 #pragma unroll
@@ -434,7 +442,7 @@ __device__ __attribute__((always_inline)) inline void wave_group_scan(uint64_t* 
 // thread gets the exclusive prefix (through *s_excl)
 __device__ inline uint32_t lookback(const KArgs& a, uint32_t chunk, uint32_t total, uint32_t epoch, uint32_t* s_excl)
 {
-    if (threadIdx.x < 64) wave_group_scan<1>(a.tstate, a.gstate, chunk, &total, epoch, &a.ctl->error, s_excl);
+    if (threadIdx.x < 64) wave_group_scan<1, true>(a.tstate, a.gstate, chunk, &total, epoch, &a.ctl->error, s_excl);
     __syncthreads();
     return *s_excl;
 }
@@ -710,8 +718,8 @@ __device__ inline void block_add(uint64_t* acc, uint32_t c, uint32_t* s_red)
 
 // per-tile state of a few-path compaction (by list positions)
 struct PosShared {
-    uint32_t cnt[7 * kPosItems * 4];   // kept candidates per direction, item and wave
-    uint32_t excl[7 * kPosItems * 4];  // their exclusive prefix within the tile, per direction
+    uint32_t cnt[7 * 16];   // kept candidates per direction (16 each), item and wave
+    uint32_t excl[7 * 16];  // their exclusive prefix within the tile, per direction
     uint32_t tot[7];                   // the tile's kept candidates per direction
     uint32_t chain[7];                 // the chunk's, summed over its tiles
     uint32_t pre[7];                   // kept candidates of direction e in the chunks before
@@ -921,7 +929,8 @@ __device__ inline void compact_pos_bases(const uint32_t* d, PosShared& ps)
 // ps.excl, ps.tot
 __device__ __attribute__((always_inline)) inline void compact_pos_counts(const uint32_t* kb, PosShared& ps)
 {
-    constexpr int kSeg = kPosItems * 4;  // counts per direction
+    constexpr int kSeg = 16;  // counts per direction: items (at most 4) x waves, the rest zero
+    static_assert(kPosItems <= 4, "the counts of a direction fill one row of 16 lanes");
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
 #pragma unroll
@@ -929,15 +938,14 @@ __device__ __attribute__((always_inline)) inline void compact_pos_counts(const u
 #pragma unroll
         for (int j = 0; j < kPosItems; ++j) {
             const uint32_t c = (uint32_t)__popcll(__ballot((kb[j] >> e) & 1u));
-            if (lane == 0) ps.cnt[(e * kPosItems + j) * 4 + wave] = c;
+            if (lane == 0) ps.cnt[e * 16 + j * 4 + wave] = c;
         }
     __syncthreads();
     if (threadIdx.x < 64) {
 #pragma unroll
         for (int r = 0; r < (7 * kSeg + 63) / 64; ++r) {
             const int i = r * 64 + lane;
-            const uint32_t cc = i < 7 * kSeg ? ps.cnt[i] : 0u;
-            static_assert(kSeg == 16, "one direction per row of 16 lanes");
+            const uint32_t cc = (i < 7 * kSeg && (i % kSeg) / 4 < kPosItems) ? ps.cnt[i] : 0u;
             const uint32_t v = row_inclusive_scan(cc);
             if (i < 7 * kSeg) {
                 ps.excl[i] = v - cc;
@@ -965,7 +973,7 @@ __device__ __attribute__((always_inline)) inline void compact_pos_place(const KA
             const bool k = ((kb[j] >> e) & 1u) != 0;
             const uint64_t bal = __ballot(k);
             if (k) {
-                const uint32_t at = r0 + ps.excl[(e * kPosItems + j) * 4 + wave] + (uint32_t)__popcll(bal & lt);
+                const uint32_t at = r0 + ps.excl[e * 16 + j * 4 + wave] + (uint32_t)__popcll(bal & lt);
                 const uint32_t v = neighbor_in(a.g, vx[j], (uint32_t)e);
                 out[at] = v;
                 a.pos[v] = at;
@@ -1007,7 +1015,7 @@ __device__ inline bool compact_pos_chunk(const KArgs& a, int q, const CompactPla
         if (threadIdx.x < 7) ps.chain[threadIdx.x] = acc;
     }
     __syncthreads();
-    if (threadIdx.x < 64) wave_group_scan<7>(a.tstate, a.gstate, chunk, ps.chain, epoch, &a.ctl->error, ps.pre);
+    if (threadIdx.x < 64) wave_group_scan<7, true>(a.tstate, a.gstate, chunk, ps.chain, epoch, &a.ctl->error, ps.pre);
     __syncthreads();
     bool over = false;
 #pragma unroll
