@@ -55,7 +55,8 @@ constexpr float kConnTol = 0.1f;              // UpdateLevelSetVolume.cu:282-287
 constexpr int kBlock = 256;                   // threads per workgroup (4 waves of 64)
 constexpr int kItems = 8;                     // compaction slots per thread
 constexpr int kScanTile = kBlock * kItems;    // slots per look-back tile
-constexpr int kKeepItems = 2;                 // S positions per thread in the keep step
+constexpr int kKeepItems = 1;                 // S positions per thread in the keep step (measured:
+                                              // 2 costs 4% on the phantom, 4 costs 10%)
 constexpr int kKeepTile = kBlock * kKeepItems;
 constexpr int kPosItems = 4;                  // list positions per thread in a few-path compaction (<= 4;
                                               // 2 measured 4% slower on the phantom, 9% on the vessels)
