@@ -57,7 +57,9 @@ constexpr int kItems = 8;                     // compaction slots per thread
 constexpr int kScanTile = kBlock * kItems;    // slots per look-back tile
 constexpr int kKeepItems = 1;                 // S positions per thread in the keep step (measured:
                                               // 2 costs 4% on the phantom, 4 costs 10%)
-constexpr int kKeepTile = kBlock * kKeepItems;
+constexpr int kKeepItemsBatch = 2;            // the same in the batched keep step (1 costs 5% on
+                                              // 64 volumes: its scans share one grid)
+constexpr int kKeepTile = kBlock * kKeepItems;  // the smallest keep tile (state sizing)
 constexpr int kPosItems = 4;                  // list positions per thread in a few-path compaction (<= 4;
                                               // 2 measured 4% slower on the phantom, 9% on the vessels)
 constexpr int kPosTile = kBlock * kPosItems;
@@ -1136,14 +1138,16 @@ __device__ __attribute__((always_inline)) inline bool compact_preload(const KArg
 // exactly once; when the compaction takes the dense path it is also marked in
 // `memb`.  The last chunk sets the count of the next list and the statistics.
 // ---------------------------------------------------------------------------
+template <int kI>
 struct KeepShared {
-    uint32_t cnt[kKeepItems * 4];
-    uint32_t excl[kKeepItems * 4];
+    uint32_t cnt[kI * 4];
+    uint32_t excl[kI * 4];
     uint32_t dir[2][7 * 4];  // kept candidates per direction and wave, by sub-tile parity
     uint32_t prefix, total;
 };
 
-__device__ inline uint32_t keep_tiles(uint32_t n) { return (n + kKeepTile - 1) / kKeepTile; }
+template <int kI>
+__device__ inline uint32_t keep_tiles(uint32_t n) { return (n + kBlock * kI - 1) / (kBlock * kI); }
 
 // this lane's share of the candidate counters (run by the 64 lanes of a wave)
 __device__ inline uint64_t candidate_share(const KArgs& a)
@@ -1193,12 +1197,13 @@ __device__ inline void keep_finish(const KArgs& a, int q, uint32_t n, uint32_t m
 // the traversal entries of keep sub-tile `sub` of list q and their emission
 // masks; positions are bounded by N (every one a valid address), and
 // keep_decide discards the entries at or beyond the count
+template <int kI>
 __device__ __attribute__((always_inline)) inline void keep_voxels(const KArgs& a, int q, uint32_t sub, uint32_t* us,
                                                                   uint32_t* ms)
 {
-    const uint64_t base = (uint64_t)sub * kKeepTile;
+    const uint64_t base = (uint64_t)sub * (kBlock * kI);
 #pragma unroll
-    for (int j = 0; j < kKeepItems; ++j) {
+    for (int j = 0; j < kI; ++j) {
         const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
         const uint64_t ic = i < a.g.N ? i : 0;
         us[j] = a.S[q][ic];
@@ -1209,30 +1214,31 @@ __device__ __attribute__((always_inline)) inline void keep_voxels(const KArgs& a
 // the kept directions kb of the voxels us (emission masks ms) of keep sub-tile
 // `sub`, written to keep[pos[u]] as well; entries past the count become voxel
 // 0 with kb = 0
+template <int kI>
 __device__ __attribute__((always_inline)) inline void keep_decide(const KArgs& a, uint32_t n, uint32_t sub,
                                                                   uint32_t* us, const uint32_t* ms, uint32_t* kb)
 {
     const uint64_t* __restrict__ Ck = a.C;
-    const uint64_t base = (uint64_t)sub * kKeepTile;
+    const uint64_t base = (uint64_t)sub * (kBlock * kI);
     // the positions and the records of the emitted candidates together (other
     // lanes load the voxel's own record and ignore it)
-    uint32_t ps[kKeepItems];
+    uint32_t ps[kI];
 #pragma unroll
-    for (int j = 0; j < kKeepItems; ++j)
+    for (int j = 0; j < kI; ++j)
         if (base + (uint64_t)j * kBlock + threadIdx.x >= n) us[j] = 0;
 #pragma unroll
-    for (int j = 0; j < kKeepItems; ++j) ps[j] = a.pos[us[j]];
+    for (int j = 0; j < kI; ++j) ps[j] = a.pos[us[j]];
     // a -z candidate (e = 0) comes first in any record, so only e >= 1 needs one
-    uint64_t rec[kKeepItems][7];
+    uint64_t rec[kI][7];
 #pragma unroll
-    for (int j = 0; j < kKeepItems; ++j) {
+    for (int j = 0; j < kI; ++j) {
         const bool valid = base + (uint64_t)j * kBlock + threadIdx.x < n;
         const uint32_t m = valid ? ms[j] : 0u;
 #pragma unroll
         for (int e = 1; e < 7; ++e) rec[j][e] = Ck[((m >> e) & 1u) ? neighbor_in(a.g, us[j], (uint32_t)e) : us[j]];
     }
 #pragma unroll
-    for (int j = 0; j < kKeepItems; ++j) {
+    for (int j = 0; j < kI; ++j) {
         const bool valid = base + (uint64_t)j * kBlock + threadIdx.x < n;
         const uint32_t m = valid ? ms[j] : 0u;
         uint32_t k = m & 1u;
@@ -1247,13 +1253,14 @@ __device__ __attribute__((always_inline)) inline void keep_decide(const KArgs& a
 
 // offsets of the kept voxels of a sub-tile: ex within the wave, sh.excl per
 // wave and item; returns the sub-tile's total in every thread
+template <int kI>
 __device__ __attribute__((always_inline)) inline uint32_t keep_counts(const uint32_t* kb, uint32_t* ex,
-                                                                      KeepShared& sh)
+                                                                      KeepShared<kI>& sh)
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int j = 0; j < kKeepItems; ++j) {
+    for (int j = 0; j < kI; ++j) {
         const uint32_t c = (uint32_t)__popc(kb[j]);
         const uint32_t incl = wave_inclusive_scan(c, lane);
         ex[j] = incl - c;
@@ -1261,9 +1268,9 @@ __device__ __attribute__((always_inline)) inline uint32_t keep_counts(const uint
     }
     __syncthreads();
     if (threadIdx.x < 64) {
-        const uint32_t cc = lane < kKeepItems * 4 ? sh.cnt[lane] : 0u;
+        const uint32_t cc = lane < kI * 4 ? sh.cnt[lane] : 0u;
         const uint32_t incl = wave_inclusive_scan(cc, lane);
-        if (lane < kKeepItems * 4) sh.excl[lane] = incl - cc;
+        if (lane < kI * 4) sh.excl[lane] = incl - cc;
         if (lane == 63) sh.total = incl;
     }
     __syncthreads();
@@ -1275,19 +1282,21 @@ __device__ __attribute__((always_inline)) inline uint32_t keep_counts(const uint
 // keep_dirs_count before keep_counts (whose barriers publish sh.dir),
 // keep_dirs_add after it.  Buffer `buf` alternates between sub-tiles: a wave
 // may count the next sub-tile while wave 0 still adds this one's.
-__device__ __attribute__((always_inline)) inline void keep_dirs_count(const uint32_t* kb, KeepShared& sh, int buf)
+template <int kI>
+__device__ __attribute__((always_inline)) inline void keep_dirs_count(const uint32_t* kb, KeepShared<kI>& sh, int buf)
 {
     const int wave = threadIdx.x >> 6;
 #pragma unroll
     for (int e = 0; e < 7; ++e) {
         uint32_t c = 0;
 #pragma unroll
-        for (int j = 0; j < kKeepItems; ++j) c += (uint32_t)__popcll(__ballot((kb[j] >> e) & 1u));
+        for (int j = 0; j < kI; ++j) c += (uint32_t)__popcll(__ballot((kb[j] >> e) & 1u));
         if ((threadIdx.x & 63) == 0) sh.dir[buf][e * 4 + wave] = c;
     }
 }
 
-__device__ __attribute__((always_inline)) inline void keep_dirs_add(const KArgs& a, int q, const KeepShared& sh,
+template <int kI>
+__device__ __attribute__((always_inline)) inline void keep_dirs_add(const KArgs& a, int q, const KeepShared<kI>& sh,
                                                                     int buf)
 {
     if (threadIdx.x < 7) {
@@ -1298,15 +1307,16 @@ __device__ __attribute__((always_inline)) inline void keep_dirs_add(const KArgs&
 }
 
 // appends the kept voxels of a counted sub-tile to the next S from `prefix` on
+template <int kI>
 __device__ __attribute__((always_inline)) inline void keep_append(const KArgs& a, int q, bool dense,
                                                                   const uint32_t* us, const uint32_t* kb,
                                                                   const uint32_t* ex, uint32_t prefix,
-                                                                  KeepShared& sh)
+                                                                  KeepShared<kI>& sh)
 {
     uint32_t* __restrict__ Sn = a.S[q ^ 1];
     const int wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int j = 0; j < kKeepItems; ++j) {
+    for (int j = 0; j < kI; ++j) {
         uint32_t off = prefix + sh.excl[j * 4 + wave] + ex[j];
         const uint32_t k = kb[j];
         if (k) {
@@ -1326,31 +1336,32 @@ __device__ __attribute__((always_inline)) inline void keep_append(const KArgs& a
 // emission masks of its first sub-tile.  With several sub-tiles the first pass
 // replaces each emission mask by the kept directions for the second.  Returns
 // false (with the error flag set) if the next list would overflow.
+template <int kI>
 __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a, int q, uint32_t n,
                                                                  const ChunkPlan& cp, uint32_t chunk,
-                                                                 uint32_t epoch, KeepShared& sh, uint32_t* us,
+                                                                 uint32_t epoch, KeepShared<kI>& sh, uint32_t* us,
                                                                  uint32_t* ms)
 {
     const bool dense = !path_is_few(n, a.g.N);
-    const uint32_t s0 = chunk * cp.r, s1 = min(s0 + cp.r, keep_tiles(n));
+    const uint32_t s0 = chunk * cp.r, s1 = min(s0 + cp.r, keep_tiles<kI>(n));
     const bool last = chunk == cp.chunks - 1;
     // the last chunk loads the candidate counters and the control fields now
     const uint64_t cshare = (last && threadIdx.x < 64) ? candidate_share(a) : 0ull;
     const CtlSnapshot snap = last ? ctl_snapshot(a.ctl) : CtlSnapshot{0, 0, 0, 0};
-    uint32_t kb[kKeepItems], ex[kKeepItems];
+    uint32_t kb[kI], ex[kI];
     uint32_t total = 0;
     if (cp.r == 1) {
-        keep_decide(a, n, s0, us, ms, kb);
+        keep_decide<kI>(a, n, s0, us, ms, kb);
         keep_dirs_count(kb, sh, 0);
         total = keep_counts(kb, ex, sh);
         keep_dirs_add(a, q, sh, 0);
     } else {
         for (uint32_t sub = s0; sub < s1; ++sub) {
-            keep_voxels(a, q, sub, us, ms);
-            keep_decide(a, n, sub, us, ms, kb);
-            const uint64_t base = (uint64_t)sub * kKeepTile;
+            keep_voxels<kI>(a, q, sub, us, ms);
+            keep_decide<kI>(a, n, sub, us, ms, kb);
+            const uint64_t base = (uint64_t)sub * (kBlock * kI);
 #pragma unroll
-            for (int j = 0; j < kKeepItems; ++j) {
+            for (int j = 0; j < kI; ++j) {
                 const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
                 if (i < n) a.emit[i] = (uint8_t)kb[j];  // read back by the second pass
             }
@@ -1369,10 +1380,10 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
     } else {
         uint32_t run = prefix;
         for (uint32_t sub = s0; sub < s1; ++sub) {
-            keep_voxels(a, q, sub, us, ms);  // ms: the kept directions of the first pass
-            const uint64_t base = (uint64_t)sub * kKeepTile;
+            keep_voxels<kI>(a, q, sub, us, ms);  // ms: the kept directions of the first pass
+            const uint64_t base = (uint64_t)sub * (kBlock * kI);
 #pragma unroll
-            for (int j = 0; j < kKeepItems; ++j) kb[j] = base + (uint64_t)j * kBlock + threadIdx.x < n ? ms[j] : 0u;
+            for (int j = 0; j < kI; ++j) kb[j] = base + (uint64_t)j * kBlock + threadIdx.x < n ? ms[j] : 0u;
             const uint32_t t = keep_counts(kb, ex, sh);
             keep_append(a, q, dense, us, kb, ex, run, sh);
             run += t;
@@ -1385,12 +1396,12 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
 
 __global__ __launch_bounds__(kBlock) void k_keep(KArgs a, int q)
 {
-    __shared__ KeepShared sh;
+    __shared__ KeepShared<kKeepItems> sh;
     uint32_t us[kKeepItems], ms[kKeepItems];
-    keep_voxels(a, q, blockIdx.x, us, ms);  // with the count: this workgroup's sub-tile when chunks are single
+    keep_voxels<kKeepItems>(a, q, blockIdx.x, us, ms);  // with the count: this workgroup's sub-tile when chunks are single
     const uint32_t epoch = a.ctl->epoch;
     const uint32_t n = a.ctl->n[q];
-    const ChunkPlan cp = chunk_plan(keep_tiles(n), gridDim.x);
+    const ChunkPlan cp = chunk_plan(keep_tiles<kKeepItems>(n), gridDim.x);
     if (blockIdx.x < cp.chunks) (void)keep_chunk(a, q, n, cp, blockIdx.x, epoch, sh, us, ms);
     if (cp.chunks == 0 && blockIdx.x == 0 && threadIdx.x < 64)
         keep_finish(a, q, 0, 0, epoch, candidate_share(a), ctl_snapshot(a.ctl));
@@ -1632,22 +1643,22 @@ __device__ inline uint32_t batch_chunk_r(uint32_t total, uint32_t count, uint32_
 
 __global__ __launch_bounds__(kBlock) void k_keep_batch(BatchArgs b, int q)
 {
-    __shared__ KeepShared sh;
+    __shared__ KeepShared<kKeepItemsBatch> sh;
     __shared__ uint32_t s_pre[kMaxBatch + 1];
     __shared__ uint32_t s_tmp[kBlock / 64];
-    batch_prefix(b.count, [&](uint32_t v) { return keep_tiles(b.ctls[v].n[q]); }, s_pre, s_tmp);
+    batch_prefix(b.count, [&](uint32_t v) { return keep_tiles<kKeepItemsBatch>(b.ctls[v].n[q]); }, s_pre, s_tmp);
     const uint32_t r = batch_chunk_r(s_pre[b.count], b.count, gridDim.x);
     __syncthreads();
-    batch_prefix(b.count, [&](uint32_t v) { return (keep_tiles(b.ctls[v].n[q]) + r - 1) / r; }, s_pre, s_tmp);
+    batch_prefix(b.count, [&](uint32_t v) { return (keep_tiles<kKeepItemsBatch>(b.ctls[v].n[q]) + r - 1) / r; }, s_pre, s_tmp);
     const uint32_t j = blockIdx.x;
     if (j < s_pre[b.count]) {
         const uint32_t v = batch_find(s_pre, b.count, j);
         const KArgs& a = b.vols[v];
         const uint32_t n = b.ctls[v].n[q];
-        const ChunkPlan cp{r, (keep_tiles(n) + r - 1) / r};
+        const ChunkPlan cp{r, (keep_tiles<kKeepItemsBatch>(n) + r - 1) / r};
         const uint32_t chunk = j - s_pre[v];
-        uint32_t us[kKeepItems], ms[kKeepItems];
-        keep_voxels(a, q, chunk * r, us, ms);
+        uint32_t us[kKeepItemsBatch], ms[kKeepItemsBatch];
+        keep_voxels<kKeepItemsBatch>(a, q, chunk * r, us, ms);
         (void)keep_chunk(a, q, n, cp, chunk, b.ctls[v].epoch, sh, us, ms);
     }
     // volumes with an empty list still advance their counts
