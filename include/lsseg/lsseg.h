@@ -332,8 +332,11 @@ lsseg_status lsseg_batch_autotune(lsseg_batch* batch, const lsseg_params* params
 
 /* Runs `iters` iterations of lsseg_iterate from the current state (which
    lsseg_set_phi normally made: both buffers = phi0) and records them.  The
-   context ends in the same state as after lsseg_iterate.  Uses checkpoint slot
-   LSSEG_MAX_CHECKPOINTS - 1. */
+   context ends in the same state as after lsseg_iterate.  One run records; the
+   record's size is known only on the device, so the run starts from a guess
+   (the context's last record, scaled) and runs again with the exact size when
+   the guess was short.  Uses checkpoint slot LSSEG_MAX_CHECKPOINTS - 1.
+   Recorded lists are returned in the tiled index tau, like lsseg_get_active. */
 lsseg_status lsseg_fwd_record(lsseg_ctx* ctx, const lsseg_params* params, int32_t iters,
                               int32_t checkpoint_every, lsseg_record** out);
 /* A record may outlive its context: destroying the context detaches the
