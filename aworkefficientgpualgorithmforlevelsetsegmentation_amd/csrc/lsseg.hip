@@ -1635,7 +1635,8 @@ struct lsseg_record {
     uint32_t* lists = nullptr;
     uint8_t* dec = nullptr;
     std::vector<float*> ckA, ckB;     // (read, write) buffers at k = 0, every, 2 every, ...
-    float* seg = nullptr;             // read buffers A_k of one checkpoint segment
+    float* undo = nullptr;            // values a segment's replay overwrites (n_k per iteration)
+    uint64_t undo_cap = 0;
     float* work[2] = {nullptr, nullptr};
     float* adj[2] = {nullptr, nullptr};
     float* lin = nullptr;             // unpadded staging
@@ -1687,12 +1688,13 @@ LSSEG_DTYPE_KERNEL(k_adjoint)
 
 uint32_t list_grid(uint64_t n) { return dense_grid(std::max<uint64_t>(n, 1)); }
 
-void launch_replay(lsseg_ctx* c, lsseg_record* r, const KArgs& ra, int k, const float* R, float* W)
+void launch_replay(lsseg_ctx* c, lsseg_record* r, const KArgs& ra, int k, const float* R, float* W,
+                   float* undo = nullptr)
 {
     const uint32_t* V = r->lists + r->off[k];
     const uint8_t* D = r->dec + r->off[k];
     uint32_t n = r->n[k];
-    void* args[] = {(void*)&ra, (void*)&V, (void*)&D, (void*)&n, (void*)&R, (void*)&W};
+    void* args[] = {(void*)&ra, (void*)&V, (void*)&D, (void*)&n, (void*)&R, (void*)&W, (void*)&undo};
     (void)hipLaunchKernel(k_replay_kernel(c->dtype), dim3(list_grid(n)), dim3(kBlock), args, 0, c->stream);
 }
 
@@ -1834,13 +1836,12 @@ lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t ite
     r->params = *prm;
     // everything but the lists in one allocation
     const int nck = (iters + every - 1) / every;
-    const int seg = std::min(every, iters);
     const size_t NF = N * sizeof(float);
     RecordLayout lay;
     const size_t o_prm = lay.take(sizeof(DevParams)), o_prm_replay = lay.take(sizeof(DevParams));
     const size_t o_theta = lay.take(kCountSlots * kCountStride * sizeof(double));
     const size_t o_offs = lay.take(((size_t)iters + 1) * sizeof(uint64_t));
-    const size_t o_ck = lay.take(2 * (size_t)nck * NF), o_seg = lay.take((size_t)seg * NF);
+    const size_t o_ck = lay.take(2 * (size_t)nck * NF);
     const size_t o_work = lay.take(4 * NF), o_lin = lay.take((size_t)c->nx * c->ny * c->nz * sizeof(float));
     void* p = nullptr;
     if ((st = ralloc(c, r, &p, lay.end)) != LSSEG_OK) return bail(st);
@@ -1853,7 +1854,6 @@ lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t ite
         r->ckA.push_back((float*)(base + o_ck + (size_t)(2 * i) * NF));
         r->ckB.push_back((float*)(base + o_ck + (size_t)(2 * i + 1) * NF));
     }
-    r->seg = (float*)(base + o_seg);
     for (int i = 0; i < 2; ++i) {
         r->work[i] = (float*)(base + o_work + (size_t)i * NF);
         r->adj[i] = (float*)(base + o_work + (size_t)(2 + i) * NF);
@@ -1959,29 +1959,50 @@ lsseg_status lsseg_bwd(lsseg_ctx* c, lsseg_record* r, const float* dphi, float* 
     const KArgs ra = record_args(c, r->d_prm);
     int ab = 0;  // adj[ab] = Abar, adj[ab ^ 1] = Bbar
     const int nck = (int)r->ckA.size();
+    // undo log for the longest checkpoint segment
+    uint64_t need = 0;
+    for (int sgi = 0; sgi < nck; ++sgi) {
+        const int k0 = sgi * r->every, k1 = std::min(r->iters, k0 + r->every);
+        need = std::max<uint64_t>(need, r->off[k1] - r->off[k0]);
+    }
+    if (need > r->undo_cap) {
+        void* p = nullptr;
+        if ((st = ralloc(c, r, &p, need * sizeof(float))) != LSSEG_OK) return st;
+        r->undo = (float*)p;
+        r->undo_cap = need;
+    }
     for (int sgi = nck - 1; sgi >= 0; --sgi) {
         const int k0 = sgi * r->every, k1 = std::min(r->iters, k0 + r->every);
-        // recompute the read buffers A_k of the segment from its checkpoint
+        // replay the segment from its checkpoint up to the state before its
+        // last iteration, keeping the values each replay overwrites
         HIP_TRY(c, hipMemcpyAsync(r->work[0], r->ckA[sgi], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
         HIP_TRY(c, hipMemcpyAsync(r->work[1], r->ckB[sgi], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
         int rd = 0;
-        for (int k = k0; k < k1; ++k) {
-            HIP_TRY(c, hipMemcpyAsync(r->seg + (size_t)(k - k0) * N, r->work[rd], N * sizeof(float),
-                                      hipMemcpyDeviceToDevice, c->stream));
-            if (k + 1 < k1) launch_replay(c, r, ra, k, r->work[rd], r->work[rd ^ 1]);
+        for (int k = k0; k + 1 < k1; ++k) {
+            launch_replay(c, r, ra, k, r->work[rd], r->work[rd ^ 1],
+                          k > k0 ? r->undo + (r->off[k] - r->off[k0]) : nullptr);
             rd ^= 1;
         }
+        // iteration k reads A_k = work[rd]; stepping back, A_{k-1} is the other
+        // buffer and the write buffer of iteration k-1 is work[rd] with that
+        // iteration's overwritten values put back (by the adjoint launch of
+        // k-1, which reads only A_{k-1}).  The segment's first write buffer is
+        // never read again, so its values are neither kept nor restored.
         for (int k = k1 - 1; k >= k0; --k) {
             const uint32_t* V = r->lists + r->off[k];
             const uint8_t* D = r->dec + r->off[k];
             uint32_t n = r->n[k];
-            const float* A = r->seg + (size_t)(k - k0) * N;
+            const float* A = r->work[rd];
             float* X = r->adj[ab];
             float* Y = r->adj[ab ^ 1];
             double* th = r->d_theta;
-            void* args[] = {(void*)&ra, (void*)&V, (void*)&D, (void*)&n, (void*)&A, (void*)&X, (void*)&Y, (void*)&th};
+            float* Bres = r->work[rd ^ 1];
+            const float* U = (k > k0 && k + 1 < k1) ? r->undo + (r->off[k] - r->off[k0]) : nullptr;
+            void* args[] = {(void*)&ra, (void*)&V, (void*)&D, (void*)&n, (void*)&A,
+                            (void*)&X, (void*)&Y, (void*)&th, (void*)&Bres, (void*)&U};
             (void)hipLaunchKernel(k_adjoint_kernel(c->dtype), dim3(list_grid(n)), dim3(kBlock), args, 0, c->stream);
             ab ^= 1;
+            rd ^= 1;
         }
     }
     // phi0 fed both buffers: its adjoint is Abar_0 + Bbar_0

@@ -1772,10 +1772,14 @@ __global__ __launch_bounds__(kBlock) void k_record(KArgs a, int q, uint32_t k, u
     }
 }
 
+// One recorded update from R into W.  With `undo` the values of W it
+// overwrites are kept (undo[p] for list position p), so that a backward sweep
+// can step the buffer pair back without a copy of the volume.
 template <typename SrcT, bool kFloatSrc>
 __global__ __launch_bounds__(kBlock) void k_replay(KArgs a, const uint32_t* __restrict__ V,
                                                    const uint8_t* __restrict__ dec, uint32_t n,
-                                                   const float* __restrict__ R, float* __restrict__ W)
+                                                   const float* __restrict__ R, float* __restrict__ W,
+                                                   float* __restrict__ undo)
 {
     const DevParams P = *a.prm;
     LSSEG_FOR_EACH(p, n)
@@ -1783,6 +1787,7 @@ __global__ __launch_bounds__(kBlock) void k_replay(KArgs a, const uint32_t* __re
         const uint32_t u = V[p];
         uint32_t nb[7], inb, d = dec[p];
         neighbors(a.g, u, nb, &inb);
+        if (undo) undo[p] = W[u];
         W[u] = update_core<kDecideReplay>(P, R[nb[0]], R[nb[1]], R[nb[2]], R[u], R[nb[4]], R[nb[5]], R[nb[6]],
                                           density_replay<SrcT, kFloatSrc>(a, P, u, d), &d);
     }
@@ -1833,7 +1838,8 @@ template <typename SrcT, bool kFloatSrc>
 __global__ __launch_bounds__(kBlock) void k_adjoint(KArgs a, const uint32_t* __restrict__ V,
                                                     const uint8_t* __restrict__ dec, uint32_t n,
                                                     const float* __restrict__ A, float* __restrict__ X,
-                                                    float* __restrict__ Y, double* theta)
+                                                    float* __restrict__ Y, double* theta,
+                                                    float* __restrict__ Bres, const float* __restrict__ undo)
 {
     __shared__ double s_red[3 * kBlock / 64];
     const DevParams P = *a.prm;
@@ -1843,6 +1849,9 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(KArgs a, const uint32_t* __r
     {
         const uint32_t u = V[p];
         const uint32_t d = dec[p];
+        // the write buffer of iteration k back to its values before the update
+        // (undo log of the replay); this kernel reads the other buffer, A
+        if (undo) Bres[u] = undo[p];
         const float gbar = X[u];
         X[u] = 0.0f;
         if ((d & (kDecLo1 | kDecHi1 | kDecLo2 | kDecHi2)) || gbar == 0.0f) continue;
