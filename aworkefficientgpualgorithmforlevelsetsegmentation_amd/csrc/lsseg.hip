@@ -1681,7 +1681,6 @@ lsseg_status ralloc(lsseg_ctx* c, lsseg_record* r, void** p, size_t bytes)
         default: return (const void*)&name<float, true>;                                  \
         }                                                                                 \
     }
-LSSEG_DTYPE_KERNEL(k_record)
 LSSEG_DTYPE_KERNEL(k_replay)
 LSSEG_DTYPE_KERNEL(k_adjoint)
 #undef LSSEG_DTYPE_KERNEL
@@ -1747,12 +1746,34 @@ lsseg_status record_lists_alloc(lsseg_ctx* c, lsseg_record* r, uint64_t cap)
     return LSSEG_OK;
 }
 
+// the update launch of a recording run (FP32 phi): the decisions of every
+// update go to bits 1-7 of its temporal-flag byte
+const void* update_record_kernel(int32_t dtype, bool fused)
+{
+    switch (dtype) {
+    case LSSEG_U8: return fused ? (const void*)&k_update_fused<uint8_t, false, float, true>
+                                : (const void*)&k_update<uint8_t, false, float, true>;
+    case LSSEG_I8: return fused ? (const void*)&k_update_fused<int8_t, false, float, true>
+                                : (const void*)&k_update<int8_t, false, float, true>;
+    case LSSEG_U16: return fused ? (const void*)&k_update_fused<uint16_t, false, float, true>
+                                 : (const void*)&k_update<uint16_t, false, float, true>;
+    case LSSEG_I16: return fused ? (const void*)&k_update_fused<int16_t, false, float, true>
+                                 : (const void*)&k_update<int16_t, false, float, true>;
+    case LSSEG_U32: return fused ? (const void*)&k_update_fused<uint32_t, false, float, true>
+                                 : (const void*)&k_update<uint32_t, false, float, true>;
+    case LSSEG_I32: return fused ? (const void*)&k_update_fused<int32_t, false, float, true>
+                                 : (const void*)&k_update<int32_t, false, float, true>;
+    default: return fused ? (const void*)&k_update_fused<float, true, float, true>
+                          : (const void*)&k_update<float, true, float, true>;
+    }
+}
+
 // One recording run of `iters` iterations from the context's state: the
 // launches of lsseg_iterate (each compaction in the next update's launch),
-// phi checkpointed every `every` iterations, and the recording of iteration
-// k after its update launch: it reads list[q], which that launch completes,
-// and the read buffer, which the update leaves alone.  V_k and its decisions
-// go to the device offsets r->offs.
+// with phi checkpointed every `every` iterations.  The update keeps each
+// voxel's decisions beside its temporal flag, and the marking, which runs
+// after the compaction that placed list[q], writes V_k in list order with its
+// decisions at the device offsets r->offs.
 lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
 {
     const uint64_t N = c->a.g.N;
@@ -1767,7 +1788,7 @@ lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
     c->h_ctl->many_iters = 0;
     HIP_TRY(c, hipMemcpyAsync(c->a.ctl, c->h_ctl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
     int q = c->q;
-    const uint32_t grid = std::min<uint32_t>(c->sgrid, list_grid(N));
+    const RecordOut ro{r->offs, r->cap, r->lists, r->dec};
     for (int k = 0; k < iters; ++k) {
         if (k % r->every == 0) {
             HIP_TRY(c, hipMemcpyAsync(r->ckA[k / r->every], c->a.phi[q], N * sizeof(float), hipMemcpyDeviceToDevice,
@@ -1775,12 +1796,10 @@ lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
             HIP_TRY(c, hipMemcpyAsync(r->ckB[k / r->every], c->a.phi[q ^ 1], N * sizeof(float),
                                       hipMemcpyDeviceToDevice, c->stream));
         }
-        launch_head(c, q, k > 0);
-        uint32_t kk = (uint32_t)k;
-        void* args[] = {(void*)&c->a, (void*)&q, (void*)&kk, (void*)&r->offs, (void*)&r->cap,
-                        (void*)&r->lists, (void*)&r->dec};
-        (void)hipLaunchKernel(k_record_kernel(c->dtype), dim3(grid), dim3(kBlock), args, 0, c->stream);
-        launch_tail(c, q);
+        launch(c, update_record_kernel(c->dtype, k > 0), k > 0 ? c->grid : c->sgrid, q);
+        void* args[] = {(void*)&c->a, (void*)&q, (void*)&ro};
+        (void)hipLaunchKernel((const void*)&k_mark_record, dim3(c->sgrid), dim3(kBlock), args, 0, c->stream);
+        launch(c, (const void*)&k_keep, c->grid, q);
         q ^= 1;
     }
     if (iters > 0) launch(c, (const void*)&k_compact<kCompact>, c->grid, q ^ 1);

@@ -651,31 +651,39 @@ struct ListWalk {
 // tile; the single-volume kernels and the batched kernels (which take each
 // volume's arguments from a table) share them.
 // ---------------------------------------------------------------------------
-// the update of active voxel u at traversal position p
-template <typename SrcT, bool kFloatSrc, typename PhiT>
+// the update of active voxel u at traversal position p.  The byte moved[p]
+// holds the temporal flag in bit 0 and, with kRec (a recording run of the
+// differentiable form), the update's decisions in bits 1-7.
+template <typename SrcT, bool kFloatSrc, typename PhiT, bool kRec = false>
 __device__ inline void update_at(const KArgs& a, const DevParams& P, const PhiT* __restrict__ R,
                                  PhiT* __restrict__ W, uint32_t u, uint64_t p)
 {
-    uint32_t nb[7], inb;
+    uint32_t nb[7], inb, d = 0;
     neighbors(a.g, u, nb, &inb);
     const float u4 = phi_load(R, u);
-    const float nv = update_core(P, phi_load(R, nb[0]), phi_load(R, nb[1]), phi_load(R, nb[2]), u4,
-                                 phi_load(R, nb[4]), phi_load(R, nb[5]), phi_load(R, nb[6]),
-                                 density_factor<SrcT, kFloatSrc>(a, P, u));
+    const float nv = update_core<kRec ? kDecideRecord : kDecideNone>(
+        P, phi_load(R, nb[0]), phi_load(R, nb[1]), phi_load(R, nb[2]), u4, phi_load(R, nb[4]), phi_load(R, nb[5]),
+        phi_load(R, nb[6]), density_factor<SrcT, kFloatSrc>(a, P, u), &d);
     phi_store(W, u, nv);
     a.E[u] = export_byte(nv);
     a.C[u] = 0ull;  // the record of the candidates that produced this list
     // the temporal flag |phi_W - phi_R| > 0.008 of the value as computed (before
     // a fixed-point store quantises it; UpdateLevelSetVolume.cu:327-331), in
     // traversal order for the marking
-    a.moved[p] = fabsf(nv - u4) > kTemporalThreshold ? 1 : 0;
+    uint32_t mv = fabsf(nv - u4) > kTemporalThreshold ? 1u : 0u;
+    if constexpr (kRec) {
+        if constexpr (kFloatSrc)
+            if (static_cast<const float*>(a.src)[u] - P.Tf < 0.0f) d |= kDecBelow;
+        mv |= d << 1;
+    }
+    a.moved[p] = (uint8_t)mv;
 }
 
 // marking of active voxel u (UpdateActiveElements.cu:75-103) whose update set
 // the temporal flag `moved`; returns its emission mask
 template <typename PhiT>
 __device__ inline uint32_t mark_at(const KArgs& a, bool ref, bool moved, const PhiT* __restrict__ W,
-                                   uint8_t* __restrict__ Cb, uint32_t u)
+                                   uint8_t* __restrict__ Cb, uint32_t u, uint32_t* list_pos = nullptr)
 {
     uint32_t m = 0;
     // the position and every operand are loaded together (all addresses are
@@ -683,6 +691,7 @@ __device__ inline uint32_t mark_at(const KArgs& a, bool ref, bool moved, const P
     uint32_t nb[7], inb;
     neighbors(a.g, u, nb, &inb);
     const uint32_t pos = a.pos[u];
+    if (list_pos) *list_pos = pos;
     const float nv = phi_load(W, u);
     float wn[7];
 #pragma unroll
@@ -1434,19 +1443,19 @@ __device__ inline void walk_list(const KArgs& a, int q, uint32_t first, uint32_t
     }
 }
 
-template <typename SrcT, bool kFloatSrc, typename PhiT>
+template <typename SrcT, bool kFloatSrc, typename PhiT, bool kRec = false>
 __device__ inline void update_walk(const KArgs& a, int q, uint32_t first = 0, uint32_t count = 0)
 {
     const DevParams P = *a.prm;
     walk_list(a, q, first, count, [&](uint32_t u, uint64_t p) {
-        update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), u, p);
+        update_at<SrcT, kFloatSrc, PhiT, kRec>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), u, p);
     });
 }
 
-template <typename SrcT, bool kFloatSrc, typename PhiT>
+template <typename SrcT, bool kFloatSrc, typename PhiT, bool kRec = false>
 __global__ __launch_bounds__(kBlock) void k_update(KArgs a, int q)
 {
-    update_walk<SrcT, kFloatSrc, PhiT>(a, q);
+    update_walk<SrcT, kFloatSrc, PhiT, kRec>(a, q);
 }
 
 // the compaction of list[q ^ 1] into list[q] and the update over S[q].  The
@@ -1454,7 +1463,7 @@ __global__ __launch_bounds__(kBlock) void k_update(KArgs a, int q)
 // no more than the compaction has sub-tiles) take the compaction, the others
 // the update, so neither waits behind the other.  The grid is a multiple of 8
 // and at least 16.
-template <typename SrcT, bool kFloatSrc, typename PhiT>
+template <typename SrcT, bool kFloatSrc, typename PhiT, bool kRec = false>
 __global__ __launch_bounds__(kBlock) void k_update_fused(KArgs a, int q)
 {
     __shared__ CompactShared sh;
@@ -1468,7 +1477,7 @@ __global__ __launch_bounds__(kBlock) void k_update_fused(KArgs a, int q)
         compact_all<kCompact>(a, q ^ 1, c, epoch, sh, gc, kb, vx, d, loaded);
     } else {
         if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kCompact>(a, q ^ 1, c, 0, epoch);
-        update_walk<SrcT, kFloatSrc, PhiT>(a, q, gc, gridDim.x - gc);
+        update_walk<SrcT, kFloatSrc, PhiT, kRec>(a, q, gc, gridDim.x - gc);
     }
 }
 
@@ -1494,18 +1503,59 @@ __global__ __launch_bounds__(kBlock) void k_empty_iterations(KArgs a, uint32_t c
     }
 }
 
-template <typename PhiT>
-__global__ __launch_bounds__(kBlock) void k_mark(KArgs a, int q)
+// Where a recording run of the differentiable form puts iteration k's active
+// list V_k (list order) and the decisions of its updates: from offs[k] on, with
+// offs[0] = 0.  Entries beyond `cap` are not written; the host then sizes the
+// record from offs and records again.
+struct RecordOut {
+    uint64_t* offs;
+    uint64_t cap;
+    uint32_t* lists;
+    uint8_t* dec;
+};
+
+// The marking over S[q]; with kRec also the recording of the iteration, k =
+// ctl->iter: V_k at list positions offs[k] + pos[u] with the decisions that the
+// update left in bits 1-7 of moved[p], and offs[k + 1].
+template <typename PhiT, bool kRec>
+__device__ inline void mark_walk(const KArgs& a, int q, const RecordOut& r)
 {
     __shared__ uint32_t s_red[kBlock / 64];
     const bool ref = a.prm->refMode != 0;
+    uint64_t base = 0;
+    bool rec = false;
+    if constexpr (kRec) {
+        const uint32_t k = a.ctl->iter, n = a.ctl->n[q];
+        base = r.offs[k];
+        if (blockIdx.x == 0 && threadIdx.x == 0) r.offs[k + 1] = base + n;
+        rec = base + n <= r.cap;
+    }
     uint32_t emitted = 0;
     walk_list(a, q, 0, 0, [&](uint32_t u, uint64_t p) {
-        const uint32_t m = mark_at<PhiT>(a, ref, a.moved[p] != 0, phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, u);
+        const uint32_t mv = a.moved[p];
+        uint32_t lp = 0;
+        const uint32_t m =
+            mark_at<PhiT>(a, ref, (mv & 1u) != 0, phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, u, kRec ? &lp : nullptr);
         a.emit[p] = (uint8_t)m;
         emitted += (uint32_t)__popc(m);
+        if constexpr (kRec)
+            if (rec) {
+                r.lists[base + lp] = u;
+                r.dec[base + lp] = (uint8_t)(mv >> 1);
+            }
     });
     block_add(&a.cpart[(blockIdx.x % kCountSlots) * kCountStride], emitted, s_red);
+}
+
+template <typename PhiT>
+__global__ __launch_bounds__(kBlock) void k_mark(KArgs a, int q)
+{
+    mark_walk<PhiT, false>(a, q, RecordOut{});
+}
+
+__global__ __launch_bounds__(kBlock) void k_mark_record(KArgs a, int q, RecordOut r)
+{
+    mark_walk<float, true>(a, q, r);
 }
 
 template <int kMode>
@@ -1620,7 +1670,7 @@ __global__ __launch_bounds__(kBlock) void k_mark_batch(BatchArgs b, int q)
         const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
         uint32_t emitted = 0;
         if (p < b.ctls[v].n[q]) {
-            const uint32_t m = mark_at<PhiT>(a, a.prm->refMode != 0, a.moved[p] != 0, phi_buf<PhiT>(a, q ^ 1),
+            const uint32_t m = mark_at<PhiT>(a, a.prm->refMode != 0, (a.moved[p] & 1u) != 0, phi_buf<PhiT>(a, q ^ 1),
                                              (uint8_t*)a.C, a.S[q][p]);
             a.emit[p] = (uint8_t)m;
             emitted = (uint32_t)__popc(m);
@@ -1740,38 +1790,6 @@ __global__ __launch_bounds__(kBlock) void k_update_fused_batch(BatchArgs b, int 
 // k_adjoint reads and clears X = Abar' on V_k (leaving Bbar) and scatters the
 // J_v^T terms into Y = Bbar' (leaving Abar).
 // ---------------------------------------------------------------------------
-// Iteration k of a recording run appends V_k at offs[k] and sets offs[k + 1]
-// (offs[0] = 0): the sizes come from the device counts, so one run records.
-// Entries beyond `cap` are not written; the host then sizes the record from
-// offs and records again.
-template <typename SrcT, bool kFloatSrc>
-__global__ __launch_bounds__(kBlock) void k_record(KArgs a, int q, uint32_t k, uint64_t* __restrict__ offs,
-                                                   uint64_t cap, uint32_t* __restrict__ lists,
-                                                   uint8_t* __restrict__ dec)
-{
-    const uint32_t n = a.ctl->n[q];
-    const uint64_t base = offs[k];
-    if (blockIdx.x == 0 && threadIdx.x == 0) offs[k + 1] = base + n;
-    if (base + n > cap) return;
-    const DevParams P = *a.prm;
-    const uint32_t* __restrict__ V = a.list[q];
-    const float* __restrict__ R = phi_buf<float>(a, q);  // FP32 storage only
-    lists += base;
-    dec += base;
-    LSSEG_FOR_EACH(p, n)
-    {
-        const uint32_t u = V[p];
-        uint32_t nb[7], inb, d = 0;
-        neighbors(a.g, u, nb, &inb);
-        (void)update_core<kDecideRecord>(P, R[nb[0]], R[nb[1]], R[nb[2]], R[u], R[nb[4]], R[nb[5]], R[nb[6]],
-                                         density_factor<SrcT, kFloatSrc>(a, P, u), &d);
-        if constexpr (kFloatSrc)
-            if (static_cast<const float*>(a.src)[u] - P.Tf < 0.0f) d |= kDecBelow;
-        lists[p] = u;
-        dec[p] = (uint8_t)d;
-    }
-}
-
 // One recorded update from R into W.  With `undo` the values of W it
 // overwrites are kept (undo[p] for list position p), so that a backward sweep
 // can step the buffer pair back without a copy of the volume.
