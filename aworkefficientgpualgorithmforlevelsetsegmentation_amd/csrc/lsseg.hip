@@ -627,11 +627,18 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     g.N = N;
     g.dTX = make_fastdiv(g.TX);
     g.dTXY = make_fastdiv(g.TX * g.TY);
+    // few-path compactions of lists of up to 2^17 positions take tiles of
+    // kBlock positions: four times the workgroups place the list, at the price
+    // of a longer look-back (phantom 31.5 -> 29.6 ms; 2^16 and 2^17 measured
+    // alike, 2^18 slower; tiles of 2 kBlock above gained nothing)
+    g.small_n = (uint32_t)std::max(0, env_int("LSSEG_SMALL_TILE_N", 131072));
+    g.spec_tiles = (uint32_t)std::max(0, env_int("LSSEG_SPEC_TILES", (int)kSpecTiles));
 
     void* p = nullptr;
     // look-back states: one per keep or dense-compaction tile, seven per
     // few-path compaction tile
-    const uint64_t ntiles = std::max<uint64_t>(N / std::min(kScanTile, kKeepTile) + 2, 7 * (N / kPosTile + 2));
+    const uint64_t ntiles = std::max<uint64_t>(N / std::min(kScanTile, kKeepTile) + 2,
+                                               7 * (std::max<uint64_t>(N / kPosTile, g.small_n / kBlock) + 2));
     lsseg_status st;
 #define ALLOC(field, type, count)                                          \
     if ((st = dalloc(c, &p, (size_t)(count) * sizeof(type))) != LSSEG_OK) \
@@ -1478,7 +1485,10 @@ lsseg_status lsseg_batch_iterate(lsseg_batch* b, const lsseg_params* params, int
         if ((st = bforward(b, v, normalise_parity(c))) != LSSEG_OK) return st;
     }
     b->h_table.resize(cnt);
-    for (uint32_t v = 0; v < cnt; ++v) b->h_table[v] = b->vols[v]->a;
+    for (uint32_t v = 0; v < cnt; ++v) {
+        b->h_table[v] = b->vols[v]->a;
+        b->h_table[v].g.small_n = 0;  // the batch's compactions share one grid: one tile size
+    }
     BHIP_TRY(b, hipMemcpyAsync(b->d_table, b->h_table.data(), cnt * sizeof(KArgs), hipMemcpyHostToDevice, b->stream));
     // reset the per-call counters (keep n[], epoch)
     BHIP_TRY(b, hipMemcpyAsync(b->h_ctls, b->d_ctls, cnt * sizeof(DevCtl), hipMemcpyDeviceToHost, b->stream));

@@ -61,9 +61,10 @@ constexpr int kKeepItemsBatch = 2;            // the same in the batched keep st
                                               // 64 volumes: its scans share one grid)
 constexpr int kKeepTile = kBlock * kKeepItems;  // the smallest keep tile (state sizing)
 constexpr int kPosItems = 4;                  // list positions per thread in a few-path compaction (<= 4;
-                                              // 2 measured 4% slower on the phantom, 9% on the vessels)
+                                              // 2 measured 4% slower on the phantom, 9% on the vessels);
+                                              // lists of at most Geom::small_n positions take one
 constexpr int kPosTile = kBlock * kPosItems;
-constexpr uint32_t kSpecTiles = 64;           // workgroups that load their compaction tile with the counts
+constexpr uint32_t kSpecTiles = 256;          // default of Geom::spec_tiles
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded look-back wait
 
 enum : uint32_t { kErrSpin = 1u, kErrOverflow = 2u, kErrCount = 8u };
@@ -110,6 +111,9 @@ struct Geom {
     uint32_t sTY, sTZ;       // tiled strides of one tile row (y) and one tile slab (z)
     uint64_t N;
     FastDiv dTX, dTXY;
+    uint32_t small_n;        // few-path compactions of at most this many list positions take
+                             // tiles of kBlock positions, the others of kPosTile (0: never)
+    uint32_t spec_tiles;     // workgroups of a compaction that load their tile with the counts
 };
 
 // Voxel ids.  Voxel (x, y, z) of the padded volume is stored at
@@ -755,7 +759,8 @@ struct CompactPlan {
     uint32_t n_dst;   // the next list's count, set by the keep step (loaded early)
     bool dense;
     uint64_t slots;   // list positions (few path) or voxels (dense path)
-    uint32_t ntiles;  // sub-tiles of kPosTile positions or kScanTile voxels
+    uint32_t ntiles;  // sub-tiles of kBlock * items positions or kScanTile voxels
+    uint32_t items;   // few path: list positions per thread of a sub-tile (1 or kPosItems)
 };
 
 template <int kMode>
@@ -766,7 +771,8 @@ __device__ inline CompactPlan compact_plan(const Geom& g, uint32_t n_src)
     c.n_dst = 0;
     c.dense = (kMode == kInit) || !path_is_few(n_src, g.N);
     c.slots = c.dense ? g.N : (uint64_t)n_src;
-    const uint32_t tile = c.dense ? kScanTile : kPosTile;
+    c.items = (!c.dense && n_src <= g.small_n) ? 1u : (uint32_t)kPosItems;
+    const uint32_t tile = c.dense ? kScanTile : kBlock * c.items;
     c.ntiles = (uint32_t)((c.slots + tile - 1) / tile);
     return c;
 }
@@ -899,28 +905,31 @@ __device__ inline void dir_clear(const KArgs& a, int q)
     for (int e = 0; e < 7; ++e) *dir_counter(a, q, e, lane) = 0;
 }
 
-// keep bytes and list entries of position tile `tile` of list[q]; positions
-// are bounded by N (every one a valid address), and compact_pos_mask drops the
-// ones at or beyond the count
+// keep bytes and list entries of position tile `tile` (kBlock * items
+// positions, items <= kPosItems) of list[q]; positions are bounded by N (every
+// one a valid address), and compact_pos_mask drops the ones at or beyond the
+// count and the items past `items`
 __device__ __attribute__((always_inline)) inline void compact_pos_load(const KArgs& a, int q, uint32_t tile,
-                                                                       uint32_t* kb, uint32_t* vx)
+                                                                       uint32_t items, uint32_t* kb, uint32_t* vx)
 {
-    const uint64_t base = (uint64_t)tile * kPosTile;
-#pragma unroll
-    for (int j = 0; j < kPosItems; ++j) {
-        const uint64_t p = base + (uint64_t)j * kBlock + threadIdx.x;
-        const uint64_t pc = p < a.g.N ? p : 0;
-        kb[j] = a.keep[pc];
-        vx[j] = a.list[q][pc];
-    }
-}
-
-__device__ __attribute__((always_inline)) inline void compact_pos_mask(uint32_t n, uint32_t tile, uint32_t* kb)
-{
-    const uint64_t base = (uint64_t)tile * kPosTile;
+    const uint64_t base = (uint64_t)tile * kBlock * items;
 #pragma unroll
     for (int j = 0; j < kPosItems; ++j)
-        if (base + (uint64_t)j * kBlock + threadIdx.x >= n) kb[j] = 0;
+        if (j < (int)items) {
+            const uint64_t p = base + (uint64_t)j * kBlock + threadIdx.x;
+            const uint64_t pc = p < a.g.N ? p : 0;
+            kb[j] = a.keep[pc];
+            vx[j] = a.list[q][pc];
+        }
+}
+
+__device__ __attribute__((always_inline)) inline void compact_pos_mask(uint32_t n, uint32_t tile, uint32_t items,
+                                                                       uint32_t* kb)
+{
+    const uint64_t base = (uint64_t)tile * kBlock * items;
+#pragma unroll
+    for (int j = 0; j < kPosItems; ++j)
+        if (j >= (int)items || base + (uint64_t)j * kBlock + threadIdx.x >= n) kb[j] = 0;
 }
 
 // base_e from the direction counters d (each lane's, see dir_load), into
@@ -939,7 +948,8 @@ __device__ inline void compact_pos_bases(const uint32_t* d, PosShared& ps)
 
 // per-direction counts of a loaded tile in position order (item, wave, lane):
 // ps.excl, ps.tot
-__device__ __attribute__((always_inline)) inline void compact_pos_counts(const uint32_t* kb, PosShared& ps)
+__device__ __attribute__((always_inline)) inline void compact_pos_counts(const uint32_t* kb, uint32_t items,
+                                                                         PosShared& ps)
 {
     constexpr int kSeg = 16;  // counts per direction: items (at most 4) x waves, the rest zero
     static_assert(kPosItems <= 4, "the counts of a direction fill one row of 16 lanes");
@@ -948,16 +958,17 @@ __device__ __attribute__((always_inline)) inline void compact_pos_counts(const u
 #pragma unroll
     for (int e = 0; e < 7; ++e)
 #pragma unroll
-        for (int j = 0; j < kPosItems; ++j) {
-            const uint32_t c = (uint32_t)__popcll(__ballot((kb[j] >> e) & 1u));
-            if (lane == 0) ps.cnt[e * 16 + j * 4 + wave] = c;
-        }
+        for (int j = 0; j < kPosItems; ++j)
+            if (j < (int)items) {
+                const uint32_t c = (uint32_t)__popcll(__ballot((kb[j] >> e) & 1u));
+                if (lane == 0) ps.cnt[e * 16 + j * 4 + wave] = c;
+            }
     __syncthreads();
     if (threadIdx.x < 64) {
 #pragma unroll
         for (int r = 0; r < (7 * kSeg + 63) / 64; ++r) {
             const int i = r * 64 + lane;
-            const uint32_t cc = (i < 7 * kSeg && (i % kSeg) / 4 < kPosItems) ? ps.cnt[i] : 0u;
+            const uint32_t cc = (i < 7 * kSeg && (uint32_t)((i % kSeg) / 4) < items) ? ps.cnt[i] : 0u;
             const uint32_t v = row_inclusive_scan(cc);
             if (i < 7 * kSeg) {
                 ps.excl[i] = v - cc;
@@ -970,8 +981,9 @@ __device__ __attribute__((always_inline)) inline void compact_pos_counts(const u
 
 // places the kept candidates of a loaded and counted tile, direction e from
 // list offset ps.run[e] on
-__device__ __attribute__((always_inline)) inline void compact_pos_place(const KArgs& a, int q, const uint32_t* kb,
-                                                                        const uint32_t* vx, const PosShared& ps)
+__device__ __attribute__((always_inline)) inline void compact_pos_place(const KArgs& a, int q, uint32_t items,
+                                                                        const uint32_t* kb, const uint32_t* vx,
+                                                                        const PosShared& ps)
 {
     uint32_t* __restrict__ out = a.list[q ^ 1];
     const int lane = threadIdx.x & 63;
@@ -982,6 +994,7 @@ __device__ __attribute__((always_inline)) inline void compact_pos_place(const KA
         const uint32_t r0 = ps.run[e];
 #pragma unroll
         for (int j = 0; j < kPosItems; ++j) {
+            if (j >= (int)items) break;
             const bool k = ((kb[j] >> e) & 1u) != 0;
             const uint64_t bal = __ballot(k);
             if (k) {
@@ -1009,19 +1022,19 @@ __device__ inline bool compact_pos_chunk(const KArgs& a, int q, const CompactPla
     const bool last = chunk == cp.chunks - 1;
     if (!loaded) {
         if (threadIdx.x < 64) dir_load(a, q, d);
-        if (cp.r == 1) compact_pos_load(a, q, s0, kb, vx);
+        if (cp.r == 1) compact_pos_load(a, q, s0, c.items, kb, vx);
     }
     if (threadIdx.x < 64) compact_pos_bases(d, ps);  // before the barriers below
     if (cp.r == 1) {
-        compact_pos_mask(c.n_src, s0, kb);
-        compact_pos_counts(kb, ps);
+        compact_pos_mask(c.n_src, s0, c.items, kb);
+        compact_pos_counts(kb, c.items, ps);
         if (threadIdx.x < 7) ps.chain[threadIdx.x] = ps.tot[threadIdx.x];
     } else {
         uint32_t acc = 0;
         for (uint32_t sub = s0; sub < s1; ++sub) {
-            compact_pos_load(a, q, sub, kb, vx);
-            compact_pos_mask(c.n_src, sub, kb);
-            compact_pos_counts(kb, ps);
+            compact_pos_load(a, q, sub, c.items, kb, vx);
+            compact_pos_mask(c.n_src, sub, c.items, kb);
+            compact_pos_counts(kb, c.items, ps);
             if (threadIdx.x < 7) acc += ps.tot[threadIdx.x];
         }
         if (threadIdx.x < 7) ps.chain[threadIdx.x] = acc;
@@ -1039,13 +1052,13 @@ __device__ inline bool compact_pos_chunk(const KArgs& a, int q, const CompactPla
     if (threadIdx.x < 7) ps.run[threadIdx.x] = ps.base[threadIdx.x] + ps.pre[threadIdx.x];
     __syncthreads();
     if (cp.r == 1) {
-        compact_pos_place(a, q, kb, vx, ps);
+        compact_pos_place(a, q, c.items, kb, vx, ps);
     } else {
         for (uint32_t sub = s0; sub < s1; ++sub) {
-            compact_pos_load(a, q, sub, kb, vx);
-            compact_pos_mask(c.n_src, sub, kb);
-            compact_pos_counts(kb, ps);
-            compact_pos_place(a, q, kb, vx, ps);
+            compact_pos_load(a, q, sub, c.items, kb, vx);
+            compact_pos_mask(c.n_src, sub, c.items, kb);
+            compact_pos_counts(kb, c.items, ps);
+            compact_pos_place(a, q, c.items, kb, vx, ps);
             __syncthreads();
             if (threadIdx.x < 7) ps.run[threadIdx.x] += ps.tot[threadIdx.x];
             __syncthreads();
@@ -1126,14 +1139,33 @@ __device__ inline void compact_all(const KArgs& a, int q, const CompactPlan& c, 
 }
 
 // the first workgroups' inputs of a few-path compaction of list[q], loaded
-// before the counts are known (see compact_pos_chunk)
+// before the counts are known (see compact_pos_chunk): the tile of kPosTile
+// positions and, when small tiles are enabled, the one of kBlock positions.
+// Once the plan is known compact_pick_preload selects the one it uses.
 __device__ __attribute__((always_inline)) inline bool compact_preload(const KArgs& a, int q, uint32_t* kb,
-                                                                      uint32_t* vx, uint32_t* d)
+                                                                      uint32_t* vx, uint32_t* d, uint32_t* small)
 {
-    if (blockIdx.x >= kSpecTiles) return false;
-    compact_pos_load(a, q, blockIdx.x, kb, vx);
+    if (blockIdx.x >= a.g.spec_tiles) return false;
+    compact_pos_load(a, q, blockIdx.x, kPosItems, kb, vx);
+    if (a.g.small_n) {
+        const uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+        const uint64_t pc = p < a.g.N ? p : 0;
+        small[0] = a.keep[pc];
+        small[1] = a.list[q][pc];
+    }
     if (threadIdx.x < 64) dir_load(a, q, d);
     return true;
+}
+
+__device__ __attribute__((always_inline)) inline bool compact_pick_preload(const CompactPlan& c, bool loaded,
+                                                                           uint32_t* kb, uint32_t* vx,
+                                                                           const uint32_t* small)
+{
+    if (loaded && !c.dense && c.items == 1) {
+        kb[0] = small[0];
+        vx[0] = small[1];
+    }
+    return loaded;
 }
 
 // ---------------------------------------------------------------------------
@@ -1467,11 +1499,12 @@ template <typename SrcT, bool kFloatSrc, typename PhiT, bool kRec = false>
 __global__ __launch_bounds__(kBlock) void k_update_fused(KArgs a, int q)
 {
     __shared__ CompactShared sh;
-    uint32_t kb[kPosItems], vx[kPosItems], d[7];
-    const bool loaded = compact_preload(a, q ^ 1, kb, vx, d);  // with the counts
+    uint32_t kb[kPosItems], vx[kPosItems], d[7], small[2] = {0u, 0u};
+    bool loaded = compact_preload(a, q ^ 1, kb, vx, d, small);  // with the counts
     const uint32_t epoch = a.ctl->epoch;
     CompactPlan c = compact_plan<kCompact>(a.g, a.ctl->n[q ^ 1]);
     c.n_dst = a.ctl->n[q];
+    loaded = compact_pick_preload(c, loaded, kb, vx, small);
     const uint32_t gc = min((c.ntiles + 7u) & ~7u, (gridDim.x / 4u * 3u) & ~7u);
     if (blockIdx.x < gc) {
         compact_all<kCompact>(a, q ^ 1, c, epoch, sh, gc, kb, vx, d, loaded);
@@ -1562,11 +1595,12 @@ template <int kMode>
 __global__ __launch_bounds__(kBlock) void k_compact(KArgs a, int q)
 {
     __shared__ CompactShared sh;
-    uint32_t kb[kPosItems], vx[kPosItems], d[7];
-    const bool loaded = kMode == kCompact && compact_preload(a, q, kb, vx, d);
+    uint32_t kb[kPosItems], vx[kPosItems], d[7], small[2] = {0u, 0u};
+    bool loaded = kMode == kCompact && compact_preload(a, q, kb, vx, d, small);
     const uint32_t epoch = a.ctl->epoch;
     CompactPlan c = compact_plan<kMode>(a.g, a.ctl->n[q]);
     c.n_dst = a.ctl->n[q ^ 1];
+    loaded = compact_pick_preload(c, loaded, kb, vx, small);
     compact_all<kMode>(a, q, c, epoch, sh, gridDim.x, kb, vx, d, loaded);
 }
 

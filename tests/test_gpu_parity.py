@@ -211,6 +211,35 @@ def test_fused_and_separate_launches_agree(monkeypatch, wide):
     assert a[3] == 3 * 5 + 1 and b[3] == 4 * 5
 
 
+def test_compaction_tile_sizes_agree(monkeypatch):
+    """Few-path compactions take tiles of 1024 list positions, or of 256 for
+    lists up to LSSEG_SMALL_TILE_N positions, and the workgroups that load
+    their tile together with the counts keep whichever layout the plan needs.
+    Large tiles only, small tiles only, and a threshold inside the run's range
+    of list sizes give the same states, counts and statistics."""
+    w = synth.ellipsoids(edge=64, iterations=70)
+    out = {}
+    for small in ("0", "1000000000", None):
+        if small is None:  # the median list size of the first run: the run crosses it
+            small = str(int(np.median(out["0"][1].n)))
+        _say(f"compaction tiles: small up to {small} positions")
+        monkeypatch.setenv("LSSEG_SMALL_TILE_N", small)
+        ctx = Context(w.volume)
+        ctx.set_phi(w.phi0, w.band)
+        st = ctx.iterate(w.T, w.eps, w.alpha, w.dt, w.iterations)
+        out[small] = (st, ctx.iteration_counts(), ctx.state())
+        ctx.close()
+    runs = list(out.values())
+    n = runs[0][1].n
+    assert n.min() <= int(list(out)[2]) < n.max()
+    for b in runs[1:]:
+        assert runs[0][0].sum_active == b[0].sum_active
+        for f in ("n", "c", "m"):
+            assert np.array_equal(getattr(runs[0][1], f), getattr(b[1], f)), f
+        for x, y in zip(runs[0][2], b[2]):
+            assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
+
+
 @pytest.mark.parametrize("wide", [False, True], ids=["few", "dense"])
 def test_chunks_of_several_subtiles(monkeypatch, wide):
     """With a 16-workgroup grid the keep step, the compaction (8 workgroups in
