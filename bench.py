@@ -282,6 +282,9 @@ def _soft_dice_grad(phi, mask, temp=0.5):
     return 1.0 - 2.0 * a / b, (-2.0 * (m * b - a) / (b * b)) * (-s * (1.0 - s) / temp)
 
 
+REPS = 3  # timed calls per volume and side in the differentiable-form comparison
+
+
 def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
     """The differentiable form on the 16-volume 96^3 float workload.  Timed:
     the plain forward (lsseg_iterate) against forward + loss + backward through
@@ -307,12 +310,18 @@ def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
         ctx.set_phi(w.phi0, w.band)
         ctx.checkpoint_save(0)
         ctx.iterate(T, eps, alpha, w.dt, K)  # warm-up
-        ctx.checkpoint_restore(0)
-        device_sync()
-        t0 = time.perf_counter()
-        st = ctx.iterate(T, eps, alpha, w.dt, K)
-        device_sync()
-        t_plain += time.perf_counter() - t0
+        # each side is timed REPS times per volume and its median counted, so
+        # that one slow call (host jitter, a first touch of fresh device
+        # memory) does not decide the ratio
+        reps = []
+        for _ in range(REPS):
+            ctx.checkpoint_restore(0)
+            device_sync()
+            t0 = time.perf_counter()
+            st = ctx.iterate(T, eps, alpha, w.dt, K)
+            device_sync()
+            reps.append(time.perf_counter() - t0)
+        t_plain += float(np.median(reps))
         upd += int(st.sum_active)
         # the torch path, timed: phi0 and the mask already on the GPU
         phi0_t = torch.from_numpy(np.ascontiguousarray(w.phi0)).cuda().requires_grad_()
@@ -323,13 +332,18 @@ def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
                                                    checkpoint_every=every), mask_t).backward()
             phi0_t.grad = None
             theta_t.grad = None
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        phi_t = adjoint.segment(ctx, phi0_t, theta_t, dt=w.dt, iters=K, band=w.band, checkpoint_every=every)
-        adjoint.soft_dice_loss(phi_t, mask_t).backward()
-        torch.cuda.synchronize()
-        t_adj += time.perf_counter() - t0
-        del phi_t
+        reps = []
+        for _ in range(REPS):
+            phi0_t.grad = None
+            theta_t.grad = None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            phi_t = adjoint.segment(ctx, phi0_t, theta_t, dt=w.dt, iters=K, band=w.band, checkpoint_every=every)
+            adjoint.soft_dice_loss(phi_t, mask_t).backward()
+            torch.cuda.synchronize()
+            reps.append(time.perf_counter() - t0)
+            del phi_t
+        t_adj += float(np.median(reps))
         # the same gradients through the host path, for the checks
         ctx.checkpoint_restore(0)
         rec = ctx.fwd_record(T, eps, alpha, w.dt, K, checkpoint_every=every)
@@ -391,6 +405,7 @@ def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
         "workload": f"{len(ws)} float32 volumes 96^3 (ellipsoids / 255), K={K} iterations, dt=0.5, T=0.6 eps=0.15 "
                     f"alpha=0.1, phi checkpointed every {every}; loss 1 - softDice(sigmoid(-phi/0.5), mask)",
         "plain_forward_ms": 1e3 * t_plain, "forward_record_plus_adjoint_ms": 1e3 * t_adj,
+        "timing": f"per volume the median of {REPS} timed calls of each side, summed over the volumes",
         "timed_path": "adjoint.segment (torch.autograd.Function) + soft-Dice loss + backward, GPU tensors",
         "torch_vs_host_max_rel_err_theta": worst_torch,
         "adjoint_over_plain": t_adj / t_plain if t_plain > 0 else None,
