@@ -631,7 +631,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     // kBlock positions: four times the workgroups place the list, at the price
     // of a longer look-back (phantom 31.5 -> 29.6 ms; 2^16 and 2^17 measured
     // alike, 2^18 slower; tiles of 2 kBlock above gained nothing)
-    g.small_n = (uint32_t)std::max(0, env_int("LSSEG_SMALL_TILE_N", 131072));
+    g.small_n = (uint32_t)std::min<uint64_t>(N, (uint64_t)std::max(0, env_int("LSSEG_SMALL_TILE_N", 131072)));
     g.spec_tiles = (uint32_t)std::max(0, env_int("LSSEG_SPEC_TILES", (int)kSpecTiles));
 
     void* p = nullptr;
