@@ -1781,9 +1781,8 @@ const void* update_record_kernel(int32_t dtype, bool fused)
 // One recording run of `iters` iterations from the context's state: the
 // launches of lsseg_iterate (each compaction in the next update's launch),
 // with phi checkpointed every `every` iterations.  The update keeps each
-// voxel's decisions beside its temporal flag, and the marking, which runs
-// after the compaction that placed list[q], writes V_k in list order with its
-// decisions at the device offsets r->offs.
+// voxel's decisions beside its temporal flag, and the marking writes V_k in
+// traversal order with its decisions at the device offsets r->offs.
 lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
 {
     const uint64_t N = c->a.g.N;

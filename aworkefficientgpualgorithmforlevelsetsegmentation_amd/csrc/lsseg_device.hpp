@@ -687,7 +687,7 @@ __device__ inline void update_at(const KArgs& a, const DevParams& P, const PhiT*
 // the temporal flag `moved`; returns its emission mask
 template <typename PhiT>
 __device__ inline uint32_t mark_at(const KArgs& a, bool ref, bool moved, const PhiT* __restrict__ W,
-                                   uint8_t* __restrict__ Cb, uint32_t u, uint32_t* list_pos = nullptr)
+                                   uint8_t* __restrict__ Cb, uint32_t u)
 {
     uint32_t m = 0;
     // the position and every operand are loaded together (all addresses are
@@ -695,7 +695,6 @@ __device__ inline uint32_t mark_at(const KArgs& a, bool ref, bool moved, const P
     uint32_t nb[7], inb;
     neighbors(a.g, u, nb, &inb);
     const uint32_t pos = a.pos[u];
-    if (list_pos) *list_pos = pos;
     const float nv = phi_load(W, u);
     float wn[7];
 #pragma unroll
@@ -1537,8 +1536,8 @@ __global__ __launch_bounds__(kBlock) void k_empty_iterations(KArgs a, uint32_t c
 }
 
 // Where a recording run of the differentiable form puts iteration k's active
-// list V_k (list order) and the decisions of its updates: from offs[k] on, with
-// offs[0] = 0.  Entries beyond `cap` are not written; the host then sizes the
+// list V_k and the decisions of its updates: from offs[k] on, with offs[0] =
+// 0.  Entries beyond `cap` are not written; the host then sizes the
 // record from offs and records again.
 struct RecordOut {
     uint64_t* offs;
@@ -1548,8 +1547,10 @@ struct RecordOut {
 };
 
 // The marking over S[q]; with kRec also the recording of the iteration, k =
-// ctl->iter: V_k at list positions offs[k] + pos[u] with the decisions that the
-// update left in bits 1-7 of moved[p], and offs[k + 1].
+// ctl->iter: V_k in traversal order from offs[k] on, with the decisions that
+// the update left in bits 1-7 of moved[p], and offs[k + 1].  The replay and the
+// adjoint take V_k in any order; the traversal order keeps their gathers and
+// atomics as coherent as the forward's, and the recording's stores coalesced.
 template <typename PhiT, bool kRec>
 __device__ inline void mark_walk(const KArgs& a, int q, const RecordOut& r)
 {
@@ -1566,15 +1567,13 @@ __device__ inline void mark_walk(const KArgs& a, int q, const RecordOut& r)
     uint32_t emitted = 0;
     walk_list(a, q, 0, 0, [&](uint32_t u, uint64_t p) {
         const uint32_t mv = a.moved[p];
-        uint32_t lp = 0;
-        const uint32_t m =
-            mark_at<PhiT>(a, ref, (mv & 1u) != 0, phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, u, kRec ? &lp : nullptr);
+        const uint32_t m = mark_at<PhiT>(a, ref, (mv & 1u) != 0, phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, u);
         a.emit[p] = (uint8_t)m;
         emitted += (uint32_t)__popc(m);
         if constexpr (kRec)
             if (rec) {
-                r.lists[base + lp] = u;
-                r.dec[base + lp] = (uint8_t)(mv >> 1);
+                r.lists[base + p] = u;
+                r.dec[base + p] = (uint8_t)(mv >> 1);
             }
     });
     block_add(&a.cpart[(blockIdx.x % kCountSlots) * kCountStride], emitted, s_red);

@@ -345,7 +345,9 @@ lsseg_status lsseg_record_destroy(lsseg_record* rec);
 lsseg_status lsseg_record_info(const lsseg_record* rec, int32_t* iters, int32_t* checkpoint_every,
                                uint64_t* total_active);
 /* Recorded n_k (iters entries), the concatenated lists V_0..V_{iters-1} and one
-   decision byte per list entry (total_active entries each; host or device). */
+   decision byte per list entry (total_active entries each; host or device).
+   Each V_k holds the voxels of iteration k's active list in the engine's
+   spatially coherent traversal order, not in the list's own order. */
 lsseg_status lsseg_record_get(const lsseg_record* rec, uint32_t* n_per_iter, uint32_t* lists,
                               uint8_t* decisions);
 
