@@ -6,8 +6,9 @@ the backward pass is the engine's hand-written adjoint (``lsseg_bwd``), which
 returns dL/dphi0 and dL/dtheta.  The active set and the branch decisions of
 every update are those of the forward run and are held fixed, so the gradient
 is exact for the recorded map (``Record.replay`` evaluates that map at other
-inputs, e.g. for finite differences).  phi is checkpointed every
-``checkpoint_every`` iterations and recomputed in reverse.
+inputs, e.g. for finite differences).  The record keeps the values every
+update overwrote, so the backward pass steps back from the final phi without
+recomputation.
 
 Tensors stay on the GPU: the engine reads phi0 and dL/dphi from, and writes
 phi and dL/dphi0 to, torch's device memory directly.  torch is the plumbing
@@ -35,7 +36,7 @@ from .engine import MODE_REF, Context  # noqa: E402
 
 class LevelSetFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, phi0, theta, engine: Context, dt, iters, band, checkpoint_every, mode):
+    def forward(ctx, phi0, theta, engine: Context, dt, iters, band, mode):
         if not phi0.is_cuda:
             raise ValueError("phi0 must be a CUDA (HIP) tensor")
         if tuple(phi0.shape) != (engine.nz, engine.ny, engine.nx):
@@ -45,7 +46,7 @@ class LevelSetFunction(torch.autograd.Function):
         T, eps, alpha = (float(v) for v in theta.detach().to("cpu", torch.float64).tolist())
         torch.cuda.current_stream(p0.device).synchronize()
         engine.set_phi_ptr(p0.data_ptr(), band)
-        rec = engine.fwd_record(T, eps, alpha, dt, iters, checkpoint_every, mode)
+        rec = engine.fwd_record(T, eps, alpha, dt, iters, mode)
         out = torch.empty_like(p0)
         engine.phi_into_ptr(out.data_ptr(), 0)
         ctx.rec = rec
@@ -59,13 +60,13 @@ class LevelSetFunction(torch.autograd.Function):
         dphi0 = torch.empty_like(g)
         th = ctx.rec.bwd_ptr(g.data_ptr(), dphi0.data_ptr())
         dtype, device = ctx.theta_meta
-        return dphi0, torch.tensor(th, dtype=dtype, device=device), None, None, None, None, None, None
+        return dphi0, torch.tensor(th, dtype=dtype, device=device), None, None, None, None, None
 
 
 def segment(engine: Context, phi0: torch.Tensor, theta: torch.Tensor, dt: float = 0.5, iters: int = 200,
-            band: float = 1.0, checkpoint_every: int = 20, mode: int = MODE_REF) -> torch.Tensor:
+            band: float = 1.0, mode: int = MODE_REF) -> torch.Tensor:
     """phi after `iters` iterations from phi0 with theta = (T, eps, alpha), differentiable in phi0 and theta."""
-    return LevelSetFunction.apply(phi0, theta, engine, dt, iters, band, checkpoint_every, mode)
+    return LevelSetFunction.apply(phi0, theta, engine, dt, iters, band, mode)
 
 
 def soft_dice_loss(phi: torch.Tensor, mask: torch.Tensor, temperature: float = 0.5) -> torch.Tensor:

@@ -1632,7 +1632,7 @@ lsseg_status lsseg_batch_autotune(lsseg_batch* b, const lsseg_params* params, in
 // ---------------------------------------------------------------------------
 struct lsseg_record {
     lsseg_ctx* ctx = nullptr;
-    int32_t iters = 0, every = 1;
+    int32_t iters = 0;
     lsseg_params params{};
     DevParams* d_prm = nullptr;       // the recorded run's parameters
     DevParams* d_prm_replay = nullptr;
@@ -1641,12 +1641,11 @@ struct lsseg_record {
     uint64_t total = 0;
     uint64_t* offs = nullptr;         // the same offsets, written by the recording run
     uint64_t cap = 0;                 // entries that lists / dec hold
-    void* lists_block = nullptr;      // lists and dec
+    void* lists_block = nullptr;      // lists, undo and dec
     uint32_t* lists = nullptr;
+    float* undo = nullptr;            // per entry the write buffer's value its update overwrote
     uint8_t* dec = nullptr;
-    std::vector<float*> ckA, ckB;     // (read, write) buffers at k = 0, every, 2 every, ...
-    float* undo = nullptr;            // values a segment's replay overwrites (n_k per iteration)
-    uint64_t undo_cap = 0;
+    float* fin[2] = {nullptr, nullptr};  // (read, write) buffers after the last iteration
     float* work[2] = {nullptr, nullptr};
     float* adj[2] = {nullptr, nullptr};
     float* lin = nullptr;             // unpadded staging
@@ -1697,13 +1696,12 @@ LSSEG_DTYPE_KERNEL(k_adjoint)
 
 uint32_t list_grid(uint64_t n) { return dense_grid(std::max<uint64_t>(n, 1)); }
 
-void launch_replay(lsseg_ctx* c, lsseg_record* r, const KArgs& ra, int k, const float* R, float* W,
-                   float* undo = nullptr)
+void launch_replay(lsseg_ctx* c, lsseg_record* r, const KArgs& ra, int k, const float* R, float* W)
 {
     const uint32_t* V = r->lists + r->off[k];
     const uint8_t* D = r->dec + r->off[k];
     uint32_t n = r->n[k];
-    void* args[] = {(void*)&ra, (void*)&V, (void*)&D, (void*)&n, (void*)&R, (void*)&W, (void*)&undo};
+    void* args[] = {(void*)&ra, (void*)&V, (void*)&D, (void*)&n, (void*)&R, (void*)&W};
     (void)hipLaunchKernel(k_replay_kernel(c->dtype), dim3(list_grid(n)), dim3(kBlock), args, 0, c->stream);
 }
 
@@ -1736,7 +1734,8 @@ struct RecordLayout {
 // bound of the first guess of sum_k n_k without an earlier record (5 bytes each)
 constexpr uint64_t kRecordFirstGuess = 1ull << 27;
 
-// lists and decisions of `cap` entries, replacing any earlier pair
+// lists, undo values and decisions of `cap` entries (9 bytes each), replacing
+// any earlier block
 lsseg_status record_lists_alloc(lsseg_ctx* c, lsseg_record* r, uint64_t cap)
 {
     if (r->lists_block) {
@@ -1747,42 +1746,45 @@ lsseg_status record_lists_alloc(lsseg_ctx* c, lsseg_record* r, uint64_t cap)
     }
     const size_t lb = (size_t)(cap * sizeof(uint32_t) + 255) / 256 * 256;
     void* p = nullptr;
-    lsseg_status st = ralloc(c, r, &p, lb + cap);
+    lsseg_status st = ralloc(c, r, &p, 2 * lb + cap);
     if (st != LSSEG_OK) return st;
     r->lists_block = p;
     r->lists = (uint32_t*)p;
-    r->dec = (uint8_t*)p + lb;
+    r->undo = (float*)((uint8_t*)p + lb);
+    r->dec = (uint8_t*)p + 2 * lb;
     r->cap = cap;
     return LSSEG_OK;
 }
 
 // the update launch of a recording run (FP32 phi): the decisions of every
-// update go to bits 1-7 of its temporal-flag byte
+// update go to bits 1-7 of its temporal-flag byte, the value it overwrites to
+// the record's undo slots
 const void* update_record_kernel(int32_t dtype, bool fused)
 {
     switch (dtype) {
-    case LSSEG_U8: return fused ? (const void*)&k_update_fused<uint8_t, false, float, true>
-                                : (const void*)&k_update<uint8_t, false, float, true>;
-    case LSSEG_I8: return fused ? (const void*)&k_update_fused<int8_t, false, float, true>
-                                : (const void*)&k_update<int8_t, false, float, true>;
-    case LSSEG_U16: return fused ? (const void*)&k_update_fused<uint16_t, false, float, true>
-                                 : (const void*)&k_update<uint16_t, false, float, true>;
-    case LSSEG_I16: return fused ? (const void*)&k_update_fused<int16_t, false, float, true>
-                                 : (const void*)&k_update<int16_t, false, float, true>;
-    case LSSEG_U32: return fused ? (const void*)&k_update_fused<uint32_t, false, float, true>
-                                 : (const void*)&k_update<uint32_t, false, float, true>;
-    case LSSEG_I32: return fused ? (const void*)&k_update_fused<int32_t, false, float, true>
-                                 : (const void*)&k_update<int32_t, false, float, true>;
-    default: return fused ? (const void*)&k_update_fused<float, true, float, true>
-                          : (const void*)&k_update<float, true, float, true>;
+    case LSSEG_U8: return fused ? (const void*)&k_update_fused_record<uint8_t, false>
+                                : (const void*)&k_update_record<uint8_t, false>;
+    case LSSEG_I8: return fused ? (const void*)&k_update_fused_record<int8_t, false>
+                                : (const void*)&k_update_record<int8_t, false>;
+    case LSSEG_U16: return fused ? (const void*)&k_update_fused_record<uint16_t, false>
+                                 : (const void*)&k_update_record<uint16_t, false>;
+    case LSSEG_I16: return fused ? (const void*)&k_update_fused_record<int16_t, false>
+                                 : (const void*)&k_update_record<int16_t, false>;
+    case LSSEG_U32: return fused ? (const void*)&k_update_fused_record<uint32_t, false>
+                                 : (const void*)&k_update_record<uint32_t, false>;
+    case LSSEG_I32: return fused ? (const void*)&k_update_fused_record<int32_t, false>
+                                 : (const void*)&k_update_record<int32_t, false>;
+    default: return fused ? (const void*)&k_update_fused_record<float, true>
+                          : (const void*)&k_update_record<float, true>;
     }
 }
 
 // One recording run of `iters` iterations from the context's state: the
-// launches of lsseg_iterate (each compaction in the next update's launch),
-// with phi checkpointed every `every` iterations.  The update keeps each
-// voxel's decisions beside its temporal flag, and the marking writes V_k in
-// traversal order with its decisions at the device offsets r->offs.
+// launches of lsseg_iterate (each compaction in the next update's launch).
+// The update keeps each voxel's decisions beside its temporal flag and the
+// value it overwrites in the record, and the marking writes V_k in traversal
+// order with its decisions at the device offsets r->offs.  The final buffer
+// pair is kept: the backward pass steps back from it.
 lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
 {
     const uint64_t N = c->a.g.N;
@@ -1797,21 +1799,18 @@ lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
     c->h_ctl->many_iters = 0;
     HIP_TRY(c, hipMemcpyAsync(c->a.ctl, c->h_ctl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
     int q = c->q;
-    const RecordOut ro{r->offs, r->cap, r->lists, r->dec};
+    const RecordOut ro{r->offs, r->cap, r->lists, r->dec, r->undo};
     for (int k = 0; k < iters; ++k) {
-        if (k % r->every == 0) {
-            HIP_TRY(c, hipMemcpyAsync(r->ckA[k / r->every], c->a.phi[q], N * sizeof(float), hipMemcpyDeviceToDevice,
-                                      c->stream));
-            HIP_TRY(c, hipMemcpyAsync(r->ckB[k / r->every], c->a.phi[q ^ 1], N * sizeof(float),
-                                      hipMemcpyDeviceToDevice, c->stream));
-        }
-        launch(c, update_record_kernel(c->dtype, k > 0), k > 0 ? c->grid : c->sgrid, q);
         void* args[] = {(void*)&c->a, (void*)&q, (void*)&ro};
+        (void)hipLaunchKernel(update_record_kernel(c->dtype, k > 0), dim3(k > 0 ? c->grid : c->sgrid), dim3(kBlock),
+                              args, 0, c->stream);
         (void)hipLaunchKernel((const void*)&k_mark_record, dim3(c->sgrid), dim3(kBlock), args, 0, c->stream);
         launch(c, (const void*)&k_keep, c->grid, q);
         q ^= 1;
     }
     if (iters > 0) launch(c, (const void*)&k_compact<kCompact>, c->grid, q ^ 1);
+    HIP_TRY(c, hipMemcpyAsync(r->fin[0], c->a.phi[q], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(r->fin[1], c->a.phi[q ^ 1], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipGetLastError());
     lsseg_status st = sync_and_check(c);
     if (st != LSSEG_OK) return st;
@@ -1837,12 +1836,11 @@ lsseg_status lsseg_record_destroy(lsseg_record* r)
     return LSSEG_OK;
 }
 
-lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t iters, int32_t every,
-                              lsseg_record** out)
+lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t iters, lsseg_record** out)
 {
     if (!c || !prm || !out) return LSSEG_ERR_INVALID_ARG;
     *out = nullptr;
-    if (iters <= 0 || every <= 0) return fail(c, LSSEG_ERR_INVALID_ARG, "iterations and checkpoint interval must be positive");
+    if (iters <= 0) return fail(c, LSSEG_ERR_INVALID_ARG, "the iteration count must be positive");
     if (c->phi8) return fail(c, LSSEG_ERR_UNSUPPORTED, "the differentiable form needs FP32 phi storage");
     DevParams P;
     lsseg_status st = make_params(c, prm, &P);
@@ -1860,16 +1858,14 @@ lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t ite
     };
     r->ctx = c;
     r->iters = iters;
-    r->every = every;
     r->params = *prm;
     // everything but the lists in one allocation
-    const int nck = (iters + every - 1) / every;
     const size_t NF = N * sizeof(float);
     RecordLayout lay;
     const size_t o_prm = lay.take(sizeof(DevParams)), o_prm_replay = lay.take(sizeof(DevParams));
     const size_t o_theta = lay.take(kCountSlots * kCountStride * sizeof(double));
     const size_t o_offs = lay.take(((size_t)iters + 1) * sizeof(uint64_t));
-    const size_t o_ck = lay.take(2 * (size_t)nck * NF);
+    const size_t o_fin = lay.take(2 * NF);
     const size_t o_work = lay.take(4 * NF), o_lin = lay.take((size_t)c->nx * c->ny * c->nz * sizeof(float));
     void* p = nullptr;
     if ((st = ralloc(c, r, &p, lay.end)) != LSSEG_OK) return bail(st);
@@ -1878,11 +1874,8 @@ lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t ite
     r->d_prm_replay = (DevParams*)(base + o_prm_replay);
     r->d_theta = (double*)(base + o_theta);  // per-workgroup slots of dL/dT, dL/deps, dL/dalpha
     r->offs = (uint64_t*)(base + o_offs);
-    for (int i = 0; i < nck; ++i) {
-        r->ckA.push_back((float*)(base + o_ck + (size_t)(2 * i) * NF));
-        r->ckB.push_back((float*)(base + o_ck + (size_t)(2 * i + 1) * NF));
-    }
     for (int i = 0; i < 2; ++i) {
+        r->fin[i] = (float*)(base + o_fin + (size_t)i * NF);
         r->work[i] = (float*)(base + o_work + (size_t)i * NF);
         r->adj[i] = (float*)(base + o_work + (size_t)(2 + i) * NF);
     }
@@ -1918,11 +1911,10 @@ lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t ite
     return LSSEG_OK;
 }
 
-lsseg_status lsseg_record_info(const lsseg_record* r, int32_t* iters, int32_t* every, uint64_t* total_active)
+lsseg_status lsseg_record_info(const lsseg_record* r, int32_t* iters, uint64_t* total_active)
 {
     if (!r) return LSSEG_ERR_INVALID_ARG;
     if (iters) *iters = r->iters;
-    if (every) *every = r->every;
     if (total_active) *total_active = r->total;
     return LSSEG_OK;
 }
@@ -1986,52 +1978,29 @@ lsseg_status lsseg_bwd(lsseg_ctx* c, lsseg_record* r, const float* dphi, float* 
     HIP_TRY(c, hipMemsetAsync(r->d_theta, 0, kCountSlots * kCountStride * sizeof(double), c->stream));
     const KArgs ra = record_args(c, r->d_prm);
     int ab = 0;  // adj[ab] = Abar, adj[ab ^ 1] = Bbar
-    const int nck = (int)r->ckA.size();
-    // undo log for the longest checkpoint segment
-    uint64_t need = 0;
-    for (int sgi = 0; sgi < nck; ++sgi) {
-        const int k0 = sgi * r->every, k1 = std::min(r->iters, k0 + r->every);
-        need = std::max<uint64_t>(need, r->off[k1] - r->off[k0]);
-    }
-    if (need > r->undo_cap) {
-        void* p = nullptr;
-        if ((st = ralloc(c, r, &p, need * sizeof(float))) != LSSEG_OK) return st;
-        r->undo = (float*)p;
-        r->undo_cap = need;
-    }
-    for (int sgi = nck - 1; sgi >= 0; --sgi) {
-        const int k0 = sgi * r->every, k1 = std::min(r->iters, k0 + r->every);
-        // replay the segment from its checkpoint up to the state before its
-        // last iteration, keeping the values each replay overwrites
-        HIP_TRY(c, hipMemcpyAsync(r->work[0], r->ckA[sgi], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(r->work[1], r->ckB[sgi], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-        int rd = 0;
-        for (int k = k0; k + 1 < k1; ++k) {
-            launch_replay(c, r, ra, k, r->work[rd], r->work[rd ^ 1],
-                          k > k0 ? r->undo + (r->off[k] - r->off[k0]) : nullptr);
-            rd ^= 1;
-        }
-        // iteration k reads A_k = work[rd]; stepping back, A_{k-1} is the other
-        // buffer and the write buffer of iteration k-1 is work[rd] with that
-        // iteration's overwritten values put back (by the adjoint launch of
-        // k-1, which reads only A_{k-1}).  The segment's first write buffer is
-        // never read again, so its values are neither kept nor restored.
-        for (int k = k1 - 1; k >= k0; --k) {
-            const uint32_t* V = r->lists + r->off[k];
-            const uint8_t* D = r->dec + r->off[k];
-            uint32_t n = r->n[k];
-            const float* A = r->work[rd];
-            float* X = r->adj[ab];
-            float* Y = r->adj[ab ^ 1];
-            double* th = r->d_theta;
-            float* Bres = r->work[rd ^ 1];
-            const float* U = (k > k0 && k + 1 < k1) ? r->undo + (r->off[k] - r->off[k0]) : nullptr;
-            void* args[] = {(void*)&ra, (void*)&V, (void*)&D, (void*)&n, (void*)&A,
-                            (void*)&X, (void*)&Y, (void*)&th, (void*)&Bres, (void*)&U};
-            (void)hipLaunchKernel(k_adjoint_kernel(c->dtype), dim3(list_grid(n)), dim3(kBlock), args, 0, c->stream);
-            ab ^= 1;
-            rd ^= 1;
-        }
+    // From the final pair (A_K, B_K) backwards: iteration k read A_k = B_{k+1}
+    // and turned B_k into A_{k+1} at V_k, so the adjoint launch of iteration k
+    // reads A_k and puts the recorded values of V_k back into the other
+    // buffer, which then holds B_k = A_{k-1}.  Iteration 0's write buffer is
+    // never read again.
+    HIP_TRY(c, hipMemcpyAsync(r->work[0], r->fin[0], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(r->work[1], r->fin[1], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    int rd = 1;  // A_{K-1} = B_K
+    for (int k = r->iters - 1; k >= 0; --k) {
+        const uint32_t* V = r->lists + r->off[k];
+        const uint8_t* D = r->dec + r->off[k];
+        uint32_t n = r->n[k];
+        const float* A = r->work[rd];
+        float* X = r->adj[ab];
+        float* Y = r->adj[ab ^ 1];
+        double* th = r->d_theta;
+        float* Bres = r->work[rd ^ 1];
+        const float* U = k > 0 ? r->undo + r->off[k] : nullptr;
+        void* args[] = {(void*)&ra, (void*)&V, (void*)&D, (void*)&n, (void*)&A,
+                        (void*)&X, (void*)&Y, (void*)&th, (void*)&Bres, (void*)&U};
+        (void)hipLaunchKernel(k_adjoint_kernel(c->dtype), dim3(list_grid(n)), dim3(kBlock), args, 0, c->stream);
+        ab ^= 1;
+        rd ^= 1;
     }
     // phi0 fed both buffers: its adjoint is Abar_0 + Bbar_0
     hipLaunchKernelGGL(k_tiled_sum_to_lin, dim3(dense_grid(cnt)), dim3(kBlock), 0, c->stream, c->a.g, r->adj[0],

@@ -655,16 +655,42 @@ struct ListWalk {
 // tile; the single-volume kernels and the batched kernels (which take each
 // volume's arguments from a table) share them.
 // ---------------------------------------------------------------------------
+// Where a recording run of the differentiable form puts iteration k's active
+// list V_k (in traversal order), the decisions of its updates and the values
+// of the write buffer that they overwrite: from offs[k] on, with offs[0] = 0.
+// Entries beyond `cap` are not written; the host then sizes the record from
+// offs and records again.
+struct RecordOut {
+    uint64_t* offs;
+    uint64_t cap;
+    uint32_t* lists;
+    uint8_t* dec;
+    float* undo;
+};
+
+// the undo slots of a recording run's current iteration k = ctl->iter (list
+// q), by traversal position, or nullptr past the capacity
+__device__ inline float* record_undo(const KArgs& a, int q, const RecordOut& r)
+{
+    const uint32_t k = a.ctl->iter, n = a.ctl->n[q];
+    const uint64_t base = r.offs[k];
+    return base + n <= r.cap ? r.undo + base : nullptr;
+}
+
 // the update of active voxel u at traversal position p.  The byte moved[p]
 // holds the temporal flag in bit 0 and, with kRec (a recording run of the
-// differentiable form), the update's decisions in bits 1-7.
+// differentiable form, FP32 phi), the update's decisions in bits 1-7; the
+// value of W it overwrites goes to undo[p] (when undo is not null).
 template <typename SrcT, bool kFloatSrc, typename PhiT, bool kRec = false>
 __device__ inline void update_at(const KArgs& a, const DevParams& P, const PhiT* __restrict__ R,
-                                 PhiT* __restrict__ W, uint32_t u, uint64_t p)
+                                 PhiT* __restrict__ W, uint32_t u, uint64_t p, float* __restrict__ undo = nullptr)
 {
     uint32_t nb[7], inb, d = 0;
     neighbors(a.g, u, nb, &inb);
     const float u4 = phi_load(R, u);
+    float old = 0.0f;
+    if constexpr (kRec)
+        if (undo) old = phi_load(W, u);
     const float nv = update_core<kRec ? kDecideRecord : kDecideNone>(
         P, phi_load(R, nb[0]), phi_load(R, nb[1]), phi_load(R, nb[2]), u4, phi_load(R, nb[4]), phi_load(R, nb[5]),
         phi_load(R, nb[6]), density_factor<SrcT, kFloatSrc>(a, P, u), &d);
@@ -679,6 +705,7 @@ __device__ inline void update_at(const KArgs& a, const DevParams& P, const PhiT*
         if constexpr (kFloatSrc)
             if (static_cast<const float*>(a.src)[u] - P.Tf < 0.0f) d |= kDecBelow;
         mv |= d << 1;
+        if (undo) undo[p] = old;
     }
     a.moved[p] = (uint8_t)mv;
 }
@@ -1475,18 +1502,26 @@ __device__ inline void walk_list(const KArgs& a, int q, uint32_t first, uint32_t
 }
 
 template <typename SrcT, bool kFloatSrc, typename PhiT, bool kRec = false>
-__device__ inline void update_walk(const KArgs& a, int q, uint32_t first = 0, uint32_t count = 0)
+__device__ inline void update_walk(const KArgs& a, int q, uint32_t first = 0, uint32_t count = 0,
+                                   float* undo = nullptr)
 {
     const DevParams P = *a.prm;
     walk_list(a, q, first, count, [&](uint32_t u, uint64_t p) {
-        update_at<SrcT, kFloatSrc, PhiT, kRec>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), u, p);
+        update_at<SrcT, kFloatSrc, PhiT, kRec>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), u, p, undo);
     });
 }
 
-template <typename SrcT, bool kFloatSrc, typename PhiT, bool kRec = false>
+template <typename SrcT, bool kFloatSrc, typename PhiT>
 __global__ __launch_bounds__(kBlock) void k_update(KArgs a, int q)
 {
-    update_walk<SrcT, kFloatSrc, PhiT, kRec>(a, q);
+    update_walk<SrcT, kFloatSrc, PhiT>(a, q);
+}
+
+// the update of a recording run's first iteration (FP32 phi)
+template <typename SrcT, bool kFloatSrc>
+__global__ __launch_bounds__(kBlock) void k_update_record(KArgs a, int q, RecordOut r)
+{
+    update_walk<SrcT, kFloatSrc, float, true>(a, q, 0, 0, record_undo(a, q, r));
 }
 
 // the compaction of list[q ^ 1] into list[q] and the update over S[q].  The
@@ -1494,8 +1529,8 @@ __global__ __launch_bounds__(kBlock) void k_update(KArgs a, int q)
 // no more than the compaction has sub-tiles) take the compaction, the others
 // the update, so neither waits behind the other.  The grid is a multiple of 8
 // and at least 16.
-template <typename SrcT, bool kFloatSrc, typename PhiT, bool kRec = false>
-__global__ __launch_bounds__(kBlock) void k_update_fused(KArgs a, int q)
+template <typename SrcT, bool kFloatSrc, typename PhiT, bool kRec>
+__device__ inline void update_fused_body(const KArgs& a, int q, const RecordOut& r)
 {
     __shared__ CompactShared sh;
     uint32_t kb[kPosItems], vx[kPosItems], d[7], small[2] = {0u, 0u};
@@ -1509,8 +1544,23 @@ __global__ __launch_bounds__(kBlock) void k_update_fused(KArgs a, int q)
         compact_all<kCompact>(a, q ^ 1, c, epoch, sh, gc, kb, vx, d, loaded);
     } else {
         if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kCompact>(a, q ^ 1, c, 0, epoch);
-        update_walk<SrcT, kFloatSrc, PhiT, kRec>(a, q, gc, gridDim.x - gc);
+        float* undo = nullptr;
+        if constexpr (kRec) undo = record_undo(a, q, r);
+        update_walk<SrcT, kFloatSrc, PhiT, kRec>(a, q, gc, gridDim.x - gc, undo);
     }
+}
+
+template <typename SrcT, bool kFloatSrc, typename PhiT>
+__global__ __launch_bounds__(kBlock) void k_update_fused(KArgs a, int q)
+{
+    update_fused_body<SrcT, kFloatSrc, PhiT, false>(a, q, RecordOut{});
+}
+
+// k_update_fused of a recording run (FP32 phi)
+template <typename SrcT, bool kFloatSrc>
+__global__ __launch_bounds__(kBlock) void k_update_fused_record(KArgs a, int q, RecordOut r)
+{
+    update_fused_body<SrcT, kFloatSrc, float, true>(a, q, r);
 }
 
 // The counts and statistics of `count` iterations over an empty list, which
@@ -1534,17 +1584,6 @@ __global__ __launch_bounds__(kBlock) void k_empty_iterations(KArgs a, uint32_t c
         ctl->iter = k0 + count;
     }
 }
-
-// Where a recording run of the differentiable form puts iteration k's active
-// list V_k and the decisions of its updates: from offs[k] on, with offs[0] =
-// 0.  Entries beyond `cap` are not written; the host then sizes the
-// record from offs and records again.
-struct RecordOut {
-    uint64_t* offs;
-    uint64_t cap;
-    uint32_t* lists;
-    uint8_t* dec;
-};
 
 // The marking over S[q]; with kRec also the recording of the iteration, k =
 // ctl->iter: V_k in traversal order from offs[k] on, with the decisions that
@@ -1823,14 +1862,11 @@ __global__ __launch_bounds__(kBlock) void k_update_fused_batch(BatchArgs b, int 
 // k_adjoint reads and clears X = Abar' on V_k (leaving Bbar) and scatters the
 // J_v^T terms into Y = Bbar' (leaving Abar).
 // ---------------------------------------------------------------------------
-// One recorded update from R into W.  With `undo` the values of W it
-// overwrites are kept (undo[p] for list position p), so that a backward sweep
-// can step the buffer pair back without a copy of the volume.
+// One recorded update from R into W.
 template <typename SrcT, bool kFloatSrc>
 __global__ __launch_bounds__(kBlock) void k_replay(KArgs a, const uint32_t* __restrict__ V,
                                                    const uint8_t* __restrict__ dec, uint32_t n,
-                                                   const float* __restrict__ R, float* __restrict__ W,
-                                                   float* __restrict__ undo)
+                                                   const float* __restrict__ R, float* __restrict__ W)
 {
     const DevParams P = *a.prm;
     LSSEG_FOR_EACH(p, n)
@@ -1838,7 +1874,6 @@ __global__ __launch_bounds__(kBlock) void k_replay(KArgs a, const uint32_t* __re
         const uint32_t u = V[p];
         uint32_t nb[7], inb, d = dec[p];
         neighbors(a.g, u, nb, &inb);
-        if (undo) undo[p] = W[u];
         W[u] = update_core<kDecideReplay>(P, R[nb[0]], R[nb[1]], R[nb[2]], R[u], R[nb[4]], R[nb[5]], R[nb[6]],
                                           density_replay<SrcT, kFloatSrc>(a, P, u, d), &d);
     }
@@ -1901,7 +1936,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(KArgs a, const uint32_t* __r
         const uint32_t u = V[p];
         const uint32_t d = dec[p];
         // the write buffer of iteration k back to its values before the update
-        // (undo log of the replay); this kernel reads the other buffer, A
+        // (kept by the recording run); this kernel reads the other buffer, A
         if (undo) Bres[u] = undo[p];
         const float gbar = X[u];
         X[u] = 0.0f;

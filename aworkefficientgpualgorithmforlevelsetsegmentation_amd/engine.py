@@ -151,9 +151,9 @@ def lib():
             "lsseg_batch_get_tuning": (i32, [vp, P(Tuning)]),
             "lsseg_batch_set_tuning": (i32, [vp, P(Tuning)]),
             "lsseg_batch_autotune": (i32, [vp, P(Params), i32, P(Tuning)]),
-            "lsseg_fwd_record": (i32, [vp, P(Params), i32, i32, P(vp)]),
+            "lsseg_fwd_record": (i32, [vp, P(Params), i32, P(vp)]),
             "lsseg_record_destroy": (i32, [vp]),
-            "lsseg_record_info": (i32, [vp, P(i32), P(i32), P(u64)]),
+            "lsseg_record_info": (i32, [vp, P(i32), P(u64)]),
             "lsseg_record_get": (i32, [vp, vp, vp, vp]),
             "lsseg_fwd_replay": (i32, [vp, vp, vp, P(Params), vp]),
             "lsseg_bwd": (i32, [vp, vp, vp, vp, vp]),
@@ -260,11 +260,10 @@ class Context:
         """phi (unpadded float32) into raw memory (host or device)."""
         self._check(lib().lsseg_get_phi(self._h, which, ctypes.c_void_p(address)))
 
-    def fwd_record(self, T, eps, alpha, dt, iters, checkpoint_every=20, mode=MODE_REF) -> "Record":
+    def fwd_record(self, T, eps, alpha, dt, iters, mode=MODE_REF) -> "Record":
         prm = Params(T, eps, alpha, dt, mode)
         h = ctypes.c_void_p()
-        self._check(lib().lsseg_fwd_record(self._h, ctypes.byref(prm), int(iters), int(checkpoint_every),
-                                           ctypes.byref(h)))
+        self._check(lib().lsseg_fwd_record(self._h, ctypes.byref(prm), int(iters), ctypes.byref(h)))
         return Record(self, h)
 
     def seed_sphere(self, cx: float, cy: float, cz: float, radius: int):
@@ -468,9 +467,9 @@ class Record:
         self.ctx = ctx
         self._h = handle
         _track(self)
-        it, ev, tot = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_uint64()
-        ctx._check(lib().lsseg_record_info(self._h, ctypes.byref(it), ctypes.byref(ev), ctypes.byref(tot)))
-        self.iters, self.checkpoint_every, self.total_active = it.value, ev.value, tot.value
+        it, tot = ctypes.c_int32(), ctypes.c_uint64()
+        ctx._check(lib().lsseg_record_info(self._h, ctypes.byref(it), ctypes.byref(tot)))
+        self.iters, self.total_active = it.value, tot.value
 
     def close(self):
         if getattr(self, "_h", None):

@@ -285,7 +285,7 @@ def _soft_dice_grad(phi, mask, temp=0.5):
 REPS = 3  # timed calls per volume and side in the differentiable-form comparison
 
 
-def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
+def adjoint_vs_plain(iterations, fd_h=1e-3):
     """The differentiable form on the 16-volume 96^3 float workload.  Timed:
     the plain forward (lsseg_iterate) against forward + loss + backward through
     the torch.autograd.Function (adjoint.segment, soft-Dice loss on GPU
@@ -328,8 +328,8 @@ def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
         mask_t = torch.from_numpy(np.ascontiguousarray(w.truth.astype(np.float32))).cuda()
         theta_t = torch.tensor([T, eps, alpha], dtype=torch.float64, requires_grad=True)
         if i == 0:  # torch's own first-call set-up stays out of the timing, as the plain warm-up does
-            adjoint.soft_dice_loss(adjoint.segment(ctx, phi0_t, theta_t, dt=w.dt, iters=K, band=w.band,
-                                                   checkpoint_every=every), mask_t).backward()
+            adjoint.soft_dice_loss(adjoint.segment(ctx, phi0_t, theta_t, dt=w.dt, iters=K, band=w.band),
+                                   mask_t).backward()
             phi0_t.grad = None
             theta_t.grad = None
         reps = []
@@ -338,7 +338,7 @@ def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
             theta_t.grad = None
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            phi_t = adjoint.segment(ctx, phi0_t, theta_t, dt=w.dt, iters=K, band=w.band, checkpoint_every=every)
+            phi_t = adjoint.segment(ctx, phi0_t, theta_t, dt=w.dt, iters=K, band=w.band)
             adjoint.soft_dice_loss(phi_t, mask_t).backward()
             torch.cuda.synchronize()
             reps.append(time.perf_counter() - t0)
@@ -346,7 +346,7 @@ def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
         t_adj += float(np.median(reps))
         # the same gradients through the host path, for the checks
         ctx.checkpoint_restore(0)
-        rec = ctx.fwd_record(T, eps, alpha, w.dt, K, checkpoint_every=every)
+        rec = ctx.fwd_record(T, eps, alpha, w.dt, K)
         loss, dphi = _soft_dice_grad(ctx.phi(0), w.truth)
         g0, th = rec.bwd(dphi.astype(np.float32))
         torch_vs_host = max(abs(float(theta_t.grad[k]) - th[k]) / max(abs(th[k]), 1e-12) for k in range(3))
@@ -383,7 +383,7 @@ def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
     w = synth.adjoint(count=1, edge=64, iterations=K)[0]
     ctx = Context(w.volume)
     ctx.set_phi(w.phi0, w.band)
-    rec = ctx.fwd_record(w.T, w.eps, w.alpha, w.dt, K, checkpoint_every=every)
+    rec = ctx.fwd_record(w.T, w.eps, w.alpha, w.dt, K)
     _, dphi = _soft_dice_grad(ctx.phi(0), w.truth)
     g0, th = rec.bwd(dphi.astype(np.float32))
     n, lists, dec = rec.lists()
@@ -403,7 +403,7 @@ def adjoint_vs_plain(iterations, every=20, fd_h=1e-3):
     del hist
     out = {
         "workload": f"{len(ws)} float32 volumes 96^3 (ellipsoids / 255), K={K} iterations, dt=0.5, T=0.6 eps=0.15 "
-                    f"alpha=0.1, phi checkpointed every {every}; loss 1 - softDice(sigmoid(-phi/0.5), mask)",
+                    f"alpha=0.1; loss 1 - softDice(sigmoid(-phi/0.5), mask)",
         "plain_forward_ms": 1e3 * t_plain, "forward_record_plus_adjoint_ms": 1e3 * t_adj,
         "timing": f"per volume the median of {REPS} timed calls of each side, summed over the volumes",
         "timed_path": "adjoint.segment (torch.autograd.Function) + soft-Dice loss + backward, GPU tensors",

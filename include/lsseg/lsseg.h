@@ -323,9 +323,11 @@ lsseg_status lsseg_batch_autotune(lsseg_batch* batch, const lsseg_params* params
    fixed: the map from
    (phi0, T, eps, alpha) to the final phi is then smooth, lsseg_fwd_replay
    evaluates it for any inputs, and lsseg_bwd is its exact adjoint (up to FP32
-   rounding).  phi is checkpointed every `checkpoint_every` iterations and
-   recomputed segment by segment in reverse.  For integer sources T and eps are
-   truncated (Segmenter.cpp:817-818), so their derivatives are zero.
+   rounding).  The record also keeps, per update, the value of the write buffer
+   it overwrote, and the final buffer pair, so lsseg_bwd steps the buffers back
+   from the end without checkpoints or recomputation (9 bytes per recorded
+   entry and 8 per voxel in all).  For integer sources T and eps are truncated
+   (Segmenter.cpp:817-818), so their derivatives are zero.
 
    Pointers marked "host or device" may point to either (unified addressing);
    volumes are linear, x fastest, unpadded. */
@@ -338,12 +340,11 @@ lsseg_status lsseg_batch_autotune(lsseg_batch* batch, const lsseg_params* params
    the guess was short.  Uses checkpoint slot LSSEG_MAX_CHECKPOINTS - 1.
    Recorded lists are returned in the tiled index tau, like lsseg_get_active. */
 lsseg_status lsseg_fwd_record(lsseg_ctx* ctx, const lsseg_params* params, int32_t iters,
-                              int32_t checkpoint_every, lsseg_record** out);
+                              lsseg_record** out);
 /* A record may outlive its context: destroying the context detaches the
    record, which then refuses every call but lsseg_record_destroy. */
 lsseg_status lsseg_record_destroy(lsseg_record* rec);
-lsseg_status lsseg_record_info(const lsseg_record* rec, int32_t* iters, int32_t* checkpoint_every,
-                               uint64_t* total_active);
+lsseg_status lsseg_record_info(const lsseg_record* rec, int32_t* iters, uint64_t* total_active);
 /* Recorded n_k (iters entries), the concatenated lists V_0..V_{iters-1} and one
    decision byte per list entry (total_active entries each; host or device).
    Each V_k holds the voxels of iteration k's active list in the engine's

@@ -78,7 +78,7 @@ def test_record_and_replay_match_oracle_and_forward():
     _say(f"record {vol.shape} K={K}")
     ctx = Context(vol)
     ctx.set_phi(phi0, 1.0)
-    rec = ctx.fwd_record(T, eps, alpha, DT, K, checkpoint_every=7)
+    rec = ctx.fwd_record(T, eps, alpha, DT, K)
     n, lists, dec = rec.lists()
     on, olists, odec, ost = O.record_f32(vol, phi0, 1.0, T, eps, alpha, DT, K)
     assert _same_record(n, lists, dec, on, olists, odec)
@@ -112,7 +112,7 @@ def test_record_sizes_itself(monkeypatch, capacity):
     ctx = Context(vol)
     for rep in range(2):
         ctx.set_phi(phi0, 1.0)
-        rec = ctx.fwd_record(T, eps, alpha, DT, K, checkpoint_every=4)
+        rec = ctx.fwd_record(T, eps, alpha, DT, K)
         n, lists, dec = rec.lists()
         assert _same_record(n, lists, dec, on, olists, odec), rep
         for a, b in zip(ctx.state(), plain.state()):
@@ -130,7 +130,7 @@ def test_adjoint_matches_fp64_oracle_and_finite_differences():
     _say(f"adjoint {vol.shape} K={K}")
     ctx = Context(vol)
     ctx.set_phi(phi0, 1.0)
-    rec = ctx.fwd_record(T, eps, alpha, DT, K, checkpoint_every=9)
+    rec = ctx.fwd_record(T, eps, alpha, DT, K)
     phiK = ctx.phi(0)
     loss, dphi = _soft_dice_and_grad(phiK, gt)
     g0, th = rec.bwd(dphi.astype(np.float32))
@@ -189,13 +189,13 @@ def test_torch_autograd_function():
     p0 = torch.tensor(phi0, device=dev, requires_grad=True)
     theta = torch.tensor([T, eps, alpha], dtype=torch.float64, device=dev, requires_grad=True)
     mask = torch.tensor(gt, device=dev)
-    phi = segment(ctx, p0, theta, dt=DT, iters=K, checkpoint_every=10)
+    phi = segment(ctx, p0, theta, dt=DT, iters=K)
     loss = soft_dice_loss(phi, mask)
     loss.backward()
     # the same through the record directly
     ref = Context(vol)
     ref.set_phi(phi0, 1.0)
-    rec = ref.fwd_record(T, eps, alpha, DT, K, checkpoint_every=10)
+    rec = ref.fwd_record(T, eps, alpha, DT, K)
     assert np.array_equal(phi.detach().cpu().numpy().view(np.uint32), ref.phi(0).view(np.uint32))
     _, dphi = _soft_dice_and_grad(ref.phi(0), gt)
     g0, th = rec.bwd(dphi.astype(np.float32))
