@@ -272,6 +272,7 @@ def test_empty_list_ends_the_run_early(monkeypatch):
     statistics, both phi buffers (and which one is current), export and mask
     equal the oracle's full run and a run that launches every iteration."""
     _say("vessels 64^3, 400 iterations, list empty from iteration 101")
+    monkeypatch.setenv("LSSEG_GRAPH", "1")  # the early stop watches the graph-replayed chunks
     w = synth.vessels(edge=64, n_tubes=4)
     ref = O.State(w.volume)
     ref.set_phi(w.phi0, w.band)
