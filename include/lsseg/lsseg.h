@@ -225,8 +225,9 @@ lsseg_status lsseg_set_state(lsseg_ctx* ctx, const float* phi_read, const float*
    buffers, export byte, active list) in slot 0..LSSEG_MAX_CHECKPOINTS-1; the slot
    is allocated on first use.  Restore makes the saved state current again.  No
    reference counterpart (the reference keeps no engine checkpoint); used for
-   resumable runs, repeated timed runs from one resident input, and the
-   adjoint's phi checkpoints. */
+   resumable runs, repeated timed runs from one resident input, and the start
+   state that lsseg_fwd_record records again from when its first size guess
+   was short. */
 #define LSSEG_MAX_CHECKPOINTS 64
 lsseg_status lsseg_checkpoint_save(lsseg_ctx* ctx, int32_t slot);
 lsseg_status lsseg_checkpoint_restore(lsseg_ctx* ctx, int32_t slot);
