@@ -100,7 +100,10 @@ def test_record_sizes_itself(monkeypatch, capacity):
     guess that is too short (forced to one entry) the record runs again from
     the start with the exact size; without it, the second record of a context
     starts from the first one's size.  Either way the lists and decisions are
-    the oracle's and the context ends in the plain run's state."""
+    the oracle's and the context ends in the plain run's state.  The backward
+    pass steps back from the record's own copy of the final buffers, so a
+    second call on the same record, and the other record, give the same
+    gradients (up to the order of the adjoint's float atomics)."""
     vol, _, phi0, (T, eps, alpha), K = _case(edge=(30, 26, 22), seed=777, iters=30)
     if capacity:
         monkeypatch.setenv("LSSEG_RECORD_CAPACITY", capacity)
@@ -110,6 +113,8 @@ def test_record_sizes_itself(monkeypatch, capacity):
     plain.set_phi(phi0, 1.0)
     plain.iterate(T, eps, alpha, DT, K)
     ctx = Context(vol)
+    dphi = np.random.default_rng(5).standard_normal(vol.shape).astype(np.float32)
+    grads = []
     for rep in range(2):
         ctx.set_phi(phi0, 1.0)
         rec = ctx.fwd_record(T, eps, alpha, DT, K)
@@ -119,7 +124,14 @@ def test_record_sizes_itself(monkeypatch, capacity):
             assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), rep
         out = rec.replay(phi0, T, eps, alpha, DT)
         assert np.array_equal(out.view(np.uint32), ctx.phi(0).view(np.uint32)), rep
+        g0, th = rec.bwd(dphi)
+        g0b, thb = rec.bwd(dphi)
+        assert np.allclose(g0b, g0, rtol=1e-5, atol=1e-6) and np.allclose(thb, th, rtol=1e-5), rep
+        grads.append((g0, th))
         rec.close()
+    assert np.abs(grads[0][0]).max() > 0
+    assert np.allclose(grads[0][0], grads[1][0], rtol=1e-5, atol=1e-6)
+    assert np.allclose(grads[0][1], grads[1][1], rtol=1e-5)
     ctx.close()
     plain.close()
 
