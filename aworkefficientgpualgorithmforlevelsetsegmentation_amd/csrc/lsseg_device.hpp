@@ -397,22 +397,21 @@ __device__ inline uint32_t chunk_count(const uint64_t* s, int c)
 }
 
 // With P = (C + 1) / 2 a chunk publishes its C counts two to a word, which
-// needs every count below 2^16; the group totals stay one per word.
-template <int C, bool kPrevDirect, int P = C>
+// needs every count below 2^16; likewise a group's totals with G = (C + 1) / 2.
+template <int C, bool kPrevDirect, int P = C, int G = C>
 __device__ __attribute__((always_inline)) inline void wave_group_scan(uint64_t* agg, uint64_t* grp, uint32_t chunk, const uint32_t* tot, uint32_t epoch,
                                 uint32_t* err, uint32_t* pre)
 {
     static_assert(P == C || P == (C + 1) / 2, "one or two counts per word");
+    static_assert(G == C || G == (C + 1) / 2, "one or two totals per word");
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t ep = epoch & 0x3FFFFFFFu;
     const uint64_t stamp = (uint64_t)ep << 34;
     const uint32_t g = chunk >> 6, r = chunk & 63u;
-    uint32_t mine = 0, word = 0;
+    uint32_t word = 0;
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-        if (lane == (uint32_t)c) mine = tot[c];
+    for (int c = 0; c < C; ++c)
         if (P == C ? lane == (uint32_t)c : lane == (uint32_t)(c >> 1)) word |= tot[c] << (P == C ? 0 : 16 * (c & 1));
-    }
     if (lane < (uint32_t)P) st_state(&agg[(uint64_t)chunk * P + lane], stamp | (1ull << 32) | word);
     // this group's earlier chunks, the previous group's chunks and the totals
     // of the 64 groups before that, loaded together
@@ -421,24 +420,27 @@ __device__ __attribute__((always_inline)) inline void wave_group_scan(uint64_t* 
     const bool need_grp = lane < ng;
     const uint64_t* ow = agg + ((uint64_t)g * 64u + lane) * P;
     const uint64_t* pw = g ? agg + ((uint64_t)(g - 1u) * 64u + lane) * P : agg;
-    const uint64_t* gw = grp + (uint64_t)lane * C;
-    uint64_t so[P], sp[P], sg[C];
+    const uint64_t* gw = grp + (uint64_t)lane * G;
+    uint64_t so[P], sp[P], sg[G];
 #pragma unroll
     for (int c = 0; c < P; ++c) {
         so[c] = need_own ? ld_state(ow + c) : 0ull;
         sp[c] = need_prev ? ld_state(pw + c) : 0ull;
     }
 #pragma unroll
-    for (int c = 0; c < C; ++c) sg[c] = need_grp ? ld_state(gw + c) : 0ull;
+    for (int c = 0; c < G; ++c) sg[c] = need_grp ? ld_state(gw + c) : 0ull;
     uint32_t spins = 0;
     wait_states<P>(ow, need_own, ep, so, spins, err);
     uint32_t own[C], prior[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) own[c] = wave_sum(need_own ? chunk_count<C, P>(so, c) : 0u);
     if (r == 63u) {
+        uint32_t gword = 0;
 #pragma unroll
         for (int c = 0; c < C; ++c)
-            if (lane == (uint32_t)c) st_state(&grp[(uint64_t)g * C + c], stamp | (2ull << 32) | (own[c] + mine));
+            if (G == C ? lane == (uint32_t)c : lane == (uint32_t)(c >> 1))
+                gword |= (own[c] + tot[c]) << (G == C ? 0 : 16 * (c & 1));
+        if (lane < (uint32_t)G) st_state(&grp[(uint64_t)g * G + lane], stamp | (2ull << 32) | gword);
     }
     wait_states<P>(pw, need_prev, ep, sp, spins, err);
 #pragma unroll
@@ -446,14 +448,14 @@ __device__ __attribute__((always_inline)) inline void wave_group_scan(uint64_t* 
     // the totals of groups 0..ng-1, 64 groups per round
     for (uint32_t g0 = 0; g0 < ng; g0 += 64) {
         const bool need = g0 + lane < ng;
-        const uint64_t* w = grp + ((uint64_t)g0 + lane) * C;
+        const uint64_t* w = grp + ((uint64_t)g0 + lane) * G;
         if (g0) {
 #pragma unroll
-            for (int c = 0; c < C; ++c) sg[c] = need ? ld_state(w + c) : 0ull;
+            for (int c = 0; c < G; ++c) sg[c] = need ? ld_state(w + c) : 0ull;
         }
-        wait_states<C>(w, need, ep, sg, spins, err);
+        wait_states<G>(w, need, ep, sg, spins, err);
 #pragma unroll
-        for (int c = 0; c < C; ++c) prior[c] += wave_sum(need ? (uint32_t)sg[c] : 0u);
+        for (int c = 0; c < C; ++c) prior[c] += wave_sum(need ? chunk_count<C, G>(sg, c) : 0u);
     }
 #pragma unroll
     for (int c = 0; c < C; ++c)
@@ -1084,10 +1086,13 @@ __device__ inline bool compact_pos_chunk(const KArgs& a, int q, const CompactPla
     // a chunk's count per direction is at most its positions: two counts to a
     // state word when that is below 2^16 (the same choice for every chunk)
     if (threadIdx.x < 64) {
-        if ((uint64_t)cp.r * kBlock * c.items < 65536u)
-            wave_group_scan<7, true, 4>(a.tstate, a.gstate, chunk, ps.chain, epoch, &a.ctl->error, ps.pre);
+        const uint64_t span = (uint64_t)cp.r * kBlock * c.items;  // positions per chunk
+        if (64u * span < 65536u)
+            wave_group_scan<7, true, 4, 4>(a.tstate, a.gstate, chunk, ps.chain, epoch, &a.ctl->error, ps.pre);
+        else if (span < 65536u)
+            wave_group_scan<7, true, 4, 7>(a.tstate, a.gstate, chunk, ps.chain, epoch, &a.ctl->error, ps.pre);
         else
-            wave_group_scan<7, true, 7>(a.tstate, a.gstate, chunk, ps.chain, epoch, &a.ctl->error, ps.pre);
+            wave_group_scan<7, true, 7, 7>(a.tstate, a.gstate, chunk, ps.chain, epoch, &a.ctl->error, ps.pre);
     }
     __syncthreads();
     bool over = false;
