@@ -35,6 +35,7 @@ using namespace lsseg;
 namespace {
 thread_local std::string g_create_error;
 constexpr int kGraphIters = 32;  // iterations per captured graph (even: the parity returns)
+constexpr int kProfileWatch = 64; // a profiled run checks for an empty list every this many iterations
 }  // namespace
 
 struct lsseg_ctx {
@@ -343,11 +344,32 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q, uint32_t* skipped
         launch_head(c, q, true);
         launch_tail(c, q);
         q ^= 1;
+        // a per-launch profiled run issues its launches one by one: it checks
+        // the list count every kProfileWatch iterations and stops on an empty
+        // list as the graph-replayed run does (the kernels' own start and stop
+        // events are unaffected by the host wait)
+        if (c->profile && i % kProfileWatch == kProfileWatch - 1 && i + 1 < count) {
+            HIP_TRY(c, hipMemcpyAsync(&c->h_watch[0], &c->a.ctl->n[q], sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                      c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            if (c->h_watch[0] == 0) {
+                skipped = (uint32_t)(count - i - 1);
+                break;
+            }
+        }
     }
     launch(c, (const void*)&k_compact<kCompact>, c->grid, q ^ 1);
     if (skipped) hipLaunchKernelGGL(k_empty_iterations, dim3(1), dim3(kBlock), 0, c->stream, c->a, skipped);
     if (skipped_out) *skipped_out = skipped;
     return LSSEG_OK;
+}
+
+// text of a device error flag (kErr* bits)
+const char* device_error_text(uint32_t err)
+{
+    return (err & kErrSpin)    ? "look-back wait exceeded its bound"
+           : (err & kErrCount) ? "the compaction's count differs from the keep step's"
+                               : "list overflow";
 }
 
 lsseg_status sync_and_check(lsseg_ctx* c)
@@ -356,9 +378,7 @@ lsseg_status sync_and_check(lsseg_ctx* c)
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (c->h_ctl->error)
         return fail(c, LSSEG_ERR_DEVICE, "device error flag 0x%x (%s)", c->h_ctl->error,
-                    (c->h_ctl->error & kErrSpin)    ? "look-back wait exceeded its bound"
-                    : (c->h_ctl->error & kErrCount) ? "the compaction's count differs from the keep step's"
-                                                    : "list overflow");
+                    device_error_text(c->h_ctl->error));
     return LSSEG_OK;
 }
 
@@ -633,6 +653,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     // alike, 2^18 slower; tiles of 2 kBlock above gained nothing)
     g.small_n = (uint32_t)std::min<uint64_t>(N, (uint64_t)std::max(0, env_int("LSSEG_SMALL_TILE_N", 131072)));
     g.spec_tiles = (uint32_t)std::max(0, env_int("LSSEG_SPEC_TILES", (int)kSpecTiles));
+    g.pack_counts = env_int("LSSEG_LOOKBACK_PACK", 1) ? 1u : 0u;
 
     void* p = nullptr;
     // look-back states: one per keep or dense-compaction tile, seven per
@@ -839,6 +860,9 @@ lsseg_status lsseg_iterate(lsseg_ctx* c, const lsseg_params* prm, int32_t iters,
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->h_ctl->iter = 0;
     c->h_ctl->stats_cap = c->stats_cap;
+    // after a call that ended with a device error, look-back states of its
+    // unfinished scans may carry the current epoch: skip past it
+    if (c->h_ctl->error) c->h_ctl->epoch += 2;
     c->h_ctl->error = 0;
     c->h_ctl->few_iters = 0;
     c->h_ctl->many_iters = 0;
@@ -1352,7 +1376,7 @@ lsseg_status batch_sync_and_check(lsseg_batch* b)
         *b->vols[v]->h_ctl = b->h_ctls[v];
         if (b->h_ctls[v].error)
             return bfail(b, LSSEG_ERR_DEVICE, "volume %zu: device error flag 0x%x (%s)", v, b->h_ctls[v].error,
-                         (b->h_ctls[v].error & kErrSpin) ? "look-back wait exceeded its bound" : "list overflow");
+                         device_error_text(b->h_ctls[v].error));
     }
     return LSSEG_OK;
 }
@@ -1499,6 +1523,7 @@ lsseg_status lsseg_batch_iterate(lsseg_batch* b, const lsseg_params* params, int
         DevCtl& h = b->h_ctls[v];
         h.iter = 0;
         h.stats_cap = b->vols[v]->stats_cap;
+        if (h.error) h.epoch += 2;  // as in lsseg_iterate
         h.error = 0;
         h.few_iters = 0;
         h.many_iters = 0;
@@ -1796,6 +1821,9 @@ lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->h_ctl->iter = 0;
     c->h_ctl->stats_cap = c->stats_cap;
+    // after a call that ended with a device error, look-back states of its
+    // unfinished scans may carry the current epoch: skip past it
+    if (c->h_ctl->error) c->h_ctl->epoch += 2;
     c->h_ctl->error = 0;
     c->h_ctl->few_iters = 0;
     c->h_ctl->many_iters = 0;

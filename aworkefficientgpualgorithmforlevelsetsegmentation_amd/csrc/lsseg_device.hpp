@@ -114,6 +114,8 @@ struct Geom {
     uint32_t small_n;        // few-path compactions of at most this many list positions take
                              // tiles of kBlock positions, the others of kPosTile (0: never)
     uint32_t spec_tiles;     // workgroups of a compaction that load their tile with the counts
+    uint32_t pack_counts;    // 1: a few-path compaction chunk publishes its seven counts two to a
+                             // look-back word where they fit below 2^16; 0: one count per word
 };
 
 // Voxel ids.  Voxel (x, y, z) of the padded volume is stored at
@@ -1087,9 +1089,9 @@ __device__ inline bool compact_pos_chunk(const KArgs& a, int q, const CompactPla
     // state word when that is below 2^16 (the same choice for every chunk)
     if (threadIdx.x < 64) {
         const uint64_t span = (uint64_t)cp.r * kBlock * c.items;  // positions per chunk
-        if (64u * span < 65536u)
+        if (a.g.pack_counts && 64u * span < 65536u)
             wave_group_scan<7, true, 4, 4>(a.tstate, a.gstate, chunk, ps.chain, epoch, &a.ctl->error, ps.pre);
-        else if (span < 65536u)
+        else if (a.g.pack_counts && span < 65536u)
             wave_group_scan<7, true, 4, 7>(a.tstate, a.gstate, chunk, ps.chain, epoch, &a.ctl->error, ps.pre);
         else
             wave_group_scan<7, true, 7, 7>(a.tstate, a.gstate, chunk, ps.chain, epoch, &a.ctl->error, ps.pre);
@@ -1099,7 +1101,13 @@ __device__ inline bool compact_pos_chunk(const KArgs& a, int q, const CompactPla
 #pragma unroll
     for (int e = 0; e < 7; ++e) over |= (uint64_t)ps.base[e] + ps.pre[e] + ps.chain[e] > a.g.N;
     if (over) {
-        if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
+        // the last chunk overflows whenever any does (its prefixes are the
+        // largest); it still advances the epoch so that no later scan reads
+        // this one's states as its own
+        if (threadIdx.x == 0) {
+            atomicOr(&a.ctl->error, kErrOverflow);
+            if (last) a.ctl->epoch = epoch + 1;
+        }
         return false;
     }
     if (threadIdx.x < 7) ps.run[threadIdx.x] = ps.base[threadIdx.x] + ps.pre[threadIdx.x];
@@ -1155,7 +1163,10 @@ __device__ inline bool compact_chunk(const KArgs& a, int q, const CompactPlan& c
     }
     const uint32_t prefix = lookback(a, chunk, total, epoch, &sh.prefix);
     if ((uint64_t)prefix + total > a.g.N) {
-        if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
+        if (threadIdx.x == 0) {
+            atomicOr(&a.ctl->error, kErrOverflow);
+            if (chunk == cp.chunks - 1) a.ctl->epoch = epoch + 1;  // as in compact_pos_chunk
+        }
         return false;
     }
     if (cp.r == 1) {
@@ -1466,7 +1477,10 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
     }
     const uint32_t prefix = lookback(a, chunk, total, epoch, &sh.prefix);
     if ((uint64_t)prefix + total > a.g.N) {
-        if (threadIdx.x == 0) atomicOr(&a.ctl->error, kErrOverflow);
+        if (threadIdx.x == 0) {
+            atomicOr(&a.ctl->error, kErrOverflow);
+            if (last) a.ctl->epoch = epoch + 1;  // as in compact_pos_chunk
+        }
         return false;
     }
     if (cp.r == 1) {

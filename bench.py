@@ -6,12 +6,19 @@ state from an HBM-resident checkpoint, then run the workload's iterations
 (update, marking, duplicate removal and compaction) through the C ABI.  Inputs
 are resident in HBM before the timed region starts.
 
+The headline workload is the 512^3 uint16 vessel tree (2000 iterations, less
+than 1% of the volume active): the configuration the HBM-roofline target is
+quoted on.  The 181x217x181 float32 phantom (1000 iterations) is reported
+beside it under "phantom", and the batched and differentiable forms under
+"batched" and "adjoint".
+
 Printed (one JSON line): active-voxel updates/s (sum of n_k over the run's
 iterations divided by the wall time), the roofline of the dominant kernel (its
 algorithmic bytes per launch, from the per-iteration counters, over its average
-launch time measured with HIP events on the engine's stream), the whole-path
-fraction of HBM peak, and the CPU baseline (the oracle: the reference's update
-equations compiled as host C, timed on this machine's cores).
+launch time measured with HIP events on the engine's stream, both over the
+launches of iterations with a non-empty list), the whole-path fraction of HBM
+peak, and the CPU baseline (the oracle: the reference's update equations
+compiled as host C with -march=native, timed on this machine's cores).
 
 Algorithmic bytes per iteration k, FP32 phi (4 bytes), source S bytes/voxel:
   update   (38 + S) n_k   coordinate, 7 phi taps, source, phi write, flag, mask byte
@@ -23,12 +30,14 @@ gather model's 10 c_k + 5 m_k for duplicate removal and compaction, split over
 the two kernels that do that work).  Each iteration is three launches: the
 update, which from the second iteration on runs in k_update_fused together with
 the previous iteration's compaction, k_mark and k_keep; the run ends with the
-last iteration's compaction (k_compact).  3 K + 1 launches for K iterations.
+last iteration's compaction (k_compact).  3 L + 1 launches for L launched
+iterations (launching stops once the list is empty).
 """
 from __future__ import annotations
 
 import argparse
-import ctypes
+import csv
+import glob
 import json
 import os
 import sys
@@ -77,62 +86,191 @@ def cpu_model() -> str:
 def algorithmic_bytes(n, c, m, S, P=4):
     """The gather model with P bytes per phi value: update n (4 + 7P + S + P + 2),
     mark n (5 + 7P), compaction 10 c + 5 m."""
-    n = n.astype(np.float64)
-    c = c.astype(np.float64)
-    m = m.astype(np.float64)
+    n = np.asarray(n, np.float64)
+    c = np.asarray(c, np.float64)
+    m = np.asarray(m, np.float64)
     return {"update": (6 + 8 * P + S) * n, "mark": (5 + 7 * P) * n, "keep": 5.0 * c, "compact": 5.0 * c + 5.0 * m}
 
 
-KERNELS = ("k_update_fused", "k_mark", "k_keep", "k_update", "k_compact")
+# ---------------------------------------------------------------------------
+# launches of a per-launch profiled run, by iteration
+# ---------------------------------------------------------------------------
+# The launch order of a run of L launched iterations: k_update (iteration 0),
+# k_mark, k_keep, then (k_update_fused, k_mark, k_keep) for iterations 1..L-1,
+# then the last compaction.  A launch belongs to a non-empty iteration when
+# that iteration's list has n_k > 0; only those enter the averages, so that the
+# iterations the early stop launches over an empty list do not dilute them.
+KERNEL_OF_PART = {"update_fused": "k_update_fused", "mark": "k_mark", "keep": "k_keep"}
 
 
-def _pmc_means(path):
-    """Mean counter value (KiB) per dispatch for each kernel in a rocprofv3
-    counter_collection.csv."""
-    import csv
-    acc = {}
+def launch_parts(per_launch, n, c, m, S, P):
+    """Per-launch times (ms) and algorithmic bytes of the non-empty iterations'
+    launches, by part."""
+    per_launch = np.asarray(per_launch, np.float64)
+    L = (per_launch.size - 1) // 3
+    assert per_launch.size == 3 * L + 1 and L >= 2, f"unexpected launch count {per_launch.size}"
+    n, c, m = (np.asarray(a)[:L] for a in (n, c, m))
+    B = algorithmic_bytes(n, c, m, S, P)
+    head = per_launch[0:3 * L:3]
+    live = n > 0
+    parts = {
+        # launch k >= 1: the update of iteration k and the compaction of k - 1
+        "update_fused": (head[1:], B["update"][1:] + B["compact"][:-1], n[1:], live[1:]),
+        "mark": (per_launch[1:3 * L:3], B["mark"], n, live),
+        "keep": (per_launch[2:3 * L:3], B["keep"], n, live),
+    }
+    out = {}
+    for k, (ms, by, nk, sel) in parts.items():
+        ms, by, nk = ms[sel], by[sel], nk[sel]
+        launches = int(ms.size)
+        tot_ms = float(ms.sum())
+        avg_us = 1e3 * tot_ms / max(launches, 1)
+        bpl = float(by.sum()) / max(launches, 1)
+        big = nk >= np.percentile(nk, 90) if launches else nk > 0
+        out[k] = {"kernel": KERNEL_OF_PART[k], "launches": launches, "avg_launch_us": avg_us,
+                  "bytes_per_launch": bpl, "achieved_GBs": bpl / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0,
+                  "total_ms": tot_ms,
+                  "busiest_decile_GBs": (float(by[big].sum()) / (float(ms[big].sum()) * 1e-3) / 1e9
+                                         if big.any() and ms[big].sum() > 0 else 0.0)}
+    return out, L
+
+
+def _kernel_key(name):
+    name = name.split("(")[0]
+    name = name[5:] if name.startswith("void ") else name
+    if name.startswith("lsseg::k_update_fused<"):
+        return "k_update_fused"
+    if name.startswith("lsseg::k_update<"):
+        return "k_update"
+    if name.startswith("lsseg::k_mark<"):
+        return "k_mark"
+    if name == "lsseg::k_keep":
+        return "k_keep"
+    if name == "lsseg::k_compact<2>":
+        return "k_compact"
+    return None
+
+
+def iteration_dispatches(rows, n):
+    """Rows (dicts with Kernel_Name, in dispatch order) of one profiled run ->
+    {kernel: [(iteration, row), ...]} for the iteration launches (the run's
+    first k_update starts them)."""
+    out = {"k_update_fused": [], "k_mark": [], "k_keep": []}
+    started = False
+    it_mark = it_keep = 0
+    it_fused = 1
+    for r in rows:
+        k = _kernel_key(r["Kernel_Name"])
+        if k == "k_update":
+            started = True
+            continue
+        if not started or k is None or k == "k_compact":
+            continue
+        if k == "k_update_fused":
+            out[k].append((it_fused, r))
+            it_fused += 1
+        elif k == "k_mark":
+            out[k].append((it_mark, r))
+            it_mark += 1
+        elif k == "k_keep":
+            out[k].append((it_keep, r))
+            it_keep += 1
+    for k in out:
+        out[k] = [(i, r) for i, r in out[k] if i < len(n) and n[i] > 0]
+    return out
+
+
+def _csv_rows(path):
     with open(path) as f:
-        for r in csv.DictReader(f):
-            name = r["Kernel_Name"].split("(")[0]
-            name = name[5:] if name.startswith("void ") else name
-            acc.setdefault(name, []).append(float(r["Counter_Value"]))
-    return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+        return list(csv.DictReader(f))
+
+
+def _dispatch_order(rows):
+    """Rows in dispatch order (by Dispatch_Id where the CSV has it, else as written)."""
+    key = next((k for k in ("Dispatch_Id", "Correlation_Id") if rows and k in rows[0]), None)
+    return sorted(rows, key=lambda r: int(r[key])) if key else rows
+
+
+def _find_csv(root, d, suffix):
+    hits = glob.glob(os.path.join(root, d, "**", f"*{suffix}"), recursive=True)
+    if not hits:
+        raise SystemExit(f"no *{suffix} under {os.path.join(root, d)}")
+    return hits[0]
+
+
+def _load_counts(root):
+    with open(os.path.join(root, "counts.json")) as f:
+        d = json.load(f)
+    return d
 
 
 def summarize_pmc(root, workload):
-    """HBM traffic per launch from two rocprofv3 --pmc runs of this benchmark
-    (FETCH_SIZE, then WRITE_SIZE) and the same two counters over a plain copy of
-    known size (--calibrate-copy), which give the scale factor of each counter."""
-    def csv_in(d):
-        import glob
-        hits = glob.glob(os.path.join(root, d, "**", "*counter_collection.csv"), recursive=True)
-        if not hits:
-            raise SystemExit(f"no counter_collection.csv under {os.path.join(root, d)}")
-        return hits[0]
+    """HBM traffic per launch from two rocprofv3 --pmc runs of `bench.py
+    --profile-only` (FETCH_SIZE, then WRITE_SIZE) and the same two counters over
+    a plain copy of known size (--calibrate-copy), which give each counter's
+    scale.  Averaged over the launches of non-empty iterations, the launches the
+    roofline's algorithmic bytes cover."""
     copy_bytes = None
     with open(os.path.join(root, "copy_fetch.json")) as f:
         for line in f:
             if line.startswith("{"):
                 copy_bytes = json.loads(line)["calibrate_copy_bytes"]
-    cf = _pmc_means(csv_in("copy_fetch"))
-    cw = _pmc_means(csv_in("copy_write"))
+
+    def means(d):
+        acc = {}
+        for r in _csv_rows(_find_csv(root, d, "counter_collection.csv")):
+            acc.setdefault(r["Kernel_Name"].split("(")[0], []).append(float(r["Counter_Value"]))
+        return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+    cf, cw = means("copy_fetch"), means("copy_write")
     copy_name = max(cf, key=lambda k: cf[k][0])
     f_scale = copy_bytes / (cf[copy_name][0] * 1024.0)
     w_scale = copy_bytes / (cw[copy_name][0] * 1024.0)
-    fe = _pmc_means(csv_in("fetch"))
-    wr = _pmc_means(csv_in("write"))
-    out = {"workload": workload, "copy_kernel": copy_name, "copy_bytes": copy_bytes,
-           "fetch_scale": f_scale, "write_scale": w_scale, "kernels": {}}
-    for k in KERNELS:
-        fk = [v for n, v in fe.items() if n == "lsseg::" + k or n.startswith("lsseg::" + k + "<")]
-        wk = [v for n, v in wr.items() if n == "lsseg::" + k or n.startswith("lsseg::" + k + "<")]
-        if not fk or not wk:
-            continue
-        fk = max(fk, key=lambda v: v[1])
-        wk = max(wk, key=lambda v: v[1])
-        out["kernels"][k] = {"dispatches": fk[1], "fetch_KiB_raw": fk[0], "write_KiB_raw": wk[0],
-                             "read_bytes": fk[0] * 1024.0 * f_scale, "write_bytes": wk[0] * 1024.0 * w_scale,
-                             "bytes_per_launch": fk[0] * 1024.0 * f_scale + wk[0] * 1024.0 * w_scale}
+    counts = _load_counts(root)
+    n = np.asarray(counts["n"])
+    S, P = counts["S"], counts["P"]
+    B = algorithmic_bytes(n, counts["c"], counts["m"], S, P)
+    alg = {"k_update_fused": lambda i: B["update"][i] + B["compact"][i - 1], "k_mark": lambda i: B["mark"][i],
+           "k_keep": lambda i: B["keep"][i]}
+    per = {}
+    for d, key in (("fetch", "fetch"), ("write", "write")):
+        rows = _dispatch_order(_csv_rows(_find_csv(root, d, "counter_collection.csv")))
+        for k, lst in iteration_dispatches(rows, n).items():
+            per.setdefault(k, {})[key] = [float(r["Counter_Value"]) for _, r in lst]
+            per[k]["iters"] = [i for i, _ in lst]
+    out = {"workload": workload, "copy_kernel": copy_name, "copy_bytes": copy_bytes, "fetch_scale": f_scale,
+           "write_scale": w_scale, "launches": "non-empty iterations of one per-launch profiled run",
+           "kernels": {}}
+    for k, v in per.items():
+        fe, wr = np.asarray(v["fetch"]), np.asarray(v["write"])
+        assert fe.size == wr.size == len(v["iters"]), k
+        rd, wt = fe.mean() * 1024.0 * f_scale, wr.mean() * 1024.0 * w_scale
+        ab = float(np.mean([alg[k](i) for i in v["iters"]]))
+        out["kernels"][k] = {"dispatches": int(fe.size), "fetch_KiB_raw": float(fe.mean()),
+                             "write_KiB_raw": float(wr.mean()), "read_bytes": rd, "write_bytes": wt,
+                             "bytes_per_launch": rd + wt, "algorithmic_bytes_per_launch": ab,
+                             "physical_over_algorithmic": (rd + wt) / ab if ab else None}
+    return out
+
+
+def summarize_trace(root):
+    """Kernel durations from a rocprofv3 --kernel-trace run of `bench.py
+    --profile-only`, over the launches of non-empty iterations (the same set as
+    the bench line's per-launch figures), and over every dispatch."""
+    counts = _load_counts(root)
+    n = np.asarray(counts["n"])
+    rows = _dispatch_order(_csv_rows(_find_csv(root, "kt", "kernel_trace.csv")))
+    out = {"workload": counts["workload"], "kernels": {}}
+    for k, lst in iteration_dispatches(rows, n).items():
+        us = np.array([(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for _, r in lst])
+        out["kernels"][k] = {"launches": int(us.size), "avg_us": float(us.mean()), "min_us": float(us.min()),
+                             "max_us": float(us.max()), "total_ms": float(us.sum() / 1e3)}
+    allk = {}
+    for r in rows:
+        allk.setdefault(r["Kernel_Name"].split("(")[0], []).append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    out["all_dispatches"] = {k: {"calls": len(v), "avg_us": float(np.mean(v)), "total_ms": float(np.sum(v) / 1e3)}
+                             for k, v in sorted(allk.items(), key=lambda kv: -sum(kv[1]))}
     return out
 
 
@@ -166,44 +304,240 @@ def describe(w, X, Y, Z):
             f"active band |phi0|<{w.band:g}")
 
 
-def cpu_baseline(w, gpu_mask, seconds_1core, phi_storage=0):
-    """The oracle on this host: full workload on all cores (also gives the mask
-    for the Dice check), plus a bounded single-core sample."""
+# ---------------------------------------------------------------------------
+# CPU baseline: the oracle compiled for this host (-march=native)
+# ---------------------------------------------------------------------------
+def _oracle_native():
     import oracle as O
+    try:
+        O.build_native()
+        O.lib(native=True)
+        return True, "-O3 -march=native"
+    except Exception as e:  # noqa: BLE001 -- fall back to the portable build, and say so
+        log(f"native oracle build failed ({e}); timing the portable x86-64-v3 build")
+        return False, "-O3 -march=x86-64-v3"
+
+
+def cpu_baseline(w, gpu_mask, seconds_1core, phi_storage=0, window_at=None, window_len=20):
+    """The oracle on this host.  All cores: the full workload (which also gives
+    the mask for the Dice check).  With window_at (the 512^3 workload): the
+    first 200 iterations and a steady-state window of window_len iterations
+    from iteration window_at, on all cores and on one core; otherwise one core
+    over the workload's first iterations.  Single-core legs are bounded by
+    seconds_1core each."""
+    import oracle as O
+    from aworkefficientgpualgorithmforlevelsetsegmentation_amd import synth
+    native, flags = _oracle_native()
     cores = host_cores()
-    ref = O.State(w.volume, phi_storage=phi_storage)
-    ref.set_phi(w.phi0, w.band)
+
+    def new_state():
+        s = O.State(w.volume, phi_storage=phi_storage, native=native)
+        s.set_phi(w.phi0, w.band)
+        return s
+
+    ref = new_state()
+    legs = {}
     t0 = time.perf_counter()
     n_tot = 0
     done = 0
+    saved = None
     while done < w.iterations:
         k = min(50, w.iterations - done)
+        if window_at is not None and done < window_at < done + k:
+            k = window_at - done
+        if window_at is not None and done == 200 and "all_first_200" not in legs:
+            legs["all_first_200"] = {"iterations": 200, "updates": n_tot, "s": time.perf_counter() - t0}
+        if window_at is not None and done == window_at and saved is None:
+            t_pause = time.perf_counter()
+            saved = ref.state(), ref.active()
+            t0 += time.perf_counter() - t_pause  # the copy is not oracle work
+            tw = time.perf_counter()
+            n, _, _ = ref.iterate(w.T, w.eps, w.alpha, w.dt, window_len, threads=cores)
+            legs["all_window"] = {"iterations": window_len, "from": window_at, "updates": int(n.sum()),
+                                  "s": time.perf_counter() - tw}
+            n_tot += int(n.sum())
+            done += window_len
+            continue
         n, _, _ = ref.iterate(w.T, w.eps, w.alpha, w.dt, k, threads=cores)
         n_tot += int(n.sum())
         done += k
     t_all = time.perf_counter() - t0
     mask = ref.mask()
-    from aworkefficientgpualgorithmforlevelsetsegmentation_amd import synth
+    ref.close()
     dice = synth.dice(gpu_mask, mask)
     identical = bool(np.array_equal(gpu_mask, mask))
-    # single core: the first iterations of the same workload, bounded in time
-    one = O.State(w.volume, phi_storage=phi_storage)
-    one.set_phi(w.phi0, w.band)
-    t0 = time.perf_counter()
-    n1 = 0
-    it1 = 0
-    while it1 < w.iterations and time.perf_counter() - t0 < seconds_1core:
-        n, _, _ = one.iterate(w.T, w.eps, w.alpha, w.dt, 10, threads=1)
-        n1 += int(n.sum())
-        it1 += 10
-    t1 = time.perf_counter() - t0
-    return {
+
+    def one_core(state, iters_max, label, start):
+        t = time.perf_counter()
+        upd = it = 0
+        while it < iters_max and time.perf_counter() - t < seconds_1core:
+            k = min(5, iters_max - it)
+            n, _, _ = state.iterate(w.T, w.eps, w.alpha, w.dt, k, threads=1)
+            upd += int(n.sum())
+            it += k
+        dt = time.perf_counter() - t
+        return {"iterations": it, "from": start, "updates": upd, "s": dt, "value": upd / dt if dt > 0 else 0.0,
+                "what": label}
+
+    one = new_state()
+    if window_at is None:
+        single = one_core(one, w.iterations, "the workload's first iterations", 0)
+        one.close()
+        single_all = [single]
+    else:
+        first = one_core(one, 200, "first 200 iterations", 0)
+        one.close()
+        (r, wr, e), lst = saved
+        win = O.State(w.volume, phi_storage=phi_storage, native=native)
+        win.set_state(r, wr, e, lst)
+        window = one_core(win, window_len, f"steady state: iterations {window_at}..{window_at + window_len - 1} "
+                                           f"from the all-core run's state", window_at)
+        win.close()
+        single_all = [first, window]
+        for k in ("all_first_200", "all_window"):
+            if k in legs:
+                legs[k]["value"] = legs[k]["updates"] / legs[k]["s"] if legs[k]["s"] > 0 else 0.0
+    out = {
         "value": n_tot / t_all, "unit": "active-voxel updates/s", "cores": cores, "kind": "port",
-        "sample": f"full workload ({w.iterations} iterations, {n_tot} updates) in {t_all:.2f} s, "
-                  f"oracle/lsoracle.c -O3 OpenMP over the active list, {cpu_model()}",
-        "single_core": {"value": n1 / t1 if t1 > 0 else 0.0, "cores": 1,
-                        "sample": f"first {it1} iterations ({n1} updates) in {t1:.2f} s"},
-    }, dice, identical
+        "sample": f"full workload ({w.iterations} iterations, {n_tot} updates) in {t_all:.2f} s on {cores} "
+                  f"threads; oracle/lsoracle.c {flags} -ffp-contract=off, OpenMP over the active list and "
+                  f"prefix-sum compactions in the reference's order; {cpu_model()}",
+        "single_core": [dict(s, cores=1) for s in single_all],
+    }
+    if legs:
+        out["all_cores_legs"] = legs
+    return out, dice, identical
+
+
+# ---------------------------------------------------------------------------
+# the plain form on one workload
+# ---------------------------------------------------------------------------
+def plain_leg(name, args, with_cpu, window_at=None):
+    from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import PHI_F32, PHI_I8, Context
+    t = time.perf_counter()
+    w = make_workload(name, args.iterations if name == args.workload else 0)
+    log(f"{name}: workload generated in {time.perf_counter() - t:.1f} s")
+    phi_storage = PHI_I8 if args.phi == "i8" else PHI_F32
+    ctx = Context(w.volume, phi_storage=phi_storage)
+    ctx.set_phi(w.phi0, w.band)
+    ctx.checkpoint_save(0)
+    S = w.volume.dtype.itemsize
+    P = 1 if args.phi == "i8" else 4
+    log(f"{name}: context ready: {describe(w, ctx.X, ctx.Y, ctx.Z)}; n0={ctx.active_count}")
+
+    for _ in range(args.warmup):
+        ctx.checkpoint_restore(0)
+        ctx.iterate(w.T, w.eps, w.alpha, w.dt, w.iterations)
+    device_sync()
+    t0 = time.perf_counter()
+    last = None
+    for _ in range(args.steps):
+        ctx.checkpoint_restore(0)
+        last = ctx.iterate(w.T, w.eps, w.alpha, w.dt, w.iterations)
+    device_sync()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / max(args.steps, 1)
+    updates_per_step = int(last.sum_active) if last else 0
+    value = updates_per_step * args.steps / elapsed if elapsed > 0 else 0.0
+    gpu_mask = ctx.mask()
+    counts = ctx.iteration_counts()
+    log(f"{name}: timed {args.steps} steps: {ms_per_step:.2f} ms/step, {updates_per_step} updates/step, "
+        f"few/many iterations {last.few_path_iters}/{last.many_path_iters}, final n={last.active_final}, "
+        f"{last.empty_skipped} trailing iterations over an empty list not launched")
+
+    # the same run from and to host arrays: context creation (upload, pad and
+    # tile the source), phi0 upload, the iterations and the mask download
+    device_sync()
+    t0 = time.perf_counter()
+    hctx = Context(w.volume, phi_storage=phi_storage)
+    hctx.set_phi(w.phi0, w.band)
+    hst = hctx.iterate(w.T, w.eps, w.alpha, w.dt, w.iterations)
+    hmask = hctx.mask()
+    host_ms = 1e3 * (time.perf_counter() - t0)
+    assert np.array_equal(hmask, gpu_mask), "host-inclusive run diverged from the timed run"
+    hctx.close()
+    host_inclusive = {"ms": host_ms, "value": int(hst.sum_active) / (host_ms * 1e-3), "unit": "active-voxel updates/s",
+                      "what": "lsseg_create (source upload, pad, tile) + lsseg_set_phi + lsseg_iterate + "
+                              "lsseg_get_mask, host arrays in and out, one run"}
+    log(f"{name}: host-inclusive run: {host_ms:.2f} ms ({host_inclusive['value']:.3e} updates/s)")
+
+    # per-launch device times (HIP events on the engine's stream), in issue order
+    ctx.checkpoint_restore(0)
+    pst, kt, per_launch = ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, w.iterations)
+    pc = ctx.iteration_counts()
+    assert np.array_equal(pc.n, counts.n), "profiled run diverged from the timed run"
+    kernels, L = launch_parts(per_launch, pc.n, pc.c, pc.m, S, P)
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    d = kernels[dom]
+    traffic, traffic_src = load_traffic(name)
+    traffic_bytes = None
+    if traffic and traffic.get("kernels", {}).get(d["kernel"]):
+        traffic_bytes = traffic["kernels"][d["kernel"]]["bytes_per_launch"]
+    B = algorithmic_bytes(pc.n, pc.c, pc.m, S, P)
+    path_bytes = float(sum(v.sum() for v in B.values()))
+    path_gbs = path_bytes / (ms_per_step * 1e-3) / 1e9
+    live = int((np.asarray(pc.n) > 0).sum())
+    log(f"{name}: per launch over the {live} non-empty of {L} launched iterations: "
+        + "; ".join(f"{k} {v['avg_launch_us']:.2f} us, {v['bytes_per_launch'] / 1e6:.2f} MB, "
+                    f"{v['achieved_GBs']:.0f} GB/s (busiest decile {v['busiest_decile_GBs']:.0f})"
+                    for k, v in kernels.items())
+        + f"; kernel time sum {kt.total_ms:.2f} ms vs {ms_per_step:.2f} ms/step")
+
+    cpu = dice = identical = None
+    if with_cpu:
+        log(f"{name}: running the CPU oracle")
+        cpu, dice, identical = cpu_baseline(w, gpu_mask, args.cpu_seconds, 1 if args.phi == "i8" else 0,
+                                            window_at=window_at)
+        log(f"{name}: cpu baseline {cpu['value']:.3e} updates/s on {cpu['cores']} cores; Dice vs oracle {dice:.6f}")
+    out = {
+        "value": value, "unit": "active-voxel updates/s", "ms_per_step": ms_per_step,
+        "config": {
+            "workload": describe(w, ctx.X, ctx.Y, ctx.Z),
+            "volume_xyz": list(w.shape_xyz), "iterations": w.iterations,
+            "active_voxel_updates_per_step": updates_per_step,
+            "non_empty_iterations": live,
+            "parallelism": "single GPU",
+            "phi_storage": "fixed-point int8 (LEVEL_SET_CHAR)" if args.phi == "i8" else "FP32",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": d["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic_bytes, "traffic_source": traffic_src,
+            "kernel": d["kernel"], "bytes_per_launch": d["bytes_per_launch"], "avg_launch_us": d["avg_launch_us"],
+            "launches": f"the {d['launches']} launches of non-empty iterations in one per-launch profiled run",
+        },
+        "path_roofline": {"achieved": path_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": path_gbs / HBM_PEAK_GBS, "bytes_per_step": path_bytes},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+        "host_inclusive": host_inclusive,
+        "dice_vs_oracle": dice,
+        "mask_identical_to_oracle": identical,
+    }
+    ctx.close()
+    return out
+
+
+def profile_only(args):
+    """One per-launch profiled run of the workload and nothing else (for
+    rocprofv3 kernel-trace and counter passes); writes the run's per-iteration
+    counts to --counts-out so that the summaries can select the launches of
+    non-empty iterations."""
+    from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import Context
+    w = make_workload(args.workload, args.iterations)
+    ctx = Context(w.volume)
+    ctx.set_phi(w.phi0, w.band)
+    st, kt, per = ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, w.iterations)
+    pc = ctx.iteration_counts()
+    kernels, L = launch_parts(per, pc.n, pc.c, pc.m, w.volume.dtype.itemsize, 4)
+    out = {"workload": describe(w, ctx.X, ctx.Y, ctx.Z), "S": w.volume.dtype.itemsize, "P": 4,
+           "n": [int(x) for x in pc.n], "c": [int(x) for x in pc.c], "m": [int(x) for x in pc.m],
+           "launched_iterations": L, "kernels": kernels}
+    if args.counts_out:
+        os.makedirs(os.path.dirname(os.path.abspath(args.counts_out)), exist_ok=True)
+        with open(args.counts_out, "w") as f:
+            json.dump(out, f)
+    print(json.dumps({k: v for k, v in out.items() if k not in ("n", "c", "m")}), flush=True)
+    ctx.close()
 
 
 def batch_vs_sequential(steps, iterations):
@@ -285,18 +619,22 @@ def _soft_dice_grad(phi, mask, temp=0.5):
 REPS = 3  # timed calls per volume and side in the differentiable-form comparison
 
 
-def adjoint_vs_plain(iterations, fd_h=1e-3):
+def adjoint_vs_plain(iterations, fd_h=1e-3, fd64_h=1e-6):
     """The differentiable form on the 16-volume 96^3 float workload.  Timed:
     the plain forward (lsseg_iterate) against forward + loss + backward through
     the torch.autograd.Function (adjoint.segment, soft-Dice loss on GPU
-    tensors, loss.backward()), volume after volume.  Checked: the gradients
-    against central finite differences of the recorded map on every volume
-    (five-point stencil at h = 1e-3, and its Richardson extrapolation from h
-    and h/2), and against the FP64 host adjoint on one 64^3 instance.  Over
-    200 iterations the map is nonlinear enough that the h = 1e-3 difference
-    carries ~1e-2 truncation error even in exact arithmetic; the extrapolation
-    removes most of it."""
+    tensors, loss.backward()), volume after volume.  Checked, on every volume,
+    the GPU adjoint against central finite differences of the recorded map:
+      - of the FP64 replay (the oracle) at h = 1e-6, three-point: the
+        derivative of the map itself;
+      - of the GPU's FP32 replay at h = 1e-3, five-point, and its Richardson
+        extrapolation from h and h/2.  Over 200 iterations the map's higher
+        derivatives are large, and the h = 1e-3 differences carry ~1e-2
+        truncation error even in FP64 (the FP64 differences at h = 1e-3 are
+        reported beside them); FP32 rounding rules out smaller steps there;
+    and against the FP64 host adjoint on one 64^3 instance."""
     import torch
+    import oracle as O
     from aworkefficientgpualgorithmforlevelsetsegmentation_amd import adjoint, synth
     from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import Context
     ws = synth.adjoint(iterations=iterations) if iterations else synth.adjoint()
@@ -304,6 +642,7 @@ def adjoint_vs_plain(iterations, fd_h=1e-3):
     t_plain = t_adj = 0.0
     upd = 0
     worst = worst_dir = worst_r = worst_dir_r = worst_torch = 0.0
+    w64 = w64_dir = w64_h3 = 0.0
     for i, w in enumerate(ws):
         T, eps, alpha = w.T, w.eps, w.alpha
         ctx = Context(w.volume)
@@ -361,25 +700,45 @@ def adjoint_vs_plain(iterations, fd_h=1e-3):
         def rel(a, b):
             return abs(a - b) / max(abs(b), 1e-12)
 
+        # the FP64 replay of the same recorded map (oracle)
+        n, lists, dec = rec.lists()
+        nz, ny, nx = w.volume.shape
+        X, Y, Z = O.padded_dims(nx, ny, nz)
+        src = O.padded_tiled_source(w.volume)
+
+        def L64(p, t, e, a):
+            out, _ = O.replay_f64(X, Y, Z, src, t, e, a, w.dt, n, lists, dec, O.pad_tile_phi(p, 1.0))
+            return _soft_dice_grad(O.unpad_untile(out, nx, ny, nz), w.truth)[0]
+
         base = [T, eps, alpha]
+        p64 = w.phi0.astype(np.float64)
         for k in range(3):
             def f(step, k=k):
                 v = list(base)
                 v[k] += step
                 return L(w.phi0, *v)
+
+            def f64(step, k=k):
+                v = list(base)
+                v[k] += step
+                return L64(p64, *v)
             d1, d2 = central(f, fd_h), central(f, fd_h / 2)
             worst = max(worst, rel(th[k], d1))
             worst_r = max(worst_r, rel(th[k], (16 * d2 - d1) / 15))
+            w64 = max(w64, rel(th[k], (f64(fd64_h) - f64(-fd64_h)) / (2 * fd64_h)))
+            w64_h3 = max(w64_h3, rel(th[k], central(f64, fd_h)))
         d = (g0 / max(np.linalg.norm(g0), 1e-30)).astype(np.float32)
         adj_d = float((g0.astype(np.float64) * d).sum())
         fdir = (lambda step: L(w.phi0 + step * d, *base))
         d1, d2 = central(fdir, fd_h), central(fdir, fd_h / 2)
         worst_dir = max(worst_dir, rel(adj_d, d1))
         worst_dir_r = max(worst_dir_r, rel(adj_d, (16 * d2 - d1) / 15))
+        d64 = d.astype(np.float64)
+        w64_dir = max(w64_dir, rel(adj_d, (L64(p64 + fd64_h * d64, *base) - L64(p64 - fd64_h * d64, *base))
+                                   / (2 * fd64_h)))
         rec.close()
         ctx.close()
     # FP64 host adjoint of the same recorded map on one 64^3 instance
-    import oracle as O
     w = synth.adjoint(count=1, edge=64, iterations=K)[0]
     ctx = Context(w.volume)
     ctx.set_phi(w.phi0, w.band)
@@ -411,46 +770,54 @@ def adjoint_vs_plain(iterations, fd_h=1e-3):
         "adjoint_over_plain": t_adj / t_plain if t_plain > 0 else None,
         "updates": upd, "value": 2 * upd / t_adj if t_adj > 0 else 0.0,
         "unit": "active-voxel updates/s (forward + adjoint, each counted once)",
+        "fd_fp64_max_rel_err_theta": w64, "fd_fp64_max_rel_err_phi0_direction": w64_dir,
+        "fd_fp64": f"central three-point differences at h = {fd64_h:g} of the FP64 replay (oracle) of each "
+                   f"volume's recorded map, all volumes, against the GPU adjoint; phi0 along the normalised gradient",
+        "fd_fp64_h1e-3_five_point_max_rel_err_theta": w64_h3,
         "fd_max_rel_err_theta": worst, "fd_max_rel_err_phi0_direction": worst_dir,
         "fd_richardson_max_rel_err_theta": worst_r, "fd_richardson_max_rel_err_phi0_direction": worst_dir_r,
-        "fd": "central five-point stencil at h = 1e-3, and (16 D(h/2) - D(h)) / 15, on the recorded map, "
-              "all volumes; phi0 along the normalised gradient",
+        "fd": "central five-point stencil at h = 1e-3 of the GPU's FP32 replay, and (16 D(h/2) - D(h)) / 15, "
+              "all volumes; the same stencil on the FP64 replay gives the truncation error alone "
+              "(fd_fp64_h1e-3_five_point_max_rel_err_theta)",
         "fp64_64cubed_max_rel_err_theta": fp64_theta, "fp64_64cubed_rel_l2_err_phi0": fp64_phi0,
     }
     log(f"adjoint: plain forward {1e3 * t_plain:.1f} ms, forward record + loss + adjoint (torch) {1e3 * t_adj:.1f} ms "
-        f"({out['adjoint_over_plain']:.2f}x; torch vs host theta {worst_torch:.1e}); FD (h=1e-3) max rel err theta {worst:.2e}, phi0 dir {worst_dir:.2e}; "
-        f"extrapolated: theta {worst_r:.2e}, phi0 dir {worst_dir_r:.2e}; "
-        f"FP64 64^3: theta {fp64_theta:.2e}, phi0 {fp64_phi0:.2e}")
+        f"({out['adjoint_over_plain']:.2f}x; torch vs host theta {worst_torch:.1e}); FD of the FP64 replay "
+        f"(h={fd64_h:g}): theta {w64:.2e}, phi0 dir {w64_dir:.2e}; FD (h=1e-3) of the FP32 replay: theta "
+        f"{worst:.2e}, phi0 dir {worst_dir:.2e} (FP64 replay at h=1e-3: theta {w64_h3:.2e}); "
+        f"FP64 adjoint 64^3: theta {fp64_theta:.2e}, phi0 {fp64_phi0:.2e}")
     return out
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="phantom", choices=["ellipsoids", "phantom", "vessels"])
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="vessels", choices=["ellipsoids", "phantom", "vessels"])
     ap.add_argument("--iterations", type=int, default=0, help="override the workload's iteration count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--phi", default="f32", choices=["f32", "i8"],
                     help="phi storage: FP32 (default) or the reference's fixed-point signed char")
+    ap.add_argument("--no-phantom", action="store_true", help="skip the phantom workload beside the headline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-sequential comparison")
     ap.add_argument("--batch-iterations", type=int, default=0, help="override the batch workload's iterations")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-form comparison")
     ap.add_argument("--adjoint-iterations", type=int, default=0, help="override the adjoint workload's iterations")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="single-core baseline sample budget")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="bound of each single-core baseline sample")
+    ap.add_argument("--window-at", type=int, default=450,
+                    help="first iteration of the 512^3 workload's steady-state CPU window")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (HBM counter calibration)")
-    ap.add_argument("--skip-profiled", action="store_true",
-                    help="stop after the timed steps (for hardware-counter passes over the timed dispatches only)")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only one per-launch profiled run of --workload (for rocprofv3 passes)")
+    ap.add_argument("--counts-out", default="", help="with --profile-only: where to write the run's counts (JSON)")
     ap.add_argument("--summarize-pmc", default="", metavar="DIR",
                     help="only summarize rocprofv3 --pmc runs under DIR (fetch/, write/, copy_fetch/, copy_write/, "
-                         "copy_fetch.json) into profiles/hbm_traffic_<workload>.json")
+                         "copy_fetch.json, counts.json) into profiles/hbm_traffic_<workload>.json")
+    ap.add_argument("--summarize-trace", default="", metavar="DIR",
+                    help="only summarize a rocprofv3 --kernel-trace run under DIR (kt/, counts.json)")
     args = ap.parse_args()
-    if not (args.no_adjoint or args.summarize_pmc or args.calibrate_copy):
-        # the differentiable form runs through torch, whose HIP runtime must be
-        # the process's: imported before the engine library is loaded
-        import torch  # noqa: F401
 
     if args.summarize_pmc:
         out = summarize_pmc(args.summarize_pmc, args.workload)
@@ -460,6 +827,15 @@ def main():
             json.dump(out, f, indent=1)
         print(json.dumps(out, indent=1))
         return
+    if args.summarize_trace:
+        print(json.dumps(summarize_trace(args.summarize_trace), indent=1))
+        return
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
 
     if args.calibrate_copy:
         import torch
@@ -473,115 +849,19 @@ def main():
         print(json.dumps({"calibrate_copy_bytes": 4 * n, "reads_per_launch_KiB": 4 * n / 1024,
                           "writes_per_launch_KiB": 4 * n / 1024}), flush=True)
         return
-
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-
-    from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import PHI_F32, PHI_I8, Context
-
-    t = time.perf_counter()
-    w = make_workload(args.workload, args.iterations)
-    log(f"workload generated in {time.perf_counter() - t:.1f} s")
-    phi_storage = PHI_I8 if args.phi == "i8" else PHI_F32
-    ctx = Context(w.volume, phi_storage=phi_storage)
-    ctx.set_phi(w.phi0, w.band)
-    ctx.checkpoint_save(0)
-    S = w.volume.dtype.itemsize
-    P = 1 if args.phi == "i8" else 4
-    log(f"context ready: {describe(w, ctx.X, ctx.Y, ctx.Z)}; n0={ctx.active_count}")
-
-    for _ in range(args.warmup):
-        ctx.checkpoint_restore(0)
-        ctx.iterate(w.T, w.eps, w.alpha, w.dt, w.iterations)
-    log("warmup done")
-
-    device_sync()
-    t0 = time.perf_counter()
-    last = None
-    for _ in range(args.steps):
-        ctx.checkpoint_restore(0)
-        last = ctx.iterate(w.T, w.eps, w.alpha, w.dt, w.iterations)
-    device_sync()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / max(args.steps, 1)
-    updates_per_step = int(last.sum_active) if last else 0
-    value = updates_per_step * args.steps / elapsed if elapsed > 0 else 0.0
-    gpu_mask = ctx.mask()
-    counts = ctx.iteration_counts()
-    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {updates_per_step} updates/step, "
-        f"few/many iterations {last.few_path_iters}/{last.many_path_iters}, final n={last.active_final}")
-    if args.skip_profiled:
-        ctx.close()
+    if args.profile_only:
+        profile_only(args)
         return
+    if not args.no_adjoint:
+        # the differentiable form runs through torch, whose HIP runtime must be
+        # the process's: imported before the engine library is loaded
+        import torch  # noqa: F401
 
-    # the same run from and to host arrays: context creation (upload, pad and
-    # tile the source), phi0 upload, the iterations and the mask download
-    device_sync()
-    t0 = time.perf_counter()
-    hctx = Context(w.volume, phi_storage=phi_storage)
-    hctx.set_phi(w.phi0, w.band)
-    hst = hctx.iterate(w.T, w.eps, w.alpha, w.dt, w.iterations)
-    hmask = hctx.mask()
-    host_ms = 1e3 * (time.perf_counter() - t0)
-    assert np.array_equal(hmask, gpu_mask), "host-inclusive run diverged from the timed run"
-    hctx.close()
-    host_inclusive = {"ms": host_ms, "value": int(hst.sum_active) / (host_ms * 1e-3), "unit": "active-voxel updates/s",
-                      "what": "lsseg_create (source upload, pad, tile) + lsseg_set_phi + lsseg_iterate + "
-                              "lsseg_get_mask, host arrays in and out, one run"}
-    log(f"host-inclusive run: {host_ms:.2f} ms ({host_inclusive['value']:.3e} updates/s)")
-
-    # per-launch device times (HIP events on the engine's stream), in issue
-    # order: update, marking and keep of each iteration (the update fused with
-    # the previous compaction from the second iteration on), then the last
-    # compaction.
-    ctx.checkpoint_restore(0)
-    pst, kt, per_launch = ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, w.iterations)
-    pc = ctx.iteration_counts()
-    assert np.array_equal(pc.n, counts.n), "profiled run diverged from the timed run"
-    B = algorithmic_bytes(pc.n, pc.c, pc.m, S, P)
-    it = w.iterations
-    assert it >= 2 and per_launch.size == 3 * it + 1, f"expected {3 * it + 1} launches, got {per_launch.size}"
-    per_launch = per_launch.astype(np.float64)
-    head = per_launch[0:3 * it:3]
-    parts = {
-        # launch k >= 1: the update of iteration k and the compaction of k - 1
-        "update_fused": {"kernel": "k_update_fused", "bytes": B["update"][1:] + B["compact"][:-1], "ms": head[1:]},
-        "mark": {"kernel": "k_mark", "bytes": B["mark"], "ms": per_launch[1:3 * it:3]},
-        "keep": {"kernel": "k_keep", "bytes": B["keep"], "ms": per_launch[2:3 * it:3]},
-        # the first update and the last compaction run on their own
-        "ends": {"kernel": "k_update + k_compact", "bytes": np.array([B["update"][0], B["compact"][-1]]),
-                 "ms": np.array([head[0], per_launch[-1]])},
-    }
-    kernels = {}
-    for k, v in parts.items():
-        tot_ms = float(v["ms"].sum())
-        launches = v["ms"].size
-        avg_us = 1e3 * tot_ms / launches
-        bpl = float(v["bytes"].sum()) / launches
-        kernels[k] = {"kernel": v["kernel"], "avg_launch_us": avg_us, "bytes_per_launch": bpl,
-                      "achieved_GBs": bpl / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0, "total_ms": tot_ms}
-        kernels[k]["launches"] = launches
-        nk = pc.n[1:] if k == "update_fused" else (pc.n if v["ms"].size == it else pc.n[[0, -1]])
-        big = nk >= np.percentile(nk, 90)
-        kernels[k]["busiest_decile_GBs"] = (float(v["bytes"][big].sum()) / (float(v["ms"][big].sum()) * 1e-3) / 1e9
-                                            if big.any() and v["ms"][big].sum() > 0 else 0.0)
-    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
-    d = kernels[dom]
-    traffic, traffic_src = load_traffic(args.workload)
-    traffic_bytes = None
-    if traffic and traffic.get("kernels", {}).get(d["kernel"]):
-        traffic_bytes = traffic["kernels"][d["kernel"]]["bytes_per_launch"]
-    avg_us, bytes_per_launch, achieved = d["avg_launch_us"], d["bytes_per_launch"], d["achieved_GBs"]
-    path_bytes = float(sum(v.sum() for v in B.values()))
-    path_gbs = path_bytes / (ms_per_step * 1e-3) / 1e9
-    log("per launch: " + "; ".join(f"{k} {v['avg_launch_us']:.2f} us, {v['bytes_per_launch'] / 1e6:.2f} MB, "
-                                   f"{v['achieved_GBs']:.0f} GB/s (busiest decile {v['busiest_decile_GBs']:.0f})"
-                                   for k, v in kernels.items())
-        + f"; kernel time sum {kt.total_ms:.2f} ms vs {ms_per_step:.2f} ms/step")
-
+    window_at = args.window_at if args.workload == "vessels" and not args.iterations else None
+    head = plain_leg(args.workload, args, not args.no_cpu_baseline, window_at=window_at)
+    phantom = None
+    if args.workload != "phantom" and not args.no_phantom:
+        phantom = plain_leg("phantom", args, not args.no_cpu_baseline)
     batched = None
     if not args.no_batch:
         batched = batch_vs_sequential(max(1, min(args.steps, 3)), args.batch_iterations)
@@ -589,53 +869,32 @@ def main():
     if not args.no_adjoint:
         adjoint = adjoint_vs_plain(args.adjoint_iterations)
 
-    cpu = None
-    dice = None
-    identical = None
-    if not args.no_cpu_baseline:
-        log("running the CPU oracle (full workload on all cores, then a single-core sample)")
-        cpu, dice, identical = cpu_baseline(w, gpu_mask, args.cpu_seconds, 1 if args.phi == "i8" else 0)
-        log(f"cpu baseline {cpu['value']:.3e} updates/s on {cpu['cores']} cores; Dice vs oracle {dice:.6f}")
-
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "active-voxel updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.phi == "f32" else "i8",
         "data": "synthetic",
-        "config": {
-            "workload": describe(w, ctx.X, ctx.Y, ctx.Z),
-            "volume_xyz": list(w.shape_xyz),
-            "iterations": w.iterations,
-            "active_voxel_updates_per_step": updates_per_step,
-            "parallelism": "single GPU",
-            "phi_storage": "fixed-point int8 (LEVEL_SET_CHAR)" if args.phi == "i8" else "FP32",
-        },
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes,
-            "traffic_source": traffic_src,
-            "kernel": d["kernel"], "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_us,
-        },
-        "path_roofline": {"achieved": path_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": path_gbs / HBM_PEAK_GBS, "bytes_per_step": path_bytes},
-        "kernels": kernels,
-        "cpu_baseline": cpu,
-        "host_inclusive": host_inclusive,
+        "config": head["config"],
+        "roofline": head["roofline"],
+        "path_roofline": head["path_roofline"],
+        "kernels": head["kernels"],
+        "cpu_baseline": head["cpu_baseline"],
+        "host_inclusive": head["host_inclusive"],
+        "dice_vs_oracle": head["dice_vs_oracle"],
+        "mask_identical_to_oracle": head["mask_identical_to_oracle"],
+        "phantom": phantom,
         "batched": batched,
         "adjoint": adjoint,
-        "dice_vs_oracle": dice,
-        "mask_identical_to_oracle": identical,
     }
     print(json.dumps(out), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":

@@ -245,12 +245,15 @@ typedef struct lsseg_kernel_times {
     double total_ms;     /* sum over all iteration launches */
     double first_ms;     /* the first launch */
     double last_ms;      /* the last launch (the final compaction) */
-    uint64_t launches;   /* 3 * iters + 1 */
+    uint64_t launches;   /* 3 * L + 1 for L launched iterations */
 } lsseg_kernel_times;
 
 /* lsseg_iterate with per-launch event timing (launches issued one by one, no
    graph).  Results and statistics are identical to lsseg_iterate.  launch_ms
-   (may be NULL) receives up to launch_cap per-launch times in launch order. */
+   (may be NULL) receives up to launch_cap per-launch times in launch order.
+   Like lsseg_iterate it stops launching once the list is empty (checked every
+   64 iterations), so L <= iters iterations are launched; stats->empty_skipped
+   counts the rest. */
 lsseg_status lsseg_iterate_profiled(lsseg_ctx* ctx, const lsseg_params* params, int32_t iters,
                                     lsseg_stats* stats, lsseg_kernel_times* times,
                                     float* launch_ms, uint64_t launch_cap);

@@ -17,7 +17,12 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "liblsoracle.so")
+# LSORACLE_LIBRARY selects another build of the same sources (the sanitizer
+# test loads the ASan/UBSan build through it)
+_LIB_PATH = os.environ.get("LSORACLE_LIBRARY") or os.path.join(_HERE, "_build", "liblsoracle.so")
+# the same sources built with -march=native on the machine that runs them (the
+# CPU baseline); the portable build above is what the tests load
+_NATIVE_PATH = os.path.join(_HERE, "_build", "liblsoracle_native.so")
 
 U8, I8, U16, I16, U32, I32, F32 = range(7)
 MODE_REF, MODE_CLEAN = 0, 1
@@ -28,7 +33,7 @@ NP_DTYPE = {U8: np.uint8, I8: np.int8, U16: np.uint16, I16: np.int16,
             U32: np.uint32, I32: np.int32, F32: np.float32}
 DTYPE_CODE = {np.dtype(v): k for k, v in NP_DTYPE.items()}
 
-_lib = None
+_libs = {}
 
 
 def build() -> str:
@@ -37,12 +42,22 @@ def build() -> str:
     return _LIB_PATH
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
-            build()
-        L = ctypes.CDLL(_LIB_PATH)
+def build_native() -> str:
+    """Compile the oracle for this machine's CPU (-march=native), for the CPU
+    baseline; built where it runs, since the build machine's vector extensions
+    need not exist on the machine that times it."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "native"], check=True)
+    return _NATIVE_PATH
+
+
+def lib(native: bool = False):
+    """The oracle library (native: the -march=native build of build_native)."""
+    L = _libs.get(native)
+    if L is None:
+        path = _NATIVE_PATH if native else _LIB_PATH
+        if not os.path.exists(path):
+            build_native() if native else build()
+        L = ctypes.CDLL(path)
         c_i32, c_u32, c_i64, c_u64, c_f, c_d, vp = (ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64,
                                                     ctypes.c_uint64, ctypes.c_float, ctypes.c_double,
                                                     ctypes.c_void_p)
@@ -66,6 +81,10 @@ def lib():
         L.lso_compact_few.argtypes = [c_i32, c_i32, c_i32, vp, c_i64, vp, vp]
         L.lso_compact_many.restype = c_i64
         L.lso_compact_many.argtypes = [c_i32, c_i32, c_i32, vp, c_i64, vp, vp]
+        L.lso_compact_few_par.restype = c_i64
+        L.lso_compact_few_par.argtypes = [c_i32, c_i32, c_i32, vp, c_i64, vp, vp, vp, vp, c_i32]
+        L.lso_compact_many_par.restype = c_i64
+        L.lso_compact_many_par.argtypes = [c_i32, c_i32, c_i32, vp, c_i64, vp, vp, vp, c_i32]
         L.lso_create.restype = vp
         L.lso_create.argtypes = [c_i32, c_i32, c_i32, c_i32, vp, c_f]
         L.lso_destroy.restype = None
@@ -104,8 +123,8 @@ def lib():
         L.lso_clear.argtypes = [vp, c_i32]
         L.lso_add_export.restype = None
         L.lso_add_export.argtypes = [vp, vp, c_i64]
-        _lib = L
-    return _lib
+        _libs[native] = L
+    return L
 
 
 def _p(a: np.ndarray):
@@ -188,6 +207,31 @@ def compact_many(X, Y, Z, V, masks):
     return out[:m].copy()
 
 
+def compact_few_par(X, Y, Z, V, masks, threads=4):
+    """compact_few with OpenMP (the CPU baseline's path); masks as the marking
+    makes them (no out-of-bounds emission)."""
+    L = lib()
+    V = np.ascontiguousarray(V, dtype=np.uint32)
+    masks = np.ascontiguousarray(masks, dtype=np.uint8)
+    out = np.zeros(max(7 * len(V), 1), np.uint32)
+    tag = np.zeros(X * Y * Z, np.uint8)
+    keep = np.zeros(max(len(V), 1), np.uint8)
+    m = L.lso_compact_few_par(X, Y, Z, _p(V), len(V), _p(masks), _p(out), _p(tag), _p(keep), threads)
+    assert not tag.any(), "the tag scratch must come back zero"
+    return out[:m].copy()
+
+
+def compact_many_par(X, Y, Z, V, masks, threads=4):
+    L = lib()
+    V = np.ascontiguousarray(V, dtype=np.uint32)
+    masks = np.ascontiguousarray(masks, dtype=np.uint8)
+    out = np.zeros(X * Y * Z, np.uint32)
+    flag = np.zeros(X * Y * Z, np.uint8)
+    m = L.lso_compact_many_par(X, Y, Z, _p(V), len(V), _p(masks), _p(out), _p(flag), threads)
+    assert not flag.any(), "the flag scratch must come back zero"
+    return out[:m].copy()
+
+
 def path_is_few(n, N):
     return bool(lib().lso_path_is_few(n, N))
 
@@ -195,9 +239,10 @@ def path_is_few(n, N):
 class State:
     """Owning wrapper of lso_state (the oracle's Segmenter)."""
 
-    def __init__(self, volume_zyx: np.ndarray, z_aniso: float = 1.0, phi_storage: int = 0):
-        """phi_storage 1: the reference's fixed-point (signed char) phi."""
-        L = lib()
+    def __init__(self, volume_zyx: np.ndarray, z_aniso: float = 1.0, phi_storage: int = 0, native: bool = False):
+        """phi_storage 1: the reference's fixed-point (signed char) phi; native:
+        run on the -march=native build (the CPU baseline)."""
+        L = self._L = lib(native)
         vol = np.ascontiguousarray(volume_zyx)
         self.dtype_code = DTYPE_CODE[vol.dtype]
         self.nz, self.ny, self.nx = vol.shape
@@ -211,7 +256,7 @@ class State:
 
     def close(self):
         if self._h:
-            lib().lso_destroy(self._h)
+            self._L.lso_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -223,44 +268,44 @@ class State:
     def set_phi(self, phi0_zyx: np.ndarray, band: float = 1.0):
         phi = np.ascontiguousarray(phi0_zyx, dtype=np.float32)
         assert phi.shape == (self.nz, self.ny, self.nx)
-        lib().lso_set_phi(self._h, _p(phi), band)
+        self._L.lso_set_phi(self._h, _p(phi), band)
 
     def seed_sphere(self, cx, cy, cz, radius):
-        lib().lso_seed_sphere(self._h, cx, cy, cz, radius)
+        self._L.lso_seed_sphere(self._h, cx, cy, cz, radius)
 
     def auto_parameters(self):
         """(T, eps, count): CalculateSegmentationParameters in the reference's float arithmetic."""
         T = ctypes.c_float()
         eps = ctypes.c_float()
-        n = lib().lso_auto_parameters(self._h, ctypes.byref(T), ctypes.byref(eps))
+        n = self._L.lso_auto_parameters(self._h, ctypes.byref(T), ctypes.byref(eps))
         return T.value, eps.value, int(n)
 
     def iterate(self, T, eps, alpha, dt, iters, mode=MODE_REF, threads=1):
         n = np.zeros(max(iters, 1), np.uint32)
         c = np.zeros(max(iters, 1), np.uint64)
         m = np.zeros(max(iters, 1), np.uint32)
-        lib().lso_iterate(self._h, T, eps, alpha, dt, mode, iters, threads, _p(n), _p(c), _p(m))
+        self._L.lso_iterate(self._h, T, eps, alpha, dt, mode, iters, threads, _p(n), _p(c), _p(m))
         return n[:iters], c[:iters], m[:iters]
 
     @property
     def n(self) -> int:
-        return int(lib().lso_active_count(self._h))
+        return int(self._L.lso_active_count(self._h))
 
     def active(self) -> np.ndarray:
         out = np.zeros(max(self.n, 1), np.uint32)
-        lib().lso_get_active(self._h, _p(out), len(out))
+        self._L.lso_get_active(self._h, _p(out), len(out))
         return out[:self.n].copy()
 
     def state(self):
         r = np.zeros(self.N, np.float32)
         w = np.zeros(self.N, np.float32)
         e = np.zeros(self.N, np.int8)
-        lib().lso_get_state(self._h, _p(r), _p(w), _p(e))
+        self._L.lso_get_state(self._h, _p(r), _p(w), _p(e))
         return r, w, e
 
     def set_state(self, phi_read, phi_write, E, V):
         V = np.ascontiguousarray(V, dtype=np.uint32)
-        rc = lib().lso_set_state(self._h, _p(np.ascontiguousarray(phi_read, np.float32)),
+        rc = self._L.lso_set_state(self._h, _p(np.ascontiguousarray(phi_read, np.float32)),
                                  _p(np.ascontiguousarray(phi_write, np.float32)),
                                  _p(np.ascontiguousarray(E, np.int8)), _p(V), len(V))
         if rc != 0:
@@ -268,22 +313,22 @@ class State:
 
     def phi(self, which: int = 0) -> np.ndarray:
         out = np.zeros((self.nz, self.ny, self.nx), np.float32)
-        lib().lso_get_phi(self._h, which, _p(out))
+        self._L.lso_get_phi(self._h, which, _p(out))
         return out
 
     def mask(self) -> np.ndarray:
         out = np.zeros((self.nz, self.ny, self.nx), np.uint8)
-        lib().lso_get_mask(self._h, _p(out))
+        self._L.lso_get_mask(self._h, _p(out))
         return out
 
     def iterate_reinit(self, T, eps, alpha, dt):
-        lib().lso_iterate_reinit(self._h, T, eps, alpha, dt)
+        self._L.lso_iterate_reinit(self._h, T, eps, alpha, dt)
 
     def freeze(self):
-        lib().lso_freeze(self._h)
+        self._L.lso_freeze(self._h)
 
     def clear(self, all_segmentations: bool = False):
-        lib().lso_clear(self._h, 1 if all_segmentations else 0)
+        self._L.lso_clear(self._h, 1 if all_segmentations else 0)
 
 
 def add_export(result: np.ndarray, add: np.ndarray) -> np.ndarray:

@@ -194,6 +194,137 @@ int64_t lso_compact_many(int32_t X, int32_t Y, int32_t Z, const uint32_t* V, int
 }
 
 /* ------------------------------------------------------------------------- */
+/* the same two compactions with OpenMP (the CPU baseline's path)             */
+/* ------------------------------------------------------------------------- */
+
+#define LSO_SCAN_BLOCK 4096 /* positions or voxels per counting block */
+
+static int lso_nthreads(int32_t threads) { return threads > 0 ? threads : 1; }
+
+/* Exclusive prefix of cnt[0..k) in place; returns the total. */
+static int64_t lso_exclusive_scan(int64_t* cnt, int64_t k)
+{
+    int64_t run = 0;
+    for (int64_t i = 0; i < k; ++i) {
+        const int64_t c = cnt[i];
+        cnt[i] = run;
+        run += c;
+    }
+    return run;
+}
+
+/* lso_compact_few with identical output, parallel over list positions.
+   Within one candidate list the targets are distinct (V is duplicate-free and
+   a direction shifts voxels injectively; out-of-bounds neighbours are never
+   emitted), so first-wins over the list-major concatenation is decided list by
+   list in list order (FilterDuplicates.cu:42-66), each list in parallel: a
+   candidate is kept iff its target is not yet tagged, and then tags it.  The
+   kept candidates are placed by a prefix sum over (list, block of positions).
+   tag (N bytes) and keep (n bytes) are caller scratch; tag must be zero on
+   entry and is zero again on return (cleared at the placed voxels, which are
+   exactly the tagged ones). */
+int64_t lso_compact_few_par(int32_t X, int32_t Y, int32_t Z, const uint32_t* V, int64_t n,
+                            const uint8_t* masks, uint32_t* out, uint8_t* tag, uint8_t* keep,
+                            int32_t threads)
+{
+    const int nt = lso_nthreads(threads);
+    (void)nt;
+    for (int32_t e = 0; e < 7; ++e) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+        for (int64_t p = 0; p < n; ++p) {
+            uint8_t k = e ? keep[p] : 0;
+            if (masks[p] & (1u << e)) {
+                const uint32_t v = lso_neighbor(X, Y, Z, V[p], e);
+                if (!tag[v]) {
+                    tag[v] = 1;
+                    k |= (uint8_t)(1u << e);
+                }
+            }
+            keep[p] = k;
+        }
+    }
+    const int64_t nb = (n + LSO_SCAN_BLOCK - 1) / LSO_SCAN_BLOCK;
+    int64_t* cnt = (int64_t*)calloc((size_t)(7 * nb + 1), sizeof(int64_t));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+    for (int64_t i = 0; i < 7 * nb; ++i) {
+        const int32_t e = (int32_t)(i / nb);
+        const int64_t p0 = (i % nb) * LSO_SCAN_BLOCK;
+        const int64_t p1 = p0 + LSO_SCAN_BLOCK < n ? p0 + LSO_SCAN_BLOCK : n;
+        int64_t c = 0;
+        for (int64_t p = p0; p < p1; ++p) c += (keep[p] >> e) & 1u;
+        cnt[i] = c;
+    }
+    const int64_t m = lso_exclusive_scan(cnt, 7 * nb);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+    for (int64_t i = 0; i < 7 * nb; ++i) {
+        const int32_t e = (int32_t)(i / nb);
+        const int64_t p0 = (i % nb) * LSO_SCAN_BLOCK;
+        const int64_t p1 = p0 + LSO_SCAN_BLOCK < n ? p0 + LSO_SCAN_BLOCK : n;
+        int64_t at = cnt[i];
+        for (int64_t p = p0; p < p1; ++p)
+            if ((keep[p] >> e) & 1u) out[at++] = lso_neighbor(X, Y, Z, V[p], e);
+    }
+    free(cnt);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+    for (int64_t i = 0; i < m; ++i) tag[out[i]] = 0;
+    return m;
+}
+
+/* lso_compact_many with identical output: dense flags set in parallel (several
+   emitters may set one flag; the stores are atomic), then the flagged voxels
+   in ascending tau by a prefix sum over blocks of voxels.  flag (N bytes) is
+   caller scratch, zero on entry and on return. */
+int64_t lso_compact_many_par(int32_t X, int32_t Y, int32_t Z, const uint32_t* V, int64_t n,
+                             const uint8_t* masks, uint32_t* out, uint8_t* flag, int32_t threads)
+{
+    const int nt = lso_nthreads(threads);
+    (void)nt;
+    const uint64_t N = (uint64_t)X * Y * Z;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+    for (int64_t p = 0; p < n; ++p)
+        for (int32_t e = 0; e < 7; ++e)
+            if (masks[p] & (1u << e)) __atomic_store_n(&flag[lso_neighbor(X, Y, Z, V[p], e)], 1, __ATOMIC_RELAXED);
+    const int64_t nb = (int64_t)((N + LSO_SCAN_BLOCK - 1) / LSO_SCAN_BLOCK);
+    int64_t* cnt = (int64_t*)calloc((size_t)(nb + 1), sizeof(int64_t));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+    for (int64_t b = 0; b < nb; ++b) {
+        const uint64_t t0 = (uint64_t)b * LSO_SCAN_BLOCK;
+        const uint64_t t1 = t0 + LSO_SCAN_BLOCK < N ? t0 + LSO_SCAN_BLOCK : N;
+        int64_t c = 0;
+        for (uint64_t t = t0; t < t1; ++t) c += flag[t] != 0;
+        cnt[b] = c;
+    }
+    const int64_t m = lso_exclusive_scan(cnt, nb);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+    for (int64_t b = 0; b < nb; ++b) {
+        const uint64_t t0 = (uint64_t)b * LSO_SCAN_BLOCK;
+        const uint64_t t1 = t0 + LSO_SCAN_BLOCK < N ? t0 + LSO_SCAN_BLOCK : N;
+        int64_t at = cnt[b];
+        for (uint64_t t = t0; t < t1; ++t)
+            if (flag[t]) {
+                out[at++] = (uint32_t)t;
+                flag[t] = 0;
+            }
+    }
+    free(cnt);
+    return m;
+}
+
+/* ------------------------------------------------------------------------- */
 /* stateful engine (Segmenter.cpp)                                            */
 /* ------------------------------------------------------------------------- */
 
@@ -214,6 +345,8 @@ struct lso_state {
     uint8_t* flags;          /* per position */
     uint8_t* masks;          /* per position */
     int32_t phi_i8;          /* fixed-point phi storage (LEVEL_SET_CHAR, Config.hpp:72-80) */
+    uint8_t* scratch;        /* N bytes, zero between iterations: tags / dense flags */
+    uint8_t* keep;           /* kept candidate lists per position */
 };
 
 static void* xcalloc(uint64_t count, uint64_t size)
@@ -255,6 +388,8 @@ lso_state* lso_create(int32_t nx, int32_t ny, int32_t nz, int32_t dtype, const v
     s->Vn = (uint32_t*)xcalloc(s->N, sizeof(uint32_t));
     s->flags = (uint8_t*)xcalloc(s->N, 1);
     s->masks = (uint8_t*)xcalloc(s->N, 1);
+    s->scratch = (uint8_t*)xcalloc(s->N, 1);
+    s->keep = (uint8_t*)xcalloc(s->N, 1);
     /* Segmenter.cpp:150-151 fills both buffers with the int8 value 127 (= 1.0)
        and the export with 127 (:265-266) */
     for (uint64_t i = 0; i < s->N; ++i) {
@@ -280,6 +415,8 @@ void lso_destroy(lso_state* s)
     free(s->Vn);
     free(s->flags);
     free(s->masks);
+    free(s->scratch);
+    free(s->keep);
     free(s);
 }
 
@@ -431,15 +568,19 @@ int32_t lso_iterate(lso_state* s, float T, float eps, float alpha, float dt, int
         s->rd = 1 - s->rd;
         lso_mark(s->X, s->Y, s->Z, s->phi[s->rd], s->flags, s->V, s->n, mode, s->masks, threads);
         uint64_t c = 0;
-        for (int64_t p = 0; p < s->n; ++p) {
-            uint8_t m = s->masks[p];
-            while (m) { c += m & 1u; m >>= 1; }
-        }
+        const uint8_t* masks = s->masks;
+        const int64_t n = s->n;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) reduction(+ : c) num_threads(threads > 0 ? threads : 1)
+#endif
+        for (int64_t p = 0; p < n; ++p) c += (uint64_t)__builtin_popcount(masks[p]);
+        /* the compactions of lso_compact_few / lso_compact_many, in parallel
+           with identical output (tests/test_oracle_lists.py pins them) */
         int64_t m;
         if (lso_path_is_few((uint64_t)s->n, s->N))
-            m = lso_compact_few(s->X, s->Y, s->Z, s->V, s->n, s->masks, s->Vn);
+            m = lso_compact_few_par(s->X, s->Y, s->Z, s->V, s->n, s->masks, s->Vn, s->scratch, s->keep, threads);
         else
-            m = lso_compact_many(s->X, s->Y, s->Z, s->V, s->n, s->masks, s->Vn);
+            m = lso_compact_many_par(s->X, s->Y, s->Z, s->V, s->n, s->masks, s->Vn, s->scratch, threads);
         if (n_hist) n_hist[it] = (uint32_t)s->n;
         if (c_hist) c_hist[it] = c;
         if (m_hist) m_hist[it] = (uint32_t)m;

@@ -71,6 +71,13 @@ int64_t lso_compact_few(int32_t X, int32_t Y, int32_t Z, const uint32_t* V, int6
                         const uint8_t* masks, uint32_t* out);
 int64_t lso_compact_many(int32_t X, int32_t Y, int32_t Z, const uint32_t* V, int64_t n,
                          const uint8_t* masks, uint32_t* out);
+/* the same compactions in parallel with identical output; tag / flag: N bytes
+   of zeroed scratch (zero again on return), keep: n bytes of scratch */
+int64_t lso_compact_few_par(int32_t X, int32_t Y, int32_t Z, const uint32_t* V, int64_t n,
+                            const uint8_t* masks, uint32_t* out, uint8_t* tag, uint8_t* keep,
+                            int32_t threads);
+int64_t lso_compact_many_par(int32_t X, int32_t Y, int32_t Z, const uint32_t* V, int64_t n,
+                             const uint8_t* masks, uint32_t* out, uint8_t* flag, int32_t threads);
 
 /* ---- stateful engine ------------------------------------------------------ */
 typedef struct lso_state lso_state;

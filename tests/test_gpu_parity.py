@@ -211,18 +211,22 @@ def test_fused_and_separate_launches_agree(monkeypatch, wide):
     assert a[3] == 3 * 5 + 1 and b[3] == 4 * 5
 
 
-def test_compaction_tile_sizes_agree(monkeypatch):
+@pytest.mark.parametrize("pack", ["1", "0"], ids=["packed-counts", "one-count-per-word"])
+def test_compaction_tile_sizes_agree(monkeypatch, pack):
     """Few-path compactions take tiles of 1024 list positions, or of 256 for
     lists up to LSSEG_SMALL_TILE_N positions, and the workgroups that load
     their tile together with the counts keep whichever layout the plan needs.
     Large tiles only, small tiles only, and a threshold inside the run's range
-    of list sizes give the same states, counts and statistics."""
+    of list sizes give the same states, counts and statistics.  With
+    LSSEG_LOOKBACK_PACK=0 each chunk publishes its seven counts one to a
+    look-back word (the layout of chunks of 2^16 positions or more)."""
+    monkeypatch.setenv("LSSEG_LOOKBACK_PACK", pack)
     w = synth.ellipsoids(edge=64, iterations=70)
     out = {}
     for small in ("0", "1000000000", None):
         if small is None:  # the median list size of the first run: the run crosses it
             small = str(int(np.median(out["0"][1].n)))
-        _say(f"compaction tiles: small up to {small} positions")
+        _say(f"compaction tiles: small up to {small} positions, LSSEG_LOOKBACK_PACK={pack}")
         monkeypatch.setenv("LSSEG_SMALL_TILE_N", small)
         ctx = Context(w.volume)
         ctx.set_phi(w.phi0, w.band)
@@ -240,14 +244,18 @@ def test_compaction_tile_sizes_agree(monkeypatch):
             assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
 
 
+@pytest.mark.parametrize("pack", ["1", "0"], ids=["packed-counts", "one-count-per-word"])
 @pytest.mark.parametrize("wide", [False, True], ids=["few", "dense"])
-def test_chunks_of_several_subtiles(monkeypatch, wide):
+def test_chunks_of_several_subtiles(monkeypatch, wide, pack):
     """With a 16-workgroup grid the keep step, the compaction (8 workgroups in
     the fused launch) and the dense initial compaction each take several
     sub-tiles per look-back chunk (counted first, placed after the prefix):
-    states and counts still equal the oracle's every chunk."""
-    _say(f"16-workgroup grid, wide band {wide}")
+    states and counts still equal the oracle's every chunk.  The few-path
+    compaction's chunks publish their seven counts two to a look-back word, or
+    (LSSEG_LOOKBACK_PACK=0) one to a word."""
+    _say(f"16-workgroup grid, wide band {wide}, LSSEG_LOOKBACK_PACK={pack}")
     monkeypatch.setenv("LSSEG_GRID_BLOCKS", "16")
+    monkeypatch.setenv("LSSEG_LOOKBACK_PACK", pack)
     if wide:
         n = 32
         z, y, x = np.mgrid[0:n, 0:n, 0:n]
