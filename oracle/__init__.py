@@ -121,6 +121,10 @@ def lib(native: bool = False):
         L.lso_auto_parameters.argtypes = [vp, vp, vp]
         L.lso_clear.restype = None
         L.lso_clear.argtypes = [vp, c_i32]
+        L.lso_literal_begin.restype = c_i32
+        L.lso_literal_begin.argtypes = [vp, c_i32]
+        L.lso_iterate_literal.restype = c_i32
+        L.lso_iterate_literal.argtypes = [vp, c_f, c_f, c_f, c_f, c_i32, c_i32, vp, vp, vp, vp]
         L.lso_add_export.restype = None
         L.lso_add_export.argtypes = [vp, vp, c_i64]
         _libs[native] = L
@@ -321,6 +325,22 @@ class State:
         self._L.lso_get_mask(self._h, _p(out))
         return out
 
+    def literal_begin(self, initial_flags: bool = True):
+        """Start the LITERAL replay (the shipped driver's scratch volumes) from
+        the current state; see lso_literal_begin."""
+        if self._L.lso_literal_begin(self._h, 1 if initial_flags else 0) != 0:
+            raise MemoryError("lso_literal_begin failed")
+
+    def iterate_literal(self, T, eps, alpha, dt, iters, threads=1):
+        """(n, c, m, events[iters, 3]) of `iters` LITERAL iterations."""
+        n = np.zeros(max(iters, 1), np.uint32)
+        c = np.zeros(max(iters, 1), np.uint64)
+        m = np.zeros(max(iters, 1), np.uint32)
+        ev = np.zeros((max(iters, 1), 3), np.uint64)
+        if self._L.lso_iterate_literal(self._h, T, eps, alpha, dt, iters, threads, _p(n), _p(c), _p(m), _p(ev)) != 0:
+            raise RuntimeError("call literal_begin first")
+        return n[:iters], c[:iters], m[:iters], ev[:iters]
+
     def iterate_reinit(self, T, eps, alpha, dt):
         self._L.lso_iterate_reinit(self._h, T, eps, alpha, dt)
 
@@ -329,6 +349,45 @@ class State:
 
     def clear(self, all_segmentations: bool = False):
         self._L.lso_clear(self._h, 1 if all_segmentations else 0)
+
+
+def literal_diagnostic(volume_zyx, phi0_zyx, band, T, eps, alpha, dt, iters, threads=1, chunk=1,
+                       initial_flags=True):
+    """REF and LITERAL runs side by side from the same start (lso_set_phi):
+    {"first_divergence": iteration whose next list differs, or None,
+     "events": summed (stale valid flags compacted, candidates meeting a stale
+     tag, repeated targets within a list), "first_event": first iteration with
+     any event, or None, "iterations": iters, "many_iterations": dense-path
+     iterations of the REF run}.  Lists are compared every `chunk` iterations
+     (and counts every iteration)."""
+    ref = State(volume_zyx)
+    lit = State(volume_zyx)
+    ref.set_phi(phi0_zyx, band)
+    lit.set_phi(phi0_zyx, band)
+    lit.literal_begin(initial_flags)
+    out = {"first_divergence": None, "events": [0, 0, 0], "first_event": None, "iterations": iters,
+           "many_iterations": 0}
+    done = 0
+    while done < iters:
+        k = min(chunk, iters - done)
+        n1, c1, m1 = ref.iterate(T, eps, alpha, dt, k, threads=threads)
+        n2, c2, m2, ev = lit.iterate_literal(T, eps, alpha, dt, k, threads=threads)
+        out["many_iterations"] += int(sum(1 for x in n1 if not path_is_few(int(x), ref.N)))
+        hit = np.nonzero(ev.sum(axis=1))[0]
+        if out["first_event"] is None and hit.size:
+            out["first_event"] = done + int(hit[0])
+        out["events"] = [a + int(b) for a, b in zip(out["events"], ev.sum(axis=0))]
+        diff = np.nonzero((n1 != n2) | (c1 != c2) | (m1 != m2))[0]
+        if diff.size:
+            out["first_divergence"] = done + int(diff[0]) - (1 if diff[0] > 0 and n1[diff[0]] != n2[diff[0]] else 0)
+            break
+        if not np.array_equal(ref.active(), lit.active()):
+            out["first_divergence"] = done + k - 1
+            break
+        done += k
+    ref.close()
+    lit.close()
+    return out
 
 
 def add_export(result: np.ndarray, add: np.ndarray) -> np.ndarray:

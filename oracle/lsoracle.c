@@ -347,6 +347,21 @@ struct lso_state {
     int32_t phi_i8;          /* fixed-point phi storage (LEVEL_SET_CHAR, Config.hpp:72-80) */
     uint8_t* scratch;        /* N bytes, zero between iterations: tags / dense flags */
     uint8_t* keep;           /* kept candidate lists per position */
+    struct lso_literal* lit; /* the shipped driver's scratch buffers (LITERAL mode), or NULL */
+};
+
+/* LITERAL mode: the scratch volumes of the shipped driver, replayed with their
+   stale contents (Segmenter.cpp:491-595, 759-1051) */
+struct lso_literal {
+    uint32_t* coords;        /* coordinate volume; its start doubles as the seven candidate
+                                lists of stride align256(n) (Segmenter.cpp:893-897) */
+    uint8_t* valid;          /* valid-element volume (0/1; u32 in the reference) */
+    uint8_t* tag;            /* tag volume: the update's temporal flags, then the
+                                duplicate removal's scratch */
+    uint32_t* stamp;         /* per voxel: (iteration, list) of its last candidate, to count
+                                duplicates within one list */
+    uint32_t stamp_next;
+    int32_t coords_initialized; /* mCoordinatesVolumeInitialized */
 };
 
 static void* xcalloc(uint64_t count, uint64_t size)
@@ -417,6 +432,13 @@ void lso_destroy(lso_state* s)
     free(s->masks);
     free(s->scratch);
     free(s->keep);
+    if (s->lit) {
+        free(s->lit->coords);
+        free(s->lit->valid);
+        free(s->lit->tag);
+        free(s->lit->stamp);
+        free(s->lit);
+    }
     free(s);
 }
 
@@ -731,4 +753,167 @@ void lso_clear(lso_state* s, int32_t all)
 void lso_add_export(int8_t* result, const int8_t* add, int64_t N)
 {
     for (int64_t i = 0; i < N; ++i) result[i] = add_export_byte(result[i], add[i]);
+}
+
+/* ------------------------------------------------------------------------- */
+/* LITERAL mode (diagnostic): the shipped driver's buffer artefacts           */
+/* ------------------------------------------------------------------------- */
+/* REF mode above gives every iteration clean scratch.  The shipped driver
+   reuses three volumes across iterations, and three artefacts follow:
+   1. the dense keep-flags that seed initialisation writes into the valid
+      volume (Segmenter.cpp:536-554) are never cleared; the few path clears the
+      valid volume only over [0, 7 align256(n)) after each compaction (:977),
+      so a flag at tau j is compacted once 7 align256(n) exceeds j, paired with
+      whatever the coordinate volume holds at j, and it escapes duplicate
+      removal, which scans only positions p < n of each list
+      (FilterDuplicates.cu:90-100, 160-170);
+   2. the many path never clears the tag volume (:985-1019), which holds the
+      update's temporal flags (UpdateLevelSetVolume.cu:327-331); after a switch
+      back to the few path, a stale flag of 1 at a voxel outside the current
+      list makes duplicate removal drop candidates of lists 1..6 for it
+      (FilterDuplicates.cu:127-153);
+   3. duplicates in a list (from 1) break the uniqueness within each candidate
+      list that duplicate removal relies on (main.tex:399): the reference's
+      read-then-tag kernel then races; this replay resolves it in position
+      order and counts it.
+   lso_literal_begin sets the buffers as the reference's initialisation leaves
+   them for the current list (coordinate volume initialised, the list's voxels
+   flagged in the valid volume, tags clear).  On the dense path the stale flags
+   of 1 are all compacted at once: a first dense iteration re-activates the
+   whole initial list.  lso_iterate_literal replays
+   iterations with them, in REF mode's marking (the p % 4 == 3 skip). */
+
+int32_t lso_literal_begin(lso_state* s, int32_t initial_flags)
+{
+    if (!s->lit) {
+        s->lit = (struct lso_literal*)calloc(1, sizeof(struct lso_literal));
+        if (!s->lit) return -1;
+        s->lit->coords = (uint32_t*)xcalloc(s->N, sizeof(uint32_t));
+        s->lit->valid = (uint8_t*)xcalloc(s->N, 1);
+        s->lit->tag = (uint8_t*)xcalloc(s->N, 1);
+        s->lit->stamp = (uint32_t*)xcalloc(s->N, sizeof(uint32_t));
+        if (!s->lit->coords || !s->lit->valid || !s->lit->tag || !s->lit->stamp) return -1;
+    }
+    struct lso_literal* L = s->lit;
+    /* CudaInitializeCoordinateVolume (InitializeCoordinateVolume.cu:37-47): the
+       entry at tau holds the voxel tau */
+    for (uint64_t t = 0; t < s->N; ++t) {
+        L->coords[t] = (uint32_t)t;
+        L->valid[t] = 0;
+        L->tag[t] = 0;
+        L->stamp[t] = 0;
+    }
+    L->stamp_next = 1;
+    L->coords_initialized = 1;
+    /* the dense keep-flags whose compaction gave the current list (initial_flags
+       = 0 starts with a clean valid volume instead, to isolate the other two
+       artefacts) */
+    if (initial_flags)
+        for (int64_t p = 0; p < s->n; ++p) L->valid[s->V[p]] = 1;
+    return 0;
+}
+
+/* events[3 * it + 0]: stale valid flags compacted (artefact 1),
+   events[3 * it + 1]: candidates of lists 1..6 whose target carried a stale
+   tag at the start of duplicate removal (artefact 2),
+   events[3 * it + 2]: valid entries repeating a target within one list (3) */
+int32_t lso_iterate_literal(lso_state* s, float T, float eps, float alpha, float dt, int32_t iters,
+                            int32_t threads, uint32_t* n_hist, uint64_t* c_hist, uint32_t* m_hist,
+                            uint64_t* events)
+{
+    struct lso_literal* L = s->lit;
+    if (!L) return -1;
+    for (int32_t it = 0; it < iters; ++it) {
+        const int64_t n = s->n;
+        const float* R = s->phi[s->rd];
+        float* W = s->phi[1 - s->rd];
+        lso_update_f32(s->X, s->Y, s->Z, s->dtype, s->src, R, W, s->E, s->flags, s->V, n, T, eps, alpha, dt,
+                       threads);
+        store_list(s, W);
+        /* the update stores the temporal flag in the tag volume, where the
+           marking reads it (UpdateActiveElements.cu:75-103) */
+        for (int64_t p = 0; p < n; ++p) L->tag[s->V[p]] = s->flags[p];
+        for (int64_t p = 0; p < n; ++p) s->flags[p] = L->tag[s->V[p]];
+        s->rd = 1 - s->rd;
+        lso_mark(s->X, s->Y, s->Z, s->phi[s->rd], s->flags, s->V, n, LSO_MODE_REF, s->masks, threads);
+        uint64_t c = 0, ev_stale = 0, ev_tag = 0, ev_dup = 0;
+        for (int64_t p = 0; p < n; ++p) c += (uint64_t)__builtin_popcount(s->masks[p]);
+        int64_t m = 0;
+        if (lso_path_is_few((uint64_t)n, s->N)) {
+            const uint64_t A = lso_align256((uint64_t)n);
+            for (uint64_t i = 0; i < 7 * A; ++i) ev_stale += L->valid[i];
+            /* CudaOutputNewActiveElements (UpdateActiveElementsVolume.cu:129-363):
+               coordinates and valid flags of the emitted slots only */
+            for (int64_t p = 0; p < n; ++p)
+                for (int32_t e = 0; e < 7; ++e)
+                    if (s->masks[p] & (1u << e)) {
+                        L->coords[(uint64_t)e * A + p] = lso_neighbor(s->X, s->Y, s->Z, s->V[p], e);
+                        L->valid[(uint64_t)e * A + p] = 1;
+                    }
+            L->coords_initialized = 0; /* UpdateHostStateOptimizeForFewActiveVoxels, :1511-1514 */
+            /* CudaMemSetCharSparse(tag, 0, n) at the current list (:933) */
+            for (int64_t p = 0; p < n; ++p) L->tag[s->V[p]] = 0;
+            /* duplicates within a list, and targets tagged before the filter runs */
+            for (int32_t e = 0; e < 7; ++e) {
+                const uint32_t st = L->stamp_next++;
+                for (int64_t p = 0; p < n; ++p) {
+                    const uint64_t i = (uint64_t)e * A + (uint64_t)p;
+                    if (!L->valid[i]) continue;
+                    const uint32_t v = L->coords[i];
+                    if (L->stamp[v] == st) ++ev_dup;
+                    L->stamp[v] = st;
+                    if (e > 0 && L->tag[v]) ++ev_tag;
+                }
+            }
+            /* CudaFilterDuplicates (FilterDuplicates.cu:16-193): positions p < n */
+            for (int64_t p = 0; p < n; ++p)
+                if (L->valid[p]) L->tag[L->coords[p]] = 1;
+            for (int32_t e = 1; e < 7; ++e)
+                for (int64_t p = 0; p < n; ++p) {
+                    const uint64_t i = (uint64_t)e * A + (uint64_t)p;
+                    if (!L->valid[i]) continue;
+                    if (L->tag[L->coords[i]] == 1) L->valid[i] = 0;
+                    else L->tag[L->coords[i]] = 1;
+                }
+            /* CudppCompact over 7 align256(n) slots (:963-970), then
+               CudaMemSet(valid, 0, 7 align256(n)) (:977) */
+            for (uint64_t i = 0; i < 7 * A; ++i)
+                if (L->valid[i]) {
+                    s->Vn[m++] = L->coords[i];
+                    L->valid[i] = 0;
+                }
+        } else {
+            if (!L->coords_initialized) { /* :992-996 */
+                for (uint64_t t = 0; t < s->N; ++t) L->coords[t] = (uint32_t)t;
+                L->coords_initialized = 1;
+            }
+            for (uint64_t t = 0; t < s->N; ++t) ev_stale += L->valid[t];
+            /* CudaUpdateActiveElementsVolume (UpdateActiveElementsVolume.cu:8-100) */
+            for (int64_t p = 0; p < n; ++p)
+                for (int32_t e = 0; e < 7; ++e)
+                    if (s->masks[p] & (1u << e)) L->valid[lso_neighbor(s->X, s->Y, s->Z, s->V[p], e)] = 1;
+            /* CudppCompact over the volume (:1009-1016), then
+               CudaMemSetIntSparse(valid, 0, n') at the new list (:1018) */
+            for (uint64_t t = 0; t < s->N; ++t)
+                if (L->valid[t]) s->Vn[m++] = L->coords[t];
+            for (int64_t p = 0; p < m; ++p) L->valid[s->Vn[p]] = 0;
+        }
+        if (L->stamp_next > 0xFFFFFF00u) { /* restamp before the counter wraps */
+            memset(L->stamp, 0, (size_t)(s->N * sizeof(uint32_t)));
+            L->stamp_next = 1;
+        }
+        if (n_hist) n_hist[it] = (uint32_t)n;
+        if (c_hist) c_hist[it] = c;
+        if (m_hist) m_hist[it] = (uint32_t)m;
+        if (events) {
+            events[3 * it + 0] = ev_stale;
+            events[3 * it + 1] = ev_tag;
+            events[3 * it + 2] = ev_dup;
+        }
+        uint32_t* tmp = s->V;
+        s->V = s->Vn;
+        s->Vn = tmp;
+        s->n = m;
+    }
+    return 0;
 }

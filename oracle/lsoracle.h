@@ -111,6 +111,14 @@ void       lso_iterate_reinit(lso_state* s, float T, float eps, float alpha, flo
 void       lso_freeze(lso_state* s);
 int64_t    lso_auto_parameters(const lso_state* s, float* T, float* eps);
 void       lso_clear(lso_state* s, int32_t all);
+/* LITERAL mode (diagnostic): replay the shipped driver's scratch volumes with
+   their stale contents (see lsoracle.c); begin from the current state, then
+   iterate; events: 3 per iteration (stale valid flags compacted, candidates
+   meeting a stale tag, repeated targets within a list) */
+int32_t    lso_literal_begin(lso_state* s, int32_t initial_flags);
+int32_t    lso_iterate_literal(lso_state* s, float T, float eps, float alpha, float dt, int32_t iters,
+                               int32_t threads, uint32_t* n_hist, uint64_t* c_hist, uint32_t* m_hist,
+                               uint64_t* events);
 /* union of two export volumes (InitializeLevelSetVolume.cu:151-169) */
 void       lso_add_export(int8_t* result, const int8_t* add, int64_t N);
 
