@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -266,6 +267,37 @@ void launch_update_only(lsseg_ctx* c, int op)
     (void)hipLaunchKernel(fn, dim3(c->grid), dim3(kBlock), args, 0, c->stream);
 }
 
+// The padded geometry and the scan settings of a volume of X x Y x Z padded voxels.
+void init_geom(Geom& g, uint32_t X, uint32_t Y, uint32_t Z)
+{
+    g.X = X;
+    g.Y = Y;
+    g.Z = Z;
+    g.TX = g.X / 16;
+    g.TY = g.Y / 16;
+    g.TZ = g.Z / 16;
+    g.sTY = g.TX * 4096u;
+    g.sTZ = g.TX * g.TY * 4096u;
+    g.N = (uint64_t)X * Y * Z;
+    g.dTX = make_fastdiv(g.TX);
+    g.dTXY = make_fastdiv(g.TX * g.TY);
+    // few-path compactions of lists of up to 2^17 positions take tiles of
+    // kBlock positions: four times the workgroups place the list, at the price
+    // of a longer look-back (phantom 31.5 -> 29.6 ms; 2^16 and 2^17 measured
+    // alike, 2^18 slower; tiles of 2 kBlock above gained nothing)
+    g.small_n = (uint32_t)std::min<uint64_t>(g.N, (uint64_t)std::max(0, env_int("LSSEG_SMALL_TILE_N", 131072)));
+    g.spec_tiles = (uint32_t)std::max(0, env_int("LSSEG_SPEC_TILES", (int)kSpecTiles));
+    g.pack_counts = env_int("LSSEG_LOOKBACK_PACK", 1) ? 1u : 0u;
+}
+
+// look-back chunk states of a volume: one per keep or dense-compaction tile,
+// seven per few-path compaction tile (group states: 7 per 64 of these, + 2)
+uint64_t lookback_states(const Geom& g)
+{
+    return std::max<uint64_t>(g.N / std::min(kScanTile, kKeepTile) + 2,
+                              7 * (std::max<uint64_t>(g.N / kPosTile, g.small_n / kBlock) + 2));
+}
+
 void fill_u64(lsseg_ctx* c, uint64_t* p, uint64_t count, uint64_t v)
 {
     hipLaunchKernelGGL(k_fill_u64, dim3(dense_grid(count)), dim3(kBlock), 0, c->stream, p, count, v);
@@ -382,8 +414,16 @@ lsseg_status sync_and_check(lsseg_ctx* c)
     return LSSEG_OK;
 }
 
-// Validates the parameters and derives the device constants.
+// Validates the parameters and derives the device constants for a source type
+// (errors go to the context, or with c == nullptr to lsseg_last_error(NULL)).
+lsseg_status make_params_for(lsseg_ctx* c, int32_t dtype, const lsseg_params* prm, DevParams* out);
+
 lsseg_status make_params(lsseg_ctx* c, const lsseg_params* prm, DevParams* out)
+{
+    return make_params_for(c, c->dtype, prm, out);
+}
+
+lsseg_status make_params_for(lsseg_ctx* c, int32_t dtype, const lsseg_params* prm, DevParams* out)
 {
     if (!prm) return fail(c, LSSEG_ERR_INVALID_ARG, "params is null");
     if (prm->mode != LSSEG_MODE_REF && prm->mode != LSSEG_MODE_CLEAN)
@@ -392,7 +432,7 @@ lsseg_status make_params(lsseg_ctx* c, const lsseg_params* prm, DevParams* out)
         !std::isfinite(prm->dt))
         return fail(c, LSSEG_ERR_INVALID_ARG, "non-finite parameter");
     DevParams P{};
-    if (c->dtype == LSSEG_F32) {
+    if (dtype == LSSEG_F32) {
         if (!(prm->eps > 0.0f)) return fail(c, LSSEG_ERR_INVALID_ARG, "eps must be positive");
         P.negInv = -1.0f / prm->eps;
         P.Tf = prm->T;
@@ -636,30 +676,10 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     memset(c->h_ctl, 0, sizeof(DevCtl));
 
     Geom& g = c->a.g;
-    g.X = (uint32_t)X;
-    g.Y = (uint32_t)Y;
-    g.Z = (uint32_t)Z;
-    g.TX = g.X / 16;
-    g.TY = g.Y / 16;
-    g.TZ = g.Z / 16;
-    g.sTY = g.TX * 4096u;
-    g.sTZ = g.TX * g.TY * 4096u;
-    g.N = N;
-    g.dTX = make_fastdiv(g.TX);
-    g.dTXY = make_fastdiv(g.TX * g.TY);
-    // few-path compactions of lists of up to 2^17 positions take tiles of
-    // kBlock positions: four times the workgroups place the list, at the price
-    // of a longer look-back (phantom 31.5 -> 29.6 ms; 2^16 and 2^17 measured
-    // alike, 2^18 slower; tiles of 2 kBlock above gained nothing)
-    g.small_n = (uint32_t)std::min<uint64_t>(N, (uint64_t)std::max(0, env_int("LSSEG_SMALL_TILE_N", 131072)));
-    g.spec_tiles = (uint32_t)std::max(0, env_int("LSSEG_SPEC_TILES", (int)kSpecTiles));
-    g.pack_counts = env_int("LSSEG_LOOKBACK_PACK", 1) ? 1u : 0u;
+    init_geom(g, (uint32_t)X, (uint32_t)Y, (uint32_t)Z);
 
     void* p = nullptr;
-    // look-back states: one per keep or dense-compaction tile, seven per
-    // few-path compaction tile
-    const uint64_t ntiles = std::max<uint64_t>(N / std::min(kScanTile, kKeepTile) + 2,
-                                               7 * (std::max<uint64_t>(N / kPosTile, g.small_n / kBlock) + 2));
+    const uint64_t ntiles = lookback_states(g);
     lsseg_status st;
 #define ALLOC(field, type, count)                                          \
     if ((st = dalloc(c, &p, (size_t)(count) * sizeof(type))) != LSSEG_OK) \
@@ -1201,6 +1221,279 @@ lsseg_status lsseg_get_device_buffers(const lsseg_ctx* c, void** phi_read, void*
     if (phi_read) *phi_read = c->a.phi[c->q];
     if (phi_write) *phi_write = c->a.phi[c->q ^ 1];
     if (stream) *stream = (void*)c->stream;
+    return LSSEG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// one segmentation run that allocates nothing (lsseg_segment_noalloc)
+// ---------------------------------------------------------------------------
+// Every buffer of the run except the caller's inputs and outputs lives in one
+// static device array per GPU.  Its address is looked up once per device with
+// hipGetSymbolAddress (host code never passes the array's name to a kernel or
+// a copy: on the host that name is only a stand-in), and the run is carved
+// from that address.
+namespace {
+
+constexpr uint64_t kNoAllocVoxels = LSSEG_NOALLOC_MAX_VOXELS;
+constexpr uint64_t kNoAllocIters = LSSEG_NOALLOC_MAX_ITERS;
+
+// buffer offsets of a run of N padded voxels (256-byte aligned)
+struct NoAllocLayout {
+    uint64_t phi[2], moved, E, frozen, C, list[2], S[2], pos, emit, keep, memb, src;
+    uint64_t tstate, gstate, cpart, dirc, ctl, prm, sums, n_hist, c_hist, m_hist;
+    uint64_t end;
+};
+
+constexpr uint64_t align_up(uint64_t v) { return (v + 255u) & ~uint64_t(255); }
+
+NoAllocLayout noalloc_layout(const Geom& g)
+{
+    const uint64_t N = g.N;
+    const uint64_t ntiles = lookback_states(g);
+    NoAllocLayout L{};
+    uint64_t at = 0;
+    auto take = [&](uint64_t bytes) {
+        const uint64_t o = at;
+        at = align_up(at + bytes);
+        return o;
+    };
+    L.phi[0] = take(N * 4);
+    L.phi[1] = take(N * 4);
+    L.moved = take(N);
+    L.E = take(N);
+    L.frozen = take(N);
+    L.C = take(N * 8);
+    L.list[0] = take(N * 4);
+    L.list[1] = take(N * 4);
+    L.S[0] = take(N * 4);
+    L.S[1] = take(N * 4);
+    L.pos = take(N * 4);
+    L.emit = take(N);
+    L.keep = take(N);
+    L.memb = take(N);
+    L.src = take(N * 4);
+    L.tstate = take(ntiles * 8);
+    L.gstate = take(7 * (ntiles / 64 + 2) * 8);
+    L.cpart = take((uint64_t)kCountSlots * kCountStride * 8);
+    L.dirc = take(kDirCounters * 4);
+    L.ctl = take(sizeof(DevCtl));
+    L.prm = take(sizeof(DevParams));
+    L.sums = take(sizeof(IterSums));
+    L.n_hist = take(kNoAllocIters * 4);
+    L.c_hist = take(kNoAllocIters * 8);
+    L.m_hist = take(kNoAllocIters * 4);
+    L.end = at;
+    return L;
+}
+
+// the largest run's layout: a volume of kNoAllocVoxels voxels whose padded
+// dims are 16 x 16 x (kNoAllocVoxels / 256) has the most look-back states
+constexpr uint64_t kNoAllocBytes = 47ull * kNoAllocVoxels + (2ull << 20);
+static_assert(LSSEG_NOALLOC_SCRATCH_BYTES == kNoAllocBytes, "header and library agree on the scratch size");
+
+}  // namespace
+
+// the scratch array: one per GPU, held until the process ends
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+
+struct NoAllocDevice {
+    uint8_t* ptr = nullptr;  // device address of `scratch` on this GPU
+    uint32_t grid = 0, sgrid = 0;
+};
+std::mutex g_noalloc_mu;
+NoAllocDevice g_noalloc[kNoAllocDevices];
+
+// the scratch address and launch grids of the current device, set up on the
+// first call on it
+lsseg_status noalloc_device(NoAllocDevice* out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(nullptr, LSSEG_ERR_DEVICE, "no HIP device: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kNoAllocDevices) return fail(nullptr, LSSEG_ERR_UNSUPPORTED, "device %d", dev);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    NoAllocDevice& d = g_noalloc[dev];
+    if (!d.ptr) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(nullptr, LSSEG_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        int cus = 0;
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        cus = std::max(1, cus);
+        const int occ = min_occupancy({(const void*)&k_compact<kCompact>, (const void*)&k_compact<kInit>,
+                                       (const void*)&k_keep, update_fused_kernel(LSSEG_F32, false),
+                                       update_fused_kernel(LSSEG_U8, false), update_fused_kernel(LSSEG_U16, false)});
+        d.grid = std::max<uint32_t>(16u, ((uint32_t)occ * (uint32_t)cus) & ~7u);
+        d.sgrid = 8u * (uint32_t)cus;
+        d.ptr = (uint8_t*)p;
+    }
+    *out = d;
+    return LSSEG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+lsseg_status lsseg_segment_noalloc(const lsseg_volume_desc* desc, const void* src, const float* phi0, float band,
+                                   const lsseg_params* params, int32_t iters, float* phi_out, uint8_t* mask_out,
+                                   void* stream, lsseg_stats* stats)
+{
+    if (!desc || !src || !phi0 || !params || !phi_out || !mask_out)
+        return fail(nullptr, LSSEG_ERR_INVALID_ARG, "null argument");
+    if (desc->nx == 0 || desc->ny == 0 || desc->nz == 0) return fail(nullptr, LSSEG_ERR_INVALID_ARG, "zero dimension");
+    if (desc->dtype < LSSEG_U8 || desc->dtype > LSSEG_F32)
+        return fail(nullptr, LSSEG_ERR_UNSUPPORTED, "unsupported dtype %d", desc->dtype);
+    if (desc->phi_storage != LSSEG_PHI_F32 && desc->phi_storage != LSSEG_PHI_I8)
+        return fail(nullptr, LSSEG_ERR_UNSUPPORTED, "unsupported phi storage %d", desc->phi_storage);
+    if (!(band > 0.0f)) return fail(nullptr, LSSEG_ERR_INVALID_ARG, "band must be positive");
+    if (iters < 0 || (uint64_t)iters > kNoAllocIters)
+        return fail(nullptr, LSSEG_ERR_UNSUPPORTED, "lsseg_segment_noalloc runs 0..%llu iterations, not %d",
+                    (unsigned long long)kNoAllocIters, iters);
+    const uint64_t X = (desc->nx + 15ull) / 16 * 16, Y = (desc->ny + 15ull) / 16 * 16,
+                   Z = (desc->nz + 15ull) / 16 * 16;
+    if (X * Y * Z > kNoAllocVoxels)
+        return fail(nullptr, LSSEG_ERR_UNSUPPORTED,
+                    "lsseg_segment_noalloc takes at most %llu padded voxels (its static scratch holds %llu bytes); "
+                    "%llux%llux%llu pads to %llu: use lsseg_create",
+                    (unsigned long long)kNoAllocVoxels, (unsigned long long)kNoAllocBytes,
+                    (unsigned long long)desc->nx, (unsigned long long)desc->ny, (unsigned long long)desc->nz,
+                    (unsigned long long)(X * Y * Z));
+    DevParams P;
+    lsseg_status st = make_params_for(nullptr, desc->dtype, params, &P);
+    if (st != LSSEG_OK) return st;
+    NoAllocDevice dev;
+    if ((st = noalloc_device(&dev)) != LSSEG_OK) return st;
+
+    KArgs a{};
+    init_geom(a.g, (uint32_t)X, (uint32_t)Y, (uint32_t)Z);
+    const NoAllocLayout L = noalloc_layout(a.g);
+    if (L.end > kNoAllocBytes) return fail(nullptr, LSSEG_ERR_UNSUPPORTED, "scratch layout exceeds the array");
+    uint8_t* base = dev.ptr;
+    a.run_log2 = 12;  // list positions dealt to the XCDs in runs of 4096, as lsseg_create does
+    a.phi[0] = base + L.phi[0];
+    a.phi[1] = base + L.phi[1];
+    a.moved = base + L.moved;
+    a.src = base + L.src;
+    a.E = (int8_t*)(base + L.E);
+    a.frozen = (int8_t*)(base + L.frozen);
+    a.C = (uint64_t*)(base + L.C);
+    for (int i = 0; i < 2; ++i) {
+        a.list[i] = (uint32_t*)(base + L.list[i]);
+        a.S[i] = (uint32_t*)(base + L.S[i]);
+    }
+    a.pos = (uint32_t*)(base + L.pos);
+    a.emit = base + L.emit;
+    a.keep = base + L.keep;
+    a.memb = base + L.memb;
+    a.tstate = (uint64_t*)(base + L.tstate);
+    a.gstate = (uint64_t*)(base + L.gstate);
+    a.cpart = (uint64_t*)(base + L.cpart);
+    a.dirc = (uint32_t*)(base + L.dirc);
+    a.ctl = (DevCtl*)(base + L.ctl);
+    a.prm = (const DevParams*)(base + L.prm);
+    a.n_hist = (uint32_t*)(base + L.n_hist);
+    a.c_hist = (uint64_t*)(base + L.c_hist);
+    a.m_hist = (uint32_t*)(base + L.m_hist);
+
+    hipStream_t s = (hipStream_t)stream;
+    const bool phi8 = desc->phi_storage == LSSEG_PHI_I8;
+    const uint32_t S = (desc->dtype <= LSSEG_I8) ? 1 : (desc->dtype <= LSSEG_I16 ? 2 : 4);
+    const bool is_signed = desc->dtype == LSSEG_I8 || desc->dtype == LSSEG_I16 || desc->dtype == LSSEG_I32;
+    const uint64_t N = a.g.N;
+    const uint32_t dg = dense_grid(N);
+    const uint64_t ntiles = lookback_states(a.g);
+    // the scratch keeps the previous call's contents: everything a run reads
+    // before writing it is set here, with kernels only (no memset nodes, so
+    // the calls can be captured in a graph)
+    hipLaunchKernelGGL(k_pad_tile_source, dim3(dg), dim3(kBlock), 0, s, a.g, (const uint8_t*)src, (uint8_t*)a.src,
+                       desc->nx, desc->ny, desc->nz, S, (uint8_t)(is_signed ? 0x81 : 0x00));
+    hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, s, (uint8_t*)a.frozen, N, (uint8_t)127);
+    hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, s, (uint8_t*)a.pos, N * sizeof(uint32_t), (uint8_t)0);
+    hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, s, a.emit, N, (uint8_t)0);
+    hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, s, a.keep, N, (uint8_t)0);
+    hipLaunchKernelGGL(k_fill_u8, dim3(dg), dim3(kBlock), 0, s, a.memb, N, (uint8_t)0);
+    hipLaunchKernelGGL(k_fill_u64, dim3(dense_grid(ntiles)), dim3(kBlock), 0, s, a.tstate, ntiles, (uint64_t)0);
+    hipLaunchKernelGGL(k_fill_u64, dim3(dense_grid(ntiles)), dim3(kBlock), 0, s, a.gstate, 7 * (ntiles / 64 + 2),
+                       (uint64_t)0);
+    hipLaunchKernelGGL(k_fill_u64, dim3(4), dim3(kBlock), 0, s, a.cpart, (uint64_t)kCountSlots * kCountStride,
+                       (uint64_t)0);
+    hipLaunchKernelGGL(k_fill_u64, dim3(dense_grid(kDirCounters / 2)), dim3(kBlock), 0, s, (uint64_t*)a.dirc,
+                       (uint64_t)kDirCounters / 2, (uint64_t)0);
+    DevCtl c0{};
+    c0.stats_cap = (uint32_t)iters;
+    hipLaunchKernelGGL(k_put_params_ctl, dim3(1), dim3(64), 0, s, (DevParams*)a.prm, P, a.ctl, c0);
+    // lsseg_set_phi: both phi buffers, export byte, band flags, then the
+    // dense compaction of the flags into list[0]
+    if (phi8)
+        hipLaunchKernelGGL(k_set_phi<int8_t>, dim3(dg), dim3(kBlock), 0, s, a, phi0, desc->nx, desc->ny, desc->nz,
+                           band);
+    else
+        hipLaunchKernelGGL(k_set_phi<float>, dim3(dg), dim3(kBlock), 0, s, a, phi0, desc->nx, desc->ny, desc->nz,
+                           band);
+    {
+        int q1 = 1;
+        void* args[] = {(void*)&a, (void*)&q1};
+        (void)hipLaunchKernel((const void*)&k_compact<kInit>, dim3(dev.grid), dim3(kBlock), args, 0, s);
+    }
+    // lsseg_iterate: the update, marking and keep step of each iteration, each
+    // compaction in the next update's launch, the last one on its own
+    int q = 0;
+    for (int32_t i = 0; i < iters; ++i) {
+        void* args[] = {(void*)&a, (void*)&q};
+        const void* head = i ? update_fused_kernel(desc->dtype, phi8) : update_kernel(desc->dtype, phi8);
+        (void)hipLaunchKernel(head, dim3(i ? dev.grid : dev.sgrid), dim3(kBlock), args, 0, s);
+        (void)hipLaunchKernel(mark_kernel(phi8), dim3(dev.sgrid), dim3(kBlock), args, 0, s);
+        (void)hipLaunchKernel((const void*)&k_keep, dim3(dev.grid), dim3(kBlock), args, 0, s);
+        q ^= 1;
+    }
+    if (iters > 0) {
+        int qc = q ^ 1;
+        void* args[] = {(void*)&a, (void*)&qc};
+        (void)hipLaunchKernel((const void*)&k_compact<kCompact>, dim3(dev.grid), dim3(kBlock), args, 0, s);
+    }
+    // lsseg_get_phi(0) and lsseg_get_mask into the caller's buffers
+    const uint64_t cnt = (uint64_t)desc->nx * desc->ny * desc->nz;
+    if (phi8)
+        hipLaunchKernelGGL(k_get_phi<int8_t>, dim3(dense_grid(cnt)), dim3(kBlock), 0, s, a, q, phi_out, desc->nx,
+                           desc->ny, desc->nz);
+    else
+        hipLaunchKernelGGL(k_get_phi<float>, dim3(dense_grid(cnt)), dim3(kBlock), 0, s, a, q, phi_out, desc->nx,
+                           desc->ny, desc->nz);
+    hipLaunchKernelGGL(k_get_mask, dim3(dense_grid(cnt)), dim3(kBlock), 0, s, a, mask_out, desc->nx, desc->ny,
+                       desc->nz);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(nullptr, LSSEG_ERR_DEVICE, "launch failed: %s", hipGetErrorString(e));
+    if (!stats) return LSSEG_OK;
+    // statistics: summed on the device, then one copy and a wait
+    IterSums* d_sums = (IterSums*)(base + L.sums);
+    hipLaunchKernelGGL(k_sum_history, dim3(1), dim3(kBlock), 0, s, a.n_hist, a.c_hist, a.m_hist, (uint32_t)iters,
+                       d_sums);
+    struct {
+        DevCtl ctl;
+        IterSums sums;
+    } h{};
+    e = hipMemcpyAsync(&h.ctl, a.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&h.sums, d_sums, sizeof(IterSums), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(nullptr, LSSEG_ERR_DEVICE, "statistics readback failed: %s", hipGetErrorString(e));
+    if (h.ctl.error)
+        return fail(nullptr, LSSEG_ERR_DEVICE, "device error flag 0x%x (%s)", h.ctl.error, device_error_text(h.ctl.error));
+    memset(stats, 0, sizeof(*stats));
+    stats->iterations = (uint64_t)iters;
+    stats->sum_active = h.sums.n;
+    stats->sum_candidates = h.sums.c;
+    stats->sum_new_active = h.sums.m;
+    stats->active_final = h.ctl.n[q];
+    stats->few_path_iters = h.ctl.few_iters;
+    stats->many_path_iters = h.ctl.many_iters;
     return LSSEG_OK;
 }
 

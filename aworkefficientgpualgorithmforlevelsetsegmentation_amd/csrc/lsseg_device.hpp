@@ -2275,6 +2275,55 @@ __global__ void k_float_to_phi8(const float* in, int8_t* out, uint64_t count)
     }
 }
 
+// the parameters and a fresh control block, written by the device (callers
+// that stage nothing in host memory)
+__global__ void k_put_params_ctl(DevParams* prm, DevParams P, DevCtl* ctl, DevCtl c0)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *prm = P;
+        *ctl = c0;
+    }
+}
+
+// sums of the per-iteration counters n_k, c_k, m_k of `count` iterations (one
+// workgroup of kBlock threads)
+struct IterSums {
+    unsigned long long n, c, m;
+};
+
+__global__ __launch_bounds__(kBlock) void k_sum_history(const uint32_t* n, const uint64_t* c, const uint32_t* m,
+                                                        uint32_t count, IterSums* out)
+{
+    __shared__ unsigned long long part[3][kBlock / 64];
+    unsigned long long sn = 0, sc = 0, sm = 0;
+    for (uint32_t i = threadIdx.x; i < count; i += kBlock) {
+        sn += n[i];
+        sc += c[i];
+        sm += m[i];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sn += __shfl_xor(sn, o, 64);
+        sc += __shfl_xor(sc, o, 64);
+        sm += __shfl_xor(sm, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        part[0][threadIdx.x >> 6] = sn;
+        part[1][threadIdx.x >> 6] = sc;
+        part[2][threadIdx.x >> 6] = sm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        IterSums r{0, 0, 0};
+        for (int w = 0; w < kBlock / 64; ++w) {
+            r.n += part[0][w];
+            r.c += part[1][w];
+            r.m += part[2][w];
+        }
+        *out = r;
+    }
+}
+
 // GetSaveableSegmentation (Segmenter.cpp:1216-1286): export unioned with the
 // frozen export, predicate <= 0, unpadded (UnpadVolume, :1288-1337)
 __global__ void k_get_mask(KArgs a, uint8_t* out, uint32_t nx, uint32_t ny, uint32_t nz)

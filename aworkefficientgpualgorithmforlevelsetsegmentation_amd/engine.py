@@ -36,7 +36,7 @@ EXPORTS = (
     "lsseg_batch_volume", "lsseg_batch_iterate", "lsseg_batch_checkpoint_save", "lsseg_batch_checkpoint_restore",
     "lsseg_batch_get_tuning", "lsseg_batch_set_tuning", "lsseg_batch_autotune",
     "lsseg_fwd_record", "lsseg_record_destroy", "lsseg_record_info", "lsseg_record_get", "lsseg_fwd_replay",
-    "lsseg_bwd", "lsseg_auto_parameters",
+    "lsseg_bwd", "lsseg_auto_parameters", "lsseg_segment_noalloc",
 )
 
 
@@ -158,6 +158,7 @@ def lib():
             "lsseg_fwd_replay": (i32, [vp, vp, vp, P(Params), vp]),
             "lsseg_bwd": (i32, [vp, vp, vp, vp, vp]),
             "lsseg_auto_parameters": (i32, [vp, P(f32), P(f32), P(u64)]),
+            "lsseg_segment_noalloc": (i32, [P(VolumeDesc), vp, vp, f32, P(Params), i32, vp, vp, vp, P(Stats)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -265,6 +265,39 @@ lsseg_status lsseg_iterate_profiled(lsseg_ctx* ctx, const lsseg_params* params, 
 lsseg_status lsseg_get_device_buffers(const lsseg_ctx* ctx, void** phi_read, void** phi_write,
                                       void** stream);
 
+/* ---- one run that allocates nothing -------------------------------------
+
+   No counterpart in the reference.  For callers that cannot allocate device
+   memory: code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc.  One call does what lsseg_create + lsseg_set_phi +
+   lsseg_iterate + lsseg_get_phi(ctx, 0, .) + lsseg_get_mask do, with the same
+   kernels and bit-identical results, but calls no hipMalloc, hipFree or other
+   allocating routine: every buffer except the caller's inputs and outputs
+   lives in one static device array of LSSEG_NOALLOC_SCRATCH_BYTES bytes per
+   GPU, which the library holds until the process ends.
+
+   src (unpadded source, x fastest, desc->dtype elements), phi0 (unpadded
+   float) are device inputs; phi_out (unpadded float, the final read buffer)
+   and mask_out (unpadded bytes, export <= 0) are device outputs; all four in
+   device memory of the current device.  The work is queued on `stream` (a
+   hipStream_t; NULL: the null stream).  With stats == NULL the call only
+   queues kernels and returns (no wait, no memset, no host copy: it can be
+   captured in a graph); with stats it waits for the run, reports a device
+   error flag, and fills stats (device_ms and empty_skipped stay 0: every
+   iteration is launched).
+   Limits: at most LSSEG_NOALLOC_MAX_VOXELS padded voxels (each axis padded to
+   a multiple of 16) and LSSEG_NOALLOC_MAX_ITERS iterations; larger requests
+   fail with LSSEG_ERR_UNSUPPORTED.  Every call on a GPU uses that GPU's one
+   array and leaves its contents to the next call: calls that overlap on one
+   GPU (other streams, other threads) race and must be serialised by the
+   caller.  Errors: lsseg_last_error(NULL). */
+#define LSSEG_NOALLOC_MAX_VOXELS (1ull << 21)  /* 128^3 */
+#define LSSEG_NOALLOC_MAX_ITERS 16384ull
+#define LSSEG_NOALLOC_SCRATCH_BYTES (47ull * LSSEG_NOALLOC_MAX_VOXELS + (2ull << 20))
+lsseg_status lsseg_segment_noalloc(const lsseg_volume_desc* desc, const void* src, const float* phi0, float band,
+                                   const lsseg_params* params, int32_t iters, float* phi_out, uint8_t* mask_out,
+                                   void* stream, lsseg_stats* stats);
+
 /* ---- batched form --------------------------------------------------------
 
    No reference counterpart: the reference segments one volume at a time.  A
