@@ -53,6 +53,7 @@ struct lsseg_ctx {
     float z_aniso = 1.0f;
     int q = 0;                        // parity of the current list and phi read buffer
     uint32_t grid = 1024;             // co-resident workgroups of the compaction kernel
+    uint32_t fgrid = 1024;            // grid of k_update_fused (compaction + update)
     uint32_t sgrid = 2048;            // workgroups of the update and marking kernels
     uint32_t cus = 256;
     KArgs a{};
@@ -73,6 +74,7 @@ struct lsseg_ctx {
         bool valid = false;
     } ckpt[LSSEG_MAX_CHECKPOINTS];
     bool profile = false;             // time every iteration launch with its own start/stop events
+    bool checks = false;              // LSSEG_CHECKS: verify the list after every compaction
     std::vector<hipEvent_t> prof_events;
     size_t prof_next = 0;
     std::vector<void*> allocs;
@@ -85,6 +87,7 @@ struct lsseg_ctx {
 namespace {
 
 void detach_records(lsseg_ctx* c);
+int env_int(const char* name, int dflt);
 
 lsseg_status fail(lsseg_ctx* c, lsseg_status st, const char* fmt, ...)
 {
@@ -105,9 +108,31 @@ lsseg_status fail(lsseg_ctx* c, lsseg_status st, const char* fmt, ...)
             return fail((ctx), LSSEG_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));   \
     } while (0)
 
+// LSSEG_FAIL_ALLOC=k (tests of the out-of-memory paths): the k-th device
+// allocation of each context creation (lsseg_create, and each volume of
+// lsseg_batch_create) fails as if the device were out of memory.
+thread_local int t_create_allocs = 0;  // allocations of the creation in progress (0: none in progress)
+
+hipError_t device_malloc(void** p, size_t bytes)
+{
+    if (t_create_allocs > 0) {
+        const int k = env_int("LSSEG_FAIL_ALLOC", 0);
+        if (k > 0 && t_create_allocs++ == k) {
+            *p = nullptr;
+            return hipErrorOutOfMemory;
+        }
+    }
+    return hipMalloc(p, bytes);
+}
+
+struct CreateAllocScope {
+    CreateAllocScope() { t_create_allocs = 1; }
+    ~CreateAllocScope() { t_create_allocs = 0; }
+};
+
 lsseg_status dalloc(lsseg_ctx* c, void** p, size_t bytes)
 {
-    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    hipError_t e = device_malloc(p, bytes ? bytes : 1);
     if (e != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
@@ -133,6 +158,14 @@ void destroy_graphs(lsseg_ctx* c)
     }
 }
 
+// LDS bytes per CU that the grid sizing counts on (LSSEG_LDS_BUDGET, default
+// 64 KiB, the most one workgroup may use)
+int lds_budget()
+{
+    const int v = env_int("LSSEG_LDS_BUDGET", 65536);
+    return v >= 4096 ? v : 65536;
+}
+
 // Resident workgroups per CU that every listed kernel reaches.  The look-back
 // scans hand their tiles out in order, so they finish with any grid; a grid of
 // this size keeps every workgroup running from the start.  The runtime's
@@ -152,7 +185,7 @@ int min_occupancy(std::initializer_list<const void*> kernels)
         if (hipFuncGetAttributes(&fa, k) == hipSuccess) {
             const int regs = std::max(8, (fa.numRegs + 7) / 8 * 8);
             const int by_regs = 512 / regs;
-            const int by_lds = fa.sharedSizeBytes ? (int)(65536 / fa.sharedSizeBytes) : 8;
+            const int by_lds = fa.sharedSizeBytes ? (int)(lds_budget() / fa.sharedSizeBytes) : 8;
             occ = std::min(occ, std::max(1, std::min(8, std::min(by_regs, by_lds))));
         } else {
             (void)hipGetLastError();
@@ -235,12 +268,23 @@ void launch(lsseg_ctx* c, const void* fn, uint32_t grid, int q)
     (void)hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args, 0, c->stream);
 }
 
+// LSSEG_CHECKS: the invariants of list[q] once its compaction is done
+void launch_check(lsseg_ctx* c, int q)
+{
+    if (c->checks) hipLaunchKernelGGL(k_check_list, dim3(c->sgrid), dim3(kBlock), 0, c->stream, c->a, q);
+}
+
 // The update over S[q]; `fused`: in the same launch as the compaction of
-// list[q ^ 1] into list[q] that the previous iteration left pending.
+// list[q ^ 1] into list[q] that the previous iteration left pending (then
+// checked in check mode).
 void launch_head(lsseg_ctx* c, int q, bool fused)
 {
-    if (fused) launch(c, update_fused_kernel(c->dtype, c->phi8), c->grid, q);
-    else launch(c, update_kernel(c->dtype, c->phi8), c->sgrid, q);
+    if (fused) {
+        launch(c, update_fused_kernel(c->dtype, c->phi8), c->fgrid, q);
+        launch_check(c, q);
+    } else {
+        launch(c, update_kernel(c->dtype, c->phi8), c->sgrid, q);
+    }
 }
 
 // marking and keep step of the iteration over list[q]; afterwards the count
@@ -258,6 +302,7 @@ void launch_iteration(lsseg_ctx* c, int q)
     launch_head(c, q, false);
     launch_tail(c, q);
     launch(c, (const void*)&k_compact<kCompact>, c->grid, q);
+    launch_check(c, q ^ 1);
 }
 
 void launch_update_only(lsseg_ctx* c, int op)
@@ -308,6 +353,8 @@ void fill_u64(lsseg_ctx* c, uint64_t* p, uint64_t count, uint64_t v)
 lsseg_status dense_init_list(lsseg_ctx* c, int target)
 {
     launch(c, (const void*)&k_compact<kInit>, c->grid, target ^ 1);
+    if (env_int("LSSEG_CHECKS", 0))
+        hipLaunchKernelGGL(k_check_list, dim3(c->sgrid), dim3(kBlock), 0, c->stream, c->a, target);
     HIP_TRY(c, hipGetLastError());
     c->q = target;
     return LSSEG_OK;
@@ -391,6 +438,7 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q, uint32_t* skipped
         }
     }
     launch(c, (const void*)&k_compact<kCompact>, c->grid, q ^ 1);
+    launch_check(c, q);
     if (skipped) hipLaunchKernelGGL(k_empty_iterations, dim3(1), dim3(kBlock), 0, c->stream, c->a, skipped);
     if (skipped_out) *skipped_out = skipped;
     return LSSEG_OK;
@@ -400,6 +448,8 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q, uint32_t* skipped
 const char* device_error_text(uint32_t err)
 {
     return (err & kErrSpin)    ? "look-back wait exceeded its bound"
+           : (err & kErrCheck) ? "list check failed: an entry outside the volume, repeated, or missing from the "
+                                 "traversal"
            : (err & kErrCount) ? "the compaction's count differs from the keep step's"
                                : "list overflow";
 }
@@ -613,6 +663,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
 
     lsseg_ctx* c = new (std::nothrow) lsseg_ctx();
     if (!c) return fail(nullptr, LSSEG_ERR_OUT_OF_MEMORY, "host allocation failed");
+    CreateAllocScope alloc_scope;
     auto bail = [&](lsseg_status st) {
         g_create_error = c->err;
         destroy_ctx(c);
@@ -638,9 +689,15 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     const int occ = min_occupancy({(const void*)&k_compact<kCompact>, (const void*)&k_compact<kInit>,
                                    (const void*)&k_keep, update_fused_kernel(desc->dtype, c->phi8)});
     const int want = env_int("LSSEG_GRID_BLOCKS", 0);
-    c->grid = (uint32_t)occ * c->cus;
-    if (want > 0) c->grid = std::min<uint32_t>(c->grid, (uint32_t)want);
+    c->grid = want > 0 ? (uint32_t)want : (uint32_t)occ * c->cus;
     c->grid = std::max<uint32_t>(16u, c->grid & ~7u);  // k_update_fused splits it in multiples of 8
+    // k_update_fused: at most three quarters of the keep/compaction grid
+    // compact, the rest of the fused launch's grid updates
+    c->a.fused_compact_max = (c->grid / 4u * 3u) & ~7u;
+    const int fwant = env_int("LSSEG_FUSED_GRID", 0);
+    c->fgrid = fwant > 0 ? std::max<uint32_t>(((uint32_t)fwant + 7u) & ~7u, c->a.fused_compact_max + 8u) : c->grid;
+    c->a.keep_items = (uint32_t)std::max(0, env_int("LSSEG_KEEP_ITEMS", 0));
+    if (c->a.keep_items != 1 && c->a.keep_items != 2 && c->a.keep_items != 4) c->a.keep_items = 0;
     // update and marking: eight workgroups per CU (a multiple of the eight
     // XCDs), list positions dealt to the XCDs in runs of 4096
     const int swant = env_int("LSSEG_STREAM_BLOCKS", 0);
@@ -714,7 +771,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     // upload, pad and tile the source
     const uint64_t lin_bytes = (uint64_t)c->nx * c->ny * c->nz * c->S;
     void* lin = nullptr;
-    if (hipMalloc(&lin, lin_bytes) != hipSuccess) {
+    if (device_malloc(&lin, lin_bytes) != hipSuccess) {
         (void)hipGetLastError();
         c->err = "hipMalloc of the staging volume failed";
         return bail(LSSEG_ERR_OUT_OF_MEMORY);
@@ -873,6 +930,11 @@ lsseg_status lsseg_iterate(lsseg_ctx* c, const lsseg_params* prm, int32_t iters,
     DevParams P;
     lsseg_status st = make_params(c, prm, &P);
     if (st != LSSEG_OK) return st;
+    const bool checks = env_int("LSSEG_CHECKS", 0) != 0;
+    if (checks != c->checks) {
+        c->checks = checks;
+        destroy_graphs(c);  // captured without (or with) the check launches
+    }
     if ((st = ensure_stats(c, (uint64_t)iters)) != LSSEG_OK) return st;
     if ((st = upload_params(c, P)) != LSSEG_OK) return st;
     // reset the per-call counters (keep n[], epoch)
@@ -1378,6 +1440,8 @@ lsseg_status lsseg_segment_noalloc(const lsseg_volume_desc* desc, const void* sr
     if (L.end > kNoAllocBytes) return fail(nullptr, LSSEG_ERR_UNSUPPORTED, "scratch layout exceeds the array");
     uint8_t* base = dev.ptr;
     a.run_log2 = 12;  // list positions dealt to the XCDs in runs of 4096, as lsseg_create does
+    a.fused_compact_max = (dev.grid / 4u * 3u) & ~7u;
+    a.keep_items = 0;
     a.phi[0] = base + L.phi[0];
     a.phi[1] = base + L.phi[1];
     a.moved = base + L.moved;
@@ -1511,6 +1575,8 @@ struct lsseg_batch {
     uint32_t max_compact = 1024;      // resident workgroups of k_compact_batch (and above the volume count)
     uint32_t grid = 1024;             // compaction workgroups
     uint32_t sgrid = 2048;            // update and marking workgroups
+    uint32_t keep_items = kKeepItemsBatch;  // traversal positions per thread of the keep step
+    uint32_t compact_tile = kPosTile;       // list positions per few-path compaction tile (256 or 1024)
     KArgs* d_table = nullptr;
     DevCtl* d_ctls = nullptr;
     DevCtl* h_ctls = nullptr;         // pinned mirror
@@ -1563,7 +1629,7 @@ void destroy_batch_graphs(lsseg_batch* b)
 // keep step, compaction
 void launch_batch_head(lsseg_batch* b, int q, bool fused)
 {
-    BatchArgs args{b->d_table, b->d_ctls, (uint32_t)b->vols.size()};
+    BatchArgs args{b->d_table, b->d_ctls, (uint32_t)b->vols.size(), b->keep_items};
     const bool phi8 = b->vols[0]->phi8;
     if (fused) {
         uint32_t gc = b->grid;
@@ -1578,7 +1644,7 @@ void launch_batch_head(lsseg_batch* b, int q, bool fused)
 
 void launch_batch_tail(lsseg_batch* b, int q)
 {
-    BatchArgs args{b->d_table, b->d_ctls, (uint32_t)b->vols.size()};
+    BatchArgs args{b->d_table, b->d_ctls, (uint32_t)b->vols.size(), b->keep_items};
     void* kargs[] = {(void*)&args, (void*)&q};
     (void)hipLaunchKernel(mark_batch_kernel(b->vols[0]->phi8), dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
     (void)hipLaunchKernel((const void*)&k_keep_batch, dim3(b->grid), dim3(kBlock), kargs, 0, b->stream);
@@ -1586,7 +1652,7 @@ void launch_batch_tail(lsseg_batch* b, int q)
 
 void launch_batch_compact(lsseg_batch* b, int q)
 {
-    BatchArgs args{b->d_table, b->d_ctls, (uint32_t)b->vols.size()};
+    BatchArgs args{b->d_table, b->d_ctls, (uint32_t)b->vols.size(), b->keep_items};
     void* kargs[] = {(void*)&args, (void*)&q};
     (void)hipLaunchKernel((const void*)&k_compact_batch, dim3(b->grid), dim3(kBlock), kargs, 0, b->stream);
 }
@@ -1658,6 +1724,67 @@ lsseg_status normalise_parity(lsseg_ctx* c)
     c->q = 0;
     destroy_graphs(c);
     return LSSEG_OK;
+}
+
+// Autotuning results, cached per (GPU architecture, source type, phi storage,
+// volume count, total voxels to the next power of two, timed iterations) in
+// this process and, when LSSEG_TUNING_CACHE names a file, in that file (one
+// result per line, appended; read at the first autotune).
+std::mutex g_tune_mu;
+std::vector<std::pair<std::string, lsseg_tuning>> g_tune_cache;
+bool g_tune_file_read = false;
+
+std::string tuning_key(const lsseg_batch* b, int32_t iters)
+{
+    hipDeviceProp_t prop{};
+    int dev = 0;
+    std::string arch = "unknown";
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        arch = std::string(prop.gcnArchName).substr(0, std::string(prop.gcnArchName).find(':'));
+    uint64_t voxels = 0;
+    for (const lsseg_ctx* c : b->vols) voxels += c->a.g.N;
+    int lg = 0;
+    while ((1ull << lg) < voxels) ++lg;
+    char buf[160];
+    snprintf(buf, sizeof(buf), "%s/dtype%d/phi%d/volumes%zu/voxels2^%d/iters%d", arch.c_str(), b->dtype,
+             b->vols[0]->phi8 ? 1 : 0, b->vols.size(), lg, iters);
+    return buf;
+}
+
+bool tuning_cache_get(const std::string& key, lsseg_tuning* out)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    const char* path = getenv("LSSEG_TUNING_CACHE");
+    if (!g_tune_file_read && path && *path) {
+        g_tune_file_read = true;
+        if (FILE* f = fopen(path, "r")) {
+            char k[160];
+            lsseg_tuning t{};
+            while (fscanf(f, "%159s %u %u %u %u %lf", k, &t.update_blocks, &t.compact_blocks, &t.keep_items,
+                          &t.compact_tile, &t.ms) == 6)
+                g_tune_cache.emplace_back(k, t);
+            fclose(f);
+        }
+    }
+    for (auto it = g_tune_cache.rbegin(); it != g_tune_cache.rend(); ++it)
+        if (it->first == key) {
+            *out = it->second;
+            return true;
+        }
+    return false;
+}
+
+void tuning_cache_put(const std::string& key, const lsseg_tuning& t)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune_cache.emplace_back(key, t);
+    const char* path = getenv("LSSEG_TUNING_CACHE");
+    if (path && *path)
+        if (FILE* f = fopen(path, "a")) {
+            fprintf(f, "%s %u %u %u %u %.6f\n", key.c_str(), t.update_blocks, t.compact_blocks, t.keep_items,
+                    t.compact_tile, t.ms);
+            fclose(f);
+        }
 }
 
 lsseg_status batch_sync_and_check(lsseg_batch* b)
@@ -1804,9 +1931,12 @@ lsseg_status lsseg_batch_iterate(lsseg_batch* b, const lsseg_params* params, int
     b->h_table.resize(cnt);
     for (uint32_t v = 0; v < cnt; ++v) {
         b->h_table[v] = b->vols[v]->a;
-        // tiles of 1024 positions: small tiles made the 64-volume batch 6%
-        // slower (more chunks over the one grid all volumes share)
-        b->h_table[v].g.small_n = 0;
+        // compaction tiles of 1024 positions unless tuned otherwise: small
+        // tiles made the 64-volume batch 6% slower (more chunks over the one
+        // grid all volumes share)
+        const Geom& g = b->vols[v]->a.g;
+        b->h_table[v].g.small_n =
+            b->compact_tile == (uint32_t)kBlock ? (uint32_t)std::min<uint64_t>(g.N, 131072u) : 0u;
     }
     BHIP_TRY(b, hipMemcpyAsync(b->d_table, b->h_table.data(), cnt * sizeof(KArgs), hipMemcpyHostToDevice, b->stream));
     // reset the per-call counters (keep n[], epoch)
@@ -1887,6 +2017,8 @@ lsseg_status lsseg_batch_get_tuning(const lsseg_batch* b, lsseg_tuning* out)
     memset(out, 0, sizeof(*out));
     out->update_blocks = b->sgrid;
     out->compact_blocks = b->grid;
+    out->keep_items = b->keep_items;
+    out->compact_tile = b->compact_tile;
     return LSSEG_OK;
 }
 
@@ -1897,9 +2029,17 @@ lsseg_status lsseg_batch_set_tuning(lsseg_batch* b, const lsseg_tuning* t)
     if (t->update_blocks == 0 || t->compact_blocks < min_compact || t->compact_blocks > b->max_compact)
         return bfail(b, LSSEG_ERR_INVALID_ARG, "launch shape %u/%u outside 1.. / %u..%u", t->update_blocks,
                      t->compact_blocks, min_compact, b->max_compact);
-    if (t->update_blocks != b->sgrid || t->compact_blocks != b->grid) destroy_batch_graphs(b);
+    const uint32_t ki = t->keep_items ? t->keep_items : b->keep_items;
+    const uint32_t ct = t->compact_tile ? t->compact_tile : b->compact_tile;
+    if ((ki != 1 && ki != 2) || (ct != (uint32_t)kBlock && ct != (uint32_t)kPosTile))
+        return bfail(b, LSSEG_ERR_INVALID_ARG, "keep items %u (1 or 2) or compaction tile %u (%d or %d)", ki, ct,
+                     kBlock, kPosTile);
+    if (t->update_blocks != b->sgrid || t->compact_blocks != b->grid || ki != b->keep_items || ct != b->compact_tile)
+        destroy_batch_graphs(b);
     b->sgrid = t->update_blocks;
     b->grid = t->compact_blocks;
+    b->keep_items = ki;
+    b->compact_tile = ct;
     return LSSEG_OK;
 }
 
@@ -1907,9 +2047,37 @@ lsseg_status lsseg_batch_autotune(lsseg_batch* b, const lsseg_params* params, in
 {
     if (!b || !params) return LSSEG_ERR_INVALID_ARG;
     if (iters <= 0) return bfail(b, LSSEG_ERR_INVALID_ARG, "autotune needs a positive iteration count");
+    const std::string key = tuning_key(b, iters);
+    lsseg_tuning best{};
+    if (!env_int("LSSEG_AUTOTUNE_FORCE", 0) && tuning_cache_get(key, &best)) {
+        best.cached = 1;
+        best.candidates = 0;
+        lsseg_status st = lsseg_batch_set_tuning(b, &best);
+        if (st == LSSEG_OK && out) *out = best;
+        return st;
+    }
     const int32_t slot = LSSEG_MAX_CHECKPOINTS - 1;
     lsseg_status st = lsseg_batch_checkpoint_save(b, slot);
     if (st != LSSEG_OK) return st;
+    uint32_t timed = 0;
+    // device time of `iters` iterations from the saved state with tuning t;
+    // the first run captures the graphs, the second is timed
+    auto time_shape = [&](const lsseg_tuning& t, double* ms) -> lsseg_status {
+        lsseg_status s2 = lsseg_batch_set_tuning(b, &t);
+        if (s2 != LSSEG_OK) return s2;
+        for (int rep = 0; rep < 2; ++rep) {
+            if ((s2 = lsseg_batch_checkpoint_restore(b, slot)) != LSSEG_OK) return s2;
+            std::vector<lsseg_stats> sts(b->vols.size());
+            if ((s2 = lsseg_batch_iterate(b, params, iters, sts.data())) != LSSEG_OK) return s2;
+            *ms = sts[0].device_ms;
+        }
+        ++timed;
+        return LSSEG_OK;
+    };
+    // coordinate descent: the launch shape (workgroups of the update and
+    // marking launches, of the keep and compaction launches) at the default
+    // widths, then the keep step's positions per thread, then the compaction's
+    // tile of list positions
     const uint32_t occ = b->max_compact / b->cus;
     std::vector<lsseg_tuning> cands;
     for (uint32_t su : {2u, 4u, 8u, 16u})
@@ -1917,30 +2085,37 @@ lsseg_status lsseg_batch_autotune(lsseg_batch* b, const lsseg_params* params, in
             lsseg_tuning t{};
             t.update_blocks = su * b->cus;
             t.compact_blocks = std::max<uint32_t>(std::max(1u, cd) * b->cus, (uint32_t)b->vols.size() + 8u);
+            t.keep_items = kKeepItemsBatch;
+            t.compact_tile = kPosTile;
             bool dup = false;
             for (auto& o : cands) dup |= o.update_blocks == t.update_blocks && o.compact_blocks == t.compact_blocks;
             if (!dup) cands.push_back(t);
         }
-    lsseg_tuning best{};
     best.ms = 1e30;
     for (auto& t : cands) {
-        if ((st = lsseg_batch_set_tuning(b, &t)) != LSSEG_OK) return st;
-        // first run captures the graphs, second one is timed
         double ms = 0;
-        for (int rep = 0; rep < 2; ++rep) {
-            if ((st = lsseg_batch_checkpoint_restore(b, slot)) != LSSEG_OK) return st;
-            std::vector<lsseg_stats> sts(b->vols.size());
-            if ((st = lsseg_batch_iterate(b, params, iters, sts.data())) != LSSEG_OK) return st;
-            ms = sts[0].device_ms;
-        }
+        if ((st = time_shape(t, &ms)) != LSSEG_OK) return st;
         if (ms < best.ms) {
             best = t;
             best.ms = ms;
         }
     }
-    best.candidates = (uint32_t)cands.size();
+    for (int stage = 0; stage < 2; ++stage) {
+        lsseg_tuning t = best;
+        if (stage == 0) t.keep_items = best.keep_items == 1 ? 2 : 1;
+        else t.compact_tile = best.compact_tile == (uint32_t)kBlock ? (uint32_t)kPosTile : (uint32_t)kBlock;
+        double ms = 0;
+        if ((st = time_shape(t, &ms)) != LSSEG_OK) return st;
+        if (ms < best.ms) {
+            best = t;
+            best.ms = ms;
+        }
+    }
+    best.candidates = timed;
+    best.cached = 0;
     if ((st = lsseg_batch_set_tuning(b, &best)) != LSSEG_OK) return st;
     if ((st = lsseg_batch_checkpoint_restore(b, slot)) != LSSEG_OK) return st;
+    tuning_cache_put(key, best);
     if (out) *out = best;
     return LSSEG_OK;
 }

@@ -57,8 +57,8 @@ constexpr int kItems = 8;                     // compaction slots per thread
 constexpr int kScanTile = kBlock * kItems;    // slots per look-back tile
 constexpr int kKeepItems = 1;                 // S positions per thread in the keep step (measured:
                                               // 2 costs 4% on the phantom, 4 costs 10%)
-constexpr int kKeepItemsBatch = 2;            // the same in the batched keep step (1 costs 5% on
-                                              // 64 volumes: its scans share one grid)
+constexpr int kKeepItemsBatch = 2;            // the default of the batched keep step (autotuned: 1
+                                              // cost 5% on 64 volumes, whose scans share one grid)
 constexpr int kKeepTile = kBlock * kKeepItems;  // the smallest keep tile (state sizing)
 constexpr int kPosItems = 4;                  // list positions per thread in a few-path compaction (<= 4;
                                               // 2 measured 4% slower on the phantom, 9% on the vessels);
@@ -67,7 +67,7 @@ constexpr int kPosTile = kBlock * kPosItems;
 constexpr uint32_t kSpecTiles = 256;          // default of Geom::spec_tiles
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded look-back wait
 
-enum : uint32_t { kErrSpin = 1u, kErrOverflow = 2u, kErrCount = 8u };
+enum : uint32_t { kErrSpin = 1u, kErrOverflow = 2u, kErrCount = 8u, kErrCheck = 16u };
 
 enum KMode : int {
     kUpdate = 0,   // update of the current list
@@ -260,6 +260,10 @@ constexpr size_t kDirCounters = 2 * 7 * kCountSlots * kCountStride;  // words of
 struct KArgs {
     Geom g;
     uint32_t run_log2;    // log2 of the list positions per XCD run (0: plain grid stride)
+    uint32_t fused_compact_max;  // k_update_fused: at most this many workgroups (a multiple of 8) take
+                                 // the compaction, the rest of its grid the update
+    uint32_t keep_items;  // S positions per thread of the keep step: 1, 2 or 4; 0: the fewest that
+                          // leave one look-back chunk of one sub-tile per workgroup (up to 4)
     void* phi[2];         // float, or int8_t in the fixed-point storage mode
     uint8_t* moved;       // temporal-derivative flag per traversal position, written by the update
     const void* src;
@@ -1502,16 +1506,41 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
     return true;
 }
 
+// the keep step at kI positions per thread; us, ms: this workgroup's first
+// sub-tile when `loaded`
+template <int kI>
+__device__ __attribute__((always_inline)) inline void keep_step(const KArgs& a, int q, uint32_t n, uint32_t epoch,
+                                                                KeepShared<kI>& sh, uint32_t* us, uint32_t* ms,
+                                                                bool loaded)
+{
+    const ChunkPlan cp = chunk_plan(keep_tiles<kI>(n), gridDim.x);
+    if (blockIdx.x < cp.chunks) {
+        if (!loaded && cp.r == 1) keep_voxels<kI>(a, q, blockIdx.x, us, ms);
+        (void)keep_chunk<kI>(a, q, n, cp, blockIdx.x, epoch, sh, us, ms);
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_keep(KArgs a, int q)
 {
-    __shared__ KeepShared<kKeepItems> sh;
-    uint32_t us[kKeepItems], ms[kKeepItems];
-    keep_voxels<kKeepItems>(a, q, blockIdx.x, us, ms);  // with the count: this workgroup's sub-tile when chunks are single
+    __shared__ KeepShared<1> sh1;
+    __shared__ KeepShared<2> sh2;
+    __shared__ KeepShared<4> sh4;
+    uint32_t us[4], ms[4];
+    keep_voxels<1>(a, q, blockIdx.x, us, ms);  // with the count: this workgroup's sub-tile at one position per thread
     const uint32_t epoch = a.ctl->epoch;
     const uint32_t n = a.ctl->n[q];
-    const ChunkPlan cp = chunk_plan(keep_tiles<kKeepItems>(n), gridDim.x);
-    if (blockIdx.x < cp.chunks) (void)keep_chunk(a, q, n, cp, blockIdx.x, epoch, sh, us, ms);
-    if (cp.chunks == 0 && blockIdx.x == 0 && threadIdx.x < 64)
+    // one position per thread while the grid covers the list in single
+    // sub-tiles, else two or four (a single pass each), past that several
+    // sub-tiles per chunk
+    uint32_t kI = a.keep_items;
+    if (kI == 0) {
+        const uint64_t cover = (uint64_t)gridDim.x * kBlock;
+        kI = n <= cover ? 1u : (n <= 2 * cover ? 2u : 4u);
+    }
+    if (kI == 1) keep_step<1>(a, q, n, epoch, sh1, us, ms, true);
+    else if (kI == 2) keep_step<2>(a, q, n, epoch, sh2, us, ms, false);
+    else keep_step<4>(a, q, n, epoch, sh4, us, ms, false);
+    if (n == 0 && blockIdx.x == 0 && threadIdx.x < 64)
         keep_finish(a, q, 0, 0, epoch, candidate_share(a), ctl_snapshot(a.ctl));
 }
 
@@ -1580,7 +1609,7 @@ __device__ inline void update_fused_body(const KArgs& a, int q, const RecordOut&
     CompactPlan c = compact_plan<kCompact>(a.g, a.ctl->n[q ^ 1]);
     c.n_dst = a.ctl->n[q];
     loaded = compact_pick_preload(c, loaded, kb, vx, small);
-    const uint32_t gc = min((c.ntiles + 7u) & ~7u, (gridDim.x / 4u * 3u) & ~7u);
+    const uint32_t gc = min((c.ntiles + 7u) & ~7u, a.fused_compact_max);
     if (blockIdx.x < gc) {
         compact_all<kCompact>(a, q ^ 1, c, epoch, sh, gc, kb, vx, d, loaded);
     } else {
@@ -1697,6 +1726,7 @@ struct BatchArgs {
     const KArgs* vols;
     DevCtl* ctls;
     uint32_t count;
+    uint32_t keep_items;  // traversal positions per thread of k_keep_batch: 1 or 2 (autotuned)
 };
 
 // exclusive prefix s_pre[0..count] of f(v) over the volumes (all threads)
@@ -1804,26 +1834,35 @@ __device__ inline uint32_t batch_chunk_r(uint32_t total, uint32_t count, uint32_
     return (total + room - 1) / room;
 }
 
-__global__ __launch_bounds__(kBlock) void k_keep_batch(BatchArgs b, int q)
+template <int kI>
+__device__ __attribute__((always_inline)) inline void keep_batch_body(const BatchArgs& b, int q, KeepShared<kI>& sh,
+                                                                      uint32_t* s_pre, uint32_t* s_tmp)
 {
-    __shared__ KeepShared<kKeepItemsBatch> sh;
-    __shared__ uint32_t s_pre[kMaxBatch + 1];
-    __shared__ uint32_t s_tmp[kBlock / 64];
-    batch_prefix(b.count, [&](uint32_t v) { return keep_tiles<kKeepItemsBatch>(b.ctls[v].n[q]); }, s_pre, s_tmp);
+    batch_prefix(b.count, [&](uint32_t v) { return keep_tiles<kI>(b.ctls[v].n[q]); }, s_pre, s_tmp);
     const uint32_t r = batch_chunk_r(s_pre[b.count], b.count, gridDim.x);
     __syncthreads();
-    batch_prefix(b.count, [&](uint32_t v) { return (keep_tiles<kKeepItemsBatch>(b.ctls[v].n[q]) + r - 1) / r; }, s_pre, s_tmp);
+    batch_prefix(b.count, [&](uint32_t v) { return (keep_tiles<kI>(b.ctls[v].n[q]) + r - 1) / r; }, s_pre, s_tmp);
     const uint32_t j = blockIdx.x;
     if (j < s_pre[b.count]) {
         const uint32_t v = batch_find(s_pre, b.count, j);
         const KArgs& a = b.vols[v];
         const uint32_t n = b.ctls[v].n[q];
-        const ChunkPlan cp{r, (keep_tiles<kKeepItemsBatch>(n) + r - 1) / r};
+        const ChunkPlan cp{r, (keep_tiles<kI>(n) + r - 1) / r};
         const uint32_t chunk = j - s_pre[v];
-        uint32_t us[kKeepItemsBatch], ms[kKeepItemsBatch];
-        keep_voxels<kKeepItemsBatch>(a, q, chunk * r, us, ms);
-        (void)keep_chunk(a, q, n, cp, chunk, b.ctls[v].epoch, sh, us, ms);
+        uint32_t us[kI], ms[kI];
+        keep_voxels<kI>(a, q, chunk * r, us, ms);
+        (void)keep_chunk<kI>(a, q, n, cp, chunk, b.ctls[v].epoch, sh, us, ms);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_keep_batch(BatchArgs b, int q)
+{
+    __shared__ KeepShared<1> sh1;
+    __shared__ KeepShared<2> sh2;
+    __shared__ uint32_t s_pre[kMaxBatch + 1];
+    __shared__ uint32_t s_tmp[kBlock / 64];
+    if (b.keep_items == 1) keep_batch_body<1>(b, q, sh1, s_pre, s_tmp);
+    else keep_batch_body<2>(b, q, sh2, s_pre, s_tmp);
     // volumes with an empty list still advance their counts
     if (threadIdx.x < 64)
         for (uint32_t v = blockIdx.x; v < b.count; v += gridDim.x)
@@ -2273,6 +2312,28 @@ __global__ void k_float_to_phi8(const float* in, int8_t* out, uint64_t count)
         q = q > 127.0f ? 127.0f : (q < -128.0f ? -128.0f : q);
         out[i] = (int8_t)(int32_t)q;
     }
+}
+
+// Check mode (LSSEG_CHECKS): the invariants of list[q] after its compaction.
+// Every entry lies inside the volume and sits at the position that pos records
+// for it, so no voxel appears twice (the reference's duplicate removal relies
+// on duplicate-free lists, FilterDuplicates.cu:127-193, main.tex:399); every
+// traversal entry S[q][p] is a member of the list.  A violation sets kErrCheck.
+__global__ __launch_bounds__(kBlock) void k_check_list(KArgs a, int q)
+{
+    const uint32_t n = a.ctl->n[q];
+    const uint32_t* __restrict__ L = a.list[q];
+    const uint32_t* __restrict__ S = a.S[q];
+    bool bad = false;
+    LSSEG_GRID_STRIDE(p, n)
+    {
+        const uint32_t v = L[p];
+        bad |= v >= a.g.N || a.pos[v] != (uint32_t)p;
+        const uint32_t u = S[p];
+        const uint32_t pu = u < a.g.N ? a.pos[u] : 0xFFFFFFFFu;
+        bad |= pu >= n || L[pu < n ? pu : 0] != u;
+    }
+    if (bad) atomicOr(&a.ctl->error, kErrCheck);
 }
 
 // the parameters and a fresh control block, written by the device (callers

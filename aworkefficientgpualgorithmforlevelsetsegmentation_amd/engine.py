@@ -69,7 +69,8 @@ class Stats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("update_blocks", ctypes.c_uint32), ("compact_blocks", ctypes.c_uint32),
-                ("ms", ctypes.c_double), ("candidates", ctypes.c_uint32)]
+                ("ms", ctypes.c_double), ("candidates", ctypes.c_uint32), ("keep_items", ctypes.c_uint32),
+                ("compact_tile", ctypes.c_uint32), ("cached", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -451,8 +452,9 @@ class Batch:
         self._check(lib().lsseg_batch_get_tuning(self._h, ctypes.byref(t)))
         return t
 
-    def set_tuning(self, update_blocks: int, compact_blocks: int):
-        t = Tuning(update_blocks, compact_blocks, 0.0, 0)
+    def set_tuning(self, update_blocks: int, compact_blocks: int, keep_items: int = 0, compact_tile: int = 0):
+        """keep_items / compact_tile 0: unchanged."""
+        t = Tuning(update_blocks, compact_blocks, 0.0, 0, keep_items, compact_tile, 0)
         self._check(lib().lsseg_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def autotune(self, params, iters) -> Tuning:

@@ -557,8 +557,9 @@ def batch_vs_sequential(steps, iterations):
         b.volume(i).set_phi(w.phi0, w.band)
     b.checkpoint_save(0)
     tune = b.autotune(params, min(iters, 100))
-    log(f"autotuned launch shape: {tune.update_blocks} update/mark workgroups, {tune.compact_blocks} compaction "
-        f"workgroups ({tune.candidates} shapes)")
+    log(f"autotuned: {tune.update_blocks} update/mark workgroups, {tune.compact_blocks} compaction workgroups, "
+        f"keep step {tune.keep_items} positions per thread, compaction tiles of {tune.compact_tile} positions "
+        f"({tune.candidates} tunings timed{', cached' if tune.cached else ''})")
     b.checkpoint_restore(0)
     b.iterate(params, iters)  # warm-up (graph capture)
     device_sync()
@@ -598,7 +599,9 @@ def batch_vs_sequential(steps, iterations):
         "ms_per_step": 1e3 * t_batch, "value": upd / t_batch, "unit": "active-voxel updates/s",
         "sequential_ms": 1e3 * t_seq, "sequential_value": upd / t_seq,
         "speedup_vs_sequential": t_seq / t_batch, "masks_identical_to_sequential": same,
-        "tuning": {"update_blocks": tune.update_blocks, "compact_blocks": tune.compact_blocks},
+        "tuning": {"update_blocks": tune.update_blocks, "compact_blocks": tune.compact_blocks,
+                   "keep_items": tune.keep_items, "compact_tile": tune.compact_tile,
+                   "tunings_timed": tune.candidates},
     }
     log(f"batched: {1e3 * t_batch:.2f} ms/step ({upd / t_batch:.3e} updates/s); sequential plain runs: "
         f"{1e3 * t_seq:.2f} ms ({upd / t_seq:.3e} updates/s); speedup {t_seq / t_batch:.2f}x; "

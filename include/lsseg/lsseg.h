@@ -332,22 +332,33 @@ lsseg_status lsseg_batch_iterate(lsseg_batch* batch, const lsseg_params* params,
 lsseg_status lsseg_batch_checkpoint_save(lsseg_batch* batch, int32_t slot);
 lsseg_status lsseg_batch_checkpoint_restore(lsseg_batch* batch, int32_t slot);
 
-/* Launch shape of the batched kernels: workgroups of the update and marking
-   launches and of the keep and compaction launches (more than the volume
-   count, at most the resident count). */
+/* Launch shape and widths of the batched kernels: workgroups of the update
+   and marking launches and of the keep and compaction launches (more than the
+   volume count, at most the resident count), traversal positions per thread
+   of the keep step (1 or 2; 0 in set_tuning: unchanged) and list positions per
+   few-path compaction tile (256 or 1024; 0: unchanged). */
 typedef struct lsseg_tuning {
     uint32_t update_blocks;
     uint32_t compact_blocks;
-    double ms;            /* autotune: device time of the chosen shape */
-    uint32_t candidates;  /* autotune: shapes timed */
+    double ms;             /* autotune: device time of the chosen tuning */
+    uint32_t candidates;   /* autotune: tunings timed (0 when taken from the cache) */
+    uint32_t keep_items;
+    uint32_t compact_tile;
+    uint32_t cached;       /* autotune: 1 when the result came from the tuning cache */
 } lsseg_tuning;
 
 lsseg_status lsseg_batch_get_tuning(const lsseg_batch* batch, lsseg_tuning* out);
 lsseg_status lsseg_batch_set_tuning(lsseg_batch* batch, const lsseg_tuning* tuning);
 
-/* Times `iters` iterations of the batch from its current state for each
-   candidate launch shape, keeps the fastest and restores the state.  Uses
-   checkpoint slot LSSEG_MAX_CHECKPOINTS - 1 of every volume. */
+/* Times `iters` iterations of the batch from its current state for candidate
+   tunings and keeps the fastest, by coordinate descent: 12 launch shapes at
+   the default widths, then the keep step's positions per thread, then the
+   compaction tile; restores the state (uses checkpoint slot
+   LSSEG_MAX_CHECKPOINTS - 1 of every volume).  Results are cached per (GPU
+   architecture, source type, phi storage, volume count, total voxels to the
+   next power of two, iters): in the process, and in the file LSSEG_TUNING_CACHE
+   names if it is set; a cached result is applied without timing
+   (LSSEG_AUTOTUNE_FORCE=1 times again). */
 lsseg_status lsseg_batch_autotune(lsseg_batch* batch, const lsseg_params* params, int32_t iters,
                                   lsseg_tuning* out);
 

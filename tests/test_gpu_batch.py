@@ -133,10 +133,17 @@ def test_batch_checkpoint_and_autotune_keep_results():
     batch.iterate(params, 40)
     first = [batch.volume(i).state() for i in range(len(specs))]
     t = batch.autotune(params, 20)
-    _say(f"autotune: {t.update_blocks} update blocks, {t.compact_blocks} compaction blocks, "
-         f"{t.candidates} shapes, {t.ms:.3f} ms")
-    assert t.candidates >= 2 and t.update_blocks > 0 and t.compact_blocks > 0
-    assert batch.tuning().update_blocks == t.update_blocks
+    _say(f"autotune: {t.update_blocks} update blocks, {t.compact_blocks} compaction blocks, keep items "
+         f"{t.keep_items}, compaction tile {t.compact_tile}, {t.candidates} tunings, {t.ms:.3f} ms")
+    assert t.candidates >= 4 and t.update_blocks > 0 and t.compact_blocks > 0 and not t.cached
+    assert t.keep_items in (1, 2) and t.compact_tile in (256, 1024)
+    got = batch.tuning()
+    assert (got.update_blocks, got.keep_items, got.compact_tile) == (t.update_blocks, t.keep_items, t.compact_tile)
+    # the same workload again: the cached result, applied without timing
+    t2 = batch.autotune(params, 20)
+    assert t2.cached and t2.candidates == 0
+    assert (t2.update_blocks, t2.compact_blocks, t2.keep_items, t2.compact_tile) == \
+        (t.update_blocks, t.compact_blocks, t.keep_items, t.compact_tile)
     batch.checkpoint_restore(0)
     batch.iterate(params, 40)
     for i in range(len(specs)):
@@ -144,6 +151,30 @@ def test_batch_checkpoint_and_autotune_keep_results():
         assert np.array_equal(lst, first[i][3]) and np.array_equal(r.view(np.uint32), first[i][0].view(np.uint32))
     with pytest.raises(LssegError):
         batch.set_tuning(0, 1)
+    with pytest.raises(LssegError):
+        batch.set_tuning(256, 64, keep_items=3)
+    batch.close()
+
+
+@pytest.mark.parametrize("keep_items,compact_tile", [(1, 256), (2, 256), (1, 1024)])
+def test_batch_widths_give_the_solo_results(keep_items, compact_tile):
+    """Every tuning of the keep step's width and the compaction tile leaves each
+    volume bit-identical to its solo run."""
+    specs = _volumes()
+    batch = Batch([s[0] for s in specs])
+    t = batch.tuning()
+    batch.set_tuning(t.update_blocks, t.compact_blocks, keep_items, compact_tile)
+    solos = [Context(s[0]) for s in specs]
+    for i, (vol, phi0, band, prm) in enumerate(specs):
+        batch.volume(i).set_phi(phi0, band)
+        solos[i].set_phi(phi0, band)
+    params = [s[3] for s in specs]
+    batch.iterate(params, 41)
+    for i, s in enumerate(solos):
+        T, eps, alpha, dt, mode = params[i]
+        s.iterate(T, eps, alpha, dt, 41, mode=mode)
+        _assert_same(batch.volume(i), s, f"widths {keep_items}/{compact_tile} volume {i}")
+        s.close()
     batch.close()
 
 
