@@ -262,8 +262,6 @@ struct KArgs {
     uint32_t run_log2;    // log2 of the list positions per XCD run (0: plain grid stride)
     uint32_t fused_compact_max;  // k_update_fused: at most this many workgroups (a multiple of 8) take
                                  // the compaction, the rest of its grid the update
-    uint32_t keep_items;  // S positions per thread of the keep step: 1, 2 or 4; 0: the fewest that
-                          // leave one look-back chunk of one sub-tile per workgroup (up to 4)
     void* phi[2];         // float, or int8_t in the fixed-point storage mode
     uint8_t* moved;       // temporal-derivative flag per traversal position, written by the update
     const void* src;
@@ -1506,41 +1504,20 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
     return true;
 }
 
-// the keep step at kI positions per thread; us, ms: this workgroup's first
-// sub-tile when `loaded`
+// the keep step at kI traversal positions per thread (one by default, two with
+// LSSEG_KEEP_ITEMS=2), one look-back chunk per workgroup: a chunk of several
+// sub-tiles when sub-tiles outnumber workgroups
 template <int kI>
-__device__ __attribute__((always_inline)) inline void keep_step(const KArgs& a, int q, uint32_t n, uint32_t epoch,
-                                                                KeepShared<kI>& sh, uint32_t* us, uint32_t* ms,
-                                                                bool loaded)
-{
-    const ChunkPlan cp = chunk_plan(keep_tiles<kI>(n), gridDim.x);
-    if (blockIdx.x < cp.chunks) {
-        if (!loaded && cp.r == 1) keep_voxels<kI>(a, q, blockIdx.x, us, ms);
-        (void)keep_chunk<kI>(a, q, n, cp, blockIdx.x, epoch, sh, us, ms);
-    }
-}
-
 __global__ __launch_bounds__(kBlock) void k_keep(KArgs a, int q)
 {
-    __shared__ KeepShared<1> sh1;
-    __shared__ KeepShared<2> sh2;
-    __shared__ KeepShared<4> sh4;
-    uint32_t us[4], ms[4];
-    keep_voxels<1>(a, q, blockIdx.x, us, ms);  // with the count: this workgroup's sub-tile at one position per thread
+    __shared__ KeepShared<kI> sh;
+    uint32_t us[kI], ms[kI];
+    keep_voxels<kI>(a, q, blockIdx.x, us, ms);  // with the count: this workgroup's sub-tile when chunks are single
     const uint32_t epoch = a.ctl->epoch;
     const uint32_t n = a.ctl->n[q];
-    // one position per thread while the grid covers the list in single
-    // sub-tiles, else two or four (a single pass each), past that several
-    // sub-tiles per chunk
-    uint32_t kI = a.keep_items;
-    if (kI == 0) {
-        const uint64_t cover = (uint64_t)gridDim.x * kBlock;
-        kI = n <= cover ? 1u : (n <= 2 * cover ? 2u : 4u);
-    }
-    if (kI == 1) keep_step<1>(a, q, n, epoch, sh1, us, ms, true);
-    else if (kI == 2) keep_step<2>(a, q, n, epoch, sh2, us, ms, false);
-    else keep_step<4>(a, q, n, epoch, sh4, us, ms, false);
-    if (n == 0 && blockIdx.x == 0 && threadIdx.x < 64)
+    const ChunkPlan cp = chunk_plan(keep_tiles<kI>(n), gridDim.x);
+    if (blockIdx.x < cp.chunks) (void)keep_chunk<kI>(a, q, n, cp, blockIdx.x, epoch, sh, us, ms);
+    if (cp.chunks == 0 && blockIdx.x == 0 && threadIdx.x < 64)
         keep_finish(a, q, 0, 0, epoch, candidate_share(a), ctl_snapshot(a.ctl));
 }
 
