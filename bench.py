@@ -144,7 +144,7 @@ def _kernel_key(name):
         return "k_update"
     if name.startswith("lsseg::k_mark<"):
         return "k_mark"
-    if name == "lsseg::k_keep":
+    if name.startswith("lsseg::k_keep<"):
         return "k_keep"
     if name == "lsseg::k_compact<2>":
         return "k_compact"

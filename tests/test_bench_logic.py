@@ -41,10 +41,10 @@ def test_launch_parts_cover_non_empty_iterations_only():
 def test_profiler_rows_map_to_iterations():
     names = ["lsseg::k_set_phi<float>", "void lsseg::k_compact<3>(lsseg::KArgs, int)",
              "void lsseg::k_update<unsigned short, false, float>(lsseg::KArgs, int)",
-             "void lsseg::k_mark<float>(lsseg::KArgs, int)", "lsseg::k_keep(lsseg::KArgs, int)"]
+             "void lsseg::k_mark<float>(lsseg::KArgs, int)", "void lsseg::k_keep<1>(lsseg::KArgs, int)"]
     for _ in range(3):
         names += ["void lsseg::k_update_fused<unsigned short, false, float>(lsseg::KArgs, int)",
-                  "void lsseg::k_mark<float>(lsseg::KArgs, int)", "lsseg::k_keep(lsseg::KArgs, int)"]
+                  "void lsseg::k_mark<float>(lsseg::KArgs, int)", "void lsseg::k_keep<1>(lsseg::KArgs, int)"]
     names.append("void lsseg::k_compact<2>(lsseg::KArgs, int)")
     rows = [{"Kernel_Name": k, "Dispatch_Id": str(i)} for i, k in enumerate(names)]
     rows = bench._dispatch_order(rows[::-1])
