@@ -1288,6 +1288,40 @@ lsseg_status lsseg_get_device_buffers(const lsseg_ctx* c, void** phi_read, void*
     return LSSEG_OK;
 }
 
+lsseg_status lsseg_probe_random_access(void* buf, uint64_t bytes, uint32_t count, int32_t mode, int32_t reps,
+                                       void* stream, float* us_median)
+{
+    if (!buf || !us_median || count == 0 || reps < 1 || reps > 1000 || bytes < 256 || (bytes & 3u) ||
+        (reinterpret_cast<uintptr_t>(buf) & 3u) || (mode != 0 && mode != 1 && mode != 2))
+        return fail(nullptr, LSSEG_ERR_INVALID_ARG, "lsseg_probe_random_access: invalid argument");
+    const void* fn = mode == 0   ? (const void*)&k_probe_access<1, false>
+                     : mode == 1 ? (const void*)&k_probe_access<7, false>
+                                 : (const void*)&k_probe_access<1, true>;
+    hipStream_t s = (hipStream_t)stream;
+    float* a = static_cast<float*>(buf);
+    float* sink = a;  // written only where a thread's loads sum to the kernel's sentinel (never on a zeroed buffer)
+    uint64_t words = bytes / 4u;
+    std::vector<float> ms;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    for (int32_t r = 0; r < reps && e == hipSuccess; ++r) {
+        uint32_t seed = 0x9e3779b9u * (uint32_t)(r + 1);
+        void* args[] = {(void*)&a, (void*)&words, (void*)&count, (void*)&seed, (void*)&sink};
+        e = hipExtLaunchKernel(fn, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), args, 0, s, e0, e1, 0);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float m = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&m, e0, e1);
+        ms.push_back(m);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(nullptr, LSSEG_ERR_DEVICE, "lsseg_probe_random_access: %s", hipGetErrorString(e));
+    std::sort(ms.begin(), ms.end());
+    *us_median = 1e3f * ms[ms.size() / 2];
+    return LSSEG_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

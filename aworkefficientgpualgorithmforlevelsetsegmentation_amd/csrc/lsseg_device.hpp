@@ -2078,6 +2078,35 @@ __global__ void k_fill_f32(float* p, uint64_t count, float v)
     LSSEG_GRID_STRIDE(i, count) p[i] = v;
 }
 
+// random-access floor (measurement aid, not on the path): thread t < count
+// makes `loads` loads (or one store) of 4 bytes, each in a pseudo-random
+// 32-byte sector of the `words` 4-byte words at a; every index is reduced
+// modulo the sector count, so all accesses stay in the buffer
+__device__ inline uint32_t probe_mix(uint32_t x)
+{
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+
+template <int kLoads, bool kStore>
+__global__ __launch_bounds__(kBlock) void k_probe_access(float* a, uint64_t words, uint32_t count, uint32_t seed,
+                                                         float* sink)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    const uint64_t sectors = words / 8u;
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < kLoads; ++k) {
+        const uint32_t i = t * (uint32_t)kLoads + (uint32_t)k;
+        const uint64_t h = ((uint64_t)probe_mix(i ^ seed) << 32) | probe_mix(i * 2654435761u + seed);
+        const uint64_t w = (h % sectors) * 8u + (i & 7u);
+        if constexpr (kStore) a[w] = 0.f;
+        else s += a[w];
+    }
+    if (!kStore && s == 12345.678f && t < words) sink[t] = s;  // keeps the loads; a zeroed buffer never takes it
+}
+
 __global__ void k_fill_u8(uint8_t* p, uint64_t count, uint8_t v)
 {
     LSSEG_GRID_STRIDE(i, count) p[i] = v;

@@ -36,7 +36,7 @@ EXPORTS = (
     "lsseg_batch_volume", "lsseg_batch_iterate", "lsseg_batch_checkpoint_save", "lsseg_batch_checkpoint_restore",
     "lsseg_batch_get_tuning", "lsseg_batch_set_tuning", "lsseg_batch_autotune",
     "lsseg_fwd_record", "lsseg_record_destroy", "lsseg_record_info", "lsseg_record_get", "lsseg_fwd_replay",
-    "lsseg_bwd", "lsseg_auto_parameters", "lsseg_segment_noalloc",
+    "lsseg_bwd", "lsseg_auto_parameters", "lsseg_segment_noalloc", "lsseg_probe_random_access",
 )
 
 
@@ -160,6 +160,7 @@ def lib():
             "lsseg_bwd": (i32, [vp, vp, vp, vp, vp]),
             "lsseg_auto_parameters": (i32, [vp, P(f32), P(f32), P(u64)]),
             "lsseg_segment_noalloc": (i32, [P(VolumeDesc), vp, vp, f32, P(Params), i32, vp, vp, vp, P(Stats)]),
+            "lsseg_probe_random_access": (i32, [vp, u64, ctypes.c_uint32, i32, i32, vp, P(f32)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -167,6 +168,24 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+PROBE_LOAD, PROBE_LOAD7, PROBE_STORE = 0, 1, 2
+
+
+def probe_random_access(buf_ptr: int, nbytes: int, count: int, mode: int, reps: int = 21, stream=None) -> float:
+    """Median launch time (us) of `count` threads making one random 4-byte load
+    (PROBE_LOAD), seven (PROBE_LOAD7) or one store (PROBE_STORE) each, in
+    32-byte sectors of the device buffer at buf_ptr (nbytes bytes; a store
+    overwrites words with zeros): the random-access floor the path's gathers
+    run against."""
+    us = ctypes.c_float(0.0)
+    L = lib()
+    st = L.lsseg_probe_random_access(ctypes.c_void_p(buf_ptr), nbytes, count, mode, reps,
+                                     ctypes.c_void_p(stream) if stream else None, ctypes.byref(us))
+    if st != 0:
+        raise LssegError(st, L.lsseg_last_error(None).decode())
+    return float(us.value)
 
 
 def _p(a: np.ndarray):

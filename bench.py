@@ -483,6 +483,10 @@ def plain_leg(name, args, with_cpu, window_at=None):
                     for k, v in kernels.items())
         + f"; kernel time sum {kt.total_ms:.2f} ms vs {ms_per_step:.2f} ms/step")
 
+    floor = random_access_floor(ctx, int(round(float(np.mean(pc.n[pc.n > 0])))), P * ctx.X * ctx.Y * ctx.Z)
+    log(f"{name}: random-access floor at {floor['threads']} threads over {floor['buffer_bytes'] / 2**20:.0f} MiB: "
+        f"one load {floor['load_us']:.2f} us, seven loads {floor['load7_us']:.2f} us, one store {floor['store_us']:.2f} us")
+
     cpu = dice = identical = None
     if with_cpu:
         log(f"{name}: running the CPU oracle")
@@ -508,12 +512,31 @@ def plain_leg(name, args, with_cpu, window_at=None):
         "path_roofline": {"achieved": path_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": path_gbs / HBM_PEAK_GBS, "bytes_per_step": path_bytes},
         "kernels": kernels,
+        "random_access_floor": floor,
         "cpu_baseline": cpu,
         "host_inclusive": host_inclusive,
         "dice_vs_oracle": dice,
         "mask_identical_to_oracle": identical,
     }
     ctx.close()
+    return out
+
+
+def random_access_floor(ctx, threads, nbytes):
+    """Launch times of the random-access floor (lsseg_probe_random_access) at
+    the workload's mean list length, over the context's phi write buffer
+    (nbytes; its contents are spent: the store mode zeroes words of it): what
+    one launch of `threads` threads costs when each makes one (or seven) loads,
+    or one store, in 32-byte sectors spread over a buffer of that size."""
+    from aworkefficientgpualgorithmforlevelsetsegmentation_amd import engine
+    _, buf, stream = ctx.device_buffers()
+    out = {"threads": threads, "buffer_bytes": nbytes}
+    for key, mode in (("load_us", engine.PROBE_LOAD), ("load7_us", engine.PROBE_LOAD7),
+                      ("store_us", engine.PROBE_STORE)):
+        out[key] = engine.probe_random_access(buf, nbytes, threads, mode, stream=stream)
+    out["what"] = ("median of 21 launches; each of `threads` threads (the mean list length of the non-empty "
+                   "iterations) makes one pseudo-random 4-byte load, seven such loads, or one store, each in its own "
+                   "32-byte sector of a buffer the size of one phi buffer")
     return out
 
 
@@ -528,7 +551,10 @@ def profile_only(args):
     ctx.set_phi(w.phi0, w.band)
     st, kt, per = ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, w.iterations)
     pc = ctx.iteration_counts()
-    kernels, L = launch_parts(per, pc.n, pc.c, pc.m, w.volume.dtype.itemsize, 4)
+    if os.environ.get("LSSEG_FUSE", "1") == "0":  # four launches per iteration: no per-part summary
+        kernels, L = None, int(np.count_nonzero(pc.n))
+    else:
+        kernels, L = launch_parts(per, pc.n, pc.c, pc.m, w.volume.dtype.itemsize, 4)
     out = {"workload": describe(w, ctx.X, ctx.Y, ctx.Z), "S": w.volume.dtype.itemsize, "P": 4,
            "n": [int(x) for x in pc.n], "c": [int(x) for x in pc.c], "m": [int(x) for x in pc.m],
            "launched_iterations": L, "kernels": kernels}

@@ -412,6 +412,20 @@ lsseg_status lsseg_fwd_replay(lsseg_ctx* ctx, lsseg_record* rec, const float* ph
 lsseg_status lsseg_bwd(lsseg_ctx* ctx, lsseg_record* rec, const float* dphi, float* dphi0,
                        double* dtheta);
 
+/* ---- measurement aid ----------------------------------------------------
+
+   No counterpart in the reference.  The random-access floor that the path's
+   gathers and scatters run against: `reps` launches (one at a time on
+   `stream`, each timed by its own start and stop events) in which each of
+   `count` threads makes one pseudo-random 4-byte load (mode 0), seven such
+   loads (mode 1, the fan-out of the update's stencil without its locality) or
+   one such store (mode 2) in the caller's device buffer `buf` of `bytes`
+   bytes, each access in its own 32-byte sector.  Mode 2 overwrites words of
+   the buffer with zeros.  *us_median receives the median launch time in
+   microseconds. */
+lsseg_status lsseg_probe_random_access(void* buf, uint64_t bytes, uint32_t count, int32_t mode, int32_t reps,
+                                       void* stream, float* us_median);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,14 +7,21 @@
 - LSSEG_FAIL_ALLOC=k fails the k-th device allocation of a context creation:
   every k makes lsseg_create return LSSEG_ERR_OUT_OF_MEMORY with a message and
   no leaked device memory, and the library works normally afterwards.
+- lsseg_probe_random_access (the measurement aid behind the bench line's
+  random-access floor) times its three modes, writes only zeros, and refuses
+  bad arguments.
 """
 import ctypes
 
 import numpy as np
 import pytest
 
-from aworkefficientgpualgorithmforlevelsetsegmentation_amd import engine, synth
-from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import Batch, Context, LssegError
+# torch ships its own HIP runtime and must provide the process's runtime: it is
+# imported here, at collection, before any test loads the engine.
+torch = pytest.importorskip("torch")
+
+from aworkefficientgpualgorithmforlevelsetsegmentation_amd import engine, synth  # noqa: E402
+from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import Batch, Context, LssegError  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -92,3 +99,17 @@ def test_every_allocation_failure_of_create_is_reported_and_cleaned_up(monkeypat
     ctx.set_phi(w.phi0, w.band)
     assert ctx.iterate(w.T, w.eps, w.alpha, w.dt, 20).sum_active > 0
     ctx.close()
+
+
+def test_random_access_probe():
+    buf = torch.zeros(1 << 20, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    nbytes = buf.numel() * 4
+    for mode in (engine.PROBE_LOAD, engine.PROBE_LOAD7, engine.PROBE_STORE):
+        _say(f"probe mode {mode}: 10000 threads over {nbytes} bytes")
+        us = engine.probe_random_access(buf.data_ptr(), nbytes, 10000, mode, reps=3)
+        assert 0.0 < us < 1e5, us
+    assert float(buf.abs().sum().item()) == 0.0  # the stores write zeros, the loads nothing
+    for bad in ((nbytes, 0, 0), (100, 10, 0), (nbytes, 10, 3), (nbytes - 2, 10, 0)):
+        with pytest.raises(LssegError):
+            engine.probe_random_access(buf.data_ptr(), bad[0], bad[1], bad[2], reps=3)
