@@ -56,6 +56,8 @@ struct lsseg_ctx {
     uint32_t fgrid = 1024;            // grid of k_update_fused (compaction + update)
     uint32_t sgrid = 2048;            // workgroups of the update and marking kernels
     const void* keep_fn = nullptr;    // k_keep<1> or k_keep<2> (LSSEG_KEEP_ITEMS)
+    uint32_t grid0 = 1024;            // the keep/compaction grid chosen at creation (occupancy x CUs)
+    uint32_t small_n_max = 0;         // the small-tile bound at creation (the look-back states are sized for it)
     uint32_t cus = 256;
     KArgs a{};
     DevParams* d_prm = nullptr;
@@ -695,6 +697,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     const int want = env_int("LSSEG_GRID_BLOCKS", 0);
     c->grid = want > 0 ? (uint32_t)want : (uint32_t)occ * c->cus;
     c->grid = std::max<uint32_t>(16u, c->grid & ~7u);  // k_update_fused splits it in multiples of 8
+    c->grid0 = std::max<uint32_t>(16u, ((uint32_t)occ * c->cus) & ~7u);
     // k_update_fused: at most three quarters of the keep/compaction grid
     // compact, the rest of the fused launch's grid updates
     c->a.fused_compact_max = (c->grid / 4u * 3u) & ~7u;
@@ -736,6 +739,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
 
     Geom& g = c->a.g;
     init_geom(g, (uint32_t)X, (uint32_t)Y, (uint32_t)Z);
+    c->small_n_max = g.small_n;
 
     void* p = nullptr;
     const uint64_t ntiles = lookback_states(g);
@@ -1769,13 +1773,19 @@ std::mutex g_tune_mu;
 std::vector<std::pair<std::string, lsseg_tuning>> g_tune_cache;
 bool g_tune_file_read = false;
 
-std::string tuning_key(const lsseg_batch* b, int32_t iters)
+std::string gpu_arch()
 {
     hipDeviceProp_t prop{};
     int dev = 0;
     std::string arch = "unknown";
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
         arch = std::string(prop.gcnArchName).substr(0, std::string(prop.gcnArchName).find(':'));
+    return arch;
+}
+
+std::string tuning_key(const lsseg_batch* b, int32_t iters)
+{
+    const std::string arch = gpu_arch();
     uint64_t voxels = 0;
     for (const lsseg_ctx* c : b->vols) voxels += c->a.g.N;
     int lg = 0;
@@ -2150,6 +2160,151 @@ lsseg_status lsseg_batch_autotune(lsseg_batch* b, const lsseg_params* params, in
     best.cached = 0;
     if ((st = lsseg_batch_set_tuning(b, &best)) != LSSEG_OK) return st;
     if ((st = lsseg_batch_checkpoint_restore(b, slot)) != LSSEG_OK) return st;
+    tuning_cache_put(key, best);
+    if (out) *out = best;
+    return LSSEG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// launch shapes and widths of a single context
+// ---------------------------------------------------------------------------
+namespace {
+
+std::string ctx_tuning_key(const lsseg_ctx* c, int32_t iters)
+{
+    int lg = 0;
+    while ((1ull << lg) < c->a.g.N) ++lg;
+    char buf[160];
+    snprintf(buf, sizeof(buf), "%s/single/dtype%d/phi%d/voxels2^%d/iters%d", gpu_arch().c_str(), c->dtype,
+             c->phi8 ? 1 : 0, lg, iters);
+    return buf;
+}
+
+}  // namespace
+
+extern "C" {
+
+lsseg_status lsseg_get_tuning(const lsseg_ctx* c, lsseg_tuning* out)
+{
+    if (!c || !out) return LSSEG_ERR_INVALID_ARG;
+    *out = lsseg_tuning{};
+    out->update_blocks = c->sgrid;
+    out->compact_blocks = c->grid;
+    out->keep_items = c->keep_fn == (const void*)&k_keep<2> ? 2u : 1u;
+    out->compact_tile = c->a.g.small_n > 0 ? (uint32_t)kBlock : (uint32_t)kPosTile;
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_set_tuning(lsseg_ctx* c, const lsseg_tuning* t)
+{
+    if (!c || !t) return LSSEG_ERR_INVALID_ARG;
+    lsseg_tuning cur{};
+    (void)lsseg_get_tuning(c, &cur);
+    const uint32_t ub = t->update_blocks ? t->update_blocks : cur.update_blocks;
+    const uint32_t cb = t->compact_blocks ? t->compact_blocks : cur.compact_blocks;
+    const uint32_t ki = t->keep_items ? t->keep_items : cur.keep_items;
+    const uint32_t ct = t->compact_tile ? t->compact_tile : cur.compact_tile;
+    if (ub < 8 || (ub & 7u) || ub > (1u << 20) || cb < 16 || (cb & 7u) || cb > (1u << 20))
+        return fail(c, LSSEG_ERR_INVALID_ARG,
+                    "launch shape %u/%u: workgroup counts are multiples of 8, at least 8 (update) and 16 (scan), "
+                    "at most 2^20", ub, cb);
+    if ((ki != 1 && ki != 2) || (ct != (uint32_t)kBlock && ct != (uint32_t)kPosTile))
+        return fail(c, LSSEG_ERR_INVALID_ARG, "keep items %u (1 or 2) or compaction tile %u (%d or %d)", ki, ct,
+                    kBlock, kPosTile);
+    if (ct == (uint32_t)kBlock && c->small_n_max == 0)
+        return fail(c, LSSEG_ERR_INVALID_ARG,
+                    "compaction tiles of %d positions need a context created with them (LSSEG_SMALL_TILE_N > 0)",
+                    kBlock);
+    if (ub == cur.update_blocks && cb == cur.compact_blocks && ki == cur.keep_items && ct == cur.compact_tile)
+        return LSSEG_OK;
+    destroy_graphs(c);  // captured launch shapes and kernel arguments change
+    c->sgrid = ub;
+    c->grid = cb;
+    c->a.fused_compact_max = (cb / 4u * 3u) & ~7u;
+    c->fgrid = cb;
+    c->keep_fn = ki == 2 ? (const void*)&k_keep<2> : (const void*)&k_keep<1>;
+    c->a.g.small_n = ct == (uint32_t)kBlock ? c->small_n_max : 0u;
+    return LSSEG_OK;
+}
+
+lsseg_status lsseg_autotune(lsseg_ctx* c, const lsseg_params* params, int32_t iters, lsseg_tuning* out)
+{
+    if (!c || !params) return LSSEG_ERR_INVALID_ARG;
+    if (iters <= 0) return fail(c, LSSEG_ERR_INVALID_ARG, "autotune needs a positive iteration count");
+    const std::string key = ctx_tuning_key(c, iters);
+    lsseg_tuning best{};
+    if (!env_int("LSSEG_AUTOTUNE_FORCE", 0) && tuning_cache_get(key, &best)) {
+        if (best.compact_tile == (uint32_t)kBlock && c->small_n_max == 0) best.compact_tile = kPosTile;
+        best.cached = 1;
+        best.candidates = 0;
+        lsseg_status st = lsseg_set_tuning(c, &best);
+        if (st == LSSEG_OK && out) *out = best;
+        return st;
+    }
+    const int32_t slot = LSSEG_MAX_CHECKPOINTS - 1;
+    lsseg_status st = lsseg_checkpoint_save(c, slot);
+    if (st != LSSEG_OK) return st;
+    uint32_t timed = 0;
+    // device time of `iters` iterations from the saved state with tuning t;
+    // the first run captures the graphs, the second is timed
+    auto time_shape = [&](const lsseg_tuning& t, double* ms) -> lsseg_status {
+        lsseg_status s2 = lsseg_set_tuning(c, &t);
+        if (s2 != LSSEG_OK) return s2;
+        for (int rep = 0; rep < 2; ++rep) {
+            if ((s2 = lsseg_checkpoint_restore(c, slot)) != LSSEG_OK) return s2;
+            lsseg_stats stt{};
+            if ((s2 = lsseg_iterate(c, params, iters, &stt)) != LSSEG_OK) return s2;
+            *ms = stt.device_ms;
+        }
+        ++timed;
+        return LSSEG_OK;
+    };
+    // coordinate descent: the launch shape (update and marking workgroups,
+    // keep and compaction workgroups) at the current widths, then the keep
+    // step's positions per thread, then the compaction's tile
+    lsseg_tuning start{};
+    (void)lsseg_get_tuning(c, &start);
+    std::vector<lsseg_tuning> cands;
+    const uint32_t half = std::max<uint32_t>(16u, (c->grid0 / 2u) & ~7u);
+    const uint32_t two = std::max<uint32_t>(16u, (2u * c->cus) & ~7u);
+    for (uint32_t su : {4u, 8u, 16u})
+        for (uint32_t cb : {c->grid0, half, two}) {
+            lsseg_tuning t = start;
+            t.update_blocks = std::max<uint32_t>(8u, (su * c->cus) & ~7u);
+            t.compact_blocks = cb;
+            bool dup = false;
+            for (auto& o : cands) dup |= o.update_blocks == t.update_blocks && o.compact_blocks == t.compact_blocks;
+            if (!dup) cands.push_back(t);
+        }
+    best.ms = 1e30;
+    for (auto& t : cands) {
+        double ms = 0;
+        if ((st = time_shape(t, &ms)) != LSSEG_OK) return st;
+        if (ms < best.ms) {
+            best = t;
+            best.ms = ms;
+        }
+    }
+    for (int stage = 0; stage < 2; ++stage) {
+        lsseg_tuning t = best;
+        if (stage == 0) t.keep_items = best.keep_items == 1 ? 2 : 1;
+        else if (c->small_n_max > 0)
+            t.compact_tile = best.compact_tile == (uint32_t)kBlock ? (uint32_t)kPosTile : (uint32_t)kBlock;
+        else
+            continue;
+        double ms = 0;
+        if ((st = time_shape(t, &ms)) != LSSEG_OK) return st;
+        if (ms < best.ms) {
+            best = t;
+            best.ms = ms;
+        }
+    }
+    best.candidates = timed;
+    best.cached = 0;
+    if ((st = lsseg_set_tuning(c, &best)) != LSSEG_OK) return st;
+    if ((st = lsseg_checkpoint_restore(c, slot)) != LSSEG_OK) return st;
     tuning_cache_put(key, best);
     if (out) *out = best;
     return LSSEG_OK;

@@ -37,6 +37,7 @@ EXPORTS = (
     "lsseg_batch_get_tuning", "lsseg_batch_set_tuning", "lsseg_batch_autotune",
     "lsseg_fwd_record", "lsseg_record_destroy", "lsseg_record_info", "lsseg_record_get", "lsseg_fwd_replay",
     "lsseg_bwd", "lsseg_auto_parameters", "lsseg_segment_noalloc", "lsseg_probe_random_access",
+    "lsseg_get_tuning", "lsseg_set_tuning", "lsseg_autotune",
 )
 
 
@@ -161,6 +162,9 @@ def lib():
             "lsseg_auto_parameters": (i32, [vp, P(f32), P(f32), P(u64)]),
             "lsseg_segment_noalloc": (i32, [P(VolumeDesc), vp, vp, f32, P(Params), i32, vp, vp, vp, P(Stats)]),
             "lsseg_probe_random_access": (i32, [vp, u64, ctypes.c_uint32, i32, i32, vp, P(f32)]),
+            "lsseg_get_tuning": (i32, [vp, P(Tuning)]),
+            "lsseg_set_tuning": (i32, [vp, P(Tuning)]),
+            "lsseg_autotune": (i32, [vp, P(Params), i32, P(Tuning)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -385,6 +389,24 @@ class Context:
         if r.size != self.N or w.size != self.N or e.size != self.N:
             raise ValueError("state arrays must have X*Y*Z entries")
         self._check(lib().lsseg_set_state(self._h, _p(r), _p(w), _p(e), _p(v) if v.size else None, v.size))
+
+    def tuning(self) -> "Tuning":
+        t = Tuning()
+        self._check(lib().lsseg_get_tuning(self._h, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, update_blocks: int = 0, compact_blocks: int = 0, keep_items: int = 0, compact_tile: int = 0):
+        """0 in a field: unchanged."""
+        t = Tuning(update_blocks, compact_blocks, 0.0, 0, keep_items, compact_tile, 0)
+        self._check(lib().lsseg_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, T: float, eps: float, alpha: float, dt: float, iters: int, mode=MODE_REF) -> "Tuning":
+        """Times the launch shapes and widths on `iters` iterations from the
+        current state, keeps the fastest and restores the state."""
+        p = Params(T, eps, alpha, dt, mode)
+        t = Tuning()
+        self._check(lib().lsseg_autotune(self._h, ctypes.byref(p), iters, ctypes.byref(t)))
+        return t
 
     def device_buffers(self):
         r, w, s = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()

@@ -152,19 +152,17 @@ def _kernel_key(name):
 
 
 def iteration_dispatches(rows, n):
-    """Rows (dicts with Kernel_Name, in dispatch order) of one profiled run ->
-    {kernel: [(iteration, row), ...]} for the iteration launches (the run's
-    first k_update starts them)."""
+    """Rows (dicts with Kernel_Name, in dispatch order) of a process whose last
+    iteration run is the profiled one -> {kernel: [(iteration, row), ...]} for
+    that run's iteration launches (its k_update, the last in the process,
+    starts them; earlier runs, such as the autotuner's, are skipped)."""
     out = {"k_update_fused": [], "k_mark": [], "k_keep": []}
-    started = False
+    keys = [_kernel_key(r["Kernel_Name"]) for r in rows]
+    first = max((i for i, k in enumerate(keys) if k == "k_update"), default=len(rows))
     it_mark = it_keep = 0
     it_fused = 1
-    for r in rows:
-        k = _kernel_key(r["Kernel_Name"])
-        if k == "k_update":
-            started = True
-            continue
-        if not started or k is None or k == "k_compact":
+    for r, k in zip(rows[first + 1:], keys[first + 1:]):
+        if k is None or k == "k_compact":
             continue
         if k == "k_update_fused":
             out[k].append((it_fused, r))
@@ -424,6 +422,13 @@ def plain_leg(name, args, with_cpu, window_at=None):
     S = w.volume.dtype.itemsize
     P = 1 if args.phi == "i8" else 4
     log(f"{name}: context ready: {describe(w, ctx.X, ctx.Y, ctx.Z)}; n0={ctx.active_count}")
+    # launch shapes and widths timed on this workload (outside the timed region)
+    t = ctx.autotune(w.T, w.eps, w.alpha, w.dt, w.iterations)
+    tuning = {"update_blocks": t.update_blocks, "compact_blocks": t.compact_blocks, "keep_items": t.keep_items,
+              "compact_tile": t.compact_tile, "tunings_timed": t.candidates, "cached": bool(t.cached)}
+    log(f"{name}: autotuned: {t.update_blocks} update/mark workgroups, {t.compact_blocks} keep/compaction "
+        f"workgroups, keep step {t.keep_items} positions per thread, compaction tiles of {t.compact_tile} "
+        f"positions below the small-list bound ({t.candidates} tunings timed)")
 
     for _ in range(args.warmup):
         ctx.checkpoint_restore(0)
@@ -502,6 +507,7 @@ def plain_leg(name, args, with_cpu, window_at=None):
             "non_empty_iterations": live,
             "parallelism": "single GPU",
             "phi_storage": "fixed-point int8 (LEVEL_SET_CHAR)" if args.phi == "i8" else "FP32",
+            "tuning": tuning,
         },
         "roofline": {
             "bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -549,6 +555,9 @@ def profile_only(args):
     w = make_workload(args.workload, args.iterations)
     ctx = Context(w.volume)
     ctx.set_phi(w.phi0, w.band)
+    # the bench line's tuning; under a profiler, run this once beforehand with
+    # LSSEG_TUNING_CACHE set so that the profiled runs take it from the cache
+    t = ctx.autotune(w.T, w.eps, w.alpha, w.dt, w.iterations)
     st, kt, per = ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, w.iterations)
     pc = ctx.iteration_counts()
     if os.environ.get("LSSEG_FUSE", "1") == "0":  # four launches per iteration: no per-part summary
@@ -557,7 +566,9 @@ def profile_only(args):
         kernels, L = launch_parts(per, pc.n, pc.c, pc.m, w.volume.dtype.itemsize, 4)
     out = {"workload": describe(w, ctx.X, ctx.Y, ctx.Z), "S": w.volume.dtype.itemsize, "P": 4,
            "n": [int(x) for x in pc.n], "c": [int(x) for x in pc.c], "m": [int(x) for x in pc.m],
-           "launched_iterations": L, "kernels": kernels}
+           "launched_iterations": L, "kernels": kernels,
+           "tuning": {"update_blocks": t.update_blocks, "compact_blocks": t.compact_blocks,
+                      "keep_items": t.keep_items, "compact_tile": t.compact_tile, "cached": bool(t.cached)}}
     if args.counts_out:
         os.makedirs(os.path.dirname(os.path.abspath(args.counts_out)), exist_ok=True)
         with open(args.counts_out, "w") as f:

@@ -362,6 +362,24 @@ lsseg_status lsseg_batch_set_tuning(lsseg_batch* batch, const lsseg_tuning* tuni
 lsseg_status lsseg_batch_autotune(lsseg_batch* batch, const lsseg_params* params, int32_t iters,
                                   lsseg_tuning* out);
 
+/* The same for one context (the reference hard-codes its launch shapes,
+   UpdateLevelSetVolume.cu:42-55, Cuda.cpp:699-758).  Fields: update_blocks =
+   workgroups of the update and marking launches, compact_blocks = workgroups of
+   the keep, compaction and fused launches (multiples of 8, at least 8 and 16),
+   keep_items = traversal positions per thread of the keep step (1 or 2),
+   compact_tile = 256: few-path compactions of lists up to the context's
+   small-list bound (LSSEG_SMALL_TILE_N at creation) take tiles of 256
+   positions, 1024: every compaction takes tiles of 1024; 0 in a field of
+   set_tuning leaves it unchanged.  Results are identical under every tuning.
+   lsseg_autotune times `iters` iterations from the current state for 9 launch
+   shapes, then the other keep width, then the other compaction tile, keeps
+   the fastest and restores the state (checkpoint slot LSSEG_MAX_CHECKPOINTS -
+   1); it caches like lsseg_batch_autotune, per (GPU architecture, source
+   type, phi storage, voxels to the next power of two, iters). */
+lsseg_status lsseg_get_tuning(const lsseg_ctx* ctx, lsseg_tuning* out);
+lsseg_status lsseg_set_tuning(lsseg_ctx* ctx, const lsseg_tuning* tuning);
+lsseg_status lsseg_autotune(lsseg_ctx* ctx, const lsseg_params* params, int32_t iters, lsseg_tuning* out);
+
 /* ---- differentiable form -------------------------------------------------
 
    No reference counterpart.  The adjoint of the update lets a caller tune T,

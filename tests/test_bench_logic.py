@@ -52,3 +52,11 @@ def test_profiler_rows_map_to_iterations():
     assert [i for i, _ in got["k_update_fused"]] == [1]
     assert [i for i, _ in got["k_mark"]] == [0, 1]
     assert [i for i, _ in got["k_keep"]] == [0, 1]
+    # an earlier run in the same process (the autotuner's) is skipped: the
+    # profiled run starts at the last k_update
+    earlier = names[2:5] + names[5:8] * 2 + [names[-1]]
+    rows2 = [{"Kernel_Name": k, "Dispatch_Id": str(i)} for i, k in enumerate(names[:2] + earlier + names[2:])]
+    got2 = bench.iteration_dispatches(bench._dispatch_order(rows2), np.array([5, 6, 0, 0]))
+    for k in got:
+        assert [(i, r["Kernel_Name"]) for i, r in got2[k]] == [(i, r["Kernel_Name"]) for i, r in got[k]]
+        assert all(int(r["Dispatch_Id"]) > len(earlier) + 1 for _, r in got2[k])

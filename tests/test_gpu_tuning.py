@@ -1,0 +1,110 @@
+"""Launch shapes and widths of a single context (lsseg_get_tuning,
+lsseg_set_tuning, lsseg_autotune) through the C ABI (needs a GPU).
+
+- Every tuning (update/marking workgroups, keep/compaction workgroups, keep
+  width of one or two positions per thread, compaction tiles of 256 or 1024
+  positions) gives the oracle's lists, phi and export bit for bit, on the few
+  and the dense path.
+- lsseg_autotune leaves the state as it found it, applies the fastest tuning
+  it timed, and a second context of the same workload takes the cached result
+  without timing.
+- Bad tunings are refused with a message.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from aworkefficientgpualgorithmforlevelsetsegmentation_amd import synth
+from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import Context, LssegError
+
+pytestmark = pytest.mark.gpu
+
+TUNINGS = [(64, 16, 1, 256), (2048, 64, 2, 1024), (4096, 2048, 2, 256), (8, 24, 1, 1024)]
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _dense():
+    n = 32
+    z, y, x = np.mgrid[0:n, 0:n, 0:n]
+    vol = ((x + y + z) % 7 * 30).astype(np.uint8)
+    phi0 = np.clip((x - 15.5) / 3.0, -4, 4).astype(np.float32)
+    return synth.Workload("dense", vol, phi0, 100.0, 60.0, 0.2, 0.5, 40, band=3.5)
+
+
+def _same_state(ctx, ref, where):
+    r, w, e, lst = ctx.state()
+    rr, rw, re_ = ref.state()
+    assert np.array_equal(lst, ref.active()), f"{where}: lists differ"
+    assert np.array_equal(r.view(np.uint32), rr.view(np.uint32)), f"{where}: phi read differs"
+    assert np.array_equal(w.view(np.uint32), rw.view(np.uint32)), f"{where}: phi write differs"
+    assert np.array_equal(e, re_), f"{where}: export differs"
+
+
+def _fields(t):
+    return (t.update_blocks, t.compact_blocks, t.keep_items, t.compact_tile)
+
+
+@pytest.mark.parametrize("case", ["few", "dense"])
+@pytest.mark.parametrize("tuning", TUNINGS, ids=["-".join(map(str, t)) for t in TUNINGS])
+def test_tunings_match_the_oracle(case, tuning):
+    w, iters = (synth.ellipsoids(edge=64), 120) if case == "few" else (_dense(), 40)
+    _say(f"{case}: tuning {tuning}, {iters} iterations")
+    ref = O.State(w.volume)
+    ref.set_phi(w.phi0, w.band)
+    ref.iterate(w.T, w.eps, w.alpha, w.dt, iters, threads=8)
+    ctx = Context(w.volume)
+    ctx.set_phi(w.phi0, w.band)
+    ctx.set_tuning(*tuning)
+    assert _fields(ctx.tuning()) == tuning
+    ctx.iterate(w.T, w.eps, w.alpha, w.dt, iters)
+    _same_state(ctx, ref, f"{case} {tuning}")
+    assert np.array_equal(ctx.mask(), ref.mask())
+    ctx.close()
+
+
+def test_autotune_restores_the_state_and_caches(monkeypatch, tmp_path):
+    monkeypatch.setenv("LSSEG_TUNING_CACHE", str(tmp_path / "tuning.txt"))
+    w = synth.ellipsoids(edge=64)
+    ctx = Context(w.volume)
+    ctx.set_phi(w.phi0, w.band)
+    before = ctx.state()
+    _say("autotune: ellipsoids 64^3, 60 iterations")
+    t = ctx.autotune(w.T, w.eps, w.alpha, w.dt, 60)
+    _say(f"autotune: {_fields(t)}, {t.candidates} tunings timed, {t.ms:.3f} ms")
+    assert t.cached == 0 and t.candidates >= 9 and t.ms > 0
+    assert t.keep_items in (1, 2) and t.compact_tile in (256, 1024)
+    assert _fields(ctx.tuning()) == _fields(t)
+    for x, y in zip(before, ctx.state()):
+        assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
+    ref = O.State(w.volume)
+    ref.set_phi(w.phi0, w.band)
+    ref.iterate(w.T, w.eps, w.alpha, w.dt, 120, threads=8)
+    ctx.iterate(w.T, w.eps, w.alpha, w.dt, 120)
+    _same_state(ctx, ref, f"autotuned {_fields(t)}")
+    ctx2 = Context(w.volume)
+    ctx2.set_phi(w.phi0, w.band)
+    t2 = ctx2.autotune(w.T, w.eps, w.alpha, w.dt, 60)
+    assert t2.cached == 1 and t2.candidates == 0 and _fields(t2) == _fields(t)
+    ctx.close()
+    ctx2.close()
+
+
+def test_bad_tunings_are_refused(monkeypatch):
+    w = synth.ellipsoids(edge=32)
+    ctx = Context(w.volume)
+    ctx.set_phi(w.phi0, w.band)
+    start = _fields(ctx.tuning())
+    for bad in ((12, 0, 0, 0), (0, 8, 0, 0), (0, 20, 0, 0), (0, 0, 3, 0), (0, 0, 0, 512)):
+        with pytest.raises(LssegError):
+            ctx.set_tuning(*bad)
+    assert _fields(ctx.tuning()) == start
+    ctx.close()
+    monkeypatch.setenv("LSSEG_SMALL_TILE_N", "0")
+    ctx = Context(w.volume)
+    assert ctx.tuning().compact_tile == 1024
+    with pytest.raises(LssegError, match="compaction tiles"):
+        ctx.set_tuning(0, 0, 0, 256)
+    ctx.close()
