@@ -926,6 +926,7 @@ def main():
         "roofline": head["roofline"],
         "path_roofline": head["path_roofline"],
         "kernels": head["kernels"],
+        "random_access_floor": head["random_access_floor"],
         "cpu_baseline": head["cpu_baseline"],
         "host_inclusive": head["host_inclusive"],
         "dice_vs_oracle": head["dice_vs_oracle"],
