@@ -2136,10 +2136,18 @@ lsseg_status lsseg_batch_autotune(lsseg_batch* b, const lsseg_params* params, in
             for (auto& o : cands) dup |= o.update_blocks == t.update_blocks && o.compact_blocks == t.compact_blocks;
             if (!dup) cands.push_back(t);
         }
+    lsseg_tuning start{};
+    (void)lsseg_batch_get_tuning(b, &start);
+    // a failed timing run leaves the caller's tuning and state as they were
+    auto give_up = [&](lsseg_status s2) {
+        (void)lsseg_batch_set_tuning(b, &start);
+        (void)lsseg_batch_checkpoint_restore(b, slot);
+        return s2;
+    };
     best.ms = 1e30;
     for (auto& t : cands) {
         double ms = 0;
-        if ((st = time_shape(t, &ms)) != LSSEG_OK) return st;
+        if ((st = time_shape(t, &ms)) != LSSEG_OK) return give_up(st);
         if (ms < best.ms) {
             best = t;
             best.ms = ms;
@@ -2150,7 +2158,7 @@ lsseg_status lsseg_batch_autotune(lsseg_batch* b, const lsseg_params* params, in
         if (stage == 0) t.keep_items = best.keep_items == 1 ? 2 : 1;
         else t.compact_tile = best.compact_tile == (uint32_t)kBlock ? (uint32_t)kPosTile : (uint32_t)kBlock;
         double ms = 0;
-        if ((st = time_shape(t, &ms)) != LSSEG_OK) return st;
+        if ((st = time_shape(t, &ms)) != LSSEG_OK) return give_up(st);
         if (ms < best.ms) {
             best = t;
             best.ms = ms;
@@ -2278,10 +2286,16 @@ lsseg_status lsseg_autotune(lsseg_ctx* c, const lsseg_params* params, int32_t it
             for (auto& o : cands) dup |= o.update_blocks == t.update_blocks && o.compact_blocks == t.compact_blocks;
             if (!dup) cands.push_back(t);
         }
+    // a failed timing run leaves the caller's tuning and state as they were
+    auto give_up = [&](lsseg_status s2) {
+        (void)lsseg_set_tuning(c, &start);
+        (void)lsseg_checkpoint_restore(c, slot);
+        return s2;
+    };
     best.ms = 1e30;
     for (auto& t : cands) {
         double ms = 0;
-        if ((st = time_shape(t, &ms)) != LSSEG_OK) return st;
+        if ((st = time_shape(t, &ms)) != LSSEG_OK) return give_up(st);
         if (ms < best.ms) {
             best = t;
             best.ms = ms;
@@ -2295,7 +2309,7 @@ lsseg_status lsseg_autotune(lsseg_ctx* c, const lsseg_params* params, int32_t it
         else
             continue;
         double ms = 0;
-        if ((st = time_shape(t, &ms)) != LSSEG_OK) return st;
+        if ((st = time_shape(t, &ms)) != LSSEG_OK) return give_up(st);
         if (ms < best.ms) {
             best = t;
             best.ms = ms;

@@ -8,7 +8,8 @@ lsseg_set_tuning, lsseg_autotune) through the C ABI (needs a GPU).
 - lsseg_autotune leaves the state as it found it, applies the fastest tuning
   it timed, and a second context of the same workload takes the cached result
   without timing.
-- Bad tunings are refused with a message.
+- Bad tunings are refused with a message; a failed autotune leaves the
+  tuning and the state as they were.
 """
 import numpy as np
 import pytest
@@ -107,4 +108,23 @@ def test_bad_tunings_are_refused(monkeypatch):
     assert ctx.tuning().compact_tile == 1024
     with pytest.raises(LssegError, match="compaction tiles"):
         ctx.set_tuning(0, 0, 0, 256)
+    ctx.close()
+
+
+def test_failed_autotune_leaves_tuning_and_state(monkeypatch):
+    """A timing run that fails (here: an unknown mode, refused by every
+    lsseg_iterate) makes lsseg_autotune return the error with the caller's
+    tuning and state as they were."""
+    monkeypatch.setenv("LSSEG_AUTOTUNE_FORCE", "1")
+    w = synth.ellipsoids(edge=32)
+    ctx = Context(w.volume)
+    ctx.set_phi(w.phi0, w.band)
+    ctx.set_tuning(64, 32, 2, 1024)
+    start = _fields(ctx.tuning())
+    before = ctx.state()
+    with pytest.raises(LssegError, match="mode"):
+        ctx.autotune(w.T, w.eps, w.alpha, w.dt, 30, mode=7)
+    assert _fields(ctx.tuning()) == start
+    for x, y in zip(before, ctx.state()):
+        assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
     ctx.close()
