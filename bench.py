@@ -474,7 +474,8 @@ def plain_leg(name, args, with_cpu, window_at=None):
     kernels, L = launch_parts(per_launch, pc.n, pc.c, pc.m, S, P)
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     d = kernels[dom]
-    traffic, traffic_src = load_traffic(name)
+    # the committed counter summaries are of FP32 phi runs
+    traffic, traffic_src = load_traffic(name) if P == 4 else (None, None)
     traffic_bytes = None
     if traffic and traffic.get("kernels", {}).get(d["kernel"]):
         traffic_bytes = traffic["kernels"][d["kernel"]]["bytes_per_launch"]
