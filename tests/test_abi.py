@@ -46,6 +46,16 @@ def test_null_arguments_are_rejected_without_a_device():
     assert L.lsseg_destroy(None) == engine.ERR_INVALID_ARG
     assert L.lsseg_iterate(None, None, 1, None) == engine.ERR_INVALID_ARG
     assert L.lsseg_get_mask(None, None) == engine.ERR_INVALID_ARG
+    t = engine.Tuning()
+    assert L.lsseg_get_tuning(None, ctypes.byref(t)) == engine.ERR_INVALID_ARG
+    assert L.lsseg_set_tuning(None, ctypes.byref(t)) == engine.ERR_INVALID_ARG
+    assert L.lsseg_autotune(None, None, 10, ctypes.byref(t)) == engine.ERR_INVALID_ARG
+    us = ctypes.c_float()
+    # the measurement aid checks its arguments before any device call
+    assert L.lsseg_probe_random_access(None, 4096, 10, 0, 3, None, ctypes.byref(us)) == engine.ERR_INVALID_ARG
+    assert L.lsseg_probe_random_access(ctypes.c_void_p(256), 4096, 10, 5, 3, None, ctypes.byref(us)) == \
+        engine.ERR_INVALID_ARG
+    assert b"invalid argument" in L.lsseg_last_error(None)
 
 
 def test_bad_descriptors_are_rejected_without_a_device():
