@@ -78,6 +78,8 @@ struct lsseg_ctx {
     } ckpt[LSSEG_MAX_CHECKPOINTS];
     bool profile = false;             // time every iteration launch with its own start/stop events
     bool checks = false;              // LSSEG_CHECKS: verify the list after every compaction
+    bool defer_keep = false;          // LSSEG_DEFER_KEEP: the keep step without a look-back, the next
+                                      // update rebuilding the traversal (single volumes)
     std::vector<hipEvent_t> prof_events;
     size_t prof_next = 0;
     std::vector<void*> allocs;
@@ -231,6 +233,7 @@ LSSEG_SRC_PHI_KERNEL(update_fused_kernel, k_update_fused)
 LSSEG_SRC_PHI_KERNEL(update_only_kernel, k_update_only)
 LSSEG_SRC_PHI_KERNEL(update_batch_kernel, k_update_batch)
 LSSEG_SRC_PHI_KERNEL(update_fused_batch_kernel, k_update_fused_batch)
+LSSEG_SRC_PHI_KERNEL(update_fused_defer_kernel, k_update_fused_defer)
 #undef LSSEG_SRC_PHI_KERNEL
 
 const void* mark_kernel(bool phi8) { return phi8 ? (const void*)&k_mark<int8_t> : (const void*)&k_mark<float>; }
@@ -283,7 +286,8 @@ void launch_check(lsseg_ctx* c, int q)
 void launch_head(lsseg_ctx* c, int q, bool fused)
 {
     if (fused) {
-        launch(c, update_fused_kernel(c->dtype, c->phi8), c->fgrid, q);
+        launch(c, c->a.defer ? update_fused_defer_kernel(c->dtype, c->phi8) : update_fused_kernel(c->dtype, c->phi8),
+               c->fgrid, q);
         launch_check(c, q);
     } else {
         launch(c, update_kernel(c->dtype, c->phi8), c->sgrid, q);
@@ -295,7 +299,11 @@ void launch_head(lsseg_ctx* c, int q, bool fused)
 void launch_tail(lsseg_ctx* c, int q)
 {
     launch(c, mark_kernel(c->phi8), c->sgrid, q);
-    launch(c, c->keep_fn, c->grid, q);
+    if (c->a.defer)
+        launch(c, c->keep_fn == (const void*)&k_keep<2> ? (const void*)&k_keep_defer<2> : (const void*)&k_keep_defer<1>,
+               c->grid, q);
+    else
+        launch(c, c->keep_fn, c->grid, q);
 }
 
 // one complete iteration over list[q] (update, marking, keep, compaction into
@@ -376,6 +384,15 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q, uint32_t* skipped
         for (int i = 0; i < count; ++i) launch_iteration(c, q ^ (i & 1));
         return LSSEG_OK;
     }
+    // a deferred keep step leaves the traversal to the next update launch; the
+    // chunks it counts follow the keep step's grid and width
+    c->a.defer = c->defer_keep ? 1u : 0u;
+    c->a.keep_grid = c->grid;
+    c->a.keep_items = c->keep_fn == (const void*)&k_keep<2> ? 2u : 1u;
+    struct DeferOff {
+        lsseg_ctx* c;
+        ~DeferOff() { c->a.defer = 0; }  // the other paths (records, re-initialisation) place their traversal
+    } defer_off{c};
     launch_head(c, q, false);
     launch_tail(c, q);
     q ^= 1;
@@ -441,6 +458,8 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q, uint32_t* skipped
         }
     }
     launch(c, (const void*)&k_compact<kCompact>, c->grid, q ^ 1);
+    if (c->a.defer)
+        hipLaunchKernelGGL(k_settle_traversal, dim3(c->sgrid), dim3(kBlock), 0, c->stream, c->a, q);
     launch_check(c, q);
     if (skipped) hipLaunchKernelGGL(k_empty_iterations, dim3(1), dim3(kBlock), 0, c->stream, c->a, skipped);
     if (skipped_out) *skipped_out = skipped;
@@ -740,6 +759,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     Geom& g = c->a.g;
     init_geom(g, (uint32_t)X, (uint32_t)Y, (uint32_t)Z);
     c->small_n_max = g.small_n;
+    c->defer_keep = !shared && env_int("LSSEG_DEFER_KEEP", 1) != 0;
 
     void* p = nullptr;
     const uint64_t ntiles = lookback_states(g);
@@ -766,6 +786,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     ALLOC(gstate, uint64_t, 7 * (ntiles / 64 + 2));
     ALLOC(cpart, uint64_t, kCountSlots * kCountStride);
     ALLOC(dirc, uint32_t, kDirCounters);
+    ALLOC(kcnt, uint32_t, N / kBlock + 64);  // a keep chunk holds at least one tile of kBlock positions
     ALLOC(ctl, DevCtl, 1);
 #undef ALLOC
     if ((st = dalloc(c, &p, (size_t)N * c->S)) != LSSEG_OK) return bail(st);

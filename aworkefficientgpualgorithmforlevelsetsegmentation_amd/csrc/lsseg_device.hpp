@@ -284,6 +284,10 @@ struct KArgs {
     uint32_t* n_hist;
     uint64_t* c_hist;
     uint32_t* m_hist;
+    uint32_t* kcnt;       // kept voxels per chunk of a deferred keep step (LSSEG_DEFER_KEEP)
+    uint32_t keep_grid;   // the keep step's grid and positions per thread (for the chunks of a
+    uint32_t keep_items;  // deferred keep step); defer: the compaction writes the next count
+    uint32_t defer;
 };
 
 // ---------------------------------------------------------------------------
@@ -833,6 +837,11 @@ __device__ inline void compact_finish(const KArgs& a, int q, const CompactPlan& 
     if constexpr (kMode == kInit) {
         ctl->n[q ^ 1] = m;
         ctl->n[q] = 0;
+    } else if (a.defer) {
+        // the keep step left the count to the compaction
+        ctl->n[q ^ 1] = m;
+        const uint32_t k = ctl->iter - 1u;
+        if (k < ctl->stats_cap) a.m_hist[k] = m;
     } else if (c.n_dst != m) {
         atomicOr(&ctl->error, kErrCount);
     }
@@ -1687,6 +1696,208 @@ __global__ __launch_bounds__(kBlock) void k_compact(KArgs a, int q)
     c.n_dst = a.ctl->n[q ^ 1];
     loaded = compact_pick_preload(c, loaded, kb, vx, small);
     compact_all<kMode>(a, q, c, epoch, sh, gridDim.x, kb, vx, d, loaded);
+}
+
+// ---------------------------------------------------------------------------
+// The keep step without a look-back (LSSEG_DEFER_KEEP, single volumes).  The
+// next traversal's order is free: the update, the marking and the keep step
+// give the same results whatever order they walk the list in, and only the
+// compaction follows the reference's order.  So the keep step does not place
+// the kept voxels: it leaves each traversal position's kept directions in
+// emit[p] and its chunk's count in kcnt, and stops.  The update launch that
+// follows rebuilds the next traversal from them (the kept targets of
+// positions in order, so exactly the S the keep step would have appended) as
+// it updates: a chunk's start is the sum of the counts before it, summed
+// after the launch boundary with plain loads instead of a cross-XCD
+// publication and poll.  The compaction in the same launch writes the next
+// list's count, which it knows from the direction counters.
+// ---------------------------------------------------------------------------
+
+// counts and statistics of a deferred keep step of list q (n entries): as
+// keep_finish, but the next count is written by the compaction and the
+// look-back epoch is untouched
+__device__ inline void keep_finish_defer(const KArgs& a, uint32_t n, uint64_t c, const CtlSnapshot& st)
+{
+    const int lane = threadIdx.x & 63;
+    for (int i = lane; i < kCountSlots; i += 64) a.cpart[i * kCountStride] = 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += (uint64_t)__shfl_xor((unsigned long long)c, o, 64);
+    if (lane != 0) return;
+    DevCtl* ctl = a.ctl;
+    const uint32_t k = st.iter;
+    if (k < st.stats_cap) {
+        a.n_hist[k] = n;
+        a.c_hist[k] = c;
+    }
+    if (path_is_few(n, a.g.N)) ctl->few_iters = st.few + 1;
+    else ctl->many_iters = st.many + 1;
+    ctl->iter = k + 1;
+}
+
+// chunk `chunk` of a deferred keep step: decisions (keep bytes, kept
+// directions into emit, dense marks), direction counters and the chunk's count
+template <int kI>
+__device__ __attribute__((always_inline)) inline void keep_defer_chunk(const KArgs& a, int q, uint32_t n,
+                                                                      const ChunkPlan& cp, uint32_t chunk,
+                                                                      KeepShared<kI>& sh, uint32_t* us,
+                                                                      uint32_t* ms)
+{
+    const bool dense = !path_is_few(n, a.g.N);
+    const uint32_t s0 = chunk * cp.r, s1 = min(s0 + cp.r, keep_tiles<kI>(n));
+    const bool last = chunk == cp.chunks - 1;
+    const uint64_t cshare = (last && threadIdx.x < 64) ? candidate_share(a) : 0ull;
+    const CtlSnapshot snap = last ? ctl_snapshot(a.ctl) : CtlSnapshot{0, 0, 0, 0};
+    uint32_t kb[kI], ex[kI];
+    uint32_t total = 0;
+    for (uint32_t sub = s0; sub < s1; ++sub) {
+        if (cp.r > 1) keep_voxels<kI>(a, q, sub, us, ms);
+        keep_decide<kI>(a, n, sub, us, ms, kb);
+        const uint64_t base = (uint64_t)sub * (kBlock * kI);
+#pragma unroll
+        for (int j = 0; j < kI; ++j) {
+            const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+            if (i < n) a.emit[i] = (uint8_t)kb[j];
+            if (dense && kb[j]) {
+#pragma unroll
+                for (int e = 0; e < 7; ++e)
+                    if ((kb[j] >> e) & 1u) a.memb[neighbor_in(a.g, us[j], (uint32_t)e)] = 1;
+            }
+        }
+        keep_dirs_count(kb, sh, (int)(sub & 1u));
+        total += keep_counts(kb, ex, sh);
+        keep_dirs_add(a, q, sh, (int)(sub & 1u));
+    }
+    if (threadIdx.x == 0) a.kcnt[chunk] = total;
+    if (threadIdx.x < 64 && last) keep_finish_defer(a, n, cshare, snap);
+}
+
+template <int kI>
+__global__ __launch_bounds__(kBlock) void k_keep_defer(KArgs a, int q)
+{
+    __shared__ KeepShared<kI> sh;
+    uint32_t us[kI], ms[kI];
+    keep_voxels<kI>(a, q, blockIdx.x, us, ms);  // with the count: this workgroup's sub-tile when chunks are single
+    const uint32_t n = a.ctl->n[q];
+    const ChunkPlan cp = chunk_plan(keep_tiles<kI>(n), gridDim.x);
+    if (blockIdx.x < cp.chunks) keep_defer_chunk<kI>(a, q, n, cp, blockIdx.x, sh, us, ms);
+    if (cp.chunks == 0 && blockIdx.x == 0 && threadIdx.x < 64) keep_finish_defer(a, 0, candidate_share(a), ctl_snapshot(a.ctl));
+}
+
+struct AppendShared {
+    uint32_t tgt[7 * kBlock];  // kept targets of one block of positions, in order
+    uint32_t cnt[4];           // per-wave counts
+    uint32_t excl[4];
+    uint32_t red[4];
+    uint32_t total;
+};
+
+union FusedDeferShared {
+    CompactShared c;
+    AppendShared d;
+};
+
+// sum over the block of v (every thread gets it; two barriers)
+__device__ __attribute__((always_inline)) inline uint32_t block_sum(uint32_t v, uint32_t* red)
+{
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+}
+
+// The traversal of list q rebuilt from a deferred keep step over list q ^ 1
+// (its traversal S[q ^ 1], kept directions in emit, chunk counts in kcnt),
+// over `count` workgroups from `first` (multiples of 8), each voxel updated as
+// it is placed when kUpd.  Chunks go to the XCDs in runs of 16 (4096 positions
+// of single-tile chunks), as ListWalk deals list positions.
+template <typename SrcT, bool kFloatSrc, typename PhiT, bool kUpd>
+__device__ inline void append_walk(const KArgs& a, int q, uint32_t first, uint32_t count, AppendShared& sh)
+{
+    const uint32_t nsrc = a.ctl->n[q ^ 1];
+    const uint32_t items = (uint32_t)(kBlock * a.keep_items);
+    const ChunkPlan cp = chunk_plan((nsrc + items - 1) / items, a.keep_grid);
+    const uint64_t span = (uint64_t)cp.r * items;  // positions per chunk
+    const uint32_t* __restrict__ Ss = a.S[q ^ 1];
+    uint32_t* __restrict__ Sn = a.S[q];
+    DevParams P{};
+    if constexpr (kUpd) P = *a.prm;
+    const uint32_t b = blockIdx.x - first, g8 = count >> 3, x = b & 7u;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t s = b >> 3;; s += g8) {
+        const uint32_t j = (((s >> 4) << 3) + x) * 16u + (s & 15u);
+        if ((s >> 4) * 128u >= cp.chunks || j >= cp.chunks) {
+            // past the last chunk of this XCD's runs: every later s is too
+            if ((s >> 4) * 128u >= cp.chunks) break;
+            continue;
+        }
+        // the chunk's start: the counts of the chunks before it
+        uint32_t pre = 0;
+        for (uint32_t i = threadIdx.x; i < j; i += kBlock) pre += a.kcnt[i];
+        uint32_t run = block_sum(pre, sh.red);
+        const uint64_t p0 = (uint64_t)j * span, p1 = min(p0 + span, (uint64_t)nsrc);
+        for (uint64_t blk = p0; blk < p1; blk += kBlock) {
+            const uint64_t p = blk + threadIdx.x;
+            const bool valid = p < p1;
+            const uint32_t u = valid ? Ss[p] : 0u;
+            const uint32_t kb = valid ? a.emit[p] : 0u;
+            const uint32_t c = (uint32_t)__popc(kb);
+            const uint32_t incl = wave_inclusive_scan(c, lane);
+            if (lane == 63) sh.cnt[wave] = incl;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const uint32_t c0 = sh.cnt[0], c1 = sh.cnt[1], c2 = sh.cnt[2], c3 = sh.cnt[3];
+                sh.excl[0] = 0;
+                sh.excl[1] = c0;
+                sh.excl[2] = c0 + c1;
+                sh.excl[3] = c0 + c1 + c2;
+                sh.total = c0 + c1 + c2 + c3;
+            }
+            __syncthreads();
+            uint32_t off = sh.excl[wave] + incl - c;
+#pragma unroll
+            for (int e = 0; e < 7; ++e)
+                if ((kb >> e) & 1u) sh.tgt[off++] = neighbor_in(a.g, u, (uint32_t)e);
+            const uint32_t t = sh.total;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < t; i += kBlock) {
+                const uint32_t v = sh.tgt[i];
+                Sn[run + i] = v;
+                if constexpr (kUpd)
+                    update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), v,
+                                                     run + i);
+            }
+            run += t;
+            __syncthreads();
+        }
+    }
+}
+
+// the compaction of list[q ^ 1] into list[q] and the update over list q,
+// whose traversal it rebuilds from the deferred keep step (see append_walk)
+template <typename SrcT, bool kFloatSrc, typename PhiT>
+__global__ __launch_bounds__(kBlock) void k_update_fused_defer(KArgs a, int q)
+{
+    __shared__ FusedDeferShared sh;
+    uint32_t kb[kPosItems], vx[kPosItems], d[7], small[2] = {0u, 0u};
+    bool loaded = compact_preload(a, q ^ 1, kb, vx, d, small);  // with the counts
+    const uint32_t epoch = a.ctl->epoch;
+    CompactPlan c = compact_plan<kCompact>(a.g, a.ctl->n[q ^ 1]);
+    loaded = compact_pick_preload(c, loaded, kb, vx, small);
+    const uint32_t gc = min((c.ntiles + 7u) & ~7u, a.fused_compact_max);
+    if (blockIdx.x < gc) {
+        compact_all<kCompact>(a, q ^ 1, c, epoch, sh.c, gc, kb, vx, d, loaded);
+    } else {
+        if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kCompact>(a, q ^ 1, c, 0, epoch);
+        append_walk<SrcT, kFloatSrc, PhiT, true>(a, q, gc, gridDim.x - gc, sh.d);
+    }
+}
+
+// the traversal of list q rebuilt after a call's last deferred keep step
+__global__ __launch_bounds__(kBlock) void k_settle_traversal(KArgs a, int q)
+{
+    __shared__ AppendShared sh;
+    append_walk<uint8_t, false, float, false>(a, q, 0, gridDim.x, sh);
 }
 
 // ---------------------------------------------------------------------------
