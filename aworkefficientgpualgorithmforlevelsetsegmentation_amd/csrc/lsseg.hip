@@ -970,8 +970,13 @@ lsseg_status lsseg_iterate(lsseg_ctx* c, const lsseg_params* prm, int32_t iters,
     c->h_ctl->iter = 0;
     c->h_ctl->stats_cap = c->stats_cap;
     // after a call that ended with a device error, look-back states of its
-    // unfinished scans may carry the current epoch: skip past it
-    if (c->h_ctl->error) c->h_ctl->epoch += 2;
+    // unfinished scans may carry the current epoch (skip past it), and an
+    // unfinished compaction may have left keep bytes set (clear them)
+    if (c->h_ctl->error) {
+        c->h_ctl->epoch += 2;
+        hipLaunchKernelGGL(k_fill_u8, dim3(dense_grid(c->a.g.N)), dim3(kBlock), 0, c->stream, c->a.keep, c->a.g.N,
+                           (uint8_t)0);
+    }
     c->h_ctl->error = 0;
     c->h_ctl->few_iters = 0;
     c->h_ctl->many_iters = 0;
@@ -2012,7 +2017,12 @@ lsseg_status lsseg_batch_iterate(lsseg_batch* b, const lsseg_params* params, int
         DevCtl& h = b->h_ctls[v];
         h.iter = 0;
         h.stats_cap = b->vols[v]->stats_cap;
-        if (h.error) h.epoch += 2;  // as in lsseg_iterate
+        if (h.error) {  // as in lsseg_iterate
+            h.epoch += 2;
+            const KArgs& a = b->vols[v]->a;
+            hipLaunchKernelGGL(k_fill_u8, dim3(dense_grid(a.g.N)), dim3(kBlock), 0, b->stream, a.keep, a.g.N,
+                               (uint8_t)0);
+        }
         h.error = 0;
         h.few_iters = 0;
         h.many_iters = 0;
@@ -2515,8 +2525,13 @@ lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
     c->h_ctl->iter = 0;
     c->h_ctl->stats_cap = c->stats_cap;
     // after a call that ended with a device error, look-back states of its
-    // unfinished scans may carry the current epoch: skip past it
-    if (c->h_ctl->error) c->h_ctl->epoch += 2;
+    // unfinished scans may carry the current epoch (skip past it), and an
+    // unfinished compaction may have left keep bytes set (clear them)
+    if (c->h_ctl->error) {
+        c->h_ctl->epoch += 2;
+        hipLaunchKernelGGL(k_fill_u8, dim3(dense_grid(c->a.g.N)), dim3(kBlock), 0, c->stream, c->a.keep, c->a.g.N,
+                           (uint8_t)0);
+    }
     c->h_ctl->error = 0;
     c->h_ctl->few_iters = 0;
     c->h_ctl->many_iters = 0;

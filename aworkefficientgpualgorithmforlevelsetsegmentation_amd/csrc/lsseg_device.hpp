@@ -1035,12 +1035,17 @@ __device__ __attribute__((always_inline)) inline void compact_pos_counts(const u
     __syncthreads();
 }
 
-// places the kept candidates of a loaded and counted tile, direction e from
-// list offset ps.run[e] on
-__device__ __attribute__((always_inline)) inline void compact_pos_place(const KArgs& a, int q, uint32_t items,
-                                                                        const uint32_t* kb, const uint32_t* vx,
-                                                                        const PosShared& ps)
+// places the kept candidates of loaded and counted tile `tile`, direction e
+// from list offset ps.run[e] on, and clears the tile's non-zero keep bytes
+// (the keep step stores only non-zero ones)
+__device__ __attribute__((always_inline)) inline void compact_pos_place(const KArgs& a, int q, uint32_t tile,
+                                                                        uint32_t items, const uint32_t* kb,
+                                                                        const uint32_t* vx, const PosShared& ps)
 {
+    const uint64_t tbase = (uint64_t)tile * kBlock * items;
+#pragma unroll
+    for (int j = 0; j < kPosItems; ++j)
+        if (j < (int)items && kb[j]) a.keep[tbase + (uint64_t)j * kBlock + threadIdx.x] = 0;
     uint32_t* __restrict__ out = a.list[q ^ 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -1124,13 +1129,13 @@ __device__ inline bool compact_pos_chunk(const KArgs& a, int q, const CompactPla
     if (threadIdx.x < 7) ps.run[threadIdx.x] = ps.base[threadIdx.x] + ps.pre[threadIdx.x];
     __syncthreads();
     if (cp.r == 1) {
-        compact_pos_place(a, q, c.items, kb, vx, ps);
+        compact_pos_place(a, q, s0, c.items, kb, vx, ps);
     } else {
         for (uint32_t sub = s0; sub < s1; ++sub) {
             compact_pos_load(a, q, sub, c.items, kb, vx);
             compact_pos_mask(c.n_src, sub, c.items, kb);
             compact_pos_counts(kb, c.items, ps);
-            compact_pos_place(a, q, c.items, kb, vx, ps);
+            compact_pos_place(a, q, sub, c.items, kb, vx, ps);
             __syncthreads();
             if (threadIdx.x < 7) ps.run[threadIdx.x] += ps.tot[threadIdx.x];
             __syncthreads();
@@ -1331,8 +1336,9 @@ __device__ __attribute__((always_inline)) inline void keep_voxels(const KArgs& a
 // `sub`, written to keep[pos[u]] as well; entries past the count become voxel
 // 0 with kb = 0
 template <int kI>
-__device__ __attribute__((always_inline)) inline void keep_decide(const KArgs& a, uint32_t n, uint32_t sub,
-                                                                  uint32_t* us, const uint32_t* ms, uint32_t* kb)
+__device__ __attribute__((always_inline)) inline void keep_decide(const KArgs& a, uint32_t n, bool dense,
+                                                                  uint32_t sub, uint32_t* us, const uint32_t* ms,
+                                                                  uint32_t* kb)
 {
     const uint64_t* __restrict__ Ck = a.C;
     const uint64_t base = (uint64_t)sub * (kBlock * kI);
@@ -1363,7 +1369,10 @@ __device__ __attribute__((always_inline)) inline void keep_decide(const KArgs& a
             if (((m >> e) & 1u) && (uint32_t)((__ffsll((unsigned long long)rec[j][e]) - 1) >> 3) == (uint32_t)e)
                 k |= 1u << e;
         kb[j] = k;
-        if (valid) a.keep[ps[j]] = (uint8_t)k;
+        // the keep bytes are zero between compactions (each few-path
+        // compaction clears the ones it read), so only kept lists are stored,
+        // and none for a dense compaction, which reads the marks instead
+        if (valid && k && !dense) a.keep[ps[j]] = (uint8_t)k;
     }
 }
 
@@ -1467,14 +1476,14 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
     uint32_t kb[kI], ex[kI];
     uint32_t total = 0;
     if (cp.r == 1) {
-        keep_decide<kI>(a, n, s0, us, ms, kb);
+        keep_decide<kI>(a, n, dense, s0, us, ms, kb);
         keep_dirs_count(kb, sh, 0);
         total = keep_counts(kb, ex, sh);
         keep_dirs_add(a, q, sh, 0);
     } else {
         for (uint32_t sub = s0; sub < s1; ++sub) {
             keep_voxels<kI>(a, q, sub, us, ms);
-            keep_decide<kI>(a, n, sub, us, ms, kb);
+            keep_decide<kI>(a, n, dense, sub, us, ms, kb);
             const uint64_t base = (uint64_t)sub * (kBlock * kI);
 #pragma unroll
             for (int j = 0; j < kI; ++j) {
@@ -1751,7 +1760,7 @@ __device__ __attribute__((always_inline)) inline void keep_defer_chunk(const KAr
     uint32_t total = 0;
     for (uint32_t sub = s0; sub < s1; ++sub) {
         if (cp.r > 1) keep_voxels<kI>(a, q, sub, us, ms);
-        keep_decide<kI>(a, n, sub, us, ms, kb);
+        keep_decide<kI>(a, n, dense, sub, us, ms, kb);
         const uint64_t base = (uint64_t)sub * (kBlock * kI);
 #pragma unroll
         for (int j = 0; j < kI; ++j) {

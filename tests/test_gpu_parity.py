@@ -175,10 +175,14 @@ def test_many_path_multi_iteration():
 
 @pytest.mark.parametrize("wide", [False, True], ids=["few", "dense"])
 def test_fused_and_separate_launches_agree(monkeypatch, wide):
-    """Compactions run inside the next update launch (the default) or as launches
-    of their own (LSSEG_FUSE=0): identical states, counts and statistics; the
-    profiled run issues 3 K + 1 launches, or 4 K."""
-    _say(f"fused vs separate launches, wide band {wide}")
+    """Three launch schedules give identical states, counts and statistics:
+    the default (each compaction inside the next update launch, which also
+    rebuilds the traversal that the keep step left as kept directions and
+    per-chunk counts), the same with a keep step that places the traversal
+    itself through its look-back (LSSEG_DEFER_KEEP=0), and every step in a
+    launch of its own (LSSEG_FUSE=0).  A profiled run of K iterations issues
+    3 K + 1 launches, 3 K + 1 and 4 K."""
+    _say(f"fused, placing-keep and separate launches, wide band {wide}")
     if wide:
         n = 32
         z, y, x = np.mgrid[0:n, 0:n, 0:n]
@@ -188,7 +192,8 @@ def test_fused_and_separate_launches_agree(monkeypatch, wide):
     else:
         w = synth.ellipsoids(edge=64, iterations=70)
     out = {}
-    for fuse in ("1", "0"):
+    for name, defer, fuse in (("deferred", "1", "1"), ("placing", "0", "1"), ("separate", "1", "0")):
+        monkeypatch.setenv("LSSEG_DEFER_KEEP", defer)
         monkeypatch.setenv("LSSEG_FUSE", fuse)
         ctx = Context(w.volume)
         ctx.set_phi(w.phi0, w.band)
@@ -197,18 +202,20 @@ def test_fused_and_separate_launches_agree(monkeypatch, wide):
         state = ctx.state()
         ctx.set_phi(w.phi0, w.band)
         _, kt, per = ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, 5)
-        out[fuse] = (st, cnt, state, int(kt.launches), ctx.state())
+        out[name] = (st, cnt, state, int(kt.launches), ctx.state())
         ctx.close()
-    a, b = out["1"], out["0"]
-    assert (a[0].few_path_iters, a[0].many_path_iters, a[0].sum_active) == \
-        (b[0].few_path_iters, b[0].many_path_iters, b[0].sum_active)
+    a = out["deferred"]
+    for other in ("placing", "separate"):
+        b = out[other]
+        assert (a[0].few_path_iters, a[0].many_path_iters, a[0].sum_active, a[0].sum_candidates) == \
+            (b[0].few_path_iters, b[0].many_path_iters, b[0].sum_active, b[0].sum_candidates), other
+        for f in ("n", "c", "m"):
+            assert np.array_equal(getattr(a[1], f), getattr(b[1], f)), (other, f)
+        for x, y in zip(a[2] + a[4], b[2] + b[4]):
+            assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8)), other
     if wide:
         assert a[0].many_path_iters > 0
-    for f in ("n", "c", "m"):
-        assert np.array_equal(getattr(a[1], f), getattr(b[1], f)), f
-    for x, y in zip(a[2] + a[4], b[2] + b[4]):
-        assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
-    assert a[3] == 3 * 5 + 1 and b[3] == 4 * 5
+    assert (out["deferred"][3], out["placing"][3], out["separate"][3]) == (3 * 5 + 1, 3 * 5 + 1, 4 * 5)
 
 
 @pytest.mark.parametrize("pack", ["1", "0"], ids=["packed-counts", "one-count-per-word"])
