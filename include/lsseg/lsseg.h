@@ -371,7 +371,7 @@ lsseg_status lsseg_batch_autotune(lsseg_batch* batch, const lsseg_params* params
    small-list bound (LSSEG_SMALL_TILE_N at creation) take tiles of 256
    positions, 1024: every compaction takes tiles of 1024; 0 in a field of
    set_tuning leaves it unchanged.  Results are identical under every tuning.
-   lsseg_autotune times `iters` iterations from the current state for 9 launch
+   lsseg_autotune times `iters` iterations from the current state for 15 launch
    shapes, then the other keep width, then the other compaction tile, keeps
    the fastest and restores the state (checkpoint slot LSSEG_MAX_CHECKPOINTS -
    1); it caches like lsseg_batch_autotune, per (GPU architecture, source

@@ -60,3 +60,21 @@ def test_profiler_rows_map_to_iterations():
     for k in got:
         assert [(i, r["Kernel_Name"]) for i, r in got2[k]] == [(i, r["Kernel_Name"]) for i, r in got[k]]
         assert all(int(r["Dispatch_Id"]) > len(earlier) + 1 for _, r in got2[k])
+
+
+def test_profiler_rows_of_the_deferred_keep_step_map_to_iterations():
+    """The deferred keep step's kernels (k_update_fused_defer, k_keep_defer)
+    count as the fused update and keep launches; the traversal rebuild at the
+    end of a run (k_settle_traversal) is no iteration launch."""
+    names = ["void lsseg::k_update<float, true, float>(lsseg::KArgs, int)",
+             "void lsseg::k_mark<float>(lsseg::KArgs, int)", "void lsseg::k_keep_defer<1>(lsseg::KArgs, int)"]
+    for _ in range(2):
+        names += ["void lsseg::k_update_fused_defer<float, true, float>(lsseg::KArgs, int)",
+                  "void lsseg::k_mark<float>(lsseg::KArgs, int)", "void lsseg::k_keep_defer<1>(lsseg::KArgs, int)"]
+    names += ["void lsseg::k_compact<2>(lsseg::KArgs, int)", "lsseg::k_settle_traversal(lsseg::KArgs, int)"]
+    rows = [{"Kernel_Name": k, "Dispatch_Id": str(i)} for i, k in enumerate(names)]
+    got = bench.iteration_dispatches(bench._dispatch_order(rows), np.array([4, 3, 2]))
+    assert [i for i, _ in got["k_update_fused"]] == [1, 2]
+    assert [i for i, _ in got["k_mark"]] == [0, 1, 2]
+    assert [i for i, _ in got["k_keep"]] == [0, 1, 2]
+    assert bench.SUMMARY_KEY_OF_PART["update_fused"] == "k_update_fused"
