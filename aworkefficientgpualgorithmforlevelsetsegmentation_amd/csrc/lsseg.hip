@@ -1642,6 +1642,9 @@ struct lsseg_batch {
     uint32_t sgrid = 2048;            // update and marking workgroups
     uint32_t keep_items = kKeepItemsBatch;  // traversal positions per thread of the keep step
     uint32_t compact_tile = kPosTile;       // list positions per few-path compaction tile (256 or 1024)
+    bool defer_keep = true;           // LSSEG_DEFER_KEEP at creation: the keep step leaves the traversals
+                                      // to the update launch
+    bool deferring = false;           // the call in progress defers them (not with LSSEG_FUSE=0)
     KArgs* d_table = nullptr;
     DevCtl* d_ctls = nullptr;
     DevCtl* h_ctls = nullptr;         // pinned mirror
@@ -1692,9 +1695,25 @@ void destroy_batch_graphs(lsseg_batch* b)
 // update (with `fused`, in one launch with every volume's pending compaction:
 // b->grid compaction workgroups, then b->sgrid update workgroups), marking and
 // keep step, compaction
+// The grid of a deferred batched keep step: it waits on nothing, so it runs
+// wide enough that its chunks stay single sub-tiles, and the update launch
+// that rebuilds the traversals gets as many work items (the compaction grid,
+// about 1024 workgroups for 64 volumes, gave chunks of 6 sub-tiles and a 6%
+// slower batch)
+uint32_t batch_keep_grid(const lsseg_batch* b)
+{
+    return b->deferring ? std::max<uint32_t>(b->grid, (uint32_t)env_int("LSSEG_BATCH_KEEP_GRID", 8192)) : b->grid;
+}
+
+BatchArgs batch_args(const lsseg_batch* b)
+{
+    return BatchArgs{b->d_table, b->d_ctls, (uint32_t)b->vols.size(), b->keep_items, batch_keep_grid(b),
+                     b->deferring ? 1u : 0u};
+}
+
 void launch_batch_head(lsseg_batch* b, int q, bool fused)
 {
-    BatchArgs args{b->d_table, b->d_ctls, (uint32_t)b->vols.size(), b->keep_items};
+    const BatchArgs args = batch_args(b);
     const bool phi8 = b->vols[0]->phi8;
     if (fused) {
         uint32_t gc = b->grid;
@@ -1709,15 +1728,15 @@ void launch_batch_head(lsseg_batch* b, int q, bool fused)
 
 void launch_batch_tail(lsseg_batch* b, int q)
 {
-    BatchArgs args{b->d_table, b->d_ctls, (uint32_t)b->vols.size(), b->keep_items};
+    const BatchArgs args = batch_args(b);
     void* kargs[] = {(void*)&args, (void*)&q};
     (void)hipLaunchKernel(mark_batch_kernel(b->vols[0]->phi8), dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
-    (void)hipLaunchKernel((const void*)&k_keep_batch, dim3(b->grid), dim3(kBlock), kargs, 0, b->stream);
+    (void)hipLaunchKernel((const void*)&k_keep_batch, dim3(batch_keep_grid(b)), dim3(kBlock), kargs, 0, b->stream);
 }
 
 void launch_batch_compact(lsseg_batch* b, int q)
 {
-    BatchArgs args{b->d_table, b->d_ctls, (uint32_t)b->vols.size(), b->keep_items};
+    const BatchArgs args = batch_args(b);
     void* kargs[] = {(void*)&args, (void*)&q};
     (void)hipLaunchKernel((const void*)&k_compact_batch, dim3(b->grid), dim3(kBlock), kargs, 0, b->stream);
 }
@@ -1770,6 +1789,11 @@ lsseg_status launch_batch_iterations(lsseg_batch* b, int count, int q)
         q ^= 1;
     }
     launch_batch_compact(b, q ^ 1);
+    if (b->deferring) {
+        const BatchArgs args = batch_args(b);
+        void* kargs[] = {(void*)&args, (void*)&q};
+        (void)hipLaunchKernel((const void*)&k_settle_batch, dim3(b->sgrid), dim3(kBlock), kargs, 0, b->stream);
+    }
     return LSSEG_OK;
 }
 
@@ -1920,6 +1944,7 @@ lsseg_status lsseg_batch_create(const lsseg_volume_desc* descs, const void* cons
         return st;
     };
     b->dtype = descs[0].dtype;
+    b->defer_keep = env_int("LSSEG_DEFER_KEEP", 1) != 0;
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess) {
@@ -2000,8 +2025,12 @@ lsseg_status lsseg_batch_iterate(lsseg_batch* b, const lsseg_params* params, int
         if ((st = bforward(b, v, normalise_parity(c))) != LSSEG_OK) return st;
     }
     b->h_table.resize(cnt);
+    // a deferred keep step leaves each volume's traversal to the next update
+    // launch (the four-launch schedule places it in the keep step)
+    b->deferring = b->defer_keep && env_int("LSSEG_FUSE", 1) != 0;
     for (uint32_t v = 0; v < cnt; ++v) {
         b->h_table[v] = b->vols[v]->a;
+        b->h_table[v].defer = b->deferring ? 1u : 0u;
         // compaction tiles of 1024 positions unless tuned otherwise: small
         // tiles made the 64-volume batch 6% slower (more chunks over the one
         // grid all volumes share)

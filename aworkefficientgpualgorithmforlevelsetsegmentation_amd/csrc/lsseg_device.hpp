@@ -1815,6 +1815,53 @@ __device__ __attribute__((always_inline)) inline uint32_t block_sum(uint32_t v, 
     return red[0] + red[1] + red[2] + red[3];
 }
 
+// Positions [p0, p1) of the traversal S[q ^ 1] expanded into their kept
+// targets (kept directions in emit), stored into S[q] from position `run` on
+// in order, each voxel updated as it is placed when kUpd.  Every thread of the
+// block calls it.
+template <typename SrcT, bool kFloatSrc, typename PhiT, bool kUpd>
+__device__ __attribute__((always_inline)) inline void append_chunk(const KArgs& a, int q, const DevParams& P,
+                                                                  uint64_t p0, uint64_t p1, uint32_t run,
+                                                                  AppendShared& sh)
+{
+    const uint32_t* __restrict__ Ss = a.S[q ^ 1];
+    uint32_t* __restrict__ Sn = a.S[q];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint64_t blk = p0; blk < p1; blk += kBlock) {
+        const uint64_t p = blk + threadIdx.x;
+        const bool valid = p < p1;
+        const uint32_t u = valid ? Ss[p] : 0u;
+        const uint32_t kb = valid ? a.emit[p] : 0u;
+        const uint32_t c = (uint32_t)__popc(kb);
+        const uint32_t incl = wave_inclusive_scan(c, lane);
+        if (lane == 63) sh.cnt[wave] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t c0 = sh.cnt[0], c1 = sh.cnt[1], c2 = sh.cnt[2], c3 = sh.cnt[3];
+            sh.excl[0] = 0;
+            sh.excl[1] = c0;
+            sh.excl[2] = c0 + c1;
+            sh.excl[3] = c0 + c1 + c2;
+            sh.total = c0 + c1 + c2 + c3;
+        }
+        __syncthreads();
+        uint32_t off = sh.excl[wave] + incl - c;
+#pragma unroll
+        for (int e = 0; e < 7; ++e)
+            if ((kb >> e) & 1u) sh.tgt[off++] = neighbor_in(a.g, u, (uint32_t)e);
+        const uint32_t t = sh.total;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < t; i += kBlock) {
+            const uint32_t v = sh.tgt[i];
+            Sn[run + i] = v;
+            if constexpr (kUpd)
+                update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), v, run + i);
+        }
+        run += t;
+        __syncthreads();
+    }
+}
+
 // The traversal of list q rebuilt from a deferred keep step over list q ^ 1
 // (its traversal S[q ^ 1], kept directions in emit, chunk counts in kcnt),
 // over `count` workgroups from `first` (multiples of 8), each voxel updated as
@@ -1827,58 +1874,19 @@ __device__ inline void append_walk(const KArgs& a, int q, uint32_t first, uint32
     const uint32_t items = (uint32_t)(kBlock * a.keep_items);
     const ChunkPlan cp = chunk_plan((nsrc + items - 1) / items, a.keep_grid);
     const uint64_t span = (uint64_t)cp.r * items;  // positions per chunk
-    const uint32_t* __restrict__ Ss = a.S[q ^ 1];
-    uint32_t* __restrict__ Sn = a.S[q];
     DevParams P{};
     if constexpr (kUpd) P = *a.prm;
     const uint32_t b = blockIdx.x - first, g8 = count >> 3, x = b & 7u;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t s = b >> 3;; s += g8) {
+        // the s-th chunk of XCD x's runs; j grows with s
         const uint32_t j = (((s >> 4) << 3) + x) * 16u + (s & 15u);
-        if ((s >> 4) * 128u >= cp.chunks || j >= cp.chunks) {
-            // past the last chunk of this XCD's runs: every later s is too
-            if ((s >> 4) * 128u >= cp.chunks) break;
-            continue;
-        }
+        if (j >= cp.chunks) break;
         // the chunk's start: the counts of the chunks before it
         uint32_t pre = 0;
         for (uint32_t i = threadIdx.x; i < j; i += kBlock) pre += a.kcnt[i];
-        uint32_t run = block_sum(pre, sh.red);
-        const uint64_t p0 = (uint64_t)j * span, p1 = min(p0 + span, (uint64_t)nsrc);
-        for (uint64_t blk = p0; blk < p1; blk += kBlock) {
-            const uint64_t p = blk + threadIdx.x;
-            const bool valid = p < p1;
-            const uint32_t u = valid ? Ss[p] : 0u;
-            const uint32_t kb = valid ? a.emit[p] : 0u;
-            const uint32_t c = (uint32_t)__popc(kb);
-            const uint32_t incl = wave_inclusive_scan(c, lane);
-            if (lane == 63) sh.cnt[wave] = incl;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                const uint32_t c0 = sh.cnt[0], c1 = sh.cnt[1], c2 = sh.cnt[2], c3 = sh.cnt[3];
-                sh.excl[0] = 0;
-                sh.excl[1] = c0;
-                sh.excl[2] = c0 + c1;
-                sh.excl[3] = c0 + c1 + c2;
-                sh.total = c0 + c1 + c2 + c3;
-            }
-            __syncthreads();
-            uint32_t off = sh.excl[wave] + incl - c;
-#pragma unroll
-            for (int e = 0; e < 7; ++e)
-                if ((kb >> e) & 1u) sh.tgt[off++] = neighbor_in(a.g, u, (uint32_t)e);
-            const uint32_t t = sh.total;
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i < t; i += kBlock) {
-                const uint32_t v = sh.tgt[i];
-                Sn[run + i] = v;
-                if constexpr (kUpd)
-                    update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), v,
-                                                     run + i);
-            }
-            run += t;
-            __syncthreads();
-        }
+        const uint32_t run = block_sum(pre, sh.red);
+        const uint64_t p0 = (uint64_t)j * span;
+        append_chunk<SrcT, kFloatSrc, PhiT, kUpd>(a, q, P, p0, min(p0 + span, (uint64_t)nsrc), run, sh);
     }
 }
 
@@ -1924,6 +1932,8 @@ struct BatchArgs {
     DevCtl* ctls;
     uint32_t count;
     uint32_t keep_items;  // traversal positions per thread of k_keep_batch: 1 or 2 (autotuned)
+    uint32_t keep_grid;   // the keep step's grid (its chunk plan, for a deferred keep step)
+    uint32_t defer;       // deferred keep step: the update launch rebuilds each volume's traversal
 };
 
 // exclusive prefix s_pre[0..count] of f(v) over the volumes (all threads)
@@ -2048,7 +2058,8 @@ __device__ __attribute__((always_inline)) inline void keep_batch_body(const Batc
         const uint32_t chunk = j - s_pre[v];
         uint32_t us[kI], ms[kI];
         keep_voxels<kI>(a, q, chunk * r, us, ms);
-        (void)keep_chunk<kI>(a, q, n, cp, chunk, b.ctls[v].epoch, sh, us, ms);
+        if (b.defer) keep_defer_chunk<kI>(a, q, n, cp, chunk, sh, us, ms);
+        else (void)keep_chunk<kI>(a, q, n, cp, chunk, b.ctls[v].epoch, sh, us, ms);
     }
 }
 
@@ -2063,8 +2074,55 @@ __global__ __launch_bounds__(kBlock) void k_keep_batch(BatchArgs b, int q)
     // volumes with an empty list still advance their counts
     if (threadIdx.x < 64)
         for (uint32_t v = blockIdx.x; v < b.count; v += gridDim.x)
-            if (s_pre[v + 1] == s_pre[v])
-                keep_finish(b.vols[v], q, 0, 0, b.ctls[v].epoch, candidate_share(b.vols[v]), ctl_snapshot(&b.ctls[v]));
+            if (s_pre[v + 1] == s_pre[v]) {
+                if (b.defer)
+                    keep_finish_defer(b.vols[v], 0, candidate_share(b.vols[v]), ctl_snapshot(&b.ctls[v]));
+                else
+                    keep_finish(b.vols[v], q, 0, 0, b.ctls[v].epoch, candidate_share(b.vols[v]),
+                                ctl_snapshot(&b.ctls[v]));
+            }
+}
+
+// The traversals of list q of every volume rebuilt from a deferred batched
+// keep step over list q ^ 1 (see append_walk), each voxel updated as it is
+// placed when kUpd: work items are the keep step's chunks, volume after
+// volume, item j taken by slot j mod nslots.
+template <typename SrcT, bool kFloatSrc, typename PhiT, bool kUpd>
+__device__ inline void append_batch_items(const BatchArgs& b, int q, uint32_t slot, uint32_t nslots,
+                                          uint32_t* s_pre, uint32_t* s_tmp, AppendShared& sh)
+{
+    const uint32_t kI = b.keep_items, items = kBlock * kI;
+    // the keep step's plan: r sub-tiles per chunk over all volumes' sub-tiles
+    batch_prefix(b.count, [&](uint32_t v) { return (b.ctls[v].n[q ^ 1] + items - 1) / items; }, s_pre, s_tmp);
+    const uint32_t r = batch_chunk_r(s_pre[b.count], b.count, b.keep_grid);
+    __syncthreads();
+    batch_prefix(b.count, [&](uint32_t v) { return ((b.ctls[v].n[q ^ 1] + items - 1) / items + r - 1) / r; }, s_pre,
+                 s_tmp);
+    const uint32_t total = s_pre[b.count];
+    const uint64_t span = (uint64_t)r * items;
+    for (uint32_t j = slot; j < total; j += nslots) {
+        const uint32_t v = batch_find(s_pre, b.count, j);
+        const KArgs& a = b.vols[v];
+        const uint32_t chunk = j - s_pre[v];
+        DevParams P{};
+        if constexpr (kUpd) P = *a.prm;
+        uint32_t pre = 0;
+        for (uint32_t i = threadIdx.x; i < chunk; i += kBlock) pre += a.kcnt[i];
+        const uint32_t run = block_sum(pre, sh.red);
+        const uint64_t p0 = (uint64_t)chunk * span;
+        append_chunk<SrcT, kFloatSrc, PhiT, kUpd>(a, q, P, p0, min(p0 + span, (uint64_t)b.ctls[v].n[q ^ 1]), run,
+                                                  sh);
+    }
+}
+
+// the traversals of list q of every volume rebuilt after a batched call's last
+// deferred keep step
+__global__ __launch_bounds__(kBlock) void k_settle_batch(BatchArgs b, int q)
+{
+    __shared__ AppendShared sh;
+    __shared__ uint32_t s_pre[kMaxBatch + 1];
+    __shared__ uint32_t s_tmp[kBlock / 64];
+    append_batch_items<uint8_t, false, float, false>(b, q, blockIdx.x, gridDim.x, s_pre, s_tmp, sh);
 }
 
 // The compaction of list[q] of every volume over the workgroups of slots
@@ -2115,11 +2173,14 @@ __global__ __launch_bounds__(kBlock) void k_compact_batch(BatchArgs b, int q)
 template <typename SrcT, bool kFloatSrc, typename PhiT>
 __global__ __launch_bounds__(kBlock) void k_update_fused_batch(BatchArgs b, int q, uint32_t gc)
 {
-    __shared__ CompactShared sh;
+    __shared__ FusedDeferShared sh;
     __shared__ uint32_t s_pre[kMaxBatch + 1];
     __shared__ uint32_t s_tmp[kBlock / 64];
-    if (blockIdx.x < gc) compact_batch_chunks(b, q ^ 1, gc, sh, s_pre, s_tmp);
-    else update_batch_items<SrcT, kFloatSrc, PhiT>(b, q, blockIdx.x - gc, gridDim.x - gc, s_pre, s_tmp);
+    if (blockIdx.x < gc) compact_batch_chunks(b, q ^ 1, gc, sh.c, s_pre, s_tmp);
+    else if (b.defer)
+        append_batch_items<SrcT, kFloatSrc, PhiT, true>(b, q, blockIdx.x - gc, gridDim.x - gc, s_pre, s_tmp, sh.d);
+    else
+        update_batch_items<SrcT, kFloatSrc, PhiT>(b, q, blockIdx.x - gc, gridDim.x - gc, s_pre, s_tmp);
 }
 
 #define LSSEG_FOR_EACH(p, count)                                                           \

@@ -187,3 +187,37 @@ def test_batch_rejects_mixed_types_and_owned_destroy():
     from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import lib, ERR_STATE
     assert lib().lsseg_destroy(batch.volume(0)._h) == ERR_STATE
     batch.close()
+
+
+@pytest.mark.parametrize("defer,fuse,keep_grid", [("0", "1", None), ("1", "0", None), ("1", "1", "64")],
+                         ids=["placing-keep", "separate-launches", "narrow-keep-grid"])
+def test_batch_schedules_give_the_solo_results(monkeypatch, defer, fuse, keep_grid):
+    """The batch's other schedules leave each volume bit-identical to its solo
+    run: the keep step that places the traversals itself (LSSEG_DEFER_KEEP=0),
+    every step in a launch of its own (LSSEG_FUSE=0), and the deferred keep
+    step on a grid narrow enough that its chunks hold several sub-tiles."""
+    specs = _volumes()
+    monkeypatch.setenv("LSSEG_DEFER_KEEP", defer)
+    monkeypatch.setenv("LSSEG_FUSE", fuse)
+    if keep_grid:
+        monkeypatch.setenv("LSSEG_BATCH_KEEP_GRID", keep_grid)
+    batch = Batch([s[0] for s in specs])
+    t = batch.tuning()
+    if keep_grid:
+        batch.set_tuning(t.update_blocks, 64, t.keep_items, t.compact_tile)
+    monkeypatch.setenv("LSSEG_DEFER_KEEP", "1")
+    monkeypatch.setenv("LSSEG_FUSE", "1")
+    solos = [Context(s[0]) for s in specs]
+    for i, (vol, phi0, band, prm) in enumerate(specs):
+        batch.volume(i).set_phi(phi0, band)
+        solos[i].set_phi(phi0, band)
+    params = [s[3] for s in specs]
+    monkeypatch.setenv("LSSEG_FUSE", fuse)
+    batch.iterate(params, 41)
+    monkeypatch.setenv("LSSEG_FUSE", "1")
+    for i, s in enumerate(solos):
+        T, eps, alpha, dt, mode = params[i]
+        s.iterate(T, eps, alpha, dt, 41, mode=mode)
+        _assert_same(batch.volume(i), s, f"schedule defer={defer} fuse={fuse} keep grid {keep_grid} volume {i}")
+        s.close()
+    batch.close()
