@@ -2542,6 +2542,20 @@ const void* update_record_kernel(int32_t dtype, bool fused)
     }
 }
 
+// the fused update launch of a recording run with the deferred keep step
+const void* update_record_defer_kernel(int32_t dtype)
+{
+    switch (dtype) {
+    case LSSEG_U8: return (const void*)&k_update_fused_defer_record<uint8_t, false>;
+    case LSSEG_I8: return (const void*)&k_update_fused_defer_record<int8_t, false>;
+    case LSSEG_U16: return (const void*)&k_update_fused_defer_record<uint16_t, false>;
+    case LSSEG_I16: return (const void*)&k_update_fused_defer_record<int16_t, false>;
+    case LSSEG_U32: return (const void*)&k_update_fused_defer_record<uint32_t, false>;
+    case LSSEG_I32: return (const void*)&k_update_fused_defer_record<int32_t, false>;
+    default: return (const void*)&k_update_fused_defer_record<float, true>;
+    }
+}
+
 // One recording run of `iters` iterations from the context's state: the
 // launches of lsseg_iterate (each compaction in the next update's launch).
 // The update keeps each voxel's decisions beside its temporal flag and the
@@ -2571,15 +2585,28 @@ lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
     HIP_TRY(c, hipMemcpyAsync(c->a.ctl, c->h_ctl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
     int q = c->q;
     const RecordOut ro{r->offs, r->cap, r->lists, r->dec, r->undo};
+    // with the deferred keep step each fused update launch rebuilds the
+    // traversal and records by the rebuilt positions, as lsseg_iterate runs it
+    const bool defer = c->defer_keep;
+    c->a.defer = defer ? 1u : 0u;
+    c->a.keep_grid = c->grid;
+    c->a.keep_items = c->keep_fn == (const void*)&k_keep<2> ? 2u : 1u;
+    const void* keep_fn = !defer ? c->keep_fn
+                          : c->a.keep_items == 2 ? (const void*)&k_keep_defer<2> : (const void*)&k_keep_defer<1>;
     for (int k = 0; k < iters; ++k) {
         void* args[] = {(void*)&c->a, (void*)&q, (void*)&ro};
-        (void)hipLaunchKernel(update_record_kernel(c->dtype, k > 0), dim3(k > 0 ? c->grid : c->sgrid), dim3(kBlock),
-                              args, 0, c->stream);
+        const void* head = k == 0 ? update_record_kernel(c->dtype, false)
+                           : defer ? update_record_defer_kernel(c->dtype) : update_record_kernel(c->dtype, true);
+        (void)hipLaunchKernel(head, dim3(k > 0 ? c->grid : c->sgrid), dim3(kBlock), args, 0, c->stream);
         (void)hipLaunchKernel((const void*)&k_mark_record, dim3(c->sgrid), dim3(kBlock), args, 0, c->stream);
-        launch(c, c->keep_fn, c->grid, q);
+        launch(c, keep_fn, c->grid, q);
         q ^= 1;
     }
-    if (iters > 0) launch(c, (const void*)&k_compact<kCompact>, c->grid, q ^ 1);
+    if (iters > 0) {
+        launch(c, (const void*)&k_compact<kCompact>, c->grid, q ^ 1);
+        if (defer) hipLaunchKernelGGL(k_settle_traversal, dim3(c->sgrid), dim3(kBlock), 0, c->stream, c->a, q);
+    }
+    c->a.defer = 0;
     HIP_TRY(c, hipMemcpyAsync(r->fin[0], c->a.phi[q], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(r->fin[1], c->a.phi[q ^ 1], N * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipGetLastError());

@@ -1819,10 +1819,11 @@ __device__ __attribute__((always_inline)) inline uint32_t block_sum(uint32_t v, 
 // targets (kept directions in emit), stored into S[q] from position `run` on
 // in order, each voxel updated as it is placed when kUpd.  Every thread of the
 // block calls it.
-template <typename SrcT, bool kFloatSrc, typename PhiT, bool kUpd>
+template <typename SrcT, bool kFloatSrc, typename PhiT, bool kUpd, bool kRec = false>
 __device__ __attribute__((always_inline)) inline void append_chunk(const KArgs& a, int q, const DevParams& P,
                                                                   uint64_t p0, uint64_t p1, uint32_t run,
-                                                                  AppendShared& sh)
+                                                                  AppendShared& sh, float* undo = nullptr,
+                                                                  uint64_t undo_room = 0)
 {
     const uint32_t* __restrict__ Ss = a.S[q ^ 1];
     uint32_t* __restrict__ Sn = a.S[q];
@@ -1854,7 +1855,10 @@ __device__ __attribute__((always_inline)) inline void append_chunk(const KArgs& 
         for (uint32_t i = threadIdx.x; i < t; i += kBlock) {
             const uint32_t v = sh.tgt[i];
             Sn[run + i] = v;
-            if constexpr (kUpd)
+            if constexpr (kRec)
+                update_at<SrcT, kFloatSrc, PhiT, true>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), v, run + i,
+                                                       run + i < undo_room ? undo : nullptr);
+            else if constexpr (kUpd)
                 update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), v, run + i);
         }
         run += t;
@@ -1867,8 +1871,9 @@ __device__ __attribute__((always_inline)) inline void append_chunk(const KArgs& 
 // over `count` workgroups from `first` (multiples of 8), each voxel updated as
 // it is placed when kUpd.  Chunks go to the XCDs in runs of 16 (4096 positions
 // of single-tile chunks), as ListWalk deals list positions.
-template <typename SrcT, bool kFloatSrc, typename PhiT, bool kUpd>
-__device__ inline void append_walk(const KArgs& a, int q, uint32_t first, uint32_t count, AppendShared& sh)
+template <typename SrcT, bool kFloatSrc, typename PhiT, bool kUpd, bool kRec = false>
+__device__ inline void append_walk(const KArgs& a, int q, uint32_t first, uint32_t count, AppendShared& sh,
+                                   const RecordOut& r = RecordOut{})
 {
     const uint32_t nsrc = a.ctl->n[q ^ 1];
     const uint32_t items = (uint32_t)(kBlock * a.keep_items);
@@ -1876,6 +1881,16 @@ __device__ inline void append_walk(const KArgs& a, int q, uint32_t first, uint32
     const uint64_t span = (uint64_t)cp.r * items;  // positions per chunk
     DevParams P{};
     if constexpr (kUpd) P = *a.prm;
+    // a recording run's undo slots of this iteration (k = ctl->iter) from
+    // offs[k] on, as many as fit below the capacity: the list's count is
+    // placed in this launch, so each entry checks its own slot
+    float* undo = nullptr;
+    uint64_t undo_room = 0;
+    if constexpr (kRec) {
+        const uint64_t base = r.offs[a.ctl->iter];
+        undo = r.undo + base;
+        undo_room = base < r.cap ? r.cap - base : 0;
+    }
     const uint32_t b = blockIdx.x - first, g8 = count >> 3, x = b & 7u;
     for (uint32_t s = b >> 3;; s += g8) {
         // the s-th chunk of XCD x's runs; j grows with s
@@ -1886,7 +1901,8 @@ __device__ inline void append_walk(const KArgs& a, int q, uint32_t first, uint32
         for (uint32_t i = threadIdx.x; i < j; i += kBlock) pre += a.kcnt[i];
         const uint32_t run = block_sum(pre, sh.red);
         const uint64_t p0 = (uint64_t)j * span;
-        append_chunk<SrcT, kFloatSrc, PhiT, kUpd>(a, q, P, p0, min(p0 + span, (uint64_t)nsrc), run, sh);
+        append_chunk<SrcT, kFloatSrc, PhiT, kUpd, kRec>(a, q, P, p0, min(p0 + span, (uint64_t)nsrc), run, sh, undo,
+                                                        undo_room);
     }
 }
 
@@ -1907,6 +1923,26 @@ __global__ __launch_bounds__(kBlock) void k_update_fused_defer(KArgs a, int q)
     } else {
         if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kCompact>(a, q ^ 1, c, 0, epoch);
         append_walk<SrcT, kFloatSrc, PhiT, true>(a, q, gc, gridDim.x - gc, sh.d);
+    }
+}
+
+// k_update_fused_defer of a recording run (FP32 phi): the update records
+// each voxel's decisions and the value it overwrites by its rebuilt position
+template <typename SrcT, bool kFloatSrc>
+__global__ __launch_bounds__(kBlock) void k_update_fused_defer_record(KArgs a, int q, RecordOut r)
+{
+    __shared__ FusedDeferShared sh;
+    uint32_t kb[kPosItems], vx[kPosItems], d[7], small[2] = {0u, 0u};
+    bool loaded = compact_preload(a, q ^ 1, kb, vx, d, small);  // with the counts
+    const uint32_t epoch = a.ctl->epoch;
+    CompactPlan c = compact_plan<kCompact>(a.g, a.ctl->n[q ^ 1]);
+    loaded = compact_pick_preload(c, loaded, kb, vx, small);
+    const uint32_t gc = min((c.ntiles + 7u) & ~7u, a.fused_compact_max);
+    if (blockIdx.x < gc) {
+        compact_all<kCompact>(a, q ^ 1, c, epoch, sh.c, gc, kb, vx, d, loaded);
+    } else {
+        if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kCompact>(a, q ^ 1, c, 0, epoch);
+        append_walk<SrcT, kFloatSrc, float, true, true>(a, q, gc, gridDim.x - gc, sh.d, r);
     }
 }
 
