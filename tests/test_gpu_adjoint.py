@@ -223,3 +223,27 @@ def test_torch_autograd_function():
     rec.close()
     ctx.close()
     del phi, loss
+
+
+def test_deferred_keep_step_records_the_placing_keep_steps_traversal(monkeypatch):
+    """The update launch rebuilds the traversal that a deferred keep step
+    leaves as kept directions in the order the placing keep step appends it,
+    so both record V_k in the same order with the same decisions, and their
+    replays agree bit for bit."""
+    vol, _, phi0, (T, eps, alpha), K = _case(edge=(34, 30, 26), seed=91, iters=36)
+    _say(f"records with the deferred and the placing keep step {vol.shape} K={K}")
+    out = {}
+    for defer in ("1", "0"):
+        monkeypatch.setenv("LSSEG_DEFER_KEEP", defer)
+        ctx = Context(vol)
+        ctx.set_phi(phi0, 1.0)
+        rec = ctx.fwd_record(T, eps, alpha, DT, K)
+        n, lists, dec = rec.lists()
+        out[defer] = (n, lists, dec, rec.replay(phi0, T, eps, alpha, DT), ctx.state())
+        rec.close()
+        ctx.close()
+    a, b = out["1"], out["0"]
+    for x, y in zip(a[:4], b[:4]):
+        assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
+    for x, y in zip(a[4], b[4]):
+        assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
