@@ -717,11 +717,11 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     c->grid = want > 0 ? (uint32_t)want : (uint32_t)occ * c->cus;
     c->grid = std::max<uint32_t>(16u, c->grid & ~7u);  // k_update_fused splits it in multiples of 8
     c->grid0 = std::max<uint32_t>(16u, ((uint32_t)occ * c->cus) & ~7u);
-    // k_update_fused: at most three quarters of the keep/compaction grid
-    // compact, the rest of the fused launch's grid updates
-    c->a.fused_compact_max = (c->grid / 4u * 3u) & ~7u;
+    // k_update_fused: at most three quarters of its grid compact, the rest
+    // updates (by default its grid is the keep/compaction grid)
     const int fwant = env_int("LSSEG_FUSED_GRID", 0);
-    c->fgrid = fwant > 0 ? std::max<uint32_t>(((uint32_t)fwant + 7u) & ~7u, c->a.fused_compact_max + 8u) : c->grid;
+    c->fgrid = fwant > 0 ? std::max<uint32_t>(16u, ((uint32_t)fwant + 7u) & ~7u) : c->grid;
+    c->a.fused_compact_max = (c->fgrid / 4u * 3u) & ~7u;
     // update and marking: eight workgroups per CU (a multiple of the eight
     // XCDs), list positions dealt to the XCDs in runs of 4096
     const int swant = env_int("LSSEG_STREAM_BLOCKS", 0);
