@@ -739,14 +739,14 @@ __device__ inline void update_at(const KArgs& a, const DevParams& P, const PhiT*
 // the temporal flag `moved`; returns its emission mask
 template <typename PhiT>
 __device__ inline uint32_t mark_at(const KArgs& a, bool ref, bool moved, const PhiT* __restrict__ W,
-                                   uint8_t* __restrict__ Cb, uint32_t u)
+                                   uint8_t* __restrict__ Cb, uint32_t u, uint32_t& pos)
 {
     uint32_t m = 0;
     // the position and every operand are loaded together (all addresses are
     // voxels of the volume); the position then decides whether u emits
     uint32_t nb[7], inb;
     neighbors(a.g, u, nb, &inb);
-    const uint32_t pos = a.pos[u];
+    pos = a.pos[u];
     const float nv = phi_load(W, u);
     float wn[7];
 #pragma unroll
@@ -1316,11 +1316,12 @@ __device__ inline void keep_finish(const KArgs& a, int q, uint32_t n, uint32_t m
 }
 
 // the traversal entries of keep sub-tile `sub` of list q and their emission
-// masks; positions are bounded by N (every one a valid address), and
-// keep_decide discards the entries at or beyond the count
+// masks, with ps their list positions as a deferring marking left them in
+// S[q ^ 1] (see mark_walk); positions are bounded by N (every one a valid
+// address), and keep_decide discards the entries at or beyond the count
 template <int kI>
 __device__ __attribute__((always_inline)) inline void keep_voxels(const KArgs& a, int q, uint32_t sub, uint32_t* us,
-                                                                  uint32_t* ms)
+                                                                  uint32_t* ms, uint32_t* ps = nullptr)
 {
     const uint64_t base = (uint64_t)sub * (kBlock * kI);
 #pragma unroll
@@ -1329,16 +1330,17 @@ __device__ __attribute__((always_inline)) inline void keep_voxels(const KArgs& a
         const uint64_t ic = i < a.g.N ? i : 0;
         us[j] = a.S[q][ic];
         ms[j] = a.emit[ic];
+        if (ps) ps[j] = a.S[q ^ 1][ic];
     }
 }
 
 // the kept directions kb of the voxels us (emission masks ms) of keep sub-tile
-// `sub`, written to keep[pos[u]] as well; entries past the count become voxel
-// 0 with kb = 0
+// `sub`, written to keep[pos[u]] as well (the positions psin when given, else
+// gathered); entries past the count become voxel 0 with kb = 0
 template <int kI>
 __device__ __attribute__((always_inline)) inline void keep_decide(const KArgs& a, uint32_t n, bool dense,
                                                                   uint32_t sub, uint32_t* us, const uint32_t* ms,
-                                                                  uint32_t* kb)
+                                                                  uint32_t* kb, const uint32_t* psin = nullptr)
 {
     const uint64_t* __restrict__ Ck = a.C;
     const uint64_t base = (uint64_t)sub * (kBlock * kI);
@@ -1349,7 +1351,7 @@ __device__ __attribute__((always_inline)) inline void keep_decide(const KArgs& a
     for (int j = 0; j < kI; ++j)
         if (base + (uint64_t)j * kBlock + threadIdx.x >= n) us[j] = 0;
 #pragma unroll
-    for (int j = 0; j < kI; ++j) ps[j] = a.pos[us[j]];
+    for (int j = 0; j < kI; ++j) ps[j] = psin ? psin[j] : a.pos[us[j]];
     // a -z candidate (e = 0) comes first in any record, so only e >= 1 needs one
     uint64_t rec[kI][7];
 #pragma unroll
@@ -1668,11 +1670,16 @@ __device__ inline void mark_walk(const KArgs& a, int q, const RecordOut& r)
         if (blockIdx.x == 0 && threadIdx.x == 0) r.offs[k + 1] = base + n;
         rec = base + n <= r.cap;
     }
+    // a deferred keep step reads the list positions in traversal order from
+    // S[q ^ 1], which is free until the next launch rebuilds the traversal
+    uint32_t* __restrict__ sp = a.defer ? a.S[q ^ 1] : nullptr;
     uint32_t emitted = 0;
     walk_list(a, q, 0, 0, [&](uint32_t u, uint64_t p) {
         const uint32_t mv = a.moved[p];
-        const uint32_t m = mark_at<PhiT>(a, ref, (mv & 1u) != 0, phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, u);
+        uint32_t pos;
+        const uint32_t m = mark_at<PhiT>(a, ref, (mv & 1u) != 0, phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, u, pos);
         a.emit[p] = (uint8_t)m;
+        if (sp) sp[p] = pos;
         emitted += (uint32_t)__popc(m);
         if constexpr (kRec)
             if (rec) {
@@ -1749,7 +1756,7 @@ template <int kI>
 __device__ __attribute__((always_inline)) inline void keep_defer_chunk(const KArgs& a, int q, uint32_t n,
                                                                       const ChunkPlan& cp, uint32_t chunk,
                                                                       KeepShared<kI>& sh, uint32_t* us,
-                                                                      uint32_t* ms)
+                                                                      uint32_t* ms, uint32_t* ps)
 {
     const bool dense = !path_is_few(n, a.g.N);
     const uint32_t s0 = chunk * cp.r, s1 = min(s0 + cp.r, keep_tiles<kI>(n));
@@ -1759,8 +1766,8 @@ __device__ __attribute__((always_inline)) inline void keep_defer_chunk(const KAr
     uint32_t kb[kI], ex[kI];
     uint32_t total = 0;
     for (uint32_t sub = s0; sub < s1; ++sub) {
-        if (cp.r > 1) keep_voxels<kI>(a, q, sub, us, ms);
-        keep_decide<kI>(a, n, dense, sub, us, ms, kb);
+        if (cp.r > 1) keep_voxels<kI>(a, q, sub, us, ms, ps);
+        keep_decide<kI>(a, n, dense, sub, us, ms, kb, ps);
         const uint64_t base = (uint64_t)sub * (kBlock * kI);
 #pragma unroll
         for (int j = 0; j < kI; ++j) {
@@ -1784,11 +1791,11 @@ template <int kI>
 __global__ __launch_bounds__(kBlock) void k_keep_defer(KArgs a, int q)
 {
     __shared__ KeepShared<kI> sh;
-    uint32_t us[kI], ms[kI];
-    keep_voxels<kI>(a, q, blockIdx.x, us, ms);  // with the count: this workgroup's sub-tile when chunks are single
+    uint32_t us[kI], ms[kI], ps[kI];
+    keep_voxels<kI>(a, q, blockIdx.x, us, ms, ps);  // with the count: this workgroup's sub-tile when chunks are single
     const uint32_t n = a.ctl->n[q];
     const ChunkPlan cp = chunk_plan(keep_tiles<kI>(n), gridDim.x);
-    if (blockIdx.x < cp.chunks) keep_defer_chunk<kI>(a, q, n, cp, blockIdx.x, sh, us, ms);
+    if (blockIdx.x < cp.chunks) keep_defer_chunk<kI>(a, q, n, cp, blockIdx.x, sh, us, ms, ps);
     if (cp.chunks == 0 && blockIdx.x == 0 && threadIdx.x < 64) keep_finish_defer(a, 0, candidate_share(a), ctl_snapshot(a.ctl));
 }
 
@@ -2056,9 +2063,11 @@ __global__ __launch_bounds__(kBlock) void k_mark_batch(BatchArgs b, int q)
         const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
         uint32_t emitted = 0;
         if (p < b.ctls[v].n[q]) {
+            uint32_t pos;
             const uint32_t m = mark_at<PhiT>(a, a.prm->refMode != 0, (a.moved[p] & 1u) != 0, phi_buf<PhiT>(a, q ^ 1),
-                                             (uint8_t*)a.C, a.S[q][p]);
+                                             (uint8_t*)a.C, a.S[q][p], pos);
             a.emit[p] = (uint8_t)m;
+            if (b.defer) a.S[q ^ 1][p] = pos;  // for the deferred keep step (see mark_walk)
             emitted = (uint32_t)__popc(m);
         }
         block_add(&a.cpart[(blockIdx.x % kCountSlots) * kCountStride], emitted, s_red);
@@ -2092,10 +2101,14 @@ __device__ __attribute__((always_inline)) inline void keep_batch_body(const Batc
         const uint32_t n = b.ctls[v].n[q];
         const ChunkPlan cp{r, (keep_tiles<kI>(n) + r - 1) / r};
         const uint32_t chunk = j - s_pre[v];
-        uint32_t us[kI], ms[kI];
-        keep_voxels<kI>(a, q, chunk * r, us, ms);
-        if (b.defer) keep_defer_chunk<kI>(a, q, n, cp, chunk, sh, us, ms);
-        else (void)keep_chunk<kI>(a, q, n, cp, chunk, b.ctls[v].epoch, sh, us, ms);
+        uint32_t us[kI], ms[kI], ps[kI];
+        if (b.defer) {
+            keep_voxels<kI>(a, q, chunk * r, us, ms, ps);
+            keep_defer_chunk<kI>(a, q, n, cp, chunk, sh, us, ms, ps);
+        } else {
+            keep_voxels<kI>(a, q, chunk * r, us, ms);
+            (void)keep_chunk<kI>(a, q, n, cp, chunk, b.ctls[v].epoch, sh, us, ms);
+        }
     }
 }
 
