@@ -218,6 +218,26 @@ def test_fused_and_separate_launches_agree(monkeypatch, wide):
     assert (out["deferred"][3], out["placing"][3], out["separate"][3]) == (3 * 5 + 1, 3 * 5 + 1, 4 * 5)
 
 
+def test_calls_alternating_launch_schedules_match_the_oracle(monkeypatch):
+    """One context whose calls alternate between the default schedule and
+    separate launches (LSSEG_FUSE=0, read per call) ends where the oracle
+    does.  A deferred call ends by rebuilding the traversal from the last keep
+    step's kept directions, a separate one with the keep step placing it, and
+    each starts from what the other left: every hand-over is exercised."""
+    w = synth.ellipsoids(edge=64, iterations=0)
+    ctx = Context(w.volume)
+    ctx.set_phi(w.phi0, w.band)
+    ref = O.State(w.volume)
+    ref.set_phi(w.phi0, w.band)
+    for i, (fuse, iters) in enumerate((("1", 9), ("0", 7), ("1", 12), ("0", 1), ("1", 1), ("1", 20), ("0", 5))):
+        _say(f"call {i}: LSSEG_FUSE={fuse}, {iters} iterations")
+        monkeypatch.setenv("LSSEG_FUSE", fuse)
+        ctx.iterate(w.T, w.eps, w.alpha, w.dt, iters)
+        ref.iterate(w.T, w.eps, w.alpha, w.dt, iters)
+        _assert_same_state(ctx, ref, f"call {i}")
+    ctx.close()
+
+
 @pytest.mark.parametrize("pack", ["1", "0"], ids=["packed-counts", "one-count-per-word"])
 def test_compaction_tile_sizes_agree(monkeypatch, pack):
     """Few-path compactions take tiles of 1024 list positions, or of 256 for
