@@ -80,6 +80,8 @@ struct lsseg_ctx {
     bool checks = false;              // LSSEG_CHECKS: verify the list after every compaction
     bool defer_keep = false;          // LSSEG_DEFER_KEEP: the keep step without a look-back, the next
                                       // update rebuilding the traversal (single volumes)
+    uint32_t stamp = 0;               // the device's DevCtl::stamp at the next marking (see
+                                      // stamp_reserve); 0: the records are cleared first
     std::vector<hipEvent_t> prof_events;
     size_t prof_next = 0;
     std::vector<void*> allocs;
@@ -306,10 +308,28 @@ void launch_tail(lsseg_ctx* c, int q)
         launch(c, c->keep_fn, c->grid, q);
 }
 
+// Before `count` more markings (issued by the caller after this, outside any
+// graph capture): the markings write the stamp DevCtl::stamp into the
+// emission records and every compaction advances it, so the stamps of the
+// markings since the last reset are distinct as long as none passes 255.  If
+// the next `count` would, every record is cleared and the stamp restarted.
+// c->stamp follows the device's stamp (each marking is followed by exactly
+// one compaction); it is reset to 0 after a device error.
+void stamp_reserve(lsseg_ctx* c, uint32_t count)
+{
+    if (c->stamp == 0 || c->stamp + count - 1u > 255u) {
+        hipLaunchKernelGGL(k_records_reset, dim3(dense_grid(c->a.g.N)), dim3(kBlock), 0, c->stream, c->a.C, c->a.g.N,
+                           c->a.ctl);
+        c->stamp = 1;
+    }
+    c->stamp += count;
+}
+
 // one complete iteration over list[q] (update, marking, keep, compaction into
 // list[q ^ 1]); the context is at rest afterwards
 void launch_iteration(lsseg_ctx* c, int q)
 {
+    stamp_reserve(c, 1);
     launch_head(c, q, false);
     launch_tail(c, q);
     launch(c, (const void*)&k_compact<kCompact>, c->grid, q);
@@ -393,6 +413,7 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q, uint32_t* skipped
         lsseg_ctx* c;
         ~DeferOff() { c->a.defer = 0; }  // the other paths (records, re-initialisation) place their traversal
     } defer_off{c};
+    stamp_reserve(c, 1);
     launch_head(c, q, false);
     launch_tail(c, q);
     q ^= 1;
@@ -432,6 +453,7 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q, uint32_t* skipped
                 return fail(c, LSSEG_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
             }
         }
+        stamp_reserve(c, (uint32_t)chunk);
         HIP_TRY(c, hipGraphLaunch(ge, c->stream));
         HIP_TRY(c, hipMemcpyAsync(&c->h_watch[slot], &c->a.ctl->n[q], sizeof(uint32_t), hipMemcpyDeviceToHost,
                                   c->stream));
@@ -440,6 +462,7 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q, uint32_t* skipped
         count -= chunk;
     }
     for (int i = 0; i < count; ++i) {
+        stamp_reserve(c, 1);
         launch_head(c, q, true);
         launch_tail(c, q);
         q ^= 1;
@@ -976,6 +999,7 @@ lsseg_status lsseg_iterate(lsseg_ctx* c, const lsseg_params* prm, int32_t iters,
         c->h_ctl->epoch += 2;
         hipLaunchKernelGGL(k_fill_u8, dim3(dense_grid(c->a.g.N)), dim3(kBlock), 0, c->stream, c->a.keep, c->a.g.N,
                            (uint8_t)0);
+        c->stamp = 0;  // the records are cleared before the next marking
     }
     c->h_ctl->error = 0;
     c->h_ctl->few_iters = 0;
@@ -1558,6 +1582,7 @@ lsseg_status lsseg_segment_noalloc(const lsseg_volume_desc* desc, const void* sr
                        (uint64_t)kDirCounters / 2, (uint64_t)0);
     DevCtl c0{};
     c0.stats_cap = (uint32_t)iters;
+    c0.stamp = 1;  // set_phi leaves flags only at the list's voxels, which the first update clears
     hipLaunchKernelGGL(k_put_params_ctl, dim3(1), dim3(64), 0, s, (DevParams*)a.prm, P, a.ctl, c0);
     // lsseg_set_phi: both phi buffers, export byte, band flags, then the
     // dense compaction of the flags into list[0]
@@ -1575,7 +1600,13 @@ lsseg_status lsseg_segment_noalloc(const lsseg_volume_desc* desc, const void* sr
     // lsseg_iterate: the update, marking and keep step of each iteration, each
     // compaction in the next update's launch, the last one on its own
     int q = 0;
+    uint32_t stamp = 1;  // as stamp_reserve
     for (int32_t i = 0; i < iters; ++i) {
+        if (stamp > 255u) {
+            hipLaunchKernelGGL(k_records_reset, dim3(dg), dim3(kBlock), 0, s, a.C, N, a.ctl);
+            stamp = 1;
+        }
+        ++stamp;
         void* args[] = {(void*)&a, (void*)&q};
         const void* head = i ? update_fused_kernel(desc->dtype, phi8) : update_kernel(desc->dtype, phi8);
         (void)hipLaunchKernel(head, dim3(i ? dev.grid : dev.sgrid), dim3(kBlock), args, 0, s);
@@ -1741,8 +1772,23 @@ void launch_batch_compact(lsseg_batch* b, int q)
     (void)hipLaunchKernel((const void*)&k_compact_batch, dim3(b->grid), dim3(kBlock), kargs, 0, b->stream);
 }
 
+// stamp_reserve for every volume of the batch (each has its own records and
+// stamp; a volume may also have been iterated on its own)
+void batch_stamp_reserve(lsseg_batch* b, uint32_t count)
+{
+    for (lsseg_ctx* c : b->vols) {
+        if (c->stamp == 0 || c->stamp + count - 1u > 255u) {
+            hipLaunchKernelGGL(k_records_reset, dim3(dense_grid(c->a.g.N)), dim3(kBlock), 0, b->stream, c->a.C,
+                               c->a.g.N, c->a.ctl);
+            c->stamp = 1;
+        }
+        c->stamp += count;
+    }
+}
+
 void launch_batch_iteration(lsseg_batch* b, int q)
 {
+    batch_stamp_reserve(b, 1);
     launch_batch_head(b, q, false);
     launch_batch_tail(b, q);
     launch_batch_compact(b, q);
@@ -1757,6 +1803,7 @@ lsseg_status launch_batch_iterations(lsseg_batch* b, int count, int q)
         for (int i = 0; i < count; ++i) launch_batch_iteration(b, q ^ (i & 1));
         return LSSEG_OK;
     }
+    batch_stamp_reserve(b, 1);
     launch_batch_head(b, q, false);
     launch_batch_tail(b, q);
     q ^= 1;
@@ -1780,10 +1827,12 @@ lsseg_status launch_batch_iterations(lsseg_batch* b, int count, int q)
                 return bfail(b, LSSEG_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
             }
         }
+        batch_stamp_reserve(b, (uint32_t)chunk);
         BHIP_TRY(b, hipGraphLaunch(ge, b->stream));
         count -= chunk;
     }
     for (int i = 0; i < count; ++i) {
+        batch_stamp_reserve(b, 1);
         launch_batch_head(b, q, true);
         launch_batch_tail(b, q);
         q ^= 1;
@@ -2051,6 +2100,7 @@ lsseg_status lsseg_batch_iterate(lsseg_batch* b, const lsseg_params* params, int
             const KArgs& a = b->vols[v]->a;
             hipLaunchKernelGGL(k_fill_u8, dim3(dense_grid(a.g.N)), dim3(kBlock), 0, b->stream, a.keep, a.g.N,
                                (uint8_t)0);
+            b->vols[v]->stamp = 0;
         }
         h.error = 0;
         h.few_iters = 0;
@@ -2578,6 +2628,7 @@ lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
         c->h_ctl->epoch += 2;
         hipLaunchKernelGGL(k_fill_u8, dim3(dense_grid(c->a.g.N)), dim3(kBlock), 0, c->stream, c->a.keep, c->a.g.N,
                            (uint8_t)0);
+        c->stamp = 0;  // the records are cleared before the next marking
     }
     c->h_ctl->error = 0;
     c->h_ctl->few_iters = 0;
@@ -2594,6 +2645,7 @@ lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
     const void* keep_fn = !defer ? c->keep_fn
                           : c->a.keep_items == 2 ? (const void*)&k_keep_defer<2> : (const void*)&k_keep_defer<1>;
     for (int k = 0; k < iters; ++k) {
+        stamp_reserve(c, 1);
         void* args[] = {(void*)&c->a, (void*)&q, (void*)&ro};
         const void* head = k == 0 ? update_record_kernel(c->dtype, false)
                            : defer ? update_record_defer_kernel(c->dtype) : update_record_kernel(c->dtype, true);

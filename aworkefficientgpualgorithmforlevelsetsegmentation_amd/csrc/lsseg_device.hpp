@@ -30,13 +30,16 @@
 // (chunk aggregates and totals of groups of 64 chunks, wave_group_scan)
 // instead of a chain of inclusive prefixes.
 //
-// Emission record C (8 bytes per voxel): byte e of C[u] is 1 iff u was emitted
-// into candidate list e (0:-z 1:-y 2:-x 3:self 4:+x 5:+y 6:+z,
-// UpdateActiveElementsVolume.cu:168-174).  Over the list-major concatenation of
-// the seven lists, first-wins keeps candidate (e, p) iff the lowest set byte
-// of C[u] is e; the dense path keeps the kept voxels (marked in memb by the
-// keep step) in tiled order.  The voxels with C[u] != 0 are exactly the next
-// list, so the next update clears C at the voxels it visits.
+// Emission record C (8 bytes per voxel): byte e of C[u] holds the iteration's
+// stamp iff u was emitted into candidate list e (0:-z 1:-y 2:-x 3:self 4:+x
+// 5:+y 6:+z, UpdateActiveElementsVolume.cu:168-174).  Over the list-major
+// concatenation of the seven lists, first-wins keeps candidate (e, p) iff the
+// lowest byte of C[u] holding the stamp is e; the dense path keeps the kept
+// voxels (marked in memb by the keep step) in tiled order.  The stamp (1..255,
+// DevCtl::stamp) advances with every compaction, so the bytes of earlier
+// iterations need no clearing: the host clears the whole record and restarts
+// the stamp before it would pass 255.  The first update of a call still
+// clears C at its voxels, which removes the flags an initialisation leaves.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -233,6 +236,8 @@ struct DevCtl {
     uint32_t stats_cap;
     uint32_t error;
     uint32_t few_iters, many_iters;
+    uint32_t stamp;       // the marking's byte value in C (1..255); +1 per compaction
+    uint32_t pad_;
 };
 
 struct DevParams {
@@ -705,8 +710,10 @@ __device__ inline float* record_undo(const KArgs& a, int q, const RecordOut& r)
 // the update of active voxel u at traversal position p.  The byte moved[p]
 // holds the temporal flag in bit 0 and, with kRec (a recording run of the
 // differentiable form, FP32 phi), the update's decisions in bits 1-7; the
-// value of W it overwrites goes to undo[p] (when undo is not null).
-template <typename SrcT, bool kFloatSrc, typename PhiT, bool kRec = false>
+// value of W it overwrites goes to undo[p] (when undo is not null).  kClear:
+// the record C[u] is cleared (the first update of a call, after which C holds
+// only stamped bytes).
+template <typename SrcT, bool kFloatSrc, typename PhiT, bool kRec = false, bool kClear = true>
 __device__ inline void update_at(const KArgs& a, const DevParams& P, const PhiT* __restrict__ R,
                                  PhiT* __restrict__ W, uint32_t u, uint64_t p, float* __restrict__ undo = nullptr)
 {
@@ -721,7 +728,7 @@ __device__ inline void update_at(const KArgs& a, const DevParams& P, const PhiT*
         phi_load(R, nb[6]), density_factor<SrcT, kFloatSrc>(a, P, u), &d);
     phi_store(W, u, nv);
     a.E[u] = export_byte(nv);
-    a.C[u] = 0ull;  // the record of the candidates that produced this list
+    if constexpr (kClear) a.C[u] = 0ull;  // flags of an initialised or injected list
     // the temporal flag |phi_W - phi_R| > 0.008 of the value as computed (before
     // a fixed-point store quantises it; UpdateLevelSetVolume.cu:327-331), in
     // traversal order for the marking
@@ -739,7 +746,7 @@ __device__ inline void update_at(const KArgs& a, const DevParams& P, const PhiT*
 // the temporal flag `moved`; returns its emission mask
 template <typename PhiT>
 __device__ inline uint32_t mark_at(const KArgs& a, bool ref, bool moved, const PhiT* __restrict__ W,
-                                   uint8_t* __restrict__ Cb, uint32_t u, uint32_t& pos)
+                                   uint8_t* __restrict__ Cb, uint32_t u, uint32_t& pos, uint8_t stamp)
 {
     uint32_t m = 0;
     // the position and every operand are loaded together (all addresses are
@@ -763,7 +770,7 @@ __device__ inline uint32_t mark_at(const KArgs& a, bool ref, bool moved, const P
         if (m) {
 #pragma unroll
             for (int e = 0; e < 7; ++e)
-                if ((m >> e) & 1u) Cb[(uint64_t)nb[e] * 8u + (uint32_t)e] = 1;
+                if ((m >> e) & 1u) Cb[(uint64_t)nb[e] * 8u + (uint32_t)e] = stamp;
         }
     }
     return m;
@@ -845,6 +852,7 @@ __device__ inline void compact_finish(const KArgs& a, int q, const CompactPlan& 
     } else if (c.n_dst != m) {
         atomicOr(&ctl->error, kErrCount);
     }
+    if constexpr (kMode == kCompact) ctl->stamp = ctl->stamp + 1u;  // no marking reads it in this launch
     ctl->epoch = epoch + 1;
 }
 
@@ -1340,9 +1348,11 @@ __device__ __attribute__((always_inline)) inline void keep_voxels(const KArgs& a
 template <int kI>
 __device__ __attribute__((always_inline)) inline void keep_decide(const KArgs& a, uint32_t n, bool dense,
                                                                   uint32_t sub, uint32_t* us, const uint32_t* ms,
-                                                                  uint32_t* kb, const uint32_t* psin = nullptr)
+                                                                  uint32_t* kb, uint32_t stamp,
+                                                                  const uint32_t* psin = nullptr)
 {
     const uint64_t* __restrict__ Ck = a.C;
+    const uint64_t ones = 0x0101010101010101ull, st8 = ones * stamp;
     const uint64_t base = (uint64_t)sub * (kBlock * kI);
     // the positions and the records of the emitted candidates together (other
     // lanes load the voxel's own record and ignore it)
@@ -1367,9 +1377,14 @@ __device__ __attribute__((always_inline)) inline void keep_decide(const KArgs& a
         const uint32_t m = valid ? ms[j] : 0u;
         uint32_t k = m & 1u;
 #pragma unroll
-        for (int e = 1; e < 7; ++e)
-            if (((m >> e) & 1u) && (uint32_t)((__ffsll((unsigned long long)rec[j][e]) - 1) >> 3) == (uint32_t)e)
+        for (int e = 1; e < 7; ++e) {
+            // the bytes equal to the stamp are the zero bytes of x; the lowest
+            // one is flagged exactly (no borrow reaches it from below)
+            const uint64_t x = rec[j][e] ^ st8;
+            const uint64_t z = (x - ones) & ~x & (ones << 7);
+            if (((m >> e) & 1u) && (uint32_t)((__ffsll((unsigned long long)z) - 1) >> 3) == (uint32_t)e)
                 k |= 1u << e;
+        }
         kb[j] = k;
         // the keep bytes are zero between compactions (each few-path
         // compaction clears the ones it read), so only kept lists are stored,
@@ -1466,8 +1481,8 @@ __device__ __attribute__((always_inline)) inline void keep_append(const KArgs& a
 template <int kI>
 __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a, int q, uint32_t n,
                                                                  const ChunkPlan& cp, uint32_t chunk,
-                                                                 uint32_t epoch, KeepShared<kI>& sh, uint32_t* us,
-                                                                 uint32_t* ms)
+                                                                 uint32_t epoch, uint32_t stamp, KeepShared<kI>& sh,
+                                                                 uint32_t* us, uint32_t* ms)
 {
     const bool dense = !path_is_few(n, a.g.N);
     const uint32_t s0 = chunk * cp.r, s1 = min(s0 + cp.r, keep_tiles<kI>(n));
@@ -1478,14 +1493,14 @@ __device__ __attribute__((always_inline)) inline bool keep_chunk(const KArgs& a,
     uint32_t kb[kI], ex[kI];
     uint32_t total = 0;
     if (cp.r == 1) {
-        keep_decide<kI>(a, n, dense, s0, us, ms, kb);
+        keep_decide<kI>(a, n, dense, s0, us, ms, kb, stamp);
         keep_dirs_count(kb, sh, 0);
         total = keep_counts(kb, ex, sh);
         keep_dirs_add(a, q, sh, 0);
     } else {
         for (uint32_t sub = s0; sub < s1; ++sub) {
             keep_voxels<kI>(a, q, sub, us, ms);
-            keep_decide<kI>(a, n, dense, sub, us, ms, kb);
+            keep_decide<kI>(a, n, dense, sub, us, ms, kb, stamp);
             const uint64_t base = (uint64_t)sub * (kBlock * kI);
 #pragma unroll
             for (int j = 0; j < kI; ++j) {
@@ -1533,10 +1548,10 @@ __global__ __launch_bounds__(kBlock) void k_keep(KArgs a, int q)
     __shared__ KeepShared<kI> sh;
     uint32_t us[kI], ms[kI];
     keep_voxels<kI>(a, q, blockIdx.x, us, ms);  // with the count: this workgroup's sub-tile when chunks are single
-    const uint32_t epoch = a.ctl->epoch;
+    const uint32_t epoch = a.ctl->epoch, stamp = a.ctl->stamp;
     const uint32_t n = a.ctl->n[q];
     const ChunkPlan cp = chunk_plan(keep_tiles<kI>(n), gridDim.x);
-    if (blockIdx.x < cp.chunks) (void)keep_chunk<kI>(a, q, n, cp, blockIdx.x, epoch, sh, us, ms);
+    if (blockIdx.x < cp.chunks) (void)keep_chunk<kI>(a, q, n, cp, blockIdx.x, epoch, stamp, sh, us, ms);
     if (cp.chunks == 0 && blockIdx.x == 0 && threadIdx.x < 64)
         keep_finish(a, q, 0, 0, epoch, candidate_share(a), ctl_snapshot(a.ctl));
 }
@@ -1568,13 +1583,14 @@ __device__ inline void walk_list(const KArgs& a, int q, uint32_t first, uint32_t
     }
 }
 
-template <typename SrcT, bool kFloatSrc, typename PhiT, bool kRec = false>
+template <typename SrcT, bool kFloatSrc, typename PhiT, bool kRec = false, bool kClear = true>
 __device__ inline void update_walk(const KArgs& a, int q, uint32_t first = 0, uint32_t count = 0,
                                    float* undo = nullptr)
 {
     const DevParams P = *a.prm;
     walk_list(a, q, first, count, [&](uint32_t u, uint64_t p) {
-        update_at<SrcT, kFloatSrc, PhiT, kRec>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), u, p, undo);
+        update_at<SrcT, kFloatSrc, PhiT, kRec, kClear>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), u, p,
+                                                       undo);
     });
 }
 
@@ -1613,7 +1629,7 @@ __device__ inline void update_fused_body(const KArgs& a, int q, const RecordOut&
         if (c.ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) compact_finish<kCompact>(a, q ^ 1, c, 0, epoch);
         float* undo = nullptr;
         if constexpr (kRec) undo = record_undo(a, q, r);
-        update_walk<SrcT, kFloatSrc, PhiT, kRec>(a, q, gc, gridDim.x - gc, undo);
+        update_walk<SrcT, kFloatSrc, PhiT, kRec, false>(a, q, gc, gridDim.x - gc, undo);
     }
 }
 
@@ -1673,11 +1689,13 @@ __device__ inline void mark_walk(const KArgs& a, int q, const RecordOut& r)
     // a deferred keep step reads the list positions in traversal order from
     // S[q ^ 1], which is free until the next launch rebuilds the traversal
     uint32_t* __restrict__ sp = a.defer ? a.S[q ^ 1] : nullptr;
+    const uint8_t stamp = (uint8_t)a.ctl->stamp;
     uint32_t emitted = 0;
     walk_list(a, q, 0, 0, [&](uint32_t u, uint64_t p) {
         const uint32_t mv = a.moved[p];
         uint32_t pos;
-        const uint32_t m = mark_at<PhiT>(a, ref, (mv & 1u) != 0, phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, u, pos);
+        const uint32_t m =
+            mark_at<PhiT>(a, ref, (mv & 1u) != 0, phi_buf<PhiT>(a, q ^ 1), (uint8_t*)a.C, u, pos, stamp);
         a.emit[p] = (uint8_t)m;
         if (sp) sp[p] = pos;
         emitted += (uint32_t)__popc(m);
@@ -1755,8 +1773,8 @@ __device__ inline void keep_finish_defer(const KArgs& a, uint32_t n, uint64_t c,
 template <int kI>
 __device__ __attribute__((always_inline)) inline void keep_defer_chunk(const KArgs& a, int q, uint32_t n,
                                                                       const ChunkPlan& cp, uint32_t chunk,
-                                                                      KeepShared<kI>& sh, uint32_t* us,
-                                                                      uint32_t* ms, uint32_t* ps)
+                                                                      uint32_t stamp, KeepShared<kI>& sh,
+                                                                      uint32_t* us, uint32_t* ms, uint32_t* ps)
 {
     const bool dense = !path_is_few(n, a.g.N);
     const uint32_t s0 = chunk * cp.r, s1 = min(s0 + cp.r, keep_tiles<kI>(n));
@@ -1767,7 +1785,7 @@ __device__ __attribute__((always_inline)) inline void keep_defer_chunk(const KAr
     uint32_t total = 0;
     for (uint32_t sub = s0; sub < s1; ++sub) {
         if (cp.r > 1) keep_voxels<kI>(a, q, sub, us, ms, ps);
-        keep_decide<kI>(a, n, dense, sub, us, ms, kb, ps);
+        keep_decide<kI>(a, n, dense, sub, us, ms, kb, stamp, ps);
         const uint64_t base = (uint64_t)sub * (kBlock * kI);
 #pragma unroll
         for (int j = 0; j < kI; ++j) {
@@ -1793,9 +1811,9 @@ __global__ __launch_bounds__(kBlock) void k_keep_defer(KArgs a, int q)
     __shared__ KeepShared<kI> sh;
     uint32_t us[kI], ms[kI], ps[kI];
     keep_voxels<kI>(a, q, blockIdx.x, us, ms, ps);  // with the count: this workgroup's sub-tile when chunks are single
-    const uint32_t n = a.ctl->n[q];
+    const uint32_t n = a.ctl->n[q], stamp = a.ctl->stamp;
     const ChunkPlan cp = chunk_plan(keep_tiles<kI>(n), gridDim.x);
-    if (blockIdx.x < cp.chunks) keep_defer_chunk<kI>(a, q, n, cp, blockIdx.x, sh, us, ms, ps);
+    if (blockIdx.x < cp.chunks) keep_defer_chunk<kI>(a, q, n, cp, blockIdx.x, stamp, sh, us, ms, ps);
     if (cp.chunks == 0 && blockIdx.x == 0 && threadIdx.x < 64) keep_finish_defer(a, 0, candidate_share(a), ctl_snapshot(a.ctl));
 }
 
@@ -1863,10 +1881,11 @@ __device__ __attribute__((always_inline)) inline void append_chunk(const KArgs& 
             const uint32_t v = sh.tgt[i];
             Sn[run + i] = v;
             if constexpr (kRec)
-                update_at<SrcT, kFloatSrc, PhiT, true>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), v, run + i,
-                                                       run + i < undo_room ? undo : nullptr);
+                update_at<SrcT, kFloatSrc, PhiT, true, false>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), v,
+                                                              run + i, run + i < undo_room ? undo : nullptr);
             else if constexpr (kUpd)
-                update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), v, run + i);
+                update_at<SrcT, kFloatSrc, PhiT, false, false>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), v,
+                                                               run + i);
         }
         run += t;
         __syncthreads();
@@ -2023,8 +2042,8 @@ __device__ inline uint32_t batch_find(const uint32_t* s_pre, uint32_t count, uin
 }
 
 // the update of every volume over its S[q]: work items of kBlock positions,
-// volume after volume, item j taken by slot j mod nslots
-template <typename SrcT, bool kFloatSrc, typename PhiT>
+// volume after volume, item j taken by slot j mod nslots (kClear: see update_at)
+template <typename SrcT, bool kFloatSrc, typename PhiT, bool kClear>
 __device__ inline void update_batch_items(const BatchArgs& b, int q, uint32_t slot, uint32_t nslots, uint32_t* s_pre,
                                           uint32_t* s_tmp)
 {
@@ -2036,7 +2055,8 @@ __device__ inline void update_batch_items(const BatchArgs& b, int q, uint32_t sl
         const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
         if (p < b.ctls[v].n[q]) {
             const DevParams P = *a.prm;
-            update_at<SrcT, kFloatSrc, PhiT>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1), a.S[q][p], p);
+            update_at<SrcT, kFloatSrc, PhiT, false, kClear>(a, P, phi_buf<PhiT>(a, q), phi_buf<PhiT>(a, q ^ 1),
+                                                            a.S[q][p], p);
         }
     }
 }
@@ -2046,7 +2066,7 @@ __global__ __launch_bounds__(kBlock) void k_update_batch(BatchArgs b, int q)
 {
     __shared__ uint32_t s_pre[kMaxBatch + 1];
     __shared__ uint32_t s_tmp[kBlock / 64];
-    update_batch_items<SrcT, kFloatSrc, PhiT>(b, q, blockIdx.x, gridDim.x, s_pre, s_tmp);
+    update_batch_items<SrcT, kFloatSrc, PhiT, true>(b, q, blockIdx.x, gridDim.x, s_pre, s_tmp);
 }
 
 template <typename PhiT>
@@ -2065,7 +2085,7 @@ __global__ __launch_bounds__(kBlock) void k_mark_batch(BatchArgs b, int q)
         if (p < b.ctls[v].n[q]) {
             uint32_t pos;
             const uint32_t m = mark_at<PhiT>(a, a.prm->refMode != 0, (a.moved[p] & 1u) != 0, phi_buf<PhiT>(a, q ^ 1),
-                                             (uint8_t*)a.C, a.S[q][p], pos);
+                                             (uint8_t*)a.C, a.S[q][p], pos, (uint8_t)b.ctls[v].stamp);
             a.emit[p] = (uint8_t)m;
             if (b.defer) a.S[q ^ 1][p] = pos;  // for the deferred keep step (see mark_walk)
             emitted = (uint32_t)__popc(m);
@@ -2102,12 +2122,13 @@ __device__ __attribute__((always_inline)) inline void keep_batch_body(const Batc
         const ChunkPlan cp{r, (keep_tiles<kI>(n) + r - 1) / r};
         const uint32_t chunk = j - s_pre[v];
         uint32_t us[kI], ms[kI], ps[kI];
+        const uint32_t stamp = b.ctls[v].stamp;
         if (b.defer) {
             keep_voxels<kI>(a, q, chunk * r, us, ms, ps);
-            keep_defer_chunk<kI>(a, q, n, cp, chunk, sh, us, ms, ps);
+            keep_defer_chunk<kI>(a, q, n, cp, chunk, stamp, sh, us, ms, ps);
         } else {
             keep_voxels<kI>(a, q, chunk * r, us, ms);
-            (void)keep_chunk<kI>(a, q, n, cp, chunk, b.ctls[v].epoch, sh, us, ms);
+            (void)keep_chunk<kI>(a, q, n, cp, chunk, b.ctls[v].epoch, stamp, sh, us, ms);
         }
     }
 }
@@ -2229,7 +2250,7 @@ __global__ __launch_bounds__(kBlock) void k_update_fused_batch(BatchArgs b, int 
     else if (b.defer)
         append_batch_items<SrcT, kFloatSrc, PhiT, true>(b, q, blockIdx.x - gc, gridDim.x - gc, s_pre, s_tmp, sh.d);
     else
-        update_batch_items<SrcT, kFloatSrc, PhiT>(b, q, blockIdx.x - gc, gridDim.x - gc, s_pre, s_tmp);
+        update_batch_items<SrcT, kFloatSrc, PhiT, false>(b, q, blockIdx.x - gc, gridDim.x - gc, s_pre, s_tmp);
 }
 
 #define LSSEG_FOR_EACH(p, count)                                                           \
@@ -2445,6 +2466,14 @@ __global__ void k_fill_u8(uint8_t* p, uint64_t count, uint8_t v)
 __global__ void k_fill_u64(uint64_t* p, uint64_t count, uint64_t v)
 {
     LSSEG_GRID_STRIDE(i, count) p[i] = v;
+}
+
+// every emission record cleared and the stamp restarted at 1 (before the
+// stamp would pass 255; see the emission record C)
+__global__ void k_records_reset(uint64_t* C, uint64_t count, DevCtl* ctl)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->stamp = 1u;
+    LSSEG_GRID_STRIDE(i, count) C[i] = 0ull;
 }
 
 // pad + tile the linear source (LoadVolumeHostPadded, Segmenter.cpp:1131-1204)

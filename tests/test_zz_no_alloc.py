@@ -100,7 +100,10 @@ def _same(a, b, where):
 def test_noalloc_run_equals_the_allocating_entry_points():
     big = synth.ellipsoids(edge=96, iterations=150)          # 96^3: 884,736 padded voxels
     small = synth.vessels(edge=48, n_tubes=3, iterations=90)  # smaller, another source type
-    for i, w in enumerate((big, small)):
+    # more than 255 markings: the run clears its emission records and
+    # restarts their stamp on the way
+    long = synth.ellipsoids(edge=48, iterations=300, seed_offset=5)
+    for i, w in enumerate((big, small, long)):
         _say(f"call {i}: {w.name} {w.volume.shape} {w.volume.dtype} {w.iterations} iterations")
         got = _noalloc(w, check_memory=i > 0)
         want = _reference(w)
