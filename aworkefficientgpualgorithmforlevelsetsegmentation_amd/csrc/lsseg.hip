@@ -312,14 +312,16 @@ void launch_tail(lsseg_ctx* c, int q)
 // graph capture): the markings write the stamp DevCtl::stamp into the
 // emission records and every compaction advances it, so the stamps of the
 // markings since the last reset are distinct as long as none passes 255.  If
-// the next `count` would, every record is cleared and the stamp restarted.
-// c->stamp follows the device's stamp (each marking is followed by exactly
-// one compaction); it is reset to 0 after a device error.
-void stamp_reserve(lsseg_ctx* c, uint32_t count)
+// the next `count` would, every record is cleared and the stamp restarted so
+// that the next marking writes 1; `pending`: the compaction of the last
+// marking runs after this (in the next fused update launch) and advances the
+// restarted stamp first.  c->stamp is the stamp of the next marking (each
+// marking is followed by exactly one compaction); 0 after a device error.
+void stamp_reserve(lsseg_ctx* c, uint32_t count, bool pending)
 {
     if (c->stamp == 0 || c->stamp + count - 1u > 255u) {
         hipLaunchKernelGGL(k_records_reset, dim3(dense_grid(c->a.g.N)), dim3(kBlock), 0, c->stream, c->a.C, c->a.g.N,
-                           c->a.ctl);
+                           c->a.ctl, pending ? 0u : 1u);
         c->stamp = 1;
     }
     c->stamp += count;
@@ -329,7 +331,7 @@ void stamp_reserve(lsseg_ctx* c, uint32_t count)
 // list[q ^ 1]); the context is at rest afterwards
 void launch_iteration(lsseg_ctx* c, int q)
 {
-    stamp_reserve(c, 1);
+    stamp_reserve(c, 1, false);
     launch_head(c, q, false);
     launch_tail(c, q);
     launch(c, (const void*)&k_compact<kCompact>, c->grid, q);
@@ -413,7 +415,7 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q, uint32_t* skipped
         lsseg_ctx* c;
         ~DeferOff() { c->a.defer = 0; }  // the other paths (records, re-initialisation) place their traversal
     } defer_off{c};
-    stamp_reserve(c, 1);
+    stamp_reserve(c, 1, false);
     launch_head(c, q, false);
     launch_tail(c, q);
     q ^= 1;
@@ -453,7 +455,7 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q, uint32_t* skipped
                 return fail(c, LSSEG_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
             }
         }
-        stamp_reserve(c, (uint32_t)chunk);
+        stamp_reserve(c, (uint32_t)chunk, true);
         HIP_TRY(c, hipGraphLaunch(ge, c->stream));
         HIP_TRY(c, hipMemcpyAsync(&c->h_watch[slot], &c->a.ctl->n[q], sizeof(uint32_t), hipMemcpyDeviceToHost,
                                   c->stream));
@@ -462,7 +464,7 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q, uint32_t* skipped
         count -= chunk;
     }
     for (int i = 0; i < count; ++i) {
-        stamp_reserve(c, 1);
+        stamp_reserve(c, 1, true);
         launch_head(c, q, true);
         launch_tail(c, q);
         q ^= 1;
@@ -496,6 +498,7 @@ const char* device_error_text(uint32_t err)
            : (err & kErrCheck) ? "list check failed: an entry outside the volume, repeated, or missing from the "
                                  "traversal"
            : (err & kErrCount) ? "the compaction's count differs from the keep step's"
+           : (err & kErrStamp) ? "a marking found its emission stamp outside 1..255"
                                : "list overflow";
 }
 
@@ -1602,8 +1605,8 @@ lsseg_status lsseg_segment_noalloc(const lsseg_volume_desc* desc, const void* sr
     int q = 0;
     uint32_t stamp = 1;  // as stamp_reserve
     for (int32_t i = 0; i < iters; ++i) {
-        if (stamp > 255u) {
-            hipLaunchKernelGGL(k_records_reset, dim3(dg), dim3(kBlock), 0, s, a.C, N, a.ctl);
+        if (stamp > 255u) {  // iteration i > 0: the last compaction is in this iteration's update launch
+            hipLaunchKernelGGL(k_records_reset, dim3(dg), dim3(kBlock), 0, s, a.C, N, a.ctl, 0u);
             stamp = 1;
         }
         ++stamp;
@@ -1774,12 +1777,12 @@ void launch_batch_compact(lsseg_batch* b, int q)
 
 // stamp_reserve for every volume of the batch (each has its own records and
 // stamp; a volume may also have been iterated on its own)
-void batch_stamp_reserve(lsseg_batch* b, uint32_t count)
+void batch_stamp_reserve(lsseg_batch* b, uint32_t count, bool pending)
 {
     for (lsseg_ctx* c : b->vols) {
         if (c->stamp == 0 || c->stamp + count - 1u > 255u) {
             hipLaunchKernelGGL(k_records_reset, dim3(dense_grid(c->a.g.N)), dim3(kBlock), 0, b->stream, c->a.C,
-                               c->a.g.N, c->a.ctl);
+                               c->a.g.N, c->a.ctl, pending ? 0u : 1u);
             c->stamp = 1;
         }
         c->stamp += count;
@@ -1788,7 +1791,7 @@ void batch_stamp_reserve(lsseg_batch* b, uint32_t count)
 
 void launch_batch_iteration(lsseg_batch* b, int q)
 {
-    batch_stamp_reserve(b, 1);
+    batch_stamp_reserve(b, 1, false);
     launch_batch_head(b, q, false);
     launch_batch_tail(b, q);
     launch_batch_compact(b, q);
@@ -1803,7 +1806,7 @@ lsseg_status launch_batch_iterations(lsseg_batch* b, int count, int q)
         for (int i = 0; i < count; ++i) launch_batch_iteration(b, q ^ (i & 1));
         return LSSEG_OK;
     }
-    batch_stamp_reserve(b, 1);
+    batch_stamp_reserve(b, 1, false);
     launch_batch_head(b, q, false);
     launch_batch_tail(b, q);
     q ^= 1;
@@ -1827,12 +1830,12 @@ lsseg_status launch_batch_iterations(lsseg_batch* b, int count, int q)
                 return bfail(b, LSSEG_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
             }
         }
-        batch_stamp_reserve(b, (uint32_t)chunk);
+        batch_stamp_reserve(b, (uint32_t)chunk, true);
         BHIP_TRY(b, hipGraphLaunch(ge, b->stream));
         count -= chunk;
     }
     for (int i = 0; i < count; ++i) {
-        batch_stamp_reserve(b, 1);
+        batch_stamp_reserve(b, 1, true);
         launch_batch_head(b, q, true);
         launch_batch_tail(b, q);
         q ^= 1;
@@ -2645,7 +2648,7 @@ lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
     const void* keep_fn = !defer ? c->keep_fn
                           : c->a.keep_items == 2 ? (const void*)&k_keep_defer<2> : (const void*)&k_keep_defer<1>;
     for (int k = 0; k < iters; ++k) {
-        stamp_reserve(c, 1);
+        stamp_reserve(c, 1, k > 0);
         void* args[] = {(void*)&c->a, (void*)&q, (void*)&ro};
         const void* head = k == 0 ? update_record_kernel(c->dtype, false)
                            : defer ? update_record_defer_kernel(c->dtype) : update_record_kernel(c->dtype, true);

@@ -70,7 +70,7 @@ constexpr int kPosTile = kBlock * kPosItems;
 constexpr uint32_t kSpecTiles = 256;          // default of Geom::spec_tiles
 constexpr uint32_t kSpinLimit = 1u << 24;     // bounded look-back wait
 
-enum : uint32_t { kErrSpin = 1u, kErrOverflow = 2u, kErrCount = 8u, kErrCheck = 16u };
+enum : uint32_t { kErrSpin = 1u, kErrOverflow = 2u, kErrCount = 8u, kErrCheck = 16u, kErrStamp = 32u };
 
 enum KMode : int {
     kUpdate = 0,   // update of the current list
@@ -1689,7 +1689,9 @@ __device__ inline void mark_walk(const KArgs& a, int q, const RecordOut& r)
     // a deferred keep step reads the list positions in traversal order from
     // S[q ^ 1], which is free until the next launch rebuilds the traversal
     uint32_t* __restrict__ sp = a.defer ? a.S[q ^ 1] : nullptr;
-    const uint8_t stamp = (uint8_t)a.ctl->stamp;
+    const uint32_t stamp32 = a.ctl->stamp;
+    const uint8_t stamp = (uint8_t)stamp32;
+    if (stamp32 - 1u > 254u && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl->error, kErrStamp);
     uint32_t emitted = 0;
     walk_list(a, q, 0, 0, [&](uint32_t u, uint64_t p) {
         const uint32_t mv = a.moved[p];
@@ -2082,6 +2084,7 @@ __global__ __launch_bounds__(kBlock) void k_mark_batch(BatchArgs b, int q)
         const KArgs& a = b.vols[v];
         const uint64_t p = (uint64_t)(j - s_pre[v]) * kBlock + threadIdx.x;
         uint32_t emitted = 0;
+        if (threadIdx.x == 0 && b.ctls[v].stamp - 1u > 254u) atomicOr(&b.ctls[v].error, kErrStamp);
         if (p < b.ctls[v].n[q]) {
             uint32_t pos;
             const uint32_t m = mark_at<PhiT>(a, a.prm->refMode != 0, (a.moved[p] & 1u) != 0, phi_buf<PhiT>(a, q ^ 1),
@@ -2468,11 +2471,12 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t count, uint64_t v)
     LSSEG_GRID_STRIDE(i, count) p[i] = v;
 }
 
-// every emission record cleared and the stamp restarted at 1 (before the
-// stamp would pass 255; see the emission record C)
-__global__ void k_records_reset(uint64_t* C, uint64_t count, DevCtl* ctl)
+// every emission record cleared and the stamp restarted (before it would pass
+// 255; see the emission record C): at `start`, 1, or 0 when the compaction of
+// the last marking is still to run and advances it to 1
+__global__ void k_records_reset(uint64_t* C, uint64_t count, DevCtl* ctl, uint32_t start)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->stamp = 1u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->stamp = start;
     LSSEG_GRID_STRIDE(i, count) C[i] = 0ull;
 }
 

@@ -11,7 +11,7 @@ import pytest
 
 import oracle as O
 from aworkefficientgpualgorithmforlevelsetsegmentation_amd import engine, synth
-from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import Context, LssegError
+from aworkefficientgpualgorithmforlevelsetsegmentation_amd.engine import Batch, Context, LssegError
 
 pytestmark = pytest.mark.gpu
 
@@ -236,6 +236,49 @@ def test_calls_alternating_launch_schedules_match_the_oracle(monkeypatch):
         ref.iterate(w.T, w.eps, w.alpha, w.dt, iters)
         _assert_same_state(ctx, ref, f"call {i}")
     ctx.close()
+
+
+def test_direct_and_graph_launches_agree_past_the_stamp_wrap(monkeypatch):
+    """600 iterations of the phantom, whose list stays non-empty, launched
+    three ways: replayed graphs of 32 iterations, one launch at a time
+    (LSSEG_GRAPH=0), and one launch at a time with per-launch events.  Each
+    marking stamps the emission records, and the records are cleared and the
+    stamp restarted twice in the call, between launches of either kind, with
+    the previous compaction still to run.  Counts and states agree, and so
+    does the batched form with and without graphs."""
+    w = synth.phantom(iterations=600)
+    out = {}
+    for name in ("graphs", "direct", "profiled"):
+        _say(f"phantom, 600 iterations, {name}")
+        monkeypatch.setenv("LSSEG_GRAPH", "0" if name == "direct" else "1")
+        ctx = Context(w.volume)
+        ctx.set_phi(w.phi0, w.band)
+        if name == "profiled":
+            ctx.iterate_profiled(w.T, w.eps, w.alpha, w.dt, w.iterations)
+        else:
+            ctx.iterate(w.T, w.eps, w.alpha, w.dt, w.iterations)
+        out[name] = (ctx.iteration_counts(), ctx.state())
+        ctx.close()
+    a = out["graphs"]
+    assert int(np.count_nonzero(a[0].n)) == w.iterations, "the list should stay non-empty"
+    for other in ("direct", "profiled"):
+        b = out[other]
+        for f in ("n", "c", "m"):
+            assert np.array_equal(getattr(a[0], f), getattr(b[0], f)), (other, f)
+        for x, y in zip(a[1], b[1]):
+            assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8)), other
+    prm = [(w.T, w.eps, w.alpha, w.dt, O.MODE_REF)]
+    for graph in ("1", "0"):
+        _say(f"phantom as a batch of one, 600 iterations, LSSEG_GRAPH={graph}")
+        monkeypatch.setenv("LSSEG_GRAPH", graph)
+        batch = Batch([w.volume])
+        batch.volume(0).set_phi(w.phi0, w.band)
+        batch.iterate(prm, w.iterations)
+        cb = batch.volume(0).iteration_counts()
+        assert np.array_equal(cb.n, a[0].n) and np.array_equal(cb.c, a[0].c), f"batch, graphs {graph}"
+        r, wr, e, lst = batch.volume(0).state()
+        assert np.array_equal(lst, a[1][3]) and np.array_equal(r.view(np.uint32), a[1][0].view(np.uint32))
+        batch.close()
 
 
 @pytest.mark.parametrize("pack", ["1", "0"], ids=["packed-counts", "one-count-per-word"])
