@@ -281,6 +281,34 @@ def test_direct_and_graph_launches_agree_past_the_stamp_wrap(monkeypatch):
         batch.close()
 
 
+def test_many_calls_of_mixed_schedules_match_one_call(monkeypatch):
+    """600 phantom iterations in 21 calls of uneven length (1 to 97
+    iterations), each with its own schedule: graphs, direct launches
+    (LSSEG_GRAPH=0) or separate launches (LSSEG_FUSE=0).  The stamp restarts
+    land at every offset within a call and a graph chunk, and between calls;
+    the end state equals one graph-replayed call of 600 iterations."""
+    w = synth.phantom(iterations=600)
+    ref = Context(w.volume)
+    ref.set_phi(w.phi0, w.band)
+    ref.iterate(w.T, w.eps, w.alpha, w.dt, w.iterations)
+    want = ref.state()
+    ref.close()
+    sizes = [1, 7, 33, 97, 2, 31, 64, 5, 40, 1, 90, 3, 32, 17, 65, 11, 29, 8, 36, 20, 8]
+    assert sum(sizes) == w.iterations
+    schedules = [("1", "1"), ("0", "1"), ("1", "0")]
+    ctx = Context(w.volume)
+    ctx.set_phi(w.phi0, w.band)
+    for i, k in enumerate(sizes):
+        graph, fuse = schedules[i % 3]
+        monkeypatch.setenv("LSSEG_GRAPH", graph)
+        monkeypatch.setenv("LSSEG_FUSE", fuse)
+        ctx.iterate(w.T, w.eps, w.alpha, w.dt, k)
+    _say(f"{len(sizes)} calls done")
+    for x, y in zip(ctx.state(), want):
+        assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
+    ctx.close()
+
+
 @pytest.mark.parametrize("pack", ["1", "0"], ids=["packed-counts", "one-count-per-word"])
 def test_compaction_tile_sizes_agree(monkeypatch, pack):
     """Few-path compactions take tiles of 1024 list positions, or of 256 for
