@@ -247,3 +247,30 @@ def test_deferred_keep_step_records_the_placing_keep_steps_traversal(monkeypatch
         assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
     for x, y in zip(a[4], b[4]):
         assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
+
+
+def test_record_past_the_stamp_wrap_matches_the_plain_run():
+    """A recording run of 300 phantom iterations, whose list stays non-empty:
+    the record's launches restart the emission stamp once in the call, with
+    the last marking's compaction still pending.  The final state and the
+    per-iteration counts equal a plain run's, and the replay of the record
+    equals the forward."""
+    w = synth.phantom(iterations=300)
+    _say(f"record of the phantom, {w.iterations} iterations")
+    ctx = Context(w.volume)
+    ctx.set_phi(w.phi0, w.band)
+    rec = ctx.fwd_record(w.T, w.eps, w.alpha, w.dt, w.iterations)
+    n, _, _ = rec.lists()
+    plain = Context(w.volume)
+    plain.set_phi(w.phi0, w.band)
+    plain.iterate(w.T, w.eps, w.alpha, w.dt, w.iterations)
+    assert int(np.count_nonzero(n)) == w.iterations
+    assert np.array_equal(np.asarray(n), plain.iteration_counts().n[:w.iterations])
+    for a, b in zip(ctx.state(), plain.state()):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    out = rec.replay(w.phi0, w.T, w.eps, w.alpha, w.dt)
+    assert np.array_equal(out.view(np.uint32), ctx.phi(0).view(np.uint32)), "replay differs from the forward"
+    rec.close()
+    ctx.close()
+    plain.close()
+
