@@ -80,6 +80,7 @@ struct lsseg_ctx {
     bool checks = false;              // LSSEG_CHECKS: verify the list after every compaction
     bool defer_keep = false;          // LSSEG_DEFER_KEEP: the keep step without a look-back, the next
                                       // update rebuilding the traversal (single volumes)
+    uint32_t keep_grid_min = 4096;    // LSSEG_KEEP_GRID: least grid of a deferred keep step
     uint32_t stamp = 0;               // the device's DevCtl::stamp at the next marking (see
                                       // stamp_reserve); 0: the records are cleared first
     std::vector<hipEvent_t> prof_events;
@@ -303,9 +304,20 @@ void launch_tail(lsseg_ctx* c, int q)
     launch(c, mark_kernel(c->phi8), c->sgrid, q);
     if (c->a.defer)
         launch(c, c->keep_fn == (const void*)&k_keep<2> ? (const void*)&k_keep_defer<2> : (const void*)&k_keep_defer<1>,
-               c->grid, q);
+               c->a.keep_grid, q);
     else
         launch(c, c->keep_fn, c->grid, q);
+}
+
+// The grid of a deferred keep step: it waits on nothing, so it need not fit
+// the resident slots as a look-back scan does, and a grid of at least
+// keep_grid_min workgroups keeps its chunks at one sub-tile of kBlock
+// positions up to that many sub-tiles (chunks of several sub-tiles loop
+// through them in one workgroup).  The update launch that rebuilds the
+// traversal plans the same chunks from KArgs::keep_grid.
+uint32_t defer_keep_grid(const lsseg_ctx* c)
+{
+    return std::max<uint32_t>(c->grid, c->keep_grid_min);
 }
 
 // Before `count` more markings (issued by the caller after this, outside any
@@ -409,7 +421,7 @@ lsseg_status launch_iterations(lsseg_ctx* c, int count, int q, uint32_t* skipped
     // a deferred keep step leaves the traversal to the next update launch; the
     // chunks it counts follow the keep step's grid and width
     c->a.defer = c->defer_keep ? 1u : 0u;
-    c->a.keep_grid = c->grid;
+    c->a.keep_grid = defer_keep_grid(c);
     c->a.keep_items = c->keep_fn == (const void*)&k_keep<2> ? 2u : 1u;
     struct DeferOff {
         lsseg_ctx* c;
@@ -786,6 +798,7 @@ lsseg_status create_ctx(const lsseg_volume_desc* desc, const void* host_volume, 
     init_geom(g, (uint32_t)X, (uint32_t)Y, (uint32_t)Z);
     c->small_n_max = g.small_n;
     c->defer_keep = !shared && env_int("LSSEG_DEFER_KEEP", 1) != 0;
+    c->keep_grid_min = (uint32_t)std::max(16, env_int("LSSEG_KEEP_GRID", 4096));
 
     void* p = nullptr;
     const uint64_t ntiles = lookback_states(g);
@@ -2643,7 +2656,7 @@ lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
     // traversal and records by the rebuilt positions, as lsseg_iterate runs it
     const bool defer = c->defer_keep;
     c->a.defer = defer ? 1u : 0u;
-    c->a.keep_grid = c->grid;
+    c->a.keep_grid = defer_keep_grid(c);
     c->a.keep_items = c->keep_fn == (const void*)&k_keep<2> ? 2u : 1u;
     const void* keep_fn = !defer ? c->keep_fn
                           : c->a.keep_items == 2 ? (const void*)&k_keep_defer<2> : (const void*)&k_keep_defer<1>;
@@ -2654,7 +2667,7 @@ lsseg_status record_run(lsseg_ctx* c, lsseg_record* r, int32_t iters)
                            : defer ? update_record_defer_kernel(c->dtype) : update_record_kernel(c->dtype, true);
         (void)hipLaunchKernel(head, dim3(k > 0 ? c->grid : c->sgrid), dim3(kBlock), args, 0, c->stream);
         (void)hipLaunchKernel((const void*)&k_mark_record, dim3(c->sgrid), dim3(kBlock), args, 0, c->stream);
-        launch(c, keep_fn, c->grid, q);
+        launch(c, keep_fn, defer ? c->a.keep_grid : c->grid, q);
         q ^= 1;
     }
     if (iters > 0) {
