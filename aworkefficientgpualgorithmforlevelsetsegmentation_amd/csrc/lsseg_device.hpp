@@ -1807,15 +1807,26 @@ __device__ __attribute__((always_inline)) inline void keep_defer_chunk(const KAr
     if (threadIdx.x < 64 && last) keep_finish_defer(a, n, cshare, snap);
 }
 
+// workgroups of a deferred keep step that load their sub-tile speculatively
+// (the resident slots of the whole chip at full occupancy)
+constexpr uint32_t kKeepSpec = 2048;
+
 template <int kI>
 __global__ __launch_bounds__(kBlock) void k_keep_defer(KArgs a, int q)
 {
     __shared__ KeepShared<kI> sh;
     uint32_t us[kI], ms[kI], ps[kI];
-    keep_voxels<kI>(a, q, blockIdx.x, us, ms, ps);  // with the count: this workgroup's sub-tile when chunks are single
+    // the first kKeepSpec workgroups load their sub-tile with the count (its
+    // chunk when chunks are single); a wider grid's others load it once they
+    // know they have one, so idle workgroups on short lists load nothing
+    const bool spec = blockIdx.x < kKeepSpec;
+    if (spec) keep_voxels<kI>(a, q, blockIdx.x, us, ms, ps);
     const uint32_t n = a.ctl->n[q], stamp = a.ctl->stamp;
     const ChunkPlan cp = chunk_plan(keep_tiles<kI>(n), gridDim.x);
-    if (blockIdx.x < cp.chunks) keep_defer_chunk<kI>(a, q, n, cp, blockIdx.x, stamp, sh, us, ms, ps);
+    if (blockIdx.x < cp.chunks) {
+        if (!spec && cp.r == 1) keep_voxels<kI>(a, q, blockIdx.x, us, ms, ps);
+        keep_defer_chunk<kI>(a, q, n, cp, blockIdx.x, stamp, sh, us, ms, ps);
+    }
     if (cp.chunks == 0 && blockIdx.x == 0 && threadIdx.x < 64) keep_finish_defer(a, 0, candidate_share(a), ctl_snapshot(a.ctl));
 }
 
