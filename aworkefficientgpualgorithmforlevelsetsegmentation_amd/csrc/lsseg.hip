@@ -203,6 +203,19 @@ int min_occupancy(std::initializer_list<const void*> kernels)
     return best;
 }
 
+// x[0] + ... + x[n - 1] by pairwise summation in FP32, whose rounding error
+// grows with log n instead of n
+float pairwise_sum(const float* x, size_t n)
+{
+    if (n <= 8) {
+        float t = 0.0f;
+        for (size_t i = 0; i < n; ++i) t += x[i];
+        return t;
+    }
+    const size_t h = n / 2;
+    return pairwise_sum(x, h) + pairwise_sum(x + h, n - h);
+}
+
 uint32_t dense_grid(uint64_t count)
 {
     const uint64_t b = (count + kBlock - 1) / kBlock;
@@ -1188,31 +1201,34 @@ lsseg_status lsseg_auto_parameters(lsseg_ctx* c, float* T, float* eps, uint64_t*
     case LSSEG_I32: fn = (const void*)&k_inside_moments<int32_t, false>; break;
     default: fn = (const void*)&k_inside_moments<float, true>; break;
     }
-    double* d_acc = nullptr;
-    lsseg_status st;
-    if ((st = dalloc(c, (void**)&d_acc, 2 * sizeof(double) + sizeof(unsigned long long))) != LSSEG_OK) return st;
-    unsigned long long* d_cnt = (unsigned long long*)(d_acc + 2);
-    struct Out {
-        double acc[2];
-        unsigned long long cnt;
-    } h{};
-    hipError_t e = hipMemsetAsync(d_acc, 0, sizeof(Out), c->stream);
     const uint32_t dg = dense_grid(c->a.g.N);
-    double mean = 0.0;
+    void* base = nullptr;
+    lsseg_status st;
+    if ((st = dalloc(c, &base, sizeof(unsigned long long) + (size_t)dg * sizeof(float))) != LSSEG_OK) return st;
+    unsigned long long* d_cnt = (unsigned long long*)base;
+    float* d_part = (float*)((char*)base + sizeof(unsigned long long));  // one partial sum per workgroup
+    std::vector<float> part(dg);
+    unsigned long long cnt = 0;
+    hipError_t e = hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), c->stream);
+    // as the reference: FP32 sums, the mean as sum / (float)count, then the
+    // squared deviations from it
+    float mean = 0.0f, sums[2] = {0.0f, 0.0f};
     for (int mode = 0; mode < 2 && e == hipSuccess; ++mode) {
-        double* acc = d_acc + mode;
-        void* args[] = {(void*)&c->a, (void*)&mode, (void*)&mean, (void*)&acc, (void*)&d_cnt};
+        void* args[] = {(void*)&c->a, (void*)&mode, (void*)&mean, (void*)&d_part, (void*)&d_cnt};
         e = hipLaunchKernel(fn, dim3(dg), dim3(kBlock), args, 0, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(&h, d_acc, sizeof(Out), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(part.data(), d_part, (size_t)dg * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        if (mode == 0 && h.cnt) mean = h.acc[0] / (double)h.cnt;
+        sums[mode] = pairwise_sum(part.data(), dg);
+        if (mode == 0 && cnt) mean = sums[0] / (float)cnt;
     }
-    dfree(c, d_acc);
+    dfree(c, base);
     if (e != hipSuccess) return fail(c, LSSEG_ERR_DEVICE, "auto parameters failed: %s", hipGetErrorString(e));
-    if (count) *count = h.cnt;
-    if (h.cnt < 2) return fail(c, LSSEG_ERR_STATE, "fewer than two voxels inside the segmentation");
-    *T = (float)mean;
-    *eps = (float)std::sqrt(h.acc[1] / (double)(h.cnt - 1));
+    if (count) *count = cnt;
+    if (cnt < 2) return fail(c, LSSEG_ERR_STATE, "fewer than two voxels inside the segmentation");
+    *T = mean;
+    *eps = std::sqrt(sums[1] / ((float)cnt - 1.0f));
     return LSSEG_OK;
 }
 
@@ -2478,7 +2494,7 @@ struct lsseg_record {
     float* work[2] = {nullptr, nullptr};
     float* adj[2] = {nullptr, nullptr};
     float* lin = nullptr;             // unpadded staging
-    double* d_theta = nullptr;
+    float* d_theta = nullptr;
     std::vector<void*> allocs;
 };
 
@@ -2729,7 +2745,7 @@ lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t ite
     const size_t NF = N * sizeof(float);
     RecordLayout lay;
     const size_t o_prm = lay.take(sizeof(DevParams)), o_prm_replay = lay.take(sizeof(DevParams));
-    const size_t o_theta = lay.take(kCountSlots * kCountStride * sizeof(double));
+    const size_t o_theta = lay.take(kCountSlots * kThetaStride * sizeof(float));
     const size_t o_offs = lay.take(((size_t)iters + 1) * sizeof(uint64_t));
     const size_t o_fin = lay.take(2 * NF);
     const size_t o_work = lay.take(4 * NF), o_lin = lay.take((size_t)c->nx * c->ny * c->nz * sizeof(float));
@@ -2738,7 +2754,7 @@ lsseg_status lsseg_fwd_record(lsseg_ctx* c, const lsseg_params* prm, int32_t ite
     char* base = static_cast<char*>(p);
     r->d_prm = (DevParams*)(base + o_prm);
     r->d_prm_replay = (DevParams*)(base + o_prm_replay);
-    r->d_theta = (double*)(base + o_theta);  // per-workgroup slots of dL/dT, dL/deps, dL/dalpha
+    r->d_theta = (float*)(base + o_theta);  // per-workgroup slots of dL/dT, dL/deps, dL/dalpha
     r->offs = (uint64_t*)(base + o_offs);
     for (int i = 0; i < 2; ++i) {
         r->fin[i] = (float*)(base + o_fin + (size_t)i * NF);
@@ -2841,7 +2857,7 @@ lsseg_status lsseg_bwd(lsseg_ctx* c, lsseg_record* r, const float* dphi, float* 
     hipLaunchKernelGGL(k_lin_to_tiled, dim3(dg), dim3(kBlock), 0, c->stream, c->a.g, r->lin, r->adj[0], c->nx, c->ny,
                        c->nz, 0.0f);
     hipLaunchKernelGGL(k_fill_f32, dim3(dg), dim3(kBlock), 0, c->stream, r->adj[1], N, 0.0f);
-    HIP_TRY(c, hipMemsetAsync(r->d_theta, 0, kCountSlots * kCountStride * sizeof(double), c->stream));
+    HIP_TRY(c, hipMemsetAsync(r->d_theta, 0, kCountSlots * kThetaStride * sizeof(float), c->stream));
     const KArgs ra = record_args(c, r->d_prm);
     int ab = 0;  // adj[ab] = Abar, adj[ab ^ 1] = Bbar
     // From the final pair (A_K, B_K) backwards: iteration k read A_k = B_{k+1}
@@ -2859,7 +2875,7 @@ lsseg_status lsseg_bwd(lsseg_ctx* c, lsseg_record* r, const float* dphi, float* 
         const float* A = r->work[rd];
         float* X = r->adj[ab];
         float* Y = r->adj[ab ^ 1];
-        double* th = r->d_theta;
+        float* th = r->d_theta;
         float* Bres = r->work[rd ^ 1];
         const float* U = k > 0 ? r->undo + r->off[k] : nullptr;
         void* args[] = {(void*)&ra, (void*)&V, (void*)&D, (void*)&n, (void*)&A,
@@ -2872,15 +2888,17 @@ lsseg_status lsseg_bwd(lsseg_ctx* c, lsseg_record* r, const float* dphi, float* 
     hipLaunchKernelGGL(k_tiled_sum_to_lin, dim3(dense_grid(cnt)), dim3(kBlock), 0, c->stream, c->a.g, r->adj[0],
                        r->adj[1], r->lin, c->nx, c->ny, c->nz);
     HIP_TRY(c, hipMemcpyAsync(dphi0, r->lin, cnt * sizeof(float), hipMemcpyDefault, c->stream));
-    std::vector<double> slots((size_t)kCountSlots * kCountStride);
-    HIP_TRY(c, hipMemcpyAsync(slots.data(), r->d_theta, slots.size() * sizeof(double), hipMemcpyDeviceToHost,
+    std::vector<float> slots((size_t)kCountSlots * kThetaStride);
+    HIP_TRY(c, hipMemcpyAsync(slots.data(), r->d_theta, slots.size() * sizeof(float), hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    double th[3] = {0, 0, 0};
-    for (int i = 0; i < kCountSlots; ++i)
-        for (int j = 0; j < 3; ++j) th[j] += slots[(size_t)i * kCountStride + j];
-    if (dtheta) memcpy(dtheta, th, sizeof(th));
+    // the slots' FP32 sums, added pairwise (returned widened to double)
+    for (int j = 0; j < 3; ++j) {
+        float col[kCountSlots];
+        for (int i = 0; i < kCountSlots; ++i) col[i] = slots[(size_t)i * kThetaStride + j];
+        if (dtheta) dtheta[j] = (double)pairwise_sum(col, kCountSlots);
+    }
     return LSSEG_OK;
 }
 

@@ -2314,10 +2314,11 @@ __device__ inline float density_replay(const KArgs& a, const DevParams& P, uint3
     }
 }
 
-// block-wide sum of three values into theta[0..2] (double atomics); every
+// block-wide sum of three values into theta[0..2] (FP32 atomics); every
 // thread calls it.  theta is the workgroup's slot of kCountSlots slots
-// kCountStride doubles apart (see kCountSlots), which the host adds up.
-__device__ inline void block_add3(double* theta, double v0, double v1, double v2, double* s_red)
+// kThetaStride floats (128 bytes) apart, which the host adds up.
+constexpr int kThetaStride = 32;
+__device__ inline void block_add3(float* theta, float v0, float v1, float v2, float* s_red)
 {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -2332,9 +2333,8 @@ __device__ inline void block_add3(double* theta, double v0, double v1, double v2
     }
     __syncthreads();
     if (threadIdx.x < 3) {
-        double t = 0.0;
-        for (int w = 0; w < kBlock / 64; ++w) t += s_red[w * 3 + threadIdx.x];
-        if (t != 0.0) atomicAdd(&theta[threadIdx.x], t);
+        const float t = (s_red[threadIdx.x] + s_red[3 + threadIdx.x]) + (s_red[6 + threadIdx.x] + s_red[9 + threadIdx.x]);
+        if (t != 0.0f) atomicAdd(&theta[threadIdx.x], t);
     }
 }
 
@@ -2346,10 +2346,11 @@ template <typename SrcT, bool kFloatSrc>
 __global__ __launch_bounds__(kBlock) void k_adjoint(KArgs a, const uint32_t* __restrict__ V,
                                                     const uint8_t* __restrict__ dec, uint32_t n,
                                                     const float* __restrict__ A, float* __restrict__ X,
-                                                    float* __restrict__ Y, double* theta,
+                                                    float* __restrict__ Y, float* theta,
                                                     float* __restrict__ Bres, const float* __restrict__ undo)
 {
-    __shared__ double s_red[3 * kBlock / 64];
+    __shared__ float s_red[3 * kBlock / 64];
+    static_assert(kBlock == 256, "block_add3 sums four waves");
     const DevParams P = *a.prm;
     const float dd = P.flipDensity ? 1.0f : -1.0f;
     float tT = 0.0f, tE = 0.0f, tA = 0.0f;
@@ -2402,7 +2403,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(KArgs a, const uint32_t* __r
             tE += dspeed * (P.negInv * P.negInv * (sgn * di));
         }
     }
-    block_add3(theta + (blockIdx.x % kCountSlots) * kCountStride, tT, tE, tA, s_red);
+    block_add3(theta + (blockIdx.x % kCountSlots) * kThetaStride, tT, tE, tA, s_red);
 }
 
 // linear unpadded <-> tiled padded
@@ -2617,23 +2618,29 @@ __device__ inline int8_t add_export(int8_t ra, int8_t rb)
 
 // CollectCurrentLevelSetVoxels / CalculateSegmentationParameters
 // (Segmenter.cpp:357-478): source values of the voxels whose export byte is
-// <= 0.  mode 0 accumulates the count and the sum, mode 1 the squared
-// deviations from `mean`; in double, so only the summation order differs from
-// the reference's sequential float sums.
+// <= 0.  mode 0 sums the values and counts them, mode 1 sums the squared
+// deviations from `mean`, in FP32 as the reference does: each thread with a
+// compensated (Kahan) sum, then a tree over the workgroup into partial[block];
+// the host adds the partials pairwise.  So only the summation order differs
+// from the reference's sequential float sums, and the error stays near one
+// rounding instead of growing with the count.
 template <typename SrcT, bool kFloatSrc>
-__global__ __launch_bounds__(kBlock) void k_inside_moments(KArgs a, int mode, double mean, double* acc,
+__global__ __launch_bounds__(kBlock) void k_inside_moments(KArgs a, int mode, float mean, float* partial,
                                                            unsigned long long* count)
 {
-    __shared__ double s_red[kBlock / 64];
+    __shared__ float s_red[kBlock / 64];
     __shared__ unsigned long long s_cnt[kBlock / 64];
-    double v = 0.0;
+    float v = 0.0f, comp = 0.0f;
     unsigned long long c = 0;
     for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < a.g.N;
          t += (uint64_t)gridDim.x * blockDim.x) {
         if (a.E[t] <= 0) {
-            const double x = kFloatSrc ? (double)static_cast<const float*>(a.src)[t]
-                                       : (double)(float)static_cast<const SrcT*>(a.src)[t];
-            v += mode == 0 ? x : (x - mean) * (x - mean);
+            const float x = kFloatSrc ? static_cast<const float*>(a.src)[t] : (float)static_cast<const SrcT*>(a.src)[t];
+            const float d = x - mean;
+            const float y = (mode == 0 ? x : d * d) - comp;
+            const float s = v + y;
+            comp = (s - v) - y;
+            v = s;
             ++c;
         }
     }
@@ -2648,13 +2655,8 @@ __global__ __launch_bounds__(kBlock) void k_inside_moments(KArgs a, int mode, do
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double tv = 0.0;
-        unsigned long long tc = 0;
-        for (int w = 0; w < kBlock / 64; ++w) {
-            tv += s_red[w];
-            tc += s_cnt[w];
-        }
-        if (tv != 0.0) atomicAdd(acc, tv);
+        partial[blockIdx.x] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+        const unsigned long long tc = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
         if (mode == 0 && tc) atomicAdd(count, tc);
     }
 }
