@@ -605,16 +605,14 @@ lsseg_status ensure_stats(lsseg_ctx* c, uint64_t iters)
 }
 
 // Makes list[0] (n entries, already on the device) the current list: rebuilds
-// the membership record C, the positions and the traversal S[0], taken from
-// S_dev (device, n entries) if given, else the list sorted by tiled index.
+// the positions and the traversal S[0], taken from S_dev (device, n entries)
+// if given, else the list sorted by tiled index.  The emission records need
+// nothing: their bytes carry the stamps of earlier markings, which no later
+// marking reuses before the records are cleared.
 lsseg_status inject_list_counts(lsseg_ctx* c, uint32_t n, const uint32_t* S_dev = nullptr)
 {
-    const uint64_t N = c->a.g.N;
     c->q = 0;
-    fill_u64(c, c->a.C, N, 0);
     if (n) {
-        hipLaunchKernelGGL(k_scatter_membership, dim3(dense_grid(n)), dim3(kBlock), 0, c->stream, c->a.list[0], n,
-                           c->a.C);
         hipLaunchKernelGGL(k_scatter_positions, dim3(dense_grid(n)), dim3(kBlock), 0, c->stream, c->a.list[0], n,
                            c->a.pos);
         if (S_dev) {

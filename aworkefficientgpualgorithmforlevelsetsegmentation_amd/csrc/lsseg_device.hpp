@@ -2594,12 +2594,6 @@ __global__ __launch_bounds__(kBlock) void k_update_only(KArgs a, int op)
     }
 }
 
-// membership record of an injected list
-__global__ void k_scatter_membership(const uint32_t* list, uint32_t n, uint64_t* C)
-{
-    LSSEG_GRID_STRIDE(p, n) C[list[p]] = 1ull;
-}
-
 // list positions of an injected list
 __global__ void k_scatter_positions(const uint32_t* list, uint32_t n, uint32_t* pos)
 {
