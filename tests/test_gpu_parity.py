@@ -309,6 +309,28 @@ def test_many_calls_of_mixed_schedules_match_one_call(monkeypatch):
     ctx.close()
 
 
+@pytest.mark.parametrize("keep_grid,scan_grid", [("16", 16), ("24", 32)], ids=["keep-16", "keep-24-scan-32"])
+def test_narrow_deferred_keep_grids_match_the_oracle(monkeypatch, keep_grid, scan_grid):
+    """The deferred keep step on a grid narrow enough that each of its chunks
+    holds many sub-tiles of 256 positions (LSSEG_KEEP_GRID with a small scan
+    grid, which the keep grid never undercuts), and the update launch that
+    rebuilds the traversal from those chunks: states equal the oracle's."""
+    monkeypatch.setenv("LSSEG_KEEP_GRID", keep_grid)
+    w = synth.ellipsoids(edge=64, iterations=80)
+    _say(f"keep grid {keep_grid}, scan grid {scan_grid}, {w.iterations} iterations")
+    ctx = Context(w.volume)
+    ctx.set_tuning(0, scan_grid)
+    ctx.set_phi(w.phi0, w.band)
+    ref = O.State(w.volume)
+    ref.set_phi(w.phi0, w.band)
+    for leg in (33, 47):
+        ctx.iterate(w.T, w.eps, w.alpha, w.dt, leg)
+        ref.iterate(w.T, w.eps, w.alpha, w.dt, leg)
+        _assert_same_state(ctx, ref, f"keep grid {keep_grid} after {leg}")
+    assert int(ctx.iteration_counts().n.max()) > 16 * 256 * 2, "chunks should hold several sub-tiles"
+    ctx.close()
+
+
 @pytest.mark.parametrize("pack", ["1", "0"], ids=["packed-counts", "one-count-per-word"])
 def test_compaction_tile_sizes_agree(monkeypatch, pack):
     """Few-path compactions take tiles of 1024 list positions, or of 256 for
